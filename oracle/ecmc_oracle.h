@@ -1,0 +1,207 @@
+/*
+ * ecmc_oracle.h -- CPU restatement of JeLLyFysh's event-chain Monte Carlo inner loop.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: the parity tests, the smoke
+ * check and the CPU-baseline leg of the benchmark load it to compare against the HIP
+ * product path.  Nothing in the product package may include, link or call it.
+ *
+ * It restates, in plain C99 with IEEE double arithmetic and no FMA contraction, the
+ * following reference files (paths relative to the reference's jellyfysh/ package):
+ *   mediator/single_process_mediator.py:84-156       (leg loop)
+ *   activator/tag_activator.py:166-316               (create / trash by tag, LIFO pools)
+ *   activator/internal_state/single_active_cell_occupancy.py:95-237
+ *   activator/internal_state/cell_occupancy/cells/cuboid_cells.py:54-297
+ *   activator/internal_state/cell_occupancy/cells/cuboid_periodic_cells.py:74-207
+ *   activator/tagger/excluded_cells_tagger.py:107-132, cell_boundary_tagger.py:147-149
+ *   setting/hypercubic_setting.py:101-191            (Python float % semantics)
+ *   base/time.py:87-182                              (Time = divmod quotient/remainder)
+ *   base/vectors.py:29-214
+ *   event_handler/two_leaf_unit_event_handler.py:105-154
+ *   event_handler/two_leaf_unit_bounding_potential_event_handler.py:112-168
+ *   event_handler/abstracts/event_handler_with_bounding_potential.py:75-101
+ *   event_handler/cell_boundary_event_handler.py:98-173
+ *   event_handler/abstracts/end_of_chain_event_handler.py:80-187
+ *   event_handler/single_independent_active_periodic_direction_end_of_chain_event_handler.py:87-237
+ *   event_handler/initial_chain_start_of_run_event_handler.py:97-131
+ *   event_handler/final_time_end_of_run_event_handler.py:51-103
+ *   event_handler/fixed_interval_sampling_event_handler.py:84-109
+ *   potential/hard_sphere_potential.py:65-99, inverse_power_potential.py:71-179,
+ *   potential/lennard_jones_potential.py:63-135, potential/abstracts.py:80-581,
+ *   potential/displaced_even_power_potential.py:73-142,
+ *   potential/inverse_power_coulomb_bounding_potential/...c:53-139,
+ *   potential/merged_image_coulomb_potential/...c:77-274
+ *   scheduler/list_scheduler.py:116-175, scheduler/heap_scheduler/heap.c:156-304
+ * plus CPython's random module (MT19937 init_by_array, random(), expovariate, uniform,
+ * randint via getrandbits) that the reference draws from.
+ *
+ * Math modes: ORA_MATH_LIBM calls glibc's pow/log/exp/erfc exactly where the reference
+ * (CPython `**`, math.log, C libm) does; ORA_MATH_CR replaces pow/log with quad-precision
+ * (libquadmath) evaluations rounded once to double, i.e. correctly rounded results.
+ */
+#ifndef ECMC_ORACLE_H
+#define ECMC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORA_MATH_LIBM = 0, ORA_MATH_CR = 1 };
+
+/* ---- CPython-compatible Mersenne twister ------------------------------------------ */
+typedef struct { uint32_t mt[624]; int32_t mti; } ora_mt;
+void ora_mt_seed(ora_mt *r, uint64_t seed);
+uint32_t ora_mt_u32(ora_mt *r);
+double ora_random(ora_mt *r);
+double ora_expovariate(ora_mt *r, double lambd, int math_mode);
+double ora_uniform(ora_mt *r, double a, double b);
+int64_t ora_randint(ora_mt *r, int64_t a, int64_t b);
+
+/* ---- Python float semantics --------------------------------------------------------- */
+double ora_py_mod(double x, double y);
+void ora_py_divmod(double x, double y, double *q, double *r);
+void ora_time_add(double q, double r, double dt, double *oq, double *or_);
+double ora_time_sub(double q1, double r1, double q2, double r2);
+
+/* ---- math switch ---------------------------------------------------------------------- */
+double ora_pow(double x, double y, int math_mode);
+double ora_log(double x, int math_mode);
+
+/* ---- potentials (scalar, reference argument conventions) --------------------------- */
+/* HardSpherePotential.displacement(velocity, separation) */
+double ora_hs_displacement(int dim, const double *velocity, const double *separation, double radius);
+/* InversePowerPotential standard_velocity_* with direction of motion `dir` and speed. */
+double ora_ip_displacement(int dim, int dir, double speed, const double *separation, double c1, double c2,
+                           double power, double prefactor, double potential_change, int math_mode);
+double ora_ip_derivative(int dim, int dir, double speed, const double *separation, double c1, double c2,
+                         double power, double prefactor, int math_mode);
+double ora_ip_potential(int dim, const double *separation, double charge_product, double power, double prefactor,
+                        int math_mode);
+/* Truncated and shifted inverse power (no reference counterpart; cutoff rc). */
+double ora_ipt_displacement(int dim, int dir, double speed, const double *separation, double c1, double c2,
+                            double power, double prefactor, double cutoff, double potential_change, int math_mode);
+/* LennardJonesPotential (Mexican hat) */
+double ora_lj_displacement(int dim, int dir, double speed, const double *separation, double prefactor,
+                           double sigma, double potential_change, int math_mode);
+double ora_lj_derivative(int dim, int dir, double speed, const double *separation, double prefactor,
+                         double sigma, int math_mode);
+/* DisplacedEvenPowerPotential (Mexican hat) */
+double ora_dep_displacement(int dim, int dir, double speed, const double *separation, double r0, int power,
+                            double prefactor, double potential_change, int math_mode);
+double ora_dep_derivative(int dim, int dir, double speed, const double *separation, double r0, int power,
+                          double prefactor, int math_mode);
+/* InversePowerCoulombBoundingPotential (x is the direction of motion; use permuted separations) */
+double ora_ipcb_derivative(double prefactor_product, double sx, double sy, double sz);
+double ora_ipcb_displacement(double prefactor_product, double sx, double sy, double sz, double potential_change,
+                             double system_length);
+/* MergedImageCoulombPotential derivative along +x (the C kernel; no prefactor/charges) */
+typedef struct ora_ewald ora_ewald;
+ora_ewald *ora_ewald_create(int fourier_cutoff, int position_cutoff, double alpha, double system_length);
+void ora_ewald_destroy(ora_ewald *e);
+double ora_ewald_derivative(const ora_ewald *e, double sx, double sy, double sz);
+
+/* ---- cells ---------------------------------------------------------------------------- */
+typedef struct ora_cells ora_cells;
+ora_cells *ora_cells_create(int dim, double system_length, const int *cells_per_side, int neighbor_layers);
+void ora_cells_destroy(ora_cells *c);
+int ora_cells_count(const ora_cells *c);
+void ora_cells_bounds(const ora_cells *c, int cell, double *cell_min, double *cell_max);
+int ora_cells_position_to_cell(const ora_cells *c, const double *position);
+int ora_cells_nearby(const ora_cells *c, int cell, int *out); /* canonical order, returns count */
+int ora_cells_neighbor(const ora_cells *c, int cell, int direction, int positive);
+
+/* ---- the chain engine ---------------------------------------------------------------- */
+enum {
+    ORA_TAG_PAIR_CELLS = 0,   /* ExcludedCellsTagger -> pair event handler */
+    ORA_TAG_PAIR_ALL = 1,     /* every other unit in index order -> pair event handler */
+    ORA_TAG_CELL_BOUNDARY = 2,
+    ORA_TAG_SAMPLING = 3,
+    ORA_TAG_END_OF_CHAIN = 4,
+    ORA_TAG_END_OF_RUN = 5,
+    ORA_TAG_START_OF_RUN = 6
+};
+enum { ORA_EH_TWO_LEAF = 0, ORA_EH_TWO_LEAF_BOUNDING = 1 };
+enum {
+    ORA_POT_HARD_SPHERE = 0,      /* par: radius */
+    ORA_POT_INVERSE_POWER = 1,    /* par: power, prefactor */
+    ORA_POT_LENNARD_JONES = 2,    /* par: prefactor, sigma */
+    ORA_POT_INVERSE_POWER_TRUNC = 3, /* par: power, prefactor, cutoff */
+    ORA_POT_MERGED_IMAGE_COULOMB = 4, /* par: alpha, fourier_cutoff, position_cutoff, prefactor */
+    ORA_POT_IPCB = 5              /* par: prefactor */
+};
+enum { ORA_SCHED_LIST = 0, ORA_SCHED_HEAP = 1 };
+
+#define ORA_MAX_TAGGERS 16
+typedef struct {
+    int32_t kind;
+    int32_t event_handler;
+    int32_t potential, bounding_potential;
+    double par[4], bpar[4];
+    int32_t use_charge;
+    int32_t n_create, create[ORA_MAX_TAGGERS];
+    int32_t n_trash, trash[ORA_MAX_TAGGERS];
+    int32_t n_handlers;          /* pool size (number_event_handlers) */
+    double value;                /* sampling_interval | chain_time | end_of_run_time | speed */
+    int32_t ivalue;              /* first_event_time_zero | initial_direction_of_motion */
+    int32_t ivalue2;             /* initial_active_identifier */
+} ora_tagger;
+
+typedef struct {
+    int32_t dim, n;
+    double system_length, beta;
+    int32_t cells_per_side[3];
+    int32_t neighbor_layers;
+    int32_t max_occupants;       /* <= 0: unbounded */
+    int32_t scheduler;
+    int32_t math_mode;
+    uint64_t seed;
+    int32_t n_taggers;
+    ora_tagger taggers[ORA_MAX_TAGGERS];
+} ora_config;
+
+/* One committed leg.  `tagger` is the tagger index of the event handler that committed. */
+typedef struct {
+    int32_t tagger;
+    int32_t flags;      /* bit0: lifting happened (velocity changed owner) */
+    int32_t active;     /* active unit before the event */
+    int32_t target;     /* pair target / new active chosen by end-of-chain / -1 */
+    double tq, tr;      /* event time (Time quotient, remainder) */
+    double pos[3];      /* committed position of the previously active unit */
+} ora_event;
+
+typedef struct ora_engine ora_engine;
+ora_engine *ora_engine_create(const ora_config *cfg, const double *positions, const double *charges);
+void ora_engine_destroy(ora_engine *e);
+/* Run until `max_events` legs were committed or the end-of-run event fired.  Returns the
+ * number of legs written to `log` (may be NULL), or a negative error code. */
+int64_t ora_engine_run(ora_engine *e, int64_t max_events, ora_event *log);
+int ora_engine_done(const ora_engine *e);
+int ora_engine_error(const ora_engine *e, char *buf, int buflen);
+void ora_engine_positions(const ora_engine *e, double *out);
+int32_t ora_engine_active(const ora_engine *e, int32_t *direction, double *tq, double *tr);
+/* Estimator sums (pair-lift separations along the direction of motion, chain displacement,
+ * counts of committed pair/cell-boundary/end-of-chain legs, lifts). */
+void ora_engine_estimators(const ora_engine *e, double *out8);
+/* Candidate trace of the last leg that created pair candidates: count and the
+ * target ids in push order (up to cap). */
+int32_t ora_engine_last_candidates(const ora_engine *e, int32_t *out, int32_t cap);
+/* Number of random words consumed so far. */
+int64_t ora_engine_rng_words(const ora_engine *e);
+
+/* Plain reference-wiring helpers for the benchmark configurations. */
+void ora_config_hard_disks(ora_config *c, int n, double system_length, double radius, int cells_per_side,
+                           double chain_time, double end_time, uint64_t seed);
+void ora_config_inverse_power_pairs(ora_config *c, int dim, int n, double system_length, double beta,
+                                    int cells_per_side, int n_factors, const double *powers,
+                                    const double *prefactors, double cutoff, double chain_time, double end_time,
+                                    uint64_t seed, int math_mode);
+
+/* Independent replicas on host threads (the CPU baseline).  Each thread copies `cfg` with
+ * seed+i and runs `events_per_replica` legs.  Returns total committed legs; wall time in *seconds. */
+int64_t ora_run_threads(const ora_config *cfg, const double *positions, const double *charges, int n_threads,
+                        int64_t events_per_replica, double *seconds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,745 @@
+/*
+ * ecmc_oracle_engine.c -- cells, cell occupancy, taggers, schedulers and the leg loop.
+ * TEST INFRASTRUCTURE ONLY (see ecmc_oracle.h).
+ */
+#include "ecmc_oracle.h"
+#include <math.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+static int time_lt_pub(double q1, double r1, double q2, double r2) { return q1 < q2 || (q1 == q2 && r1 < r2); }
+
+/* ===== CuboidPeriodicCells (cuboid_cells.py:54-297, cuboid_periodic_cells.py:74-207) === */
+struct ora_cells {
+    int dim, layers;
+    int cps[3];
+    long cum[3];
+    double side[3];
+    double L;
+    int count;
+    double *cmin, *cmax;     /* count * dim */
+    int *nearby_start, *nearby; /* canonical insertion order of _yield_nearby_cells, deduplicated */
+};
+
+static int trunc_int(double x) { return (int)x; } /* Python int() truncates toward zero */
+
+ora_cells *ora_cells_create(int dim, double L, const int *cells_per_side, int layers) {
+    ora_cells *c = (ora_cells *)calloc(1, sizeof(ora_cells));
+    c->dim = dim; c->layers = layers; c->L = L;
+    for (int d = 0; d < dim; d++) {
+        c->cps[d] = cells_per_side[d];
+        c->side[d] = L / c->cps[d];
+    }
+    c->cum[0] = 1;
+    for (int d = 1; d < dim; d++) c->cum[d] = c->cum[d - 1] * c->cps[d - 1];
+    c->count = (int)(c->cum[dim - 1] * c->cps[dim - 1]);
+    c->cmin = (double *)malloc(sizeof(double) * c->count * dim);
+    c->cmax = (double *)malloc(sizeof(double) * c->count * dim);
+    for (int cell = 0; cell < c->count; cell++) {
+        for (int d = 0; d < dim; d++) {
+            int id = (int)((cell / c->cum[d]) % c->cps[d]);
+            double lo = id * c->side[d];
+            double hi = (id + 1) * c->side[d];
+            if (lo > 0.0) {
+                while (trunc_int(lo / c->side[d]) == id) lo = nextafter(lo, -INFINITY);
+                while (trunc_int(lo / c->side[d]) < id) lo = nextafter(lo, INFINITY);
+            }
+            while (trunc_int(hi / c->side[d]) == id) hi = nextafter(hi, INFINITY);
+            while (trunc_int(hi / c->side[d]) > id) hi = nextafter(hi, -INFINITY);
+            c->cmin[cell * dim + d] = lo;
+            c->cmax[cell * dim + d] = hi;
+        }
+    }
+    int span = 2 * layers + 1;
+    int per = 1;
+    for (int d = 0; d < dim; d++) per *= span;
+    c->nearby_start = (int *)malloc(sizeof(int) * (c->count + 1));
+    c->nearby = (int *)malloc(sizeof(int) * (size_t)c->count * per);
+    int fill = 0;
+    for (int cell = 0; cell < c->count; cell++) {
+        c->nearby_start[cell] = fill;
+        int id[3];
+        for (int d = 0; d < dim; d++) id[d] = (int)((cell / c->cum[d]) % c->cps[d]);
+        /* itertools.product order: the last dimension varies fastest */
+        for (int t = 0; t < per; t++) {
+            int rem = t, off[3];
+            for (int d = dim - 1; d >= 0; d--) { off[d] = rem % span - layers; rem /= span; }
+            long idx = 0;
+            for (int d = 0; d < dim; d++) {
+                int v = id[d] + off[d];
+                v = ((v % c->cps[d]) + c->cps[d]) % c->cps[d];
+                idx += v * c->cum[d];
+            }
+            int dup = 0;
+            for (int q = c->nearby_start[cell]; q < fill; q++)
+                if (c->nearby[q] == idx) { dup = 1; break; }
+            if (!dup) c->nearby[fill++] = (int)idx;
+        }
+    }
+    c->nearby_start[c->count] = fill;
+    return c;
+}
+
+void ora_cells_destroy(ora_cells *c) {
+    if (!c) return;
+    free(c->cmin); free(c->cmax); free(c->nearby_start); free(c->nearby); free(c);
+}
+int ora_cells_count(const ora_cells *c) { return c->count; }
+void ora_cells_bounds(const ora_cells *c, int cell, double *mn, double *mx) {
+    for (int d = 0; d < c->dim; d++) { mn[d] = c->cmin[cell * c->dim + d]; mx[d] = c->cmax[cell * c->dim + d]; }
+}
+int ora_cells_position_to_cell(const ora_cells *c, const double *p) {
+    long idx = 0;
+    for (int d = 0; d < c->dim; d++) idx += (long)trunc_int(p[d] / c->side[d]) * c->cum[d];
+    if (idx < 0 || idx >= c->count) return -1;
+    return (int)idx;
+}
+int ora_cells_nearby(const ora_cells *c, int cell, int *out) {
+    int n = c->nearby_start[cell + 1] - c->nearby_start[cell];
+    memcpy(out, c->nearby + c->nearby_start[cell], sizeof(int) * n);
+    return n;
+}
+int ora_cells_neighbor(const ora_cells *c, int cell, int direction, int positive) {
+    int id = (int)((cell / c->cum[direction]) % c->cps[direction]);
+    int nid = positive ? (id + 1) % c->cps[direction] : ((id - 1) % c->cps[direction] + c->cps[direction]) % c->cps[direction];
+    return (int)(cell + (long)(nid - id) * c->cum[direction]);
+}
+
+/* ===== small growable int list ========================================================= */
+typedef struct { int *v; int n, cap; } ilist;
+static void il_push(ilist *l, int x) {
+    if (l->n == l->cap) { l->cap = l->cap ? 2 * l->cap : 8; l->v = (int *)realloc(l->v, sizeof(int) * l->cap); }
+    l->v[l->n++] = x;
+}
+static int il_remove(ilist *l, int x) { /* list.remove: first occurrence */
+    for (int i = 0; i < l->n; i++)
+        if (l->v[i] == x) { memmove(l->v + i, l->v + i + 1, sizeof(int) * (l->n - i - 1)); l->n--; return 1; }
+    return 0;
+}
+
+/* ===== handlers ======================================================================== */
+typedef struct {
+    int tagger;
+    int pending;           /* event currently stored in the scheduler */
+    double tq, tr;
+    int target;            /* pair target or new active (end of chain) */
+    /* in-state copy of the active unit, already time-sliced to the event time where the
+     * reference slices it in send_event_time */
+    double apos[3]; int aid; int adir; double aspeed; double aq, ar;
+    double tpos[3];
+    int cb_direction; double cb_boundary;
+    double last_q, last_r;  /* end of chain: _last_committed_event_time */
+    double samp_q, samp_r;  /* sampling: running _event_time */
+    uint32_t heap_counter;  /* heap scheduler: _minimal_valid_counter */
+} handler;
+
+typedef struct { double q, r; int h; uint32_t counter; } heap_entry;
+
+struct ora_engine {
+    ora_config cfg;
+    int dim, n;
+    double L, half_L;
+    double *pos;            /* n * dim */
+    double *charge;
+    int active; int adir; double aspeed; double aq, ar;   /* lifting state */
+    ora_cells *cells;
+    ilist *occ;             /* per cell */
+    ilist *surplus;         /* per cell */
+    int occ_active_id, occ_active_cell;
+    ora_mt rng;
+    handler *h; int nh;
+    ilist *running, *idle;  /* per tagger */
+    int started, done, err;
+    char errmsg[256];
+    /* list scheduler */
+    int *lst; int nlst, caplst;
+    /* heap scheduler */
+    heap_entry *heap; int hlen, hcap;
+    double last_q, last_r;
+    ora_ewald *ewald[ORA_MAX_TAGGERS];
+    double est[8];
+    int32_t *cand; int ncand, capcand;
+    int64_t rng_words;
+};
+
+static void eng_fail(ora_engine *e, const char *msg) {
+    if (!e->err) { e->err = 1; snprintf(e->errmsg, sizeof(e->errmsg), "%s", msg); }
+}
+
+/* --- schedulers ------------------------------------------------------------------------ */
+static void sched_push(ora_engine *e, int hidx, double q, double r) {
+    handler *hd = &e->h[hidx];
+    hd->tq = q; hd->tr = r; hd->pending = 1;
+    if (e->cfg.scheduler == ORA_SCHED_LIST) {
+        if (e->nlst == e->caplst) { e->caplst = e->caplst ? 2 * e->caplst : 64; e->lst = (int *)realloc(e->lst, sizeof(int) * e->caplst); }
+        e->lst[e->nlst++] = hidx;
+        return;
+    }
+    /* heap_scheduler.py:139 drops infinite times; heap.c:156-189 insert */
+    if (!time_lt_pub(q, r, INFINITY, INFINITY)) return;
+    if (e->hlen + 2 > e->hcap) {
+        int old = e->hcap;
+        e->hcap = e->hcap ? e->hcap * 2 : 64;
+        e->heap = (heap_entry *)realloc(e->heap, sizeof(heap_entry) * e->hcap);
+        if (!old) { e->heap[0].q = -INFINITY; e->heap[0].r = -INFINITY; e->heap[0].h = -1; e->hlen = 1; }
+    }
+    int pos = e->hlen++;
+    int par = pos >> 1;
+    while (q < e->heap[par].q || (q == e->heap[par].q && r < e->heap[par].r)) {
+        e->heap[pos] = e->heap[par];
+        pos = par; par = pos >> 1;
+    }
+    e->heap[pos].q = q; e->heap[pos].r = r; e->heap[pos].h = hidx; e->heap[pos].counter = hd->heap_counter;
+}
+static void heap_bubble_down(ora_engine *e, int pos) {
+    while (pos < e->hlen) {
+        int cmp = e->hlen, ch = pos << 1;
+        if (ch < e->hlen && (e->heap[ch].q < e->heap[cmp].q || (e->heap[ch].q == e->heap[cmp].q && e->heap[ch].r < e->heap[cmp].r))) cmp = ch;
+        if (ch + 1 < e->hlen && (e->heap[ch + 1].q < e->heap[cmp].q || (e->heap[ch + 1].q == e->heap[cmp].q && e->heap[ch + 1].r < e->heap[cmp].r))) cmp = ch + 1;
+        e->heap[pos] = e->heap[cmp];
+        pos = cmp;
+    }
+}
+static int sched_pop_min(ora_engine *e) {
+    int best = -1;
+    if (e->cfg.scheduler == ORA_SCHED_LIST) {
+        /* list_scheduler.py:129-156: min(..., key=time) keeps the first minimal element */
+        for (int i = 0; i < e->nlst; i++) {
+            handler *c = &e->h[e->lst[i]];
+            if (best < 0 || time_lt_pub(c->tq, c->tr, e->h[best].tq, e->h[best].tr)) best = e->lst[i];
+        }
+    } else {
+        /* heap.c:263-276 with heap_scheduler.py:340-362 (stale when counter moved on) */
+        while (e->hlen > 1 && e->h[e->heap[1].h].heap_counter > e->heap[1].counter) {
+            e->heap[1] = e->heap[--e->hlen];
+            heap_bubble_down(e, 1);
+        }
+        if (e->hlen > 1) best = e->heap[1].h;
+    }
+    if (best < 0) { eng_fail(e, "SchedulerError: scheduler holds no events"); return -1; }
+    if (time_lt_pub(e->h[best].tq, e->h[best].tr, e->last_q, e->last_r)) {
+        eng_fail(e, "SchedulerError: event time decreased"); return -1;
+    }
+    e->last_q = e->h[best].tq; e->last_r = e->h[best].tr;
+    return best;
+}
+static void sched_trash(ora_engine *e, int hidx) {
+    if (e->cfg.scheduler == ORA_SCHED_LIST) {
+        for (int i = 0; i < e->nlst; i++)
+            if (e->lst[i] == hidx) { memmove(e->lst + i, e->lst + i + 1, sizeof(int) * (e->nlst - i - 1)); e->nlst--; e->h[hidx].pending = 0; return; }
+        eng_fail(e, "SchedulerError: event handler not present in scheduler");
+        return;
+    }
+    e->h[hidx].heap_counter++;
+    e->h[hidx].pending = 0;
+}
+
+/* --- occupancy (single_active_cell_occupancy.py:95-203) ------------------------------- */
+static int occ_bounded(const ora_engine *e) { return e->cfg.max_occupants > 0; }
+static void occ_insert(ora_engine *e, int cell, int id) {
+    if (!occ_bounded(e) || e->occ[cell].n < e->cfg.max_occupants) il_push(&e->occ[cell], id);
+    else il_push(&e->surplus[cell], id);
+}
+static void occ_update(ora_engine *e) {
+    int id = e->active;
+    if (id != e->occ_active_id) {
+        if (e->occ_active_id >= 0) occ_insert(e, e->occ_active_cell, e->occ_active_id);
+        e->occ_active_cell = ora_cells_position_to_cell(e->cells, e->pos + (size_t)id * e->dim);
+        if (e->occ_active_cell < 0) { eng_fail(e, "position outside of the cell system"); return; }
+        e->occ_active_id = id;
+        if (!il_remove(&e->occ[e->occ_active_cell], id)) {
+            /* the move from the surplus list into the occupant list never triggers in the
+             * reference (the surplus dict never holds an empty list), so only remove */
+            if (!il_remove(&e->surplus[e->occ_active_cell], id)) eng_fail(e, "active unit not found in occupancy");
+        }
+    } else {
+        e->occ_active_cell = ora_cells_position_to_cell(e->cells, e->pos + (size_t)id * e->dim);
+        if (e->occ_active_cell < 0) eng_fail(e, "position outside of the cell system");
+    }
+}
+
+/* --- time slicing (event_handler/abstracts/abstracts.py:82-95) ------------------------ */
+static void slice_pos(const ora_engine *e, double *p, int dir, double speed, double q0, double r0, double q1, double r1) {
+    double dt = ora_time_sub(q1, r1, q0, r0);
+    for (int d = 0; d < e->dim; d++) {
+        double v = d == dir ? speed : 0.0;
+        p[d] = ora_py_mod(p[d] + v * dt, e->L);
+    }
+}
+
+static void separation_vector(const ora_engine *e, const double *a, const double *t, double *s) {
+    for (int d = 0; d < e->dim; d++)
+        s[d] = ora_py_mod((t[d] - a[d]) + e->half_L, e->L) - e->half_L;
+}
+
+/* --- pair event handlers --------------------------------------------------------------- */
+static double pair_displacement(ora_engine *e, const ora_tagger *tg, int tidx, const double *s, int a, int t) {
+    int mm = e->cfg.math_mode;
+    int dir = e->adir; double speed = e->aspeed;
+    double c1 = tg->use_charge ? e->charge[a] : 1.0;
+    double c2 = tg->use_charge ? e->charge[t] : 1.0;
+    if (tg->event_handler == ORA_EH_TWO_LEAF) {
+        switch (tg->potential) {
+        case ORA_POT_HARD_SPHERE: {
+            double v[3] = {0, 0, 0};
+            v[dir] = speed;
+            return ora_hs_displacement(e->dim, v, s, tg->par[0]);
+        }
+        case ORA_POT_INVERSE_POWER: {
+            double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
+            e->rng_words += 2;
+            return ora_ip_displacement(e->dim, dir, speed, s, c1, c2, tg->par[0], tg->par[1], dU, mm);
+        }
+        case ORA_POT_INVERSE_POWER_TRUNC: {
+            double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
+            e->rng_words += 2;
+            return ora_ipt_displacement(e->dim, dir, speed, s, c1, c2, tg->par[0], tg->par[1], tg->par[2], dU, mm);
+        }
+        case ORA_POT_LENNARD_JONES: {
+            double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
+            e->rng_words += 2;
+            return ora_lj_displacement(e->dim, dir, speed, s, tg->par[0], tg->par[1], dU, mm);
+        }
+        default: eng_fail(e, "unsupported potential for TwoLeafUnitEventHandler"); return INFINITY;
+        }
+    }
+    if (tg->event_handler == ORA_EH_TWO_LEAF_BOUNDING && tg->bounding_potential == ORA_POT_IPCB) {
+        /* two_leaf_unit_bounding_potential_event_handler.py:138-143 */
+        double p[3];
+        for (int d = 0; d < 3; d++) p[d] = s[(dir + d) % 3];    /* vectors.permutation_3d */
+        double pp = tg->bpar[0] * c1 * c2;
+        double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
+        e->rng_words += 2;
+        return ora_ipcb_displacement(pp, p[0], p[1], p[2], dU, e->L) / speed;
+    }
+    (void)tidx;
+    eng_fail(e, "unsupported event handler");
+    return INFINITY;
+}
+
+/* --- leg helpers ----------------------------------------------------------------------- */
+static int pool_pop(ora_engine *e, int t) {
+    ilist *idle = &e->idle[t];
+    if (idle->n == 0) { eng_fail(e, "TagActivatorError: not-running event handler list is empty"); return -1; }
+    int hidx = idle->v[--idle->n];
+    il_push(&e->running[t], hidx);
+    return hidx;
+}
+
+static void create_tagger(ora_engine *e, int t) {
+    ora_tagger *tg = &e->cfg.taggers[t];
+    int a = e->active;
+    switch (tg->kind) {
+    case ORA_TAG_PAIR_CELLS:
+    case ORA_TAG_PAIR_ALL: {
+        if (a < 0) return;
+        int cells[27 * 27];
+        int ncells = 0;
+        if (tg->kind == ORA_TAG_PAIR_CELLS) {
+            if (e->occ_active_cell < 0) return;
+            ncells = ora_cells_nearby(e->cells, e->occ_active_cell, cells);
+        }
+        int total = tg->kind == ORA_TAG_PAIR_CELLS ? 0 : e->n - 1;
+        if (tg->kind == ORA_TAG_PAIR_CELLS)
+            for (int c = 0; c < ncells; c++) total += e->occ[cells[c]].n;
+        int k = 0;
+        for (int c = 0; (tg->kind == ORA_TAG_PAIR_CELLS ? c < ncells : c < 1); c++) {
+            int cnt = tg->kind == ORA_TAG_PAIR_CELLS ? e->occ[cells[c]].n : e->n;
+            for (int o = 0; o < cnt; o++) {
+                int tid = tg->kind == ORA_TAG_PAIR_CELLS ? e->occ[cells[c]].v[o] : o;
+                if (tid == a) continue;
+                int hidx = pool_pop(e, t);
+                if (hidx < 0) return;
+                handler *hd = &e->h[hidx];
+                hd->aid = a; hd->adir = e->adir; hd->aspeed = e->aspeed; hd->aq = e->aq; hd->ar = e->ar;
+                memcpy(hd->apos, e->pos + (size_t)a * e->dim, sizeof(double) * e->dim);
+                memcpy(hd->tpos, e->pos + (size_t)tid * e->dim, sizeof(double) * e->dim);
+                hd->target = tid;
+                double s[3];
+                separation_vector(e, hd->apos, hd->tpos, s);
+                double dt = pair_displacement(e, tg, t, s, a, tid);
+                double q, r;
+                ora_time_add(e->aq, e->ar, dt, &q, &r);
+                /* send_event_time time-slices the in-state copies to the event time */
+                if (!isinf(q)) slice_pos(e, hd->apos, e->adir, e->aspeed, e->aq, e->ar, q, r);
+                hd->aq = q; hd->ar = r;
+                sched_push(e, hidx, q, r);
+                if (e->ncand == e->capcand) { e->capcand = e->capcand ? 2 * e->capcand : 1024; e->cand = (int32_t *)realloc(e->cand, sizeof(int32_t) * e->capcand); }
+                e->cand[e->ncand++] = tid;
+                k++;
+            }
+        }
+        (void)total;
+        break;
+    }
+    case ORA_TAG_CELL_BOUNDARY: {
+        if (a < 0) return;
+        int hidx = pool_pop(e, t);
+        if (hidx < 0) return;
+        handler *hd = &e->h[hidx];
+        hd->aid = a; hd->adir = e->adir; hd->aspeed = e->aspeed; hd->aq = e->aq; hd->ar = e->ar;
+        memcpy(hd->apos, e->pos + (size_t)a * e->dim, sizeof(double) * e->dim);
+        /* cell_boundary_event_handler.py:98-156 (velocity has a single positive component) */
+        int cell = ora_cells_position_to_cell(e->cells, hd->apos);
+        if (cell < 0) { eng_fail(e, "position outside of the cell system"); return; }
+        int nb = ora_cells_neighbor(e->cells, cell, e->adir, 1);
+        double mn[3], mx[3];
+        ora_cells_bounds(e->cells, nb, mn, mx);
+        double boundary = mn[e->adir];
+        double sep = boundary - hd->apos[e->adir];
+        if (sep < 0.0) sep = sep + e->L;
+        double dt = sep / e->aspeed;
+        hd->cb_boundary = boundary; hd->cb_direction = e->adir;
+        double q, r;
+        ora_time_add(e->aq, e->ar, dt, &q, &r);
+        sched_push(e, hidx, q, r);
+        break;
+    }
+    case ORA_TAG_SAMPLING: {
+        int hidx = pool_pop(e, t);
+        if (hidx < 0) return;
+        handler *hd = &e->h[hidx];
+        double q, r;
+        ora_time_add(hd->samp_q, hd->samp_r, tg->value, &q, &r);
+        hd->samp_q = q; hd->samp_r = r;
+        sched_push(e, hidx, q, r);
+        break;
+    }
+    case ORA_TAG_END_OF_CHAIN: {
+        if (a < 0) return;
+        int hidx = pool_pop(e, t);
+        if (hidx < 0) return;
+        handler *hd = &e->h[hidx];
+        double since = ora_time_sub(e->aq, e->ar, hd->last_q, hd->last_r);
+        if (!(0.0 <= since && since <= tg->value)) { eng_fail(e, "end-of-chain assertion on chain time failed"); return; }
+        double chain = ora_time_sub(hd->last_q, hd->last_r, e->aq, e->ar) + tg->value;
+        double q, r;
+        ora_time_add(e->aq, e->ar, chain, &q, &r);
+        hd->target = (int)ora_randint(&e->rng, 0, e->n - 1);
+        sched_push(e, hidx, q, r);
+        break;
+    }
+    case ORA_TAG_END_OF_RUN: {
+        int hidx = pool_pop(e, t);
+        if (hidx < 0) return;
+        double q, r;
+        ora_py_divmod(tg->value, 1.0, &q, &r);   /* Time.from_float */
+        sched_push(e, hidx, q, r);
+        break;
+    }
+    case ORA_TAG_START_OF_RUN: {
+        int hidx = pool_pop(e, t);
+        if (hidx < 0) return;
+        sched_push(e, hidx, 0.0, 0.0);
+        break;
+    }
+    }
+}
+
+static void commit_active_slice(ora_engine *e, double q, double r) {
+    double *p = e->pos + (size_t)e->active * e->dim;
+    slice_pos(e, p, e->adir, e->aspeed, e->aq, e->ar, q, r);
+    e->aq = q; e->ar = r;
+}
+
+/* out-state + commit; fills the event record */
+static int out_state(ora_engine *e, int hidx, ora_event *ev) {
+    handler *hd = &e->h[hidx];
+    ora_tagger *tg = &e->cfg.taggers[hd->tagger];
+    ev->tagger = hd->tagger; ev->flags = 0; ev->active = e->active; ev->target = -1;
+    ev->tq = hd->tq; ev->tr = hd->tr;
+    int a = e->active;
+    switch (tg->kind) {
+    case ORA_TAG_PAIR_CELLS:
+    case ORA_TAG_PAIR_ALL: {
+        ev->target = hd->target;
+        int lift = 1;
+        if (tg->event_handler == ORA_EH_TWO_LEAF_BOUNDING) {
+            /* two_leaf_unit_bounding_potential_event_handler.py:148-168 and
+             * event_handler_with_bounding_potential.py:75-101 */
+            double s[3], p[3];
+            separation_vector(e, hd->apos, hd->tpos, s);
+            for (int d = 0; d < 3; d++) p[d] = s[(hd->adir + d) % 3];
+            double c1 = tg->use_charge ? e->charge[hd->aid] : 1.0;
+            double c2 = tg->use_charge ? e->charge[hd->target] : 1.0;
+            double bound = ora_ipcb_derivative(tg->bpar[0] * c1 * c2, p[0], p[1], p[2]) * hd->aspeed;
+            double real = tg->par[3] * c1 * c2 * ora_ewald_derivative(e->ewald[hd->tagger], p[0], p[1], p[2]) * hd->aspeed;
+            lift = 0;
+            if (real > 0) {
+                double u = ora_uniform(&e->rng, 0, bound);
+                e->rng_words += 2;
+                if (u < real) lift = 1;
+            }
+        }
+        memcpy(e->pos + (size_t)a * e->dim, hd->apos, sizeof(double) * e->dim);
+        e->aq = hd->aq; e->ar = hd->ar;
+        if (lift) {
+            double sep[3];
+            separation_vector(e, hd->apos, hd->tpos, sep);
+            e->est[0] += sep[hd->adir];
+            e->est[4] += 1.0;
+            e->active = hd->target;       /* velocity and time stamp move to the target */
+            ev->flags |= 1;
+        }
+        e->est[2] += 1.0;
+        break;
+    }
+    case ORA_TAG_CELL_BOUNDARY: {
+        double *p = e->pos + (size_t)a * e->dim;
+        memcpy(p, hd->apos, sizeof(double) * e->dim);
+        slice_pos(e, p, hd->adir, hd->aspeed, hd->aq, hd->ar, hd->tq, hd->tr);
+        p[hd->cb_direction] = hd->cb_boundary;
+        e->aq = hd->tq; e->ar = hd->tr;
+        e->est[3] += 1.0;
+        break;
+    }
+    case ORA_TAG_SAMPLING:
+        commit_active_slice(e, hd->tq, hd->tr);
+        break;
+    case ORA_TAG_END_OF_RUN:
+        commit_active_slice(e, hd->tq, hd->tr);
+        e->done = 1;
+        break;
+    case ORA_TAG_END_OF_CHAIN: {
+        commit_active_slice(e, hd->tq, hd->tr);
+        hd->last_q = hd->tq; hd->last_r = hd->tr;
+        ev->target = hd->target;
+        e->est[1] += tg->value * e->aspeed;
+        e->est[5] += 1.0;
+        e->adir = (e->adir + 1) % e->dim;
+        if (hd->target != a) ev->flags |= 1;
+        e->active = hd->target;
+        break;
+    }
+    case ORA_TAG_START_OF_RUN:
+        e->active = tg->ivalue2;
+        e->adir = tg->ivalue;
+        e->aspeed = tg->value;
+        e->aq = 0.0; e->ar = 0.0;
+        break;
+    }
+    if (a < 0) memset(ev->pos, 0, sizeof(ev->pos));
+    else memcpy(ev->pos, e->pos + (size_t)a * e->dim, sizeof(double) * e->dim);
+    return 0;
+}
+
+/* ===== engine API ======================================================================= */
+ora_engine *ora_engine_create(const ora_config *cfg, const double *positions, const double *charges) {
+    ora_engine *e = (ora_engine *)calloc(1, sizeof(ora_engine));
+    e->cfg = *cfg;
+    e->dim = cfg->dim; e->n = cfg->n; e->L = cfg->system_length; e->half_L = cfg->system_length / 2.0;
+    e->pos = (double *)malloc(sizeof(double) * e->n * e->dim);
+    memcpy(e->pos, positions, sizeof(double) * e->n * e->dim);
+    e->charge = (double *)malloc(sizeof(double) * e->n);
+    for (int i = 0; i < e->n; i++) e->charge[i] = charges ? charges[i] : 1.0;
+    e->active = -1; e->adir = 0; e->aspeed = 1.0;
+    e->last_q = -INFINITY; e->last_r = -INFINITY;
+    int cps[3] = {cfg->cells_per_side[0], cfg->cells_per_side[1], cfg->cells_per_side[2]};
+    for (int d = 1; d < e->dim; d++) if (cps[d] <= 0) cps[d] = cps[0];
+    e->cells = ora_cells_create(e->dim, e->L, cps, cfg->neighbor_layers);
+    int nc = ora_cells_count(e->cells);
+    e->occ = (ilist *)calloc(nc, sizeof(ilist));
+    e->surplus = (ilist *)calloc(nc, sizeof(ilist));
+    e->occ_active_id = -1; e->occ_active_cell = -1;
+    for (int i = 0; i < e->n; i++) {
+        int c = ora_cells_position_to_cell(e->cells, e->pos + (size_t)i * e->dim);
+        if (c < 0) { eng_fail(e, "initial position outside of the cell system"); return e; }
+        occ_insert(e, c, i);
+    }
+    ora_mt_seed(&e->rng, cfg->seed);
+    int total = 0;
+    for (int t = 0; t < cfg->n_taggers; t++) total += cfg->taggers[t].n_handlers > 0 ? cfg->taggers[t].n_handlers : 1;
+    e->h = (handler *)calloc(total, sizeof(handler));
+    e->nh = total;
+    e->running = (ilist *)calloc(cfg->n_taggers, sizeof(ilist));
+    e->idle = (ilist *)calloc(cfg->n_taggers, sizeof(ilist));
+    int k = 0;
+    for (int t = 0; t < cfg->n_taggers; t++) {
+        int nhh = cfg->taggers[t].n_handlers > 0 ? cfg->taggers[t].n_handlers : 1;
+        for (int j = 0; j < nhh; j++) {
+            e->h[k].tagger = t;
+            if (cfg->taggers[t].kind == ORA_TAG_SAMPLING && cfg->taggers[t].ivalue)
+                ora_py_divmod(-cfg->taggers[t].value, 1.0, &e->h[k].samp_q, &e->h[k].samp_r);
+            il_push(&e->idle[t], k++);
+        }
+        if (cfg->taggers[t].potential == ORA_POT_MERGED_IMAGE_COULOMB)
+            e->ewald[t] = ora_ewald_create((int)cfg->taggers[t].par[1], (int)cfg->taggers[t].par[2],
+                                           cfg->taggers[t].par[0], e->L);
+    }
+    return e;
+}
+
+void ora_engine_destroy(ora_engine *e) {
+    if (!e) return;
+    int nc = ora_cells_count(e->cells);
+    for (int c = 0; c < nc; c++) { free(e->occ[c].v); free(e->surplus[c].v); }
+    for (int t = 0; t < e->cfg.n_taggers; t++) { free(e->running[t].v); free(e->idle[t].v); ora_ewald_destroy(e->ewald[t]); }
+    free(e->occ); free(e->surplus); free(e->running); free(e->idle);
+    ora_cells_destroy(e->cells);
+    free(e->pos); free(e->charge); free(e->h); free(e->lst); free(e->heap); free(e->cand);
+    free(e);
+}
+
+int64_t ora_engine_run(ora_engine *e, int64_t max_events, ora_event *log) {
+    int64_t k = 0;
+    if (e->err) return -1;
+    while (k < max_events && !e->done) {
+        if (!e->started) {
+            /* tag_activator.py:166-226: the start-of-run tagger alone */
+            int sor = -1;
+            for (int t = 0; t < e->cfg.n_taggers; t++) if (e->cfg.taggers[t].kind == ORA_TAG_START_OF_RUN) sor = t;
+            if (sor < 0) { eng_fail(e, "ConfigurationError: a StartOfRunEventHandler is required"); return -1; }
+            create_tagger(e, sor);
+            e->started = 1;
+        }
+        int win = sched_pop_min(e);
+        if (win < 0) return -1;
+        ora_event ev;
+        out_state(e, win, &ev);
+        int wt = e->h[win].tagger;
+        /* trash (tag_activator.py:291-316) */
+        ora_tagger *tg = &e->cfg.taggers[wt];
+        for (int i = 0; i < tg->n_trash; i++) {
+            int t = tg->trash[i];
+            for (int j = 0; j < e->running[t].n; j++) {
+                int hidx = e->running[t].v[j];
+                sched_trash(e, hidx);
+                il_push(&e->idle[t], hidx);
+            }
+            e->running[t].n = 0;
+        }
+        if (e->err) return -1;
+        if (log) log[k] = ev;
+        k++;
+        if (e->done) break;
+        /* next leg: occupancy update then the creates of the committed tagger */
+        occ_update(e);
+        if (e->err) return -1;
+        int had_pairs = 0;
+        for (int i = 0; i < tg->n_create; i++) {
+            int t = tg->create[i];
+            int kind = e->cfg.taggers[t].kind;
+            if ((kind == ORA_TAG_PAIR_CELLS || kind == ORA_TAG_PAIR_ALL) && !had_pairs) { e->ncand = 0; had_pairs = 1; }
+            create_tagger(e, t);
+            if (e->err) return -1;
+        }
+    }
+    return k;
+}
+
+int ora_engine_done(const ora_engine *e) { return e->done; }
+int ora_engine_error(const ora_engine *e, char *buf, int buflen) {
+    if (buf && buflen > 0) snprintf(buf, buflen, "%s", e->errmsg);
+    return e->err;
+}
+void ora_engine_positions(const ora_engine *e, double *out) { memcpy(out, e->pos, sizeof(double) * e->n * e->dim); }
+int32_t ora_engine_active(const ora_engine *e, int32_t *direction, double *tq, double *tr) {
+    if (direction) *direction = e->adir;
+    if (tq) *tq = e->aq;
+    if (tr) *tr = e->ar;
+    return e->active;
+}
+void ora_engine_estimators(const ora_engine *e, double *out8) { memcpy(out8, e->est, sizeof(e->est)); }
+int32_t ora_engine_last_candidates(const ora_engine *e, int32_t *out, int32_t cap) {
+    int n = e->ncand < cap ? e->ncand : cap;
+    if (out) memcpy(out, e->cand, sizeof(int32_t) * n);
+    return e->ncand;
+}
+int64_t ora_engine_rng_words(const ora_engine *e) { return e->rng_words; }
+
+/* ===== configuration helpers ============================================================ */
+static void tg_lists(ora_tagger *t, int nc, const int *c, int ntr, const int *tr) {
+    t->n_create = nc; for (int i = 0; i < nc; i++) t->create[i] = c[i];
+    t->n_trash = ntr; for (int i = 0; i < ntr; i++) t->trash[i] = tr[i];
+}
+
+/* Wiring of config_files/hard_disk_dipoles/hard_disk_dipoles_cells.ini without the dipole
+ * and sampling taggers: pair(0), cell_boundary(1), end_of_chain(2), end_of_run(3), start_of_run(4). */
+void ora_config_hard_disks(ora_config *c, int n, double L, double radius, int cps, double chain_time,
+                           double end_time, uint64_t seed) {
+    memset(c, 0, sizeof(*c));
+    c->dim = 2; c->n = n; c->system_length = L; c->beta = 1.0;
+    c->cells_per_side[0] = c->cells_per_side[1] = cps; c->neighbor_layers = 1;
+    c->max_occupants = -1; c->scheduler = ORA_SCHED_LIST; c->math_mode = ORA_MATH_LIBM; c->seed = seed;
+    c->n_taggers = 5;
+    int pc[2] = {0, 1}, eoc[3] = {2, 0, 1}, all[5] = {2, 0, 1, 4, 3}, eor[1] = {3}, sorc[4] = {0, 1, 2, 3}, sort[1] = {4};
+    ora_tagger *t = c->taggers;
+    t[0].kind = ORA_TAG_PAIR_CELLS; t[0].event_handler = ORA_EH_TWO_LEAF; t[0].potential = ORA_POT_HARD_SPHERE;
+    t[0].par[0] = radius; t[0].n_handlers = n; tg_lists(&t[0], 2, pc, 2, pc);
+    t[1].kind = ORA_TAG_CELL_BOUNDARY; t[1].n_handlers = 1; tg_lists(&t[1], 2, pc, 2, pc);
+    t[2].kind = ORA_TAG_END_OF_CHAIN; t[2].n_handlers = 1; t[2].value = chain_time; tg_lists(&t[2], 3, eoc, 3, eoc);
+    t[3].kind = ORA_TAG_END_OF_RUN; t[3].n_handlers = 1; t[3].value = end_time; tg_lists(&t[3], 1, eor, 5, all);
+    t[4].kind = ORA_TAG_START_OF_RUN; t[4].n_handlers = 1; t[4].value = 1.0; t[4].ivalue = 0; t[4].ivalue2 = 0;
+    tg_lists(&t[4], 4, sorc, 1, sort);
+}
+
+/* Inverse-power factors (e.g. LJ 12-6 as p=12,k=4 and p=6,k=-4), each an ExcludedCellsTagger
+ * with a TwoLeafUnitEventHandler; pair taggers 0..F-1, cell_boundary F, end_of_chain F+1,
+ * end_of_run F+2, start_of_run F+3.  cutoff > 0 selects the truncated-shifted variant. */
+void ora_config_inverse_power_pairs(ora_config *c, int dim, int n, double L, double beta, int cps, int nf,
+                                    const double *powers, const double *prefactors, double cutoff, double chain_time,
+                                    double end_time, uint64_t seed, int math_mode) {
+    memset(c, 0, sizeof(*c));
+    c->dim = dim; c->n = n; c->system_length = L; c->beta = beta;
+    for (int d = 0; d < 3; d++) c->cells_per_side[d] = cps;
+    c->neighbor_layers = 1; c->max_occupants = -1; c->scheduler = ORA_SCHED_HEAP; c->math_mode = math_mode; c->seed = seed;
+    c->n_taggers = nf + 4;
+    int cb = nf, eoc = nf + 1, eor = nf + 2, sor = nf + 3;
+    int pc[ORA_MAX_TAGGERS], np = 0;
+    for (int f = 0; f < nf; f++) pc[np++] = f;
+    pc[np++] = cb;
+    int eocl[ORA_MAX_TAGGERS], ne = 0;
+    eocl[ne++] = eoc; for (int i = 0; i < np; i++) eocl[ne++] = pc[i];
+    int all[ORA_MAX_TAGGERS], na = 0;
+    for (int i = 0; i < ne; i++) all[na++] = eocl[i];
+    all[na++] = sor; all[na++] = eor;
+    int sorc[ORA_MAX_TAGGERS], ns = 0;
+    for (int i = 0; i < np; i++) sorc[ns++] = pc[i];
+    sorc[ns++] = eoc; sorc[ns++] = eor;
+    int eorl[1] = {eor}, sorl[1] = {sor};
+    ora_tagger *t = c->taggers;
+    for (int f = 0; f < nf; f++) {
+        t[f].kind = ORA_TAG_PAIR_CELLS; t[f].event_handler = ORA_EH_TWO_LEAF;
+        t[f].potential = cutoff > 0 ? ORA_POT_INVERSE_POWER_TRUNC : ORA_POT_INVERSE_POWER;
+        t[f].par[0] = powers[f]; t[f].par[1] = prefactors[f]; t[f].par[2] = cutoff;
+        t[f].n_handlers = n; tg_lists(&t[f], np, pc, np, pc);
+    }
+    t[cb].kind = ORA_TAG_CELL_BOUNDARY; t[cb].n_handlers = 1; tg_lists(&t[cb], np, pc, np, pc);
+    t[eoc].kind = ORA_TAG_END_OF_CHAIN; t[eoc].n_handlers = 1; t[eoc].value = chain_time; tg_lists(&t[eoc], ne, eocl, ne, eocl);
+    t[eor].kind = ORA_TAG_END_OF_RUN; t[eor].n_handlers = 1; t[eor].value = end_time; tg_lists(&t[eor], 1, eorl, na, all);
+    t[sor].kind = ORA_TAG_START_OF_RUN; t[sor].n_handlers = 1; t[sor].value = 1.0; t[sor].ivalue = 0; t[sor].ivalue2 = 0;
+    tg_lists(&t[sor], ns, sorc, 1, sorl);
+}
+
+/* ===== CPU baseline: independent replicas on host threads ============================== */
+typedef struct {
+    ora_config cfg; const double *pos; const double *charge; int64_t events; int64_t done;
+} thr_arg;
+static void *thr_main(void *p) {
+    thr_arg *a = (thr_arg *)p;
+    ora_engine *e = ora_engine_create(&a->cfg, a->pos, a->charge);
+    a->done = ora_engine_run(e, a->events, NULL);
+    ora_engine_destroy(e);
+    return NULL;
+}
+int64_t ora_run_threads(const ora_config *cfg, const double *positions, const double *charges, int n_threads,
+                        int64_t events_per_replica, double *seconds) {
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * n_threads);
+    thr_arg *args = (thr_arg *)calloc(n_threads, sizeof(thr_arg));
+    struct timespec t0, t1;
+    for (int i = 0; i < n_threads; i++) {
+        args[i].cfg = *cfg; args[i].cfg.seed = cfg->seed + (uint64_t)i;
+        args[i].pos = positions; args[i].charge = charges; args[i].events = events_per_replica;
+    }
+    clock_gettime(CLOCK_MONOTONIC, &t0);
+    for (int i = 0; i < n_threads; i++) pthread_create(&th[i], NULL, thr_main, &args[i]);
+    int64_t total = 0;
+    for (int i = 0; i < n_threads; i++) { pthread_join(th[i], NULL); total += args[i].done > 0 ? args[i].done : 0; }
+    clock_gettime(CLOCK_MONOTONIC, &t1);
+    if (seconds) *seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
+    free(th); free(args);
+    return total;
+}
