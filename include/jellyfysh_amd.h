@@ -1,0 +1,171 @@
+/*
+ * jellyfysh_amd.h -- C ABI of the MI355X ECMC hot path (libjellyfysh_amd.so).
+ *
+ * Plain C: every argument is a pointer, a size or a plain struct; no torch or HIP types.
+ * Functions return 0 on success and a negative ecmc_status otherwise; ecmc_last_error()
+ * gives the message of the last failure on the calling thread.  "No event" is +INF, as in
+ * the reference (a time that the heap scheduler never stores, heap_scheduler.py:139).
+ * Device pointers (`d_` prefix) are HIP device memory; a `stream` argument is a
+ * hipStream_t passed as void* (NULL = the default stream).
+ *
+ * Reference interfaces replaced (paths relative to the reference's jellyfysh/ package):
+ *   ecmc_system_*          the mediator leg loop with its activator, cell occupancy,
+ *                          taggers, event handlers and scheduler for one chain
+ *                          (mediator/single_process_mediator.py:84-156,
+ *                          activator/tag_activator.py:166-316,
+ *                          activator/internal_state/single_active_cell_occupancy.py:95-237,
+ *                          scheduler/heap_scheduler/heap.c:156-304 + list_scheduler.py:116-175)
+ *   ecmc_pair_displacement InvertiblePotential.displacement(velocity, separation, charge_one,
+ *                          charge_two, potential_change) (potential/potential.py:216-253) for
+ *                          the hard-sphere, inverse-power, Lennard-Jones, displaced-even-power
+ *                          and Coulomb-bounding potentials, batched
+ *   ecmc_pair_derivative   Potential.derivative(velocity, separation, charges)
+ *                          (potential/potential.py:153-181), batched; for the merged-image
+ *                          Coulomb potential this replaces merged_image_coulomb_potential.c:205-274
+ *   ecmc_math_log/pow      the libm log / pow the reference's Python calls through `**` and
+ *                          math.log (correctly rounded here)
+ *   ecmc_mt_*              CPython random (MT19937, random(), expovariate, randint) as the
+ *                          reference draws from it (two_leaf_unit_event_handler.py:98-103,
+ *                          single_independent_active_periodic_direction_end_of_chain_event_handler.py:229)
+ */
+#ifndef JELLYFYSH_AMD_H
+#define JELLYFYSH_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    ECMC_OK = 0,
+    ECMC_ERR_ARGUMENT = -1,
+    ECMC_ERR_HIP = -2,
+    ECMC_ERR_CONFIGURATION = -3, /* base.exceptions.ConfigurationError */
+    ECMC_ERR_SCHEDULER = -4,     /* base.exceptions.SchedulerError */
+    ECMC_ERR_TAG_ACTIVATOR = -5, /* base.exceptions.TagActivatorError */
+    ECMC_ERR_STATE = -6,         /* a position left the cell system or an occupancy list overflowed */
+    ECMC_ERR_NUMERIC = -7        /* the Python code would raise ValueError (sqrt of a negative number) */
+} ecmc_status;
+
+/* Tagger kinds (the modules of activator/tagger/) */
+enum {
+    ECMC_TAG_PAIR_CELLS = 0,     /* ExcludedCellsTagger: active x occupants of the nearby cells */
+    ECMC_TAG_PAIR_ALL = 1,       /* every other unit in identifier order */
+    ECMC_TAG_CELL_BOUNDARY = 2,  /* CellBoundaryTagger + CellBoundaryEventHandler */
+    ECMC_TAG_SAMPLING = 3,       /* NoInStateTagger + FixedIntervalSamplingEventHandler */
+    ECMC_TAG_END_OF_CHAIN = 4,   /* ActiveGlobalStateInStateTagger + periodic-direction end of chain */
+    ECMC_TAG_END_OF_RUN = 5,     /* NoInStateTagger + FinalTimeEndOfRunEventHandler */
+    ECMC_TAG_START_OF_RUN = 6    /* NoInStateTagger + InitialChainStartOfRunEventHandler */
+};
+/* Event handlers of pair taggers */
+enum { ECMC_EH_TWO_LEAF = 0, ECMC_EH_TWO_LEAF_BOUNDING = 1 };
+/* Potentials (the modules of potential/) */
+enum {
+    ECMC_POT_HARD_SPHERE = 0,              /* par: radius */
+    ECMC_POT_INVERSE_POWER = 1,            /* par: power, prefactor */
+    ECMC_POT_LENNARD_JONES = 2,            /* par: prefactor, characteristic_length */
+    ECMC_POT_INVERSE_POWER_TRUNC = 3,      /* par: power, prefactor, cutoff (no reference counterpart) */
+    ECMC_POT_MERGED_IMAGE_COULOMB = 4,     /* par: alpha, fourier_cutoff, position_cutoff, prefactor */
+    ECMC_POT_INVERSE_POWER_COULOMB_BOUND = 5, /* par: prefactor */
+    ECMC_POT_DISPLACED_EVEN_POWER = 6      /* par: equilibrium_separation, power, prefactor */
+};
+enum { ECMC_SCHED_LIST = 0, ECMC_SCHED_HEAP = 1 };
+
+#define ECMC_MAX_TAGGERS 16
+
+/* One tagger of the TagActivator with its event handler (an INI section pair). */
+typedef struct {
+    int32_t kind;
+    int32_t event_handler;
+    int32_t potential, bounding_potential;
+    double par[4], bpar[4];
+    int32_t use_charge;
+    int32_t n_create, create[ECMC_MAX_TAGGERS]; /* tagger indices, in INI order */
+    int32_t n_trash, trash[ECMC_MAX_TAGGERS];
+    int32_t n_handlers;                         /* number_event_handlers */
+    double value;   /* sampling_interval | chain_time | end_of_run_time | speed */
+    int32_t ivalue; /* first_event_time_zero | initial_direction_of_motion */
+    int32_t ivalue2;/* initial_active_identifier */
+} ecmc_tagger_desc;
+
+/* A run configuration: HypercubicSetting + CuboidPeriodicCells + SingleActiveCellOccupancy
+ * + scheduler + taggers, and the seed of the random stream (random.seed(seed)). */
+typedef struct {
+    int32_t dim, n;
+    double system_length, beta;
+    int32_t cells_per_side[3];
+    int32_t neighbor_layers;
+    int32_t max_occupants;   /* <= 0: unbounded */
+    int32_t scheduler;
+    int32_t reserved;
+    uint64_t seed;
+    int32_t n_taggers;
+    ecmc_tagger_desc taggers[ECMC_MAX_TAGGERS];
+} ecmc_system_desc;
+
+/* One committed leg.  `ncand`/`chash` describe the pair candidates created right after
+ * this event (count and an order-dependent hash of the target identifiers). */
+typedef struct {
+    int32_t tagger;
+    int32_t flags;      /* bit0: the velocity changed owner (lifting) */
+    int32_t active;     /* active unit before the event */
+    int32_t target;     /* pair target, new active of an end of chain, or -1 */
+    double tq, tr;      /* event time as Time(quotient, remainder) */
+    double pos[3];      /* committed position of the previously active unit */
+    int32_t ncand;
+    uint32_t chash;
+} ecmc_event;
+
+typedef struct ecmc_system ecmc_system;
+
+const char *ecmc_last_error(void);
+/* Number of HIP devices visible (0 when none). */
+int ecmc_device_count(void);
+
+/* ---- one chain on the GPU (plain form) ---------------------------------------------------
+ * Create uploads the positions (n x dim, row-major) and charges (n, may be NULL) and
+ * prepares cells and occupancy exactly as the reference's initialisation does. */
+int ecmc_system_create(const ecmc_system_desc *desc, const double *positions, const double *charges,
+                       ecmc_system **out);
+void ecmc_system_destroy(ecmc_system *sys);
+/* Commit up to max_events legs (stops after the end-of-run event).  Events are copied to
+ * host_log when it is not NULL (room for max_events records). */
+int ecmc_system_run(ecmc_system *sys, int64_t max_events, ecmc_event *host_log, int64_t *n_committed);
+/* Bit t set: ecmc_system_run returns right after committing an event of tagger t (e.g. a
+ * sampling event, so that the caller can read the state at the sampling time). */
+int ecmc_system_set_stop_mask(ecmc_system *sys, int32_t mask);
+int ecmc_system_positions(const ecmc_system *sys, double *positions_out);
+int ecmc_system_active(const ecmc_system *sys, int32_t *active, int32_t *direction, double *tq, double *tr,
+                       int32_t *done);
+/* est[0] = sum of lifting separations along the direction of motion, est[1] = chain
+ * displacement of completed chains, est[2] = pair legs, est[3] = cell-boundary legs,
+ * est[4] = lifts, est[5] = end-of-chain legs, est[6] = rejected thinning candidates. */
+int ecmc_system_estimators(const ecmc_system *sys, double *est8);
+/* Number of MT words drawn so far. */
+int ecmc_system_rng_words(const ecmc_system *sys, int64_t *words);
+/* Kernel time of the last ecmc_system_run in milliseconds (HIP events on its stream). */
+int ecmc_system_last_kernel_ms(const ecmc_system *sys, double *ms);
+
+/* ---- batched per-pair evaluations on device memory ---------------------------------------
+ * Row i uses direction dir[i] (velocity speed[i] along +dir), separation sep[3*i..3*i+dim)
+ * (target minus active, minimum image), charges c1[i], c2[i] (NULL = 1.0) and potential
+ * change dU[i] (ignored where the potential takes none). */
+int ecmc_pair_displacement(int32_t potential, const double *par, int32_t dim, int64_t n, const int32_t *d_dir,
+                           const double *d_speed, const double *d_sep, const double *d_c1, const double *d_c2,
+                           const double *d_dU, double system_length, double *d_out, void *stream);
+int ecmc_pair_derivative(int32_t potential, const double *par, int32_t dim, int64_t n, const int32_t *d_dir,
+                         const double *d_speed, const double *d_sep, const double *d_c1, const double *d_c2,
+                         double system_length, double *d_out, void *stream);
+
+/* ---- math and random stream (device) ----------------------------------------------------- */
+int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream);
+int ecmc_math_pow(int64_t n, const double *d_x, const double *d_y, double *d_out, void *stream);
+/* Draw n MT19937 words after seeding like random.seed(seed), on the device (one workgroup,
+ * parallel twist). */
+int ecmc_mt_words(uint64_t seed, int64_t n, uint32_t *d_out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+"""One ECMC chain on the GPU: the host side of ecmc_system_* in the C ABI.
+
+``make_desc`` builds the run configuration from a list of tagger dictionaries that mirror
+the reference's INI sections (TagActivator taggers with their create/trash tag lists and
+event handlers, mediator/..., activator/tag_activator.py:166-316); ``DeviceChain`` owns
+the device state and advances it.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+_KINDS = {"pair_cells": L.TAG_PAIR_CELLS, "pair_all": L.TAG_PAIR_ALL, "cell_boundary": L.TAG_CELL_BOUNDARY,
+          "sampling": L.TAG_SAMPLING, "end_of_chain": L.TAG_END_OF_CHAIN, "end_of_run": L.TAG_END_OF_RUN,
+          "start_of_run": L.TAG_START_OF_RUN}
+_POTENTIALS = {"hard_sphere": L.POT_HARD_SPHERE, "inverse_power": L.POT_INVERSE_POWER,
+               "lennard_jones": L.POT_LENNARD_JONES, "inverse_power_truncated": L.POT_INVERSE_POWER_TRUNC,
+               "merged_image_coulomb": L.POT_MERGED_IMAGE_COULOMB,
+               "inverse_power_coulomb_bounding": L.POT_INVERSE_POWER_COULOMB_BOUND,
+               "displaced_even_power": L.POT_DISPLACED_EVEN_POWER}
+
+
+def potential_code(name):
+    try:
+        return _POTENTIALS[name]
+    except KeyError:
+        raise ValueError("unknown potential %r" % (name,))
+
+
+def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, scheduler="heap", neighbor_layers=1,
+              max_occupants=-1):
+    """Fill an ecmc_system_desc.
+
+    Each tagger is a dict with ``tag``, ``kind`` (pair_cells, pair_all, cell_boundary,
+    sampling, end_of_chain, end_of_run, start_of_run), ``create`` and ``trash`` (lists of
+    tags), and per kind: ``potential``/``params`` (pair taggers, params as in the reference
+    constructors), ``bounding_potential``/``bounding_params`` for the bounding handler,
+    ``charge`` (bool), ``number_event_handlers``; ``sampling_interval`` and
+    ``first_event_time_zero``; ``chain_time``; ``end_of_run_time``;
+    ``initial_direction_of_motion``, ``speed``, ``initial_active_identifier``.
+    """
+    d = L.SystemDesc()
+    d.dim, d.n, d.system_length, d.beta = int(dim), int(n), float(system_length), float(beta)
+    cps = list(cells_per_side) if np.ndim(cells_per_side) else [int(cells_per_side)]
+    for i in range(3):
+        d.cells_per_side[i] = int(cps[i] if i < len(cps) else cps[0])
+    d.neighbor_layers = int(neighbor_layers)
+    d.max_occupants = int(max_occupants)
+    d.scheduler = L.SCHED_LIST if scheduler == "list" else L.SCHED_HEAP
+    d.seed = int(seed)
+    if len(taggers) > L.MAX_TAGGERS:
+        raise ValueError("at most %d taggers" % L.MAX_TAGGERS)
+    index = {t["tag"]: i for i, t in enumerate(taggers)}
+    d.n_taggers = len(taggers)
+    for i, t in enumerate(taggers):
+        g = d.taggers[i]
+        g.kind = _KINDS[t["kind"]]
+        for lst, field, count in ((t.get("create", []), g.create, "n_create"), (t.get("trash", []), g.trash, "n_trash")):
+            for j, tag in enumerate(lst):
+                if tag not in index:
+                    raise ValueError("tag %r of tagger %r does not exist as a tagger" % (tag, t["tag"]))
+                field[j] = index[tag]
+            setattr(g, count, len(lst))
+        g.n_handlers = int(t.get("number_event_handlers", n))
+        if t["kind"] in ("pair_cells", "pair_all"):
+            g.event_handler = L.EH_TWO_LEAF_BOUNDING if t.get("bounding_potential") else L.EH_TWO_LEAF
+            g.potential = potential_code(t["potential"])
+            for j, v in enumerate(t.get("params", ())):
+                g.par[j] = float(v)
+            if t.get("bounding_potential"):
+                g.bounding_potential = potential_code(t["bounding_potential"])
+                for j, v in enumerate(t.get("bounding_params", ())):
+                    g.bpar[j] = float(v)
+            g.use_charge = 1 if t.get("charge") else 0
+        elif t["kind"] == "sampling":
+            g.value = float(t["sampling_interval"])
+            g.ivalue = 1 if t.get("first_event_time_zero") else 0
+        elif t["kind"] == "end_of_chain":
+            g.value = float(t["chain_time"])
+        elif t["kind"] == "end_of_run":
+            g.value = float(t["end_of_run_time"])
+        elif t["kind"] == "start_of_run":
+            g.value = float(t.get("speed", 1.0))
+            g.ivalue = int(t.get("initial_direction_of_motion", 0))
+            g.ivalue2 = int(t.get("initial_active_identifier", 0))
+    return d
+
+
+class DeviceChain:
+    """Device-resident state of one chain plus its event loop (ecmc_system_*)."""
+
+    def __init__(self, desc, positions, charges=None):
+        L.require_device()
+        self.desc = desc
+        pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64))
+        if pos.shape != (desc.n, desc.dim):
+            raise ValueError("positions must have shape (n, dim) = (%d, %d)" % (desc.n, desc.dim))
+        self._charges = None if charges is None else np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        h = ctypes.c_void_p()
+        pd = ctypes.POINTER(ctypes.c_double)
+        L.check(L.lib().ecmc_system_create(ctypes.byref(desc), pos.ctypes.data_as(pd),
+                                           None if self._charges is None else self._charges.ctypes.data_as(pd),
+                                           ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            L.lib().ecmc_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_stop_mask(self, mask):
+        L.check(L.lib().ecmc_system_set_stop_mask(self._h, int(mask)))
+
+    def run(self, max_events, log=True):
+        """Commit up to max_events legs; returns the event records (or their count)."""
+        n = ctypes.c_int64()
+        out = np.zeros(max_events, dtype=L.EVENT_DTYPE) if log else None
+        L.check(L.lib().ecmc_system_run(self._h, int(max_events), out.ctypes.data if log else None, ctypes.byref(n)))
+        return out[:n.value] if log else n.value
+
+    def positions(self):
+        out = np.zeros((self.desc.n, self.desc.dim))
+        L.check(L.lib().ecmc_system_positions(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def active(self):
+        a, d, done = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        q, r = ctypes.c_double(), ctypes.c_double()
+        L.check(L.lib().ecmc_system_active(self._h, ctypes.byref(a), ctypes.byref(d), ctypes.byref(q),
+                                           ctypes.byref(r), ctypes.byref(done)))
+        return a.value, d.value, q.value, r.value, bool(done.value)
+
+    def estimators(self):
+        out = np.zeros(8)
+        L.check(L.lib().ecmc_system_estimators(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
+    def rng_words(self):
+        w = ctypes.c_int64()
+        L.check(L.lib().ecmc_system_rng_words(self._h, ctypes.byref(w)))
+        return w.value
+
+    def last_kernel_ms(self):
+        ms = ctypes.c_double()
+        L.check(L.lib().ecmc_system_last_kernel_ms(self._h, ctypes.byref(ms)))
+        return ms.value

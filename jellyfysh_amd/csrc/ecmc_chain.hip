@@ -1,0 +1,1066 @@
+// ecmc_chain.hip -- one ECMC chain advanced on the GPU, leg after leg, in one persistent
+// workgroup (the plain form).
+//
+// A leg follows the reference mediator (mediator/single_process_mediator.py:84-156):
+//   update the cell occupancy for the active unit, run the event handlers of the taggers in
+//   the `create` list of the tagger that committed last (tag_activator.py:228-289), take the
+//   earliest stored candidate, build and commit its out-state, and trash by tag
+//   (tag_activator.py:291-316).
+//
+// Mapping to the hardware:
+//   * the candidate set of a pair tagger (active unit x occupants of the nearby cells,
+//     excluded_cells_tagger.py:129-132) is spread over the 256 threads of the workgroup in
+//     canonical order (nearby cells in itertools.product order, occupants in list order);
+//     positions are read from the SoA arrays in HBM (x[], y[], z[]), one candidate per lane;
+//   * each candidate's exponential draw is the 2 MT19937 words at its rank in the stream;
+//     words come from a ring in HBM that the workgroup refills 624 words at a time with the
+//     three-phase parallel twist, so the draws are those of CPython's random in the
+//     reference's call order;
+//   * the earliest candidate is found by a wavefront shuffle min-reduce over (time, push
+//     rank) followed by a 4-wave LDS reduction; the scheduler keeps, per tagger, the
+//     earliest of its pushed events (all events of one tagger are trashed together), and
+//     ties go to the first pushed event as in list_scheduler.py:129-156.
+//   * bookkeeping that is sequential in the reference (occupancy lists, end of chain,
+//     sampling, out-states) runs on thread 0 between barriers.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <string>
+
+#include "../../include/jellyfysh_amd.h"
+#include "ecmc_math.h"
+#include "ecmc_potentials.h"
+#include "ecmc_common.h"
+
+namespace ecmc {
+
+constexpr int NT = 256;
+constexpr uint32_t MT_UPPER = 0x80000000u, MT_LOWER = 0x7fffffffu, MT_MATRIX = 0x9908b0dfu;
+
+struct Slot {
+    Time t;
+    int64_t seq;
+    int32_t valid, target;
+    double apos[3];
+    Time astamp;
+    double tpos[3];
+    double cb_boundary;
+    int32_t cb_dir, adir;
+    double aspeed;
+};
+
+struct ChainState {
+    int32_t active, adir;
+    double aspeed;
+    Time at;
+    int32_t occ_id, occ_cell;
+    int32_t started, done, err, pending_creates, last_tagger, stop_reason;
+    Time last;
+    int64_t seq;
+    Time eoc_last, samp_time;
+    uint64_t cursor, produced;
+    double est[8];
+    int32_t ncand;
+    uint32_t chash;
+    Slot slot[ECMC_MAX_TAGGERS];
+};
+
+struct ChainArgs {
+    int32_t n, ncells, nb_stride, occ_cap, ring_words, n_taggers, stop_mask, ewald_fc;
+    int32_t pad0, pad1;
+    double L, half, beta;
+    int32_t cps[3], cum[3];
+    double side[3];
+    Ewald ewald;
+    double *pos;
+    const double *charge;
+    const double *cmin;
+    const int32_t *nearby;
+    const int32_t *nearby_n;
+    int32_t *occ;
+    int32_t *occ_n;
+    uint32_t *mt;
+    uint32_t *ring;
+    ChainState *st;
+    const ecmc_tagger_desc *tg;
+    ecmc_event *log;
+    int64_t max_events;
+    int64_t *n_out;
+};
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    return y;
+}
+
+// One twist of the 624-word state in LDS, then the tempered words go to the ring.
+// The three phases follow the data dependences of the sequential loop: words 0..226 read
+// only old words; 227..453 read new words 0..226; 454..623 read new words 227..396.
+__device__ void mt_generate_block(uint32_t *mt, uint32_t *ring, int ring_words, uint64_t produced) {
+    const int t = threadIdx.x;
+    uint32_t v = 0;
+    if (t < 227) {
+        uint32_t y = (mt[t] & MT_UPPER) | (mt[t + 1] & MT_LOWER);
+        v = mt[t + 397] ^ (y >> 1) ^ ((y & 1u) ? MT_MATRIX : 0u);
+    }
+    __syncthreads();
+    if (t < 227) mt[t] = v;
+    __syncthreads();
+    if (t < 227) {
+        int i = t + 227;
+        uint32_t y = (mt[i] & MT_UPPER) | (mt[i + 1] & MT_LOWER);
+        v = mt[i - 227] ^ (y >> 1) ^ ((y & 1u) ? MT_MATRIX : 0u);
+    }
+    __syncthreads();
+    if (t < 227) mt[t + 227] = v;
+    __syncthreads();
+    if (t < 170) {
+        int i = t + 454;
+        uint32_t nx = i == 623 ? mt[0] : mt[i + 1];
+        uint32_t y = (mt[i] & MT_UPPER) | (nx & MT_LOWER);
+        v = mt[i - 227] ^ (y >> 1) ^ ((y & 1u) ? MT_MATRIX : 0u);
+    }
+    __syncthreads();
+    if (t < 170) mt[t + 454] = v;
+    __syncthreads();
+    for (int j = t; j < 624; j += NT) ring[(produced + (uint64_t)j) % (uint64_t)ring_words] = mt_temper(mt[j]);
+    __syncthreads();
+}
+
+// Collective: make `need` unread words available in the ring.
+__device__ void ring_ensure(ChainState &st, uint32_t *mt, const ChainArgs &a, uint64_t need) {
+    while (st.produced < st.cursor + need) {
+        mt_generate_block(mt, a.ring, a.ring_words, st.produced);
+        if (threadIdx.x == 0) st.produced += 624;
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ uint32_t ring_word(const ChainArgs &a, uint64_t idx) {
+    return a.ring[idx % (uint64_t)a.ring_words];
+}
+
+// random() from two words (genrand_res53)
+__device__ __forceinline__ double words_to_random(uint32_t w0, uint32_t w1) {
+    uint32_t x = w0 >> 5, y = w1 >> 6;
+    return (x * 67108864.0 + y) * (1.0 / 9007199254740992.0);
+}
+
+// candidate-set hash term: order dependent, summed over the candidates of a leg
+__device__ __forceinline__ uint32_t cand_mix(uint32_t j, uint32_t target) {
+    uint32_t h = (j * 0x9E3779B9u) ^ target;
+    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+    return h;
+}
+
+template <int D>
+__device__ __forceinline__ int pos_to_cell(const ChainArgs &a, const double *p) {
+    long idx = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) idx += (long)((int)(p[d] / a.side[d])) * a.cum[d];
+    if (idx < 0 || idx >= a.ncells) return -1;
+    return (int)idx;
+}
+
+template <int D>
+__device__ __forceinline__ void load_pos(const ChainArgs &a, int id, double *p) {
+#pragma unroll
+    for (int d = 0; d < D; d++) p[d] = a.pos[(size_t)d * a.n + id];
+}
+template <int D>
+__device__ __forceinline__ void store_pos(const ChainArgs &a, int id, const double *p) {
+#pragma unroll
+    for (int d = 0; d < D; d++) a.pos[(size_t)d * a.n + id] = p[d];
+}
+
+template <int D>
+__device__ __forceinline__ void separation_vector(const ChainArgs &a, const double *ap, const double *tp, double *s) {
+#pragma unroll
+    for (int d = 0; d < D; d++) s[d] = py_mod((tp[d] - ap[d]) + a.half, a.L) - a.half;
+}
+
+// event_handler/abstracts/abstracts.py:82-95 for a velocity with one nonzero component
+template <int D>
+__device__ __forceinline__ void slice_pos(const ChainArgs &a, double *p, int dir, double speed, Time from, Time to) {
+    double dt = time_sub(to, from);
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double v = d == dir ? speed : 0.0;
+        p[d] = py_mod(p[d] + v * dt, a.L);
+    }
+}
+
+__device__ __forceinline__ bool pot_draws(const ecmc_tagger_desc &tg) {
+    if (tg.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) return true;
+    return tg.potential != ECMC_POT_HARD_SPHERE;
+}
+
+// Candidate time displacement of one pair (send_event_time of the pair event handlers).
+template <int D>
+__device__ double pair_dt(const ChainArgs &a, const ecmc_tagger_desc &tg, int dir, double speed, const double *s,
+                          double c1, double c2, double dU) {
+    if (tg.event_handler == ECMC_EH_TWO_LEAF) {
+        switch (tg.potential) {
+        case ECMC_POT_HARD_SPHERE: return hs_displacement<D>(dir, speed, s, tg.par[0]);
+        case ECMC_POT_INVERSE_POWER: return ip_displacement<D>(dir, speed, s, c1, c2, tg.par[0], tg.par[1], dU);
+        case ECMC_POT_INVERSE_POWER_TRUNC:
+            return ipt_displacement<D>(dir, speed, s, c1, c2, tg.par[0], tg.par[1], tg.par[2], dU);
+        case ECMC_POT_LENNARD_JONES: {
+            MexicanHat h = mh_lennard_jones(tg.par[0], tg.par[1]);
+            return mh_displacement<D>(h, dir, speed, s, dU);
+        }
+        case ECMC_POT_DISPLACED_EVEN_POWER: {
+            MexicanHat h = mh_displaced_even_power(tg.par[0], tg.par[1], tg.par[2]);
+            return mh_displacement<D>(h, dir, speed, s, dU);
+        }
+        default: return NAN;
+        }
+    }
+    // two_leaf_unit_bounding_potential_event_handler.py:138-143 with the Coulomb bound
+    double p[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 3; d++) p[d] = d < D ? s[(dir + d) % D] : 0.0;
+    return ipcb_displacement(tg.bpar[0] * c1 * c2, p[0], p[1], p[2], dU, a.L) / speed;
+}
+
+struct Best {
+    double q, r;
+    int32_t j;
+};
+__device__ __forceinline__ bool best_lt(const Best &x, const Best &y) {
+    if (x.q < y.q) return true;
+    if (x.q == y.q) {
+        if (x.r < y.r) return true;
+        if (x.r == y.r) return x.j < y.j;
+    }
+    return false;
+}
+
+// Collective workgroup min-reduce of (time, index) and sum of the hash terms.
+__device__ void block_reduce(Best &b, uint32_t &h, Best *sb, uint32_t *sh) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        Best o;
+        o.q = __shfl_xor(b.q, off, 64);
+        o.r = __shfl_xor(b.r, off, 64);
+        o.j = __shfl_xor(b.j, off, 64);
+        uint32_t oh = __shfl_xor(h, off, 64);
+        if (best_lt(o, b)) b = o;
+        h += oh;
+    }
+    if (lane == 0) { sb[wave] = b; sh[wave] = h; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < NT / 64; w++) {
+            if (best_lt(sb[w], sb[0])) sb[0] = sb[w];
+            sh[0] += sh[w];
+        }
+    }
+    __syncthreads();
+    b = sb[0];
+    h = sh[0];
+}
+
+template <int D>
+__device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &tg, int tix,
+                            int *nb_cell, int *nb_pref, Best *sb, uint32_t *sh) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        int nc = 0;
+        if (tg.kind == ECMC_TAG_PAIR_CELLS) {
+            int cell = st.occ_cell;
+            int cnt = a.nearby_n[cell];
+            nb_pref[0] = 0;
+            for (int c = 0; c < cnt; c++) {
+                int cc = a.nearby[(size_t)cell * a.nb_stride + c];
+                nb_cell[c] = cc;
+                nb_pref[c + 1] = nb_pref[c] + a.occ_n[cc];
+            }
+            nc = cnt;
+        } else {
+            nb_pref[0] = 0;
+            nb_pref[1] = a.n;  // identifiers 0..n-1 minus the active one
+            nc = 1;
+        }
+        nb_cell[31] = nc;
+    }
+    __syncthreads();
+    const int nc = nb_cell[31];
+    const bool all = tg.kind == ECMC_TAG_PAIR_ALL;
+    const int M = all ? a.n - 1 : nb_pref[nc];
+    const bool draws = pot_draws(tg);
+    ring_ensure(st, mt, a, draws ? 2ull * (uint64_t)M : 0ull);
+    const uint64_t base = st.cursor;
+    const int act = st.active, dir = st.adir;
+    const double speed = st.aspeed;
+    const Time at = st.at;
+    double ap[D];
+    load_pos<D>(a, act, ap);
+    const double ca = tg.use_charge ? a.charge[act] : 1.0;
+    const int jbase = st.ncand;
+    Best best = {INFINITY, INFINITY, 0x7fffffff};
+    uint32_t hsum = 0;
+    for (int j = tid; j < M; j += NT) {
+        int target;
+        if (all) {
+            target = j < act ? j : j + 1;
+        } else {
+            int c = 0;
+            while (nb_pref[c + 1] <= j) c++;
+            target = a.occ[(size_t)nb_cell[c] * a.occ_cap + (j - nb_pref[c])];
+        }
+        double tp[D], s[D];
+        load_pos<D>(a, target, tp);
+        separation_vector<D>(a, ap, tp, s);
+        double dU = 0.0;
+        if (draws) {
+            double u = words_to_random(ring_word(a, base + 2ull * j), ring_word(a, base + 2ull * j + 1));
+            dU = -cr_log(1.0 - u) / a.beta;
+        }
+        const double ct = tg.use_charge ? a.charge[target] : 1.0;
+        double dt = pair_dt<D>(a, tg, dir, speed, s, ca, ct, dU);
+        Time t = time_add(at, dt);
+        Best c = {t.q, t.r, j};
+        if (isnan(dt)) c.q = -INFINITY;  // flags a ValueError of the reference code
+        if (best_lt(c, best)) best = c;
+        hsum += cand_mix((uint32_t)(jbase + j), (uint32_t)target);
+    }
+    block_reduce(best, hsum, sb, sh);
+    if (tid == 0) {
+        if (draws) st.cursor += 2ull * (uint64_t)M;
+        st.ncand += M;
+        st.chash += hsum;
+        Slot &sl = st.slot[tix];
+        if (M == 0) {
+            sl.valid = 0;
+        } else if (best.q == -INFINITY) {
+            st.err = ECMC_ERR_NUMERIC;
+        } else {
+            int j = best.j, target;
+            if (all) {
+                target = j < act ? j : j + 1;
+            } else {
+                int c = 0;
+                while (nb_pref[c + 1] <= j) c++;
+                target = a.occ[(size_t)nb_cell[c] * a.occ_cap + (j - nb_pref[c])];
+            }
+            sl.valid = 1;
+            sl.t.q = best.q;
+            sl.t.r = best.r;
+            sl.seq = st.seq + j;
+            sl.target = target;
+            sl.adir = dir;
+            sl.aspeed = speed;
+            double tp[D];
+            load_pos<D>(a, target, tp);
+            double p[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) { p[d] = ap[d]; sl.tpos[d] = tp[d]; }
+            if (!isinf(sl.t.q)) slice_pos<D>(a, p, dir, speed, at, sl.t);
+#pragma unroll
+            for (int d = 0; d < D; d++) sl.apos[d] = p[d];
+            sl.astamp = sl.t;
+        }
+        st.seq += M;
+    }
+    __syncthreads();
+}
+
+template <int D>
+__device__ void occupancy_update(const ChainArgs &a, ChainState &st) {
+    // single_active_cell_occupancy.py:149-203 (unbounded occupant lists)
+    int id = st.active;
+    double p[D];
+    load_pos<D>(a, id, p);
+    if (id != st.occ_id) {
+        if (st.occ_id >= 0) {
+            int c = st.occ_cell;
+            int n = a.occ_n[c];
+            if (n >= a.occ_cap) { st.err = ECMC_ERR_STATE; return; }
+            a.occ[(size_t)c * a.occ_cap + n] = st.occ_id;
+            a.occ_n[c] = n + 1;
+        }
+        int c = pos_to_cell<D>(a, p);
+        if (c < 0) { st.err = ECMC_ERR_STATE; return; }
+        st.occ_cell = c;
+        st.occ_id = id;
+        int n = a.occ_n[c];
+        int32_t *L = a.occ + (size_t)c * a.occ_cap;
+        int i = 0;
+        while (i < n && L[i] != id) i++;
+        if (i == n) { st.err = ECMC_ERR_STATE; return; }
+        for (; i + 1 < n; i++) L[i] = L[i + 1];
+        a.occ_n[c] = n - 1;
+    } else {
+        int c = pos_to_cell<D>(a, p);
+        if (c < 0) { st.err = ECMC_ERR_STATE; return; }
+        st.occ_cell = c;
+    }
+}
+
+// Creates of one non-pair tagger (thread 0 only, ring words made available by the caller).
+template <int D>
+__device__ void create_single(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
+    Slot &sl = st.slot[tix];
+    switch (tg.kind) {
+    case ECMC_TAG_CELL_BOUNDARY: {
+        // cell_boundary_event_handler.py:98-156
+        double p[D];
+        load_pos<D>(a, st.active, p);
+        int cell = pos_to_cell<D>(a, p);
+        if (cell < 0) { st.err = ECMC_ERR_STATE; return; }
+        int dir = st.adir;
+        int id = (cell / a.cum[dir]) % a.cps[dir];
+        int nid = (id + 1) % a.cps[dir];
+        int nb = cell + (nid - id) * a.cum[dir];
+        double boundary = a.cmin[(size_t)nb * D + dir];
+        double sep = boundary - p[dir];
+        if (sep < 0.0) sep = sep + a.L;
+        double dt = sep / st.aspeed;
+        sl.t = time_add(st.at, dt);
+#pragma unroll
+        for (int d = 0; d < D; d++) sl.apos[d] = p[d];
+        sl.astamp = st.at;
+        sl.cb_boundary = boundary;
+        sl.cb_dir = dir;
+        sl.adir = dir;
+        sl.aspeed = st.aspeed;
+        sl.valid = 1;
+        sl.seq = st.seq++;
+        break;
+    }
+    case ECMC_TAG_SAMPLING:
+        st.samp_time = time_add(st.samp_time, tg.value);
+        sl.t = st.samp_time;
+        sl.valid = 1;
+        sl.seq = st.seq++;
+        break;
+    case ECMC_TAG_END_OF_RUN:
+        sl.t = time_from_float(tg.value);
+        sl.valid = 1;
+        sl.seq = st.seq++;
+        break;
+    case ECMC_TAG_START_OF_RUN:
+        sl.t.q = 0.0;
+        sl.t.r = 0.0;
+        sl.valid = 1;
+        sl.seq = st.seq++;
+        break;
+    default: break;
+    }
+}
+
+// end of chain: chain time and the randint of the next active unit
+// (single_independent_active_periodic_direction_end_of_chain_event_handler.py:176-237).
+// Returns false when the ring ran out of words before the draw completed.
+__device__ bool create_end_of_chain(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
+    Slot &sl = st.slot[tix];
+    if (!sl.valid || sl.target >= 0x40000000) {
+        double since = time_sub(st.at, st.eoc_last);
+        if (!(0.0 <= since && since <= tg.value)) { st.err = ECMC_ERR_CONFIGURATION; return true; }
+        double chain = time_sub(st.eoc_last, st.at) + tg.value;
+        sl.t = time_add(st.at, chain);
+        sl.valid = 1;
+        sl.target = 0x40000000;  // draw pending
+    }
+    uint32_t nrange = (uint32_t)a.n;
+    int k = 0;
+    while ((nrange >> k) != 0u) k++;
+    while (st.cursor < st.produced) {
+        uint32_t w = ring_word(a, st.cursor);
+        st.cursor++;
+        uint32_t v = k >= 32 ? w : (w >> (32 - k));
+        if (v < nrange) {
+            sl.target = (int32_t)v;
+            sl.seq = st.seq++;
+            return true;
+        }
+    }
+    return false;
+}
+
+template <int D>
+__global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
+    __shared__ ChainState st;
+    __shared__ ecmc_tagger_desc tg[ECMC_MAX_TAGGERS];
+    __shared__ uint32_t mt[624];
+    __shared__ int nb_cell[32];
+    __shared__ int nb_pref[33];
+    __shared__ Best sb[NT / 64];
+    __shared__ uint32_t sh[NT / 64];
+    __shared__ int flag;
+    const int tid = threadIdx.x;
+    if (tid == 0) st = *a.st;
+    for (int i = tid; i < a.n_taggers; i += NT) tg[i] = a.tg[i];
+    for (int i = tid; i < 624; i += NT) mt[i] = a.mt[i];
+    __syncthreads();
+    int64_t k = 0;
+    while (k < a.max_events) {
+        if (st.done || st.err) break;
+        if (!st.started) {
+            if (tid == 0) {
+                int sor = -1;
+                for (int t = 0; t < a.n_taggers; t++)
+                    if (tg[t].kind == ECMC_TAG_START_OF_RUN) sor = t;
+                if (sor < 0) st.err = ECMC_ERR_CONFIGURATION;
+                else create_single<D>(a, st, tg[sor], sor);
+                st.started = 1;
+                st.ncand = 0;
+                st.chash = 0;
+            }
+            __syncthreads();
+        } else if (st.pending_creates) {
+            if (tid == 0) {
+                occupancy_update<D>(a, st);
+                st.ncand = 0;
+                st.chash = 0;
+            }
+            __syncthreads();
+            const ecmc_tagger_desc &w = tg[st.last_tagger];
+            for (int ci = 0; ci < w.n_create && !st.err; ci++) {
+                const int t = w.create[ci];
+                const ecmc_tagger_desc &c = tg[t];
+                if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL) {
+                    create_pair<D>(a, st, mt, c, t, nb_cell, nb_pref, sb, sh);
+                } else if (c.kind == ECMC_TAG_END_OF_CHAIN) {
+                    if (tid == 0) st.slot[t].valid = 0;
+                    __syncthreads();
+                    // randint's rejection loop: each word is accepted with probability >= 1/2,
+                    // so 64 refills (40k words) without success only happen after a defect
+                    for (int attempt = 0;; attempt++) {
+                        ring_ensure(st, mt, a, 64);
+                        if (tid == 0) {
+                            flag = create_end_of_chain(a, st, c, t) ? 1 : 0;
+                            if (!flag && attempt >= 64) { st.err = ECMC_ERR_STATE; flag = 1; }
+                        }
+                        __syncthreads();
+                        if (flag) break;
+                        ring_ensure(st, mt, a, 624);
+                    }
+                } else {
+                    if (tid == 0) create_single<D>(a, st, c, t);
+                    __syncthreads();
+                }
+            }
+            if (tid == 0) st.pending_creates = 0;
+            __syncthreads();
+            if (st.err) break;
+        }
+        // earliest stored event (ties: first pushed)
+        if (tid == 0) {
+            int w = -1;
+            for (int t = 0; t < a.n_taggers; t++) {
+                const Slot &s = st.slot[t];
+                if (!s.valid) continue;
+                if (w < 0 || time_lt(s.t, st.slot[w].t) || (time_eq(s.t, st.slot[w].t) && s.seq < st.slot[w].seq)) w = t;
+            }
+            if (w < 0) st.err = ECMC_ERR_SCHEDULER;
+            else if (time_lt(st.slot[w].t, st.last)) st.err = ECMC_ERR_SCHEDULER;
+            else st.last = st.slot[w].t;
+            flag = w;
+        }
+        __syncthreads();
+        if (st.err) break;
+        const int w = flag;
+        const bool thinning = (tg[w].kind == ECMC_TAG_PAIR_CELLS || tg[w].kind == ECMC_TAG_PAIR_ALL) &&
+                              tg[w].event_handler == ECMC_EH_TWO_LEAF_BOUNDING;
+        if (thinning) ring_ensure(st, mt, a, 2);
+        if (tid == 0) {
+            Slot &sl = st.slot[w];
+            const ecmc_tagger_desc &T = tg[w];
+            ecmc_event ev;
+            ev.tagger = w;
+            ev.flags = 0;
+            ev.active = st.active;
+            ev.target = -1;
+            ev.tq = sl.t.q;
+            ev.tr = sl.t.r;
+            ev.ncand = st.ncand;
+            ev.chash = st.chash;
+            const int act = st.active;
+            double p[D];
+            switch (T.kind) {
+            case ECMC_TAG_PAIR_CELLS:
+            case ECMC_TAG_PAIR_ALL: {
+                ev.target = sl.target;
+                bool lift = true;
+                double s[D];
+                separation_vector<D>(a, sl.apos, sl.tpos, s);
+                if (T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
+                    // event_handler_with_bounding_potential.py:75-101
+                    double q3[3] = {0, 0, 0};
+                    for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
+                    const double c1 = T.use_charge ? a.charge[act] : 1.0;
+                    const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
+                    double bound = ipcb_derivative(T.bpar[0] * c1 * c2, q3[0], q3[1], q3[2]) * sl.aspeed;
+                    double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+                    lift = false;
+                    if (real > 0) {
+                        double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
+                        st.cursor += 2;
+                        u = 0 + (bound - 0) * u;
+                        if (u < real) lift = true;
+                    }
+                    if (!lift) st.est[6] += 1.0;
+                }
+                store_pos<D>(a, act, sl.apos);
+                st.at = sl.astamp;
+                if (lift) {
+                    st.est[0] += s[sl.adir];
+                    st.est[4] += 1.0;
+                    st.active = sl.target;
+                    ev.flags |= 1;
+                }
+                st.est[2] += 1.0;
+                break;
+            }
+            case ECMC_TAG_CELL_BOUNDARY: {
+#pragma unroll
+                for (int d = 0; d < D; d++) p[d] = sl.apos[d];
+                slice_pos<D>(a, p, sl.adir, sl.aspeed, sl.astamp, sl.t);
+                p[sl.cb_dir] = sl.cb_boundary;
+                store_pos<D>(a, act, p);
+                st.at = sl.t;
+                st.est[3] += 1.0;
+                break;
+            }
+            case ECMC_TAG_SAMPLING:
+            case ECMC_TAG_END_OF_RUN: {
+                load_pos<D>(a, act, p);
+                slice_pos<D>(a, p, st.adir, st.aspeed, st.at, sl.t);
+                store_pos<D>(a, act, p);
+                st.at = sl.t;
+                if (T.kind == ECMC_TAG_END_OF_RUN) st.done = 1;
+                break;
+            }
+            case ECMC_TAG_END_OF_CHAIN: {
+                load_pos<D>(a, act, p);
+                slice_pos<D>(a, p, st.adir, st.aspeed, st.at, sl.t);
+                store_pos<D>(a, act, p);
+                st.at = sl.t;
+                st.eoc_last = sl.t;
+                ev.target = sl.target;
+                st.est[1] += T.value * st.aspeed;
+                st.est[5] += 1.0;
+                st.adir = (st.adir + 1) % D;
+                if (sl.target != act) ev.flags |= 1;
+                st.active = sl.target;
+                break;
+            }
+            case ECMC_TAG_START_OF_RUN:
+                st.active = T.ivalue2;
+                st.adir = T.ivalue;
+                st.aspeed = T.value;
+                st.at.q = 0.0;
+                st.at.r = 0.0;
+                break;
+            default:
+                st.err = ECMC_ERR_CONFIGURATION;
+            }
+            ev.pos[0] = ev.pos[1] = ev.pos[2] = 0.0;
+            if (act >= 0) {
+                double q[D];
+                load_pos<D>(a, act, q);
+                for (int d = 0; d < D; d++) ev.pos[d] = q[d];
+            }
+            // trash by tag
+            for (int i = 0; i < T.n_trash; i++) st.slot[T.trash[i]].valid = 0;
+            if (a.log) a.log[k] = ev;
+            st.pending_creates = 1;
+            st.last_tagger = w;
+            if (a.stop_mask & (1 << w)) st.stop_reason = 1;
+        }
+        k++;
+        __syncthreads();
+        if (st.stop_reason) break;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        st.stop_reason = 0;
+        *a.st = st;
+        *a.n_out = k;
+    }
+    for (int i = tid; i < 624; i += NT) a.mt[i] = mt[i];
+}
+
+// ---- host side --------------------------------------------------------------------------
+static void mt_seed_host(uint64_t seed, uint32_t *mt) {
+    uint32_t key[2] = {(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32)};
+    int klen = key[1] ? 2 : 1;
+    mt[0] = 19650218u;
+    for (int i = 1; i < 624; i++) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+    int i = 1, j = 0;
+    for (int k = 624 > klen ? 624 : klen; k; k--) {
+        mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1664525u)) + key[j] + (uint32_t)j;
+        i++; j++;
+        if (i >= 624) { mt[0] = mt[623]; i = 1; }
+        if (j >= klen) j = 0;
+    }
+    for (int k = 623; k; k--) {
+        mt[i] = (mt[i] ^ ((mt[i - 1] ^ (mt[i - 1] >> 30)) * 1566083941u)) - (uint32_t)i;
+        i++;
+        if (i >= 624) { mt[0] = mt[623]; i = 1; }
+    }
+    mt[0] = 0x80000000u;
+}
+
+}  // namespace ecmc
+
+using namespace ecmc;
+
+struct ecmc_system {
+    ecmc_system_desc desc;
+    int dim, n, ncells, nb_stride, occ_cap, ring_words;
+    std::vector<double> cmin;
+    double *d_pos = nullptr, *d_charge = nullptr, *d_cmin = nullptr, *d_ewald = nullptr;
+    int32_t *d_nearby = nullptr, *d_nearby_n = nullptr, *d_occ = nullptr, *d_occ_n = nullptr;
+    uint32_t *d_mt = nullptr, *d_ring = nullptr;
+    ChainState *d_st = nullptr;
+    ecmc_tagger_desc *d_tg = nullptr;
+    ecmc_event *d_log = nullptr;
+    int64_t log_cap = 0;
+    int64_t *d_nout = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int32_t stop_mask = 0;
+    ChainArgs args;
+};
+
+extern "C" int ecmc_system_set_stop_mask(ecmc_system *sys, int32_t mask) {
+    if (!sys) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
+    sys->stop_mask = mask;
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *positions, const double *charges,
+                                  ecmc_system **out) {
+    if (!desc || !positions || !out) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    const int dim = desc->dim, n = desc->n;
+    if (dim != 2 && dim != 3) return ecmc_fail(ECMC_ERR_CONFIGURATION, "dimension must be 2 or 3");
+    if (n < 2) return ecmc_fail(ECMC_ERR_CONFIGURATION, "at least two units are required");
+    if (desc->n_taggers < 1 || desc->n_taggers > ECMC_MAX_TAGGERS)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "number of taggers out of range");
+    if (desc->max_occupants > 0)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the device chain supports unbounded cell occupancy only");
+    if (desc->neighbor_layers < 0 || desc->neighbor_layers > 1)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "neighbor_layers must be 0 or 1");
+    int ewald_tagger = -1;
+    for (int t = 0; t < desc->n_taggers; t++) {
+        const ecmc_tagger_desc &tg = desc->taggers[t];
+        if (tg.kind < 0 || tg.kind > ECMC_TAG_START_OF_RUN) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
+        if (tg.n_create < 0 || tg.n_create > ECMC_MAX_TAGGERS || tg.n_trash < 0 || tg.n_trash > ECMC_MAX_TAGGERS)
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "create/trash list length out of range");
+        if (tg.kind == ECMC_TAG_START_OF_RUN &&
+            (tg.ivalue < 0 || tg.ivalue >= dim || tg.ivalue2 < 0 || tg.ivalue2 >= n || !(tg.value > 0.0)))
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "initial direction, active identifier or speed out of range");
+        if (tg.kind == ECMC_TAG_END_OF_CHAIN && !(tg.value > 0.0))
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "chain_time must be > 0.0");
+        if (tg.kind == ECMC_TAG_SAMPLING && !(tg.value > 0.0))
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "sampling_interval must be > 0.0");
+        for (int i = 0; i < tg.n_create; i++)
+            if (tg.create[i] < 0 || tg.create[i] >= desc->n_taggers) return ecmc_fail(ECMC_ERR_CONFIGURATION, "create tag out of range");
+        for (int i = 0; i < tg.n_trash; i++)
+            if (tg.trash[i] < 0 || tg.trash[i] >= desc->n_taggers) return ecmc_fail(ECMC_ERR_CONFIGURATION, "trash tag out of range");
+        if (tg.kind == ECMC_TAG_PAIR_CELLS || tg.kind == ECMC_TAG_PAIR_ALL) {
+            if (tg.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
+                if (tg.potential != ECMC_POT_MERGED_IMAGE_COULOMB || tg.bounding_potential != ECMC_POT_INVERSE_POWER_COULOMB_BOUND || dim != 3)
+                    return ecmc_fail(ECMC_ERR_CONFIGURATION, "bounding handler needs merged-image Coulomb with the inverse-power Coulomb bound in 3D");
+                if (ewald_tagger >= 0) {
+                    const ecmc_tagger_desc &o = desc->taggers[ewald_tagger];
+                    if (o.par[0] != tg.par[0] || o.par[1] != tg.par[1] || o.par[2] != tg.par[2])
+                        return ecmc_fail(ECMC_ERR_CONFIGURATION, "one merged-image Coulomb parameter set per system");
+                }
+                ewald_tagger = t;
+            } else if (tg.potential == ECMC_POT_MERGED_IMAGE_COULOMB || tg.potential == ECMC_POT_INVERSE_POWER_COULOMB_BOUND) {
+                return ecmc_fail(ECMC_ERR_CONFIGURATION, "the Coulomb potentials are not invertible here");
+            }
+            if (tg.n_handlers > 0 && tg.n_handlers < n - 1 && tg.kind == ECMC_TAG_PAIR_ALL)
+                return ecmc_fail(ECMC_ERR_TAG_ACTIVATOR, "not-running event handler list would run empty");
+        }
+    }
+    ecmc_system *s = new ecmc_system();
+    s->desc = *desc;
+    s->dim = dim;
+    s->n = n;
+    CellGrid grid;
+    std::string err;
+    if (!build_cell_grid(dim, desc->system_length, desc->cells_per_side, desc->neighbor_layers, grid, err)) {
+        delete s;
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, err.c_str());
+    }
+    s->ncells = grid.count;
+    s->nb_stride = grid.nb_stride;
+    // initial occupancy in global-state order (single_active_cell_occupancy.py:111-121)
+    std::vector<std::vector<int32_t>> lists(grid.count);
+    for (int i = 0; i < n; i++) {
+        int c = grid.position_to_cell(positions + (size_t)i * dim);
+        if (c < 0) { delete s; return ecmc_fail(ECMC_ERR_STATE, "initial position outside of the cell system"); }
+        lists[c].push_back(i);
+    }
+    size_t maxocc = 0;
+    for (auto &l : lists) maxocc = l.size() > maxocc ? l.size() : maxocc;
+    int cap = (int)(maxocc * 4 + 32);
+    if (cap > n) cap = n;
+    s->occ_cap = cap;
+    std::vector<int32_t> occ((size_t)grid.count * cap, -1), occ_n(grid.count);
+    for (int c = 0; c < grid.count; c++) {
+        occ_n[c] = (int32_t)lists[c].size();
+        for (size_t i = 0; i < lists[c].size(); i++) occ[(size_t)c * cap + i] = lists[c][i];
+    }
+    std::vector<double> pos_soa((size_t)dim * n);
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < dim; d++) pos_soa[(size_t)d * n + i] = positions[(size_t)i * dim + d];
+    std::vector<double> ch(n, 1.0);
+    if (charges) for (int i = 0; i < n; i++) ch[i] = charges[i];
+    int max_pairs = 0;
+    for (int t = 0; t < desc->n_taggers; t++)
+        if (desc->taggers[t].kind == ECMC_TAG_PAIR_CELLS || desc->taggers[t].kind == ECMC_TAG_PAIR_ALL) max_pairs++;
+    (void)max_pairs;
+    s->ring_words = 624 * ((2 * n + 256) / 624 + 3);
+    uint32_t mt[624];
+    mt_seed_host(desc->seed, mt);
+    ChainState st;
+    memset(&st, 0, sizeof(st));
+    st.active = -1;
+    st.aspeed = 1.0;
+    st.occ_id = -1;
+    st.occ_cell = -1;
+    st.last.q = -INFINITY;
+    st.last.r = -INFINITY;
+    for (int t = 0; t < desc->n_taggers; t++) {
+        st.slot[t].valid = 0;
+        const ecmc_tagger_desc &tg = desc->taggers[t];
+        if (tg.kind == ECMC_TAG_SAMPLING && tg.ivalue) {
+            Time z = time_from_float(-tg.value);
+            st.samp_time = z;
+        }
+    }
+    std::vector<double> ewald;
+    ChainArgs &a = s->args;
+    memset(&a, 0, sizeof(a));
+    if (ewald_tagger >= 0) {
+        const ecmc_tagger_desc &tg = desc->taggers[ewald_tagger];
+        int fc = (int)tg.par[1], pc = (int)tg.par[2];
+        double alpha = tg.par[0], L = desc->system_length;
+        build_ewald_table(fc, alpha, L, ewald);
+        a.ewald.fc = fc;
+        a.ewald.pc = pc;
+        a.ewald.L = L;
+        a.ewald.aol = alpha / L;
+        a.ewald.aol2 = alpha * alpha / (L * L);
+        a.ewald.two_aol_rpi = 2.0 * alpha / (L * sqrt(M_PI));
+        a.ewald.tpol = 2.0 * M_PI / L;
+    }
+#define ECMC_ALLOC(ptr, bytes) \
+    if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) { ecmc_system_destroy(s); return ecmc_fail(ECMC_ERR_HIP, "hipMalloc failed"); }
+#define ECMC_UP(dst, src, bytes) \
+    if (hipMemcpy((dst), (src), (bytes), hipMemcpyHostToDevice) != hipSuccess) { ecmc_system_destroy(s); return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed"); }
+    ECMC_ALLOC(s->d_pos, sizeof(double) * dim * n);
+    ECMC_ALLOC(s->d_charge, sizeof(double) * n);
+    ECMC_ALLOC(s->d_cmin, sizeof(double) * grid.cmin.size());
+    ECMC_ALLOC(s->d_nearby, sizeof(int32_t) * grid.nearby.size());
+    ECMC_ALLOC(s->d_nearby_n, sizeof(int32_t) * grid.nearby_n.size());
+    ECMC_ALLOC(s->d_occ, sizeof(int32_t) * occ.size());
+    ECMC_ALLOC(s->d_occ_n, sizeof(int32_t) * occ_n.size());
+    ECMC_ALLOC(s->d_mt, sizeof(uint32_t) * 624);
+    ECMC_ALLOC(s->d_ring, sizeof(uint32_t) * s->ring_words);
+    ECMC_ALLOC(s->d_st, sizeof(ChainState));
+    ECMC_ALLOC(s->d_tg, sizeof(ecmc_tagger_desc) * ECMC_MAX_TAGGERS);
+    ECMC_ALLOC(s->d_nout, sizeof(int64_t));
+    if (!ewald.empty()) { ECMC_ALLOC(s->d_ewald, sizeof(double) * ewald.size()); }
+    ECMC_UP(s->d_pos, pos_soa.data(), sizeof(double) * dim * n);
+    ECMC_UP(s->d_charge, ch.data(), sizeof(double) * n);
+    ECMC_UP(s->d_cmin, grid.cmin.data(), sizeof(double) * grid.cmin.size());
+    ECMC_UP(s->d_nearby, grid.nearby.data(), sizeof(int32_t) * grid.nearby.size());
+    ECMC_UP(s->d_nearby_n, grid.nearby_n.data(), sizeof(int32_t) * grid.nearby_n.size());
+    ECMC_UP(s->d_occ, occ.data(), sizeof(int32_t) * occ.size());
+    ECMC_UP(s->d_occ_n, occ_n.data(), sizeof(int32_t) * occ_n.size());
+    ECMC_UP(s->d_mt, mt, sizeof(uint32_t) * 624);
+    ECMC_UP(s->d_st, &st, sizeof(ChainState));
+    ECMC_UP(s->d_tg, desc->taggers, sizeof(ecmc_tagger_desc) * ECMC_MAX_TAGGERS);
+    if (!ewald.empty()) { ECMC_UP(s->d_ewald, ewald.data(), sizeof(double) * ewald.size()); }
+#undef ECMC_ALLOC
+#undef ECMC_UP
+    if (hipStreamCreate(&s->stream) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
+        hipEventCreate(&s->ev1) != hipSuccess) {
+        ecmc_system_destroy(s);
+        return ecmc_fail(ECMC_ERR_HIP, "stream/event creation failed");
+    }
+    a.n = n;
+    a.ncells = grid.count;
+    a.nb_stride = grid.nb_stride;
+    a.occ_cap = cap;
+    a.ring_words = s->ring_words;
+    a.n_taggers = desc->n_taggers;
+    a.L = desc->system_length;
+    a.half = desc->system_length / 2.0;
+    a.beta = desc->beta;
+    for (int d = 0; d < 3; d++) {
+        a.cps[d] = d < dim ? grid.cps[d] : 1;
+        a.cum[d] = d < dim ? (int32_t)grid.cum[d] : 0;
+        a.side[d] = d < dim ? grid.side[d] : 1.0;
+    }
+    a.pos = s->d_pos;
+    a.charge = s->d_charge;
+    a.cmin = s->d_cmin;
+    a.nearby = s->d_nearby;
+    a.nearby_n = s->d_nearby_n;
+    a.occ = s->d_occ;
+    a.occ_n = s->d_occ_n;
+    a.mt = s->d_mt;
+    a.ring = s->d_ring;
+    a.st = s->d_st;
+    a.tg = s->d_tg;
+    a.ewald.f = s->d_ewald;
+    a.n_out = s->d_nout;
+    *out = s;
+    return ECMC_OK;
+}
+
+extern "C" void ecmc_system_destroy(ecmc_system *s) {
+    if (!s) return;
+    hipFree(s->d_pos); hipFree(s->d_charge); hipFree(s->d_cmin); hipFree(s->d_nearby); hipFree(s->d_nearby_n);
+    hipFree(s->d_occ); hipFree(s->d_occ_n); hipFree(s->d_mt); hipFree(s->d_ring); hipFree(s->d_st);
+    hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald);
+    if (s->ev0) hipEventDestroy(s->ev0);
+    if (s->ev1) hipEventDestroy(s->ev1);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+}
+
+static int status_of_state(const ChainState &st) {
+    switch (st.err) {
+    case 0: return ECMC_OK;
+    case ECMC_ERR_SCHEDULER: return ecmc_fail(ECMC_ERR_SCHEDULER, "SchedulerError: no event or decreasing event time");
+    case ECMC_ERR_STATE: return ecmc_fail(ECMC_ERR_STATE, "a position left the cell system or an occupancy list overflowed");
+    case ECMC_ERR_NUMERIC: return ecmc_fail(ECMC_ERR_NUMERIC, "ValueError: math domain error in a displacement");
+    case ECMC_ERR_CONFIGURATION: return ecmc_fail(ECMC_ERR_CONFIGURATION, "end-of-chain time assertion or missing start of run");
+    default: return ecmc_fail(st.err, "device error");
+    }
+}
+
+extern "C" int ecmc_system_run(ecmc_system *s, int64_t max_events, ecmc_event *host_log, int64_t *n_committed) {
+    if (!s || max_events < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (n_committed) *n_committed = 0;
+    if (max_events == 0) return ECMC_OK;
+    if (host_log && s->log_cap < max_events) {
+        hipFree(s->d_log);
+        s->d_log = nullptr;
+        if (hipMalloc((void **)&s->d_log, sizeof(ecmc_event) * max_events) != hipSuccess)
+            return ecmc_fail(ECMC_ERR_HIP, "hipMalloc of the event log failed");
+        s->log_cap = max_events;
+    }
+    ChainArgs a = s->args;
+    a.log = host_log ? s->d_log : nullptr;
+    a.max_events = max_events;
+    a.stop_mask = s->stop_mask;
+    hipEventRecord(s->ev0, s->stream);
+    if (s->dim == 2) hipLaunchKernelGGL(chain_kernel<2>, dim3(1), dim3(NT), 0, s->stream, a);
+    else hipLaunchKernelGGL(chain_kernel<3>, dim3(1), dim3(NT), 0, s->stream, a);
+    hipEventRecord(s->ev1, s->stream);
+    if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, s->ev0, s->ev1);
+    s->last_ms = ms;
+    int64_t k = 0;
+    ChainState st;
+    if (hipMemcpy(&k, s->d_nout, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&st, s->d_st, sizeof(ChainState), hipMemcpyDeviceToHost) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+    if (host_log && k > 0 &&
+        hipMemcpy(host_log, s->d_log, sizeof(ecmc_event) * k, hipMemcpyDeviceToHost) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the event log failed");
+    if (n_committed) *n_committed = k;
+    return status_of_state(st);
+}
+
+extern "C" int ecmc_system_positions(const ecmc_system *s, double *out) {
+    if (!s || !out) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    std::vector<double> soa((size_t)s->dim * s->n);
+    if (hipMemcpy(soa.data(), s->d_pos, sizeof(double) * soa.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+    for (int i = 0; i < s->n; i++)
+        for (int d = 0; d < s->dim; d++) out[(size_t)i * s->dim + d] = soa[(size_t)d * s->n + i];
+    return ECMC_OK;
+}
+
+static int read_state(const ecmc_system *s, ChainState &st) {
+    if (hipMemcpy(&st, s->d_st, sizeof(ChainState), hipMemcpyDeviceToHost) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_system_active(const ecmc_system *s, int32_t *active, int32_t *direction, double *tq, double *tr,
+                                  int32_t *done) {
+    if (!s) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
+    ChainState st;
+    int rc = read_state(s, st);
+    if (rc) return rc;
+    if (active) *active = st.active;
+    if (direction) *direction = st.adir;
+    if (tq) *tq = st.at.q;
+    if (tr) *tr = st.at.r;
+    if (done) *done = st.done;
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_system_estimators(const ecmc_system *s, double *est8) {
+    if (!s || !est8) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    ChainState st;
+    int rc = read_state(s, st);
+    if (rc) return rc;
+    memcpy(est8, st.est, sizeof(st.est));
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_system_rng_words(const ecmc_system *s, int64_t *words) {
+    if (!s || !words) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    ChainState st;
+    int rc = read_state(s, st);
+    if (rc) return rc;
+    *words = (int64_t)st.cursor;
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_system_last_kernel_ms(const ecmc_system *s, double *ms) {
+    if (!s || !ms) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    *ms = s->last_ms;
+    return ECMC_OK;
+}
+
+// ---- MT19937 words on the device (testing the parallel twist against CPython) ----------
+__global__ __launch_bounds__(NT) void mt_words_kernel(uint32_t *state, uint32_t *out, int64_t n) {
+    __shared__ uint32_t mt[624];
+    for (int i = threadIdx.x; i < 624; i += NT) mt[i] = state[i];
+    __syncthreads();
+    for (int64_t produced = 0; produced < n; produced += 624) {
+        mt_generate_block(mt, out + produced, 624, 0);
+    }
+}
+
+extern "C" int ecmc_mt_words(uint64_t seed, int64_t n, uint32_t *d_out, void *stream) {
+    if (n < 0 || (n > 0 && !d_out)) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (n % 624 != 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "n must be a multiple of 624");
+    if (n == 0) return ECMC_OK;
+    uint32_t mt[624];
+    mt_seed_host(seed, mt);
+    uint32_t *d_state = nullptr;
+    if (hipMalloc((void **)&d_state, sizeof(mt)) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "hipMalloc failed");
+    hipStream_t st = (hipStream_t)stream;
+    hipMemcpyAsync(d_state, mt, sizeof(mt), hipMemcpyHostToDevice, st);
+    hipLaunchKernelGGL(mt_words_kernel, dim3(1), dim3(NT), 0, st, d_state, d_out, n);
+    hipError_t e = hipStreamSynchronize(st);
+    hipFree(d_state);
+    if (e != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "mt kernel failed");
+    return ECMC_OK;
+}

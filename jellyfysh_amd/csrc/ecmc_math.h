@@ -1,0 +1,225 @@
+// ecmc_math.h -- FP64 arithmetic of the ECMC hot path for gfx950.
+//
+// * Python float semantics the reference relies on: `x % L` (setting/hypercubic_setting.py:117,172)
+//   and divmod(x, 1.0) inside base/time.py:101,129-131.  Both follow CPython's float_rem /
+//   float_divmod rules (remainder takes the divisor's sign, quotient snapped to an integer).
+// * Correctly rounded log and pow.  The reference evaluates `x ** y` and math.log through
+//   the C library; a correctly rounded result is the one an exact evaluation rounds to, so
+//   every platform that rounds correctly agrees bit for bit.  They are computed here in
+//   double-double arithmetic (about 2^-100 relative error before the final rounding):
+//     log: x = m 2^e, m in [sqrt(1/2), sqrt(2)), m r_i - 1 = z exactly in double-double with
+//          r_i ~ 1/(1+i/128) from ecmc_tables.h, log1p(z) = 2 atanh(z/(2+z)) by its Taylor
+//          series, plus e ln2 and -log(r_i) from the table.
+//     exp: t = k ln2/64 + r, |r| <= ln2/128, Taylor series of exp(r) to r^11, times the
+//          double-double 2^(j/64) table and 2^(k>>6).
+//   pow(x, y) = exp(y log x) with both steps in double-double.
+//
+// Everything here is compiled with -ffp-contract=off: fused operations appear only where
+// fma() is written out (the exact product error of two_prod).
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__) && !defined(ECMC_HOST_BUILD)
+#include <hip/hip_runtime.h>
+#define ECMC_FN __device__ __forceinline__
+#define ECMC_HD __host__ __device__ __forceinline__
+#define ECMC_TABLE __constant__
+#else
+#define ECMC_FN static inline
+#define ECMC_HD static inline
+#define ECMC_TABLE
+#endif
+
+#include "ecmc_tables.h"
+
+namespace ecmc {
+
+// ---- Python float semantics ------------------------------------------------------------
+ECMC_HD double py_mod(double x, double y) {
+    double m = fmod(x, y);
+    if (m != 0.0) {
+        if ((y < 0) != (m < 0)) m += y;
+    } else {
+        m = copysign(0.0, y);
+    }
+    return m;
+}
+
+ECMC_HD void py_divmod(double vx, double wx, double &q, double &r) {
+    double mod = fmod(vx, wx);
+    double div = (vx - mod) / wx;
+    if (mod != 0.0) {
+        if ((wx < 0) != (mod < 0)) { mod += wx; div -= 1.0; }
+    } else {
+        mod = copysign(0.0, wx);
+    }
+    double fl;
+    if (div != 0.0) {
+        fl = floor(div);
+        if (div - fl > 0.5) fl += 1.0;
+    } else {
+        fl = copysign(0.0, vx / wx);
+    }
+    q = fl;
+    r = mod;
+}
+
+// Event time as (quotient, remainder) of a division by one (base/time.py:87-182).
+struct Time {
+    double q, r;
+};
+ECMC_HD Time time_add(Time t, double dt) {
+    Time o;
+    if (!isinf(dt)) {
+        double aq, nr;
+        py_divmod(t.r + dt, 1.0, aq, nr);
+        o.q = t.q + aq;
+        o.r = nr;
+    } else {
+        o.q = dt;
+        o.r = dt;
+    }
+    return o;
+}
+ECMC_HD double time_sub(Time a, Time b) { return a.q - b.q + a.r - b.r; }
+ECMC_HD bool time_lt(Time a, Time b) { return a.q < b.q || (a.q == b.q && a.r < b.r); }
+ECMC_HD bool time_eq(Time a, Time b) { return a.q == b.q && a.r == b.r; }
+ECMC_HD Time time_from_float(double t) {
+    Time o;
+    if (isinf(t)) { o.q = t; o.r = t; } else { py_divmod(t, 1.0, o.q, o.r); }
+    return o;
+}
+
+// ---- double-double ----------------------------------------------------------------------
+struct dd {
+    double h, l;
+};
+ECMC_FN dd two_sum(double a, double b) {
+    double s = a + b;
+    double bb = s - a;
+    double e = (a - (s - bb)) + (b - bb);
+    return {s, e};
+}
+ECMC_FN dd quick_two_sum(double a, double b) {
+    double s = a + b;
+    double e = b - (s - a);
+    return {s, e};
+}
+ECMC_FN dd two_prod(double a, double b) {
+    double p = a * b;
+    double e = fma(a, b, -p);
+    return {p, e};
+}
+ECMC_FN dd dd_add(dd a, dd b) {
+    dd s = two_sum(a.h, b.h);
+    dd t = two_sum(a.l, b.l);
+    s.l += t.h;
+    s = quick_two_sum(s.h, s.l);
+    s.l += t.l;
+    return quick_two_sum(s.h, s.l);
+}
+ECMC_FN dd dd_neg(dd a) { return {-a.h, -a.l}; }
+ECMC_FN dd dd_sub(dd a, dd b) { return dd_add(a, dd_neg(b)); }
+ECMC_FN dd dd_mul(dd a, dd b) {
+    dd p = two_prod(a.h, b.h);
+    p.l += a.h * b.l + a.l * b.h;
+    return quick_two_sum(p.h, p.l);
+}
+ECMC_FN dd dd_mul_d(dd a, double b) {
+    dd p = two_prod(a.h, b);
+    p.l += a.l * b;
+    return quick_two_sum(p.h, p.l);
+}
+ECMC_FN dd dd_div(dd a, dd b) {
+    double q1 = a.h / b.h;
+    dd r = dd_sub(a, dd_mul_d(b, q1));
+    double q2 = r.h / b.h;
+    r = dd_sub(r, dd_mul_d(b, q2));
+    double q3 = r.h / b.h;
+    dd q = quick_two_sum(q1, q2);
+    return dd_add(q, dd{q3, 0.0});
+}
+
+// Natural logarithm of a positive, finite, normal x in double-double.
+ECMC_FN dd dd_log(double x) {
+    int e;
+    double m = frexp(x, &e);          // m in [0.5, 1)
+    m = m * 2.0;
+    e -= 1;                            // m in [1, 2)
+    if (m > 0x1.6a09e667f3bcdp+0) {   // sqrt(2)
+        m = m * 0.5;
+        e += 1;
+    }
+    int i = (int)rint((m - 1.0) * 128.0);
+    int ti = i - ECMC_LOG_TMIN;
+    double r = ecmc_log_r[ti];
+    dd p = two_prod(m, r);
+    dd z = quick_two_sum(p.h - 1.0, p.l);    // p.h - 1 is exact
+    dd w = dd_div(z, dd_add(dd{2.0, 0.0}, z));
+    dd s = dd_mul(w, w);
+    double sh = s.h;
+    double tail = (((1.0 / 15.0) * sh + 1.0 / 13.0) * sh + 1.0 / 11.0) * sh + 1.0 / 9.0;
+    dd acc = {tail, 0.0};
+    acc = dd_add(dd{ECMC_INV7_H, ECMC_INV7_L}, dd_mul(s, acc));
+    acc = dd_add(dd{ECMC_INV5_H, ECMC_INV5_L}, dd_mul(s, acc));
+    acc = dd_add(dd{ECMC_INV3_H, ECMC_INV3_L}, dd_mul(s, acc));
+    acc = dd_add(dd{1.0, 0.0}, dd_mul(s, acc));
+    dd l1p = dd_mul(dd_mul_d(w, 2.0), acc);
+    double ed = (double)e;
+    dd el = two_prod(ed, ECMC_LN2_H);
+    el = dd_add(el, dd{ed * ECMC_LN2_L, 0.0});
+    dd t = {ecmc_log_th[ti], ecmc_log_tl[ti]};
+    return dd_add(dd_add(el, t), l1p);
+}
+
+// exp of a double-double argument with |t| < 700.
+ECMC_FN dd dd_exp(dd t) {
+    double kf = rint(t.h * ECMC_INV_LN2_64);
+    int k = (int)kf;
+    dd r = two_sum(t.h - kf * ECMC_LN2_64_HI, t.l);
+    r = dd_sub(r, two_prod(kf, ECMC_LN2_64_MID));
+    r = dd_sub(r, dd{kf * ECMC_LN2_64_LO, 0.0});
+    double rh = r.h;
+    double tail = ((((ECMC_INVFACT12_H * rh + ECMC_INVFACT11_H) * rh + ECMC_INVFACT10_H) * rh + ECMC_INVFACT9_H) * rh
+                   + ECMC_INVFACT8_H) * rh + ECMC_INVFACT7_H;
+    tail = tail * rh + ECMC_INVFACT6_H;
+    dd acc = {tail, 0.0};
+    acc = dd_add(dd{ECMC_INVFACT5_H, ECMC_INVFACT5_L}, dd_mul(r, acc));
+    acc = dd_add(dd{ECMC_INVFACT4_H, ECMC_INVFACT4_L}, dd_mul(r, acc));
+    acc = dd_add(dd{ECMC_INVFACT3_H, ECMC_INVFACT3_L}, dd_mul(r, acc));
+    acc = dd_add(dd{0.5, 0.0}, dd_mul(r, acc));
+    acc = dd_add(dd{1.0, 0.0}, dd_mul(r, acc));
+    acc = dd_add(dd{1.0, 0.0}, dd_mul(r, acc));
+    int j = k & 63;
+    int sc = (k - j) / 64;
+    dd res = dd_mul(acc, dd{ecmc_exp2_h[j], ecmc_exp2_l[j]});
+    res.h = ldexp(res.h, sc);
+    res.l = ldexp(res.l, sc);
+    return res;
+}
+
+// Correctly rounded natural logarithm.
+ECMC_FN double cr_log(double x) {
+    if (!(x > 0.0) || isinf(x) || x < 0x1p-1022) return log(x);
+    if (x == 1.0) return 0.0;
+    dd l = dd_log(x);
+    return l.h;
+}
+
+// Correctly rounded x ** y for the arguments the potentials use (x > 0).
+ECMC_FN double cr_pow(double x, double y) {
+    if (y == 2.0) return x * x;
+    if (y == 0.5 && x >= 0.0) return x == 0.0 ? 0.0 : sqrt(x);
+    if (y == 1.0) return x;
+    if (y == 0.0) return 1.0;
+    if (!(x > 0.0) || isinf(x) || !isfinite(y) || x < 0x1p-1022 || x == 1.0) return pow(x, y);
+    dd l = dd_log(x);
+    dd t = dd_mul_d(l, y);
+    if (fabs(t.h) > 700.0) return pow(x, y);
+    dd e = dd_exp(t);
+    return e.h;
+}
+
+}  // namespace ecmc

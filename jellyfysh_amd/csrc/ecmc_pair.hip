@@ -1,0 +1,183 @@
+// ecmc_pair.hip -- batched per-pair evaluations: one thread per (separation, potential change)
+// row, reading SoA-free row-major inputs straight from device memory.  These are the
+// device counterparts of the reference's per-call potential methods (displacement and
+// derivative), used by the plugin classes and by the parity tests.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <vector>
+
+#include "../../include/jellyfysh_amd.h"
+#include "ecmc_common.h"
+#include "ecmc_math.h"
+#include "ecmc_potentials.h"
+
+namespace ecmc {
+
+constexpr int PT = 256;
+
+struct PairArgs {
+    int32_t potential, dim;
+    int64_t n;
+    double par[4];
+    double L;
+    const int32_t *dir;
+    const double *speed, *sep, *c1, *c2, *dU;
+    double *out;
+    Ewald ewald;  // merged-image Coulomb
+};
+
+template <int D>
+__device__ void pair_row(const PairArgs &a, int64_t i, bool derivative) {
+    double s[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) s[d] = a.sep[i * D + d];
+    const int dir = a.dir[i];
+    const double speed = a.speed ? a.speed[i] : 1.0;
+    const double c1 = a.c1 ? a.c1[i] : 1.0, c2 = a.c2 ? a.c2[i] : 1.0;
+    const double dU = a.dU ? a.dU[i] : 0.0;
+    double r = NAN;
+    double p[3] = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 3; d++) p[d] = d < D ? s[(dir + d) % D] : 0.0;
+    switch (a.potential) {
+    case POT_HARD_SPHERE:
+        r = derivative ? NAN : hs_displacement<D>(dir, speed, s, a.par[0]);
+        break;
+    case POT_INVERSE_POWER:
+        r = derivative ? ip_derivative<D>(dir, speed, s, c1, c2, a.par[0], a.par[1])
+                       : ip_displacement<D>(dir, speed, s, c1, c2, a.par[0], a.par[1], dU);
+        break;
+    case POT_INVERSE_POWER_TRUNC:
+        r = derivative ? ip_derivative<D>(dir, speed, s, c1, c2, a.par[0], a.par[1])
+                       : ipt_displacement<D>(dir, speed, s, c1, c2, a.par[0], a.par[1], a.par[2], dU);
+        break;
+    case POT_LENNARD_JONES: {
+        MexicanHat h = mh_lennard_jones(a.par[0], a.par[1]);
+        r = derivative ? mh_derivative<D>(h, dir, speed, s) : mh_displacement<D>(h, dir, speed, s, dU);
+        break;
+    }
+    case POT_DISPLACED_EVEN_POWER: {
+        MexicanHat h = mh_displaced_even_power(a.par[0], a.par[1], a.par[2]);
+        r = derivative ? mh_derivative<D>(h, dir, speed, s) : mh_displacement<D>(h, dir, speed, s, dU);
+        break;
+    }
+    case POT_INVERSE_POWER_COULOMB_BOUND: {
+        const double pp = a.par[0] * c1 * c2;
+        r = derivative ? ipcb_derivative(pp, p[0], p[1], p[2]) * speed
+                       : ipcb_displacement(pp, p[0], p[1], p[2], dU, a.L) / speed;
+        break;
+    }
+    case POT_MERGED_IMAGE_COULOMB:
+        r = derivative ? a.par[3] * c1 * c2 * ewald_derivative(a.ewald, p[0], p[1], p[2]) * speed : NAN;
+        break;
+    default:
+        break;
+    }
+    a.out[i] = r;
+}
+
+template <int D>
+__global__ __launch_bounds__(PT) void pair_kernel(PairArgs a, int derivative) {
+    int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
+    if (i >= a.n) return;
+    pair_row<D>(a, i, derivative != 0);
+}
+
+__global__ __launch_bounds__(PT) void log_kernel(int64_t n, const double *x, double *out) {
+    int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
+    if (i < n) out[i] = cr_log(x[i]);
+}
+__global__ __launch_bounds__(PT) void pow_kernel(int64_t n, const double *x, const double *y, double *out) {
+    int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
+    if (i < n) out[i] = cr_pow(x[i], y[i]);
+}
+
+static int launch_pair(int32_t potential, const double *par, int32_t dim, int64_t n, const int32_t *dir,
+                       const double *speed, const double *sep, const double *c1, const double *c2, const double *dU,
+                       double L, double *out, void *stream, int derivative) {
+    if (n < 0 || (dim != 2 && dim != 3) || !par) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (n == 0) return ECMC_OK;
+    if (!dir || !sep || !out) return ecmc_fail(ECMC_ERR_ARGUMENT, "null device pointer");
+    if (potential < 0 || potential > POT_DISPLACED_EVEN_POWER) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown potential");
+    if ((potential == POT_MERGED_IMAGE_COULOMB || potential == POT_INVERSE_POWER_COULOMB_BOUND) && dim != 3)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the Coulomb potentials need three dimensions");
+    if (potential == POT_MERGED_IMAGE_COULOMB && !derivative)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the merged-image Coulomb potential has no displacement");
+    if (potential == POT_HARD_SPHERE && derivative)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the hard-sphere potential has no derivative");
+    PairArgs a;
+    memset(&a, 0, sizeof(a));
+    a.potential = potential;
+    a.dim = dim;
+    a.n = n;
+    for (int k = 0; k < 4; k++) a.par[k] = par[k];
+    a.L = L;
+    a.dir = dir; a.speed = speed; a.sep = sep; a.c1 = c1; a.c2 = c2; a.dU = dU; a.out = out;
+    hipStream_t st = (hipStream_t)stream;
+    double *d_table = nullptr;
+    if (potential == POT_MERGED_IMAGE_COULOMB) {
+        std::vector<double> table;
+        int fc = (int)par[1], pc = (int)par[2];
+        if (fc < 0 || pc < 0 || !(par[0] > 0.0)) return ecmc_fail(ECMC_ERR_CONFIGURATION, "bad Ewald parameters");
+        build_ewald_table(fc, par[0], L, table);
+        if (hipMalloc((void **)&d_table, sizeof(double) * table.size()) != hipSuccess)
+            return ecmc_fail(ECMC_ERR_HIP, "hipMalloc failed");
+        hipMemcpyAsync(d_table, table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice, st);
+        a.ewald.f = d_table;
+        a.ewald.fc = fc; a.ewald.pc = pc; a.ewald.L = L;
+        a.ewald.aol = par[0] / L; a.ewald.aol2 = par[0] * par[0] / (L * L);
+        a.ewald.two_aol_rpi = 2.0 * par[0] / (L * sqrt(M_PI));
+        a.ewald.tpol = 2.0 * M_PI / L;
+    }
+    dim3 grid((unsigned)((n + PT - 1) / PT));
+    if (dim == 2) hipLaunchKernelGGL(pair_kernel<2>, grid, dim3(PT), 0, st, a, derivative);
+    else hipLaunchKernelGGL(pair_kernel<3>, grid, dim3(PT), 0, st, a, derivative);
+    hipError_t e = hipGetLastError();
+    if (d_table) {
+        hipStreamSynchronize(st);
+        hipFree(d_table);
+    }
+    if (e != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "pair kernel launch failed");
+    return ECMC_OK;
+}
+
+}  // namespace ecmc
+
+using namespace ecmc;
+
+extern "C" int ecmc_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int ecmc_pair_displacement(int32_t potential, const double *par, int32_t dim, int64_t n,
+                                      const int32_t *d_dir, const double *d_speed, const double *d_sep,
+                                      const double *d_c1, const double *d_c2, const double *d_dU,
+                                      double system_length, double *d_out, void *stream) {
+    return launch_pair(potential, par, dim, n, d_dir, d_speed, d_sep, d_c1, d_c2, d_dU, system_length, d_out, stream, 0);
+}
+
+extern "C" int ecmc_pair_derivative(int32_t potential, const double *par, int32_t dim, int64_t n,
+                                    const int32_t *d_dir, const double *d_speed, const double *d_sep,
+                                    const double *d_c1, const double *d_c2, double system_length, double *d_out,
+                                    void *stream) {
+    return launch_pair(potential, par, dim, n, d_dir, d_speed, d_sep, d_c1, d_c2, nullptr, system_length, d_out,
+                       stream, 1);
+}
+
+extern "C" int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream) {
+    if (n < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (n == 0) return ECMC_OK;
+    hipLaunchKernelGGL(log_kernel, dim3((unsigned)((n + PT - 1) / PT)), dim3(PT), 0, (hipStream_t)stream, n, d_x, d_out);
+    return hipGetLastError() == hipSuccess ? ECMC_OK : ecmc_fail(ECMC_ERR_HIP, "log kernel launch failed");
+}
+
+extern "C" int ecmc_math_pow(int64_t n, const double *d_x, const double *d_y, double *d_out, void *stream) {
+    if (n < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (n == 0) return ECMC_OK;
+    hipLaunchKernelGGL(pow_kernel, dim3((unsigned)((n + PT - 1) / PT)), dim3(PT), 0, (hipStream_t)stream, n, d_x, d_y,
+                       d_out);
+    return hipGetLastError() == hipSuccess ? ECMC_OK : ecmc_fail(ECMC_ERR_HIP, "pow kernel launch failed");
+}

@@ -49,7 +49,7 @@ extern "C" {
 enum { ORA_MATH_LIBM = 0, ORA_MATH_CR = 1 };
 
 /* ---- CPython-compatible Mersenne twister ------------------------------------------ */
-typedef struct { uint32_t mt[624]; int32_t mti; } ora_mt;
+typedef struct { uint32_t mt[624]; int32_t mti; uint64_t words; } ora_mt;
 void ora_mt_seed(ora_mt *r, uint64_t seed);
 uint32_t ora_mt_u32(ora_mt *r);
 double ora_random(ora_mt *r);
@@ -167,6 +167,8 @@ typedef struct {
     int32_t target;     /* pair target / new active chosen by end-of-chain / -1 */
     double tq, tr;      /* event time (Time quotient, remainder) */
     double pos[3];      /* committed position of the previously active unit */
+    int32_t ncand;      /* pair candidates created at the start of this leg */
+    uint32_t chash;     /* sum over them of an order-dependent hash of (rank, target) */
 } ora_event;
 
 typedef struct ora_engine ora_engine;

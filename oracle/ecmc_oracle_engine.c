@@ -162,8 +162,18 @@ struct ora_engine {
     ora_ewald *ewald[ORA_MAX_TAGGERS];
     double est[8];
     int32_t *cand; int ncand, capcand;
+    uint32_t chash;
+    int pending_creates, last_tagger;
     int64_t rng_words;
 };
+
+/* Order-dependent hash term of candidate j with target t, summed over a leg's candidates
+ * (a checksum for comparing candidate sets; it has no reference counterpart). */
+static uint32_t cand_mix(uint32_t j, uint32_t t) {
+    uint32_t h = (j * 0x9E3779B9u) ^ t;
+    h ^= h >> 16; h *= 0x85ebca6bu; h ^= h >> 13; h *= 0xc2b2ae35u; h ^= h >> 16;
+    return h;
+}
 
 static void eng_fail(ora_engine *e, const char *msg) {
     if (!e->err) { e->err = 1; snprintf(e->errmsg, sizeof(e->errmsg), "%s", msg); }
@@ -290,17 +300,14 @@ static double pair_displacement(ora_engine *e, const ora_tagger *tg, int tidx, c
         }
         case ORA_POT_INVERSE_POWER: {
             double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
-            e->rng_words += 2;
             return ora_ip_displacement(e->dim, dir, speed, s, c1, c2, tg->par[0], tg->par[1], dU, mm);
         }
         case ORA_POT_INVERSE_POWER_TRUNC: {
             double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
-            e->rng_words += 2;
             return ora_ipt_displacement(e->dim, dir, speed, s, c1, c2, tg->par[0], tg->par[1], tg->par[2], dU, mm);
         }
         case ORA_POT_LENNARD_JONES: {
             double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
-            e->rng_words += 2;
             return ora_lj_displacement(e->dim, dir, speed, s, tg->par[0], tg->par[1], dU, mm);
         }
         default: eng_fail(e, "unsupported potential for TwoLeafUnitEventHandler"); return INFINITY;
@@ -312,7 +319,6 @@ static double pair_displacement(ora_engine *e, const ora_tagger *tg, int tidx, c
         for (int d = 0; d < 3; d++) p[d] = s[(dir + d) % 3];    /* vectors.permutation_3d */
         double pp = tg->bpar[0] * c1 * c2;
         double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
-        e->rng_words += 2;
         return ora_ipcb_displacement(pp, p[0], p[1], p[2], dU, e->L) / speed;
     }
     (void)tidx;
@@ -368,6 +374,7 @@ static void create_tagger(ora_engine *e, int t) {
                 hd->aq = q; hd->ar = r;
                 sched_push(e, hidx, q, r);
                 if (e->ncand == e->capcand) { e->capcand = e->capcand ? 2 * e->capcand : 1024; e->cand = (int32_t *)realloc(e->cand, sizeof(int32_t) * e->capcand); }
+                e->chash += cand_mix((uint32_t)e->ncand, (uint32_t)tid);
                 e->cand[e->ncand++] = tid;
                 k++;
             }
@@ -470,12 +477,12 @@ static int out_state(ora_engine *e, int hidx, ora_event *ev) {
             lift = 0;
             if (real > 0) {
                 double u = ora_uniform(&e->rng, 0, bound);
-                e->rng_words += 2;
-                if (u < real) lift = 1;
+                    if (u < real) lift = 1;
             }
         }
         memcpy(e->pos + (size_t)a * e->dim, hd->apos, sizeof(double) * e->dim);
         e->aq = hd->aq; e->ar = hd->ar;
+        if (!lift) e->est[6] += 1.0;
         if (lift) {
             double sep[3];
             separation_vector(e, hd->apos, hd->tpos, sep);
@@ -521,8 +528,8 @@ static int out_state(ora_engine *e, int hidx, ora_event *ev) {
         e->aq = 0.0; e->ar = 0.0;
         break;
     }
-    if (a < 0) memset(ev->pos, 0, sizeof(ev->pos));
-    else memcpy(ev->pos, e->pos + (size_t)a * e->dim, sizeof(double) * e->dim);
+    memset(ev->pos, 0, sizeof(ev->pos));
+    if (a >= 0) memcpy(ev->pos, e->pos + (size_t)a * e->dim, sizeof(double) * e->dim);
     return 0;
 }
 
@@ -592,13 +599,30 @@ int64_t ora_engine_run(ora_engine *e, int64_t max_events, ora_event *log) {
             int sor = -1;
             for (int t = 0; t < e->cfg.n_taggers; t++) if (e->cfg.taggers[t].kind == ORA_TAG_START_OF_RUN) sor = t;
             if (sor < 0) { eng_fail(e, "ConfigurationError: a StartOfRunEventHandler is required"); return -1; }
+            e->ncand = 0;
+            e->chash = 0;
             create_tagger(e, sor);
             e->started = 1;
+        } else if (e->pending_creates) {
+            /* tag_activator.py:228-289: occupancy update, then the creates of the tagger
+             * that committed the preceding event, in INI order */
+            occ_update(e);
+            if (e->err) return -1;
+            e->ncand = 0;
+            e->chash = 0;
+            ora_tagger *pt = &e->cfg.taggers[e->last_tagger];
+            for (int i = 0; i < pt->n_create; i++) {
+                create_tagger(e, pt->create[i]);
+                if (e->err) return -1;
+            }
+            e->pending_creates = 0;
         }
         int win = sched_pop_min(e);
         if (win < 0) return -1;
         ora_event ev;
         out_state(e, win, &ev);
+        ev.ncand = e->ncand;
+        ev.chash = e->chash;
         int wt = e->h[win].tagger;
         /* trash (tag_activator.py:291-316) */
         ora_tagger *tg = &e->cfg.taggers[wt];
@@ -614,18 +638,8 @@ int64_t ora_engine_run(ora_engine *e, int64_t max_events, ora_event *log) {
         if (e->err) return -1;
         if (log) log[k] = ev;
         k++;
-        if (e->done) break;
-        /* next leg: occupancy update then the creates of the committed tagger */
-        occ_update(e);
-        if (e->err) return -1;
-        int had_pairs = 0;
-        for (int i = 0; i < tg->n_create; i++) {
-            int t = tg->create[i];
-            int kind = e->cfg.taggers[t].kind;
-            if ((kind == ORA_TAG_PAIR_CELLS || kind == ORA_TAG_PAIR_ALL) && !had_pairs) { e->ncand = 0; had_pairs = 1; }
-            create_tagger(e, t);
-            if (e->err) return -1;
-        }
+        e->pending_creates = 1;
+        e->last_tagger = wt;
     }
     return k;
 }
@@ -648,7 +662,7 @@ int32_t ora_engine_last_candidates(const ora_engine *e, int32_t *out, int32_t ca
     if (out) memcpy(out, e->cand, sizeof(int32_t) * n);
     return e->ncand;
 }
-int64_t ora_engine_rng_words(const ora_engine *e) { return e->rng_words; }
+int64_t ora_engine_rng_words(const ora_engine *e) { return (int64_t)e->rng.words; }
 
 /* ===== configuration helpers ============================================================ */
 static void tg_lists(ora_tagger *t, int nc, const int *c, int ntr, const int *tr) {

@@ -45,6 +45,7 @@ void ora_mt_seed(ora_mt *r, uint64_t seed) {
     }
     r->mt[0] = 0x80000000U;
     r->mti = MT_N;
+    r->words = 0;
 }
 
 uint32_t ora_mt_u32(ora_mt *r) {
@@ -65,6 +66,7 @@ uint32_t ora_mt_u32(ora_mt *r) {
         r->mti = 0;
     }
     y = r->mt[r->mti++];
+    r->words++;
     y ^= (y >> 11);
     y ^= (y << 7) & 0x9d2c5680U;
     y ^= (y << 15) & 0xefc60000U;

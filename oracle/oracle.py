@@ -41,7 +41,8 @@ class Config(ctypes.Structure):
 
 
 EVENT_DTYPE = np.dtype([("tagger", np.int32), ("flags", np.int32), ("active", np.int32), ("target", np.int32),
-                        ("tq", np.float64), ("tr", np.float64), ("pos", np.float64, (3,))])
+                        ("tq", np.float64), ("tr", np.float64), ("pos", np.float64, (3,)), ("ncand", np.int32),
+                        ("chash", np.uint32)])
 
 
 def build():
@@ -113,7 +114,7 @@ class MT:
     """CPython-compatible MT19937 stream."""
 
     def __init__(self, seed):
-        self._buf = ctypes.create_string_buffer(624 * 4 + 8)
+        self._buf = ctypes.create_string_buffer(624 * 4 + 16)
         lib().ora_mt_seed(self._buf, seed)
 
     def u32(self):

@@ -1,4 +1,8 @@
-"""Shared builders for test systems: fixture loading and oracle configurations."""
+"""Shared builders for test systems: fixtures, wirings, oracle and device configurations.
+
+A wiring is the list of tagger dictionaries understood by jellyfysh_amd.chain.make_desc;
+the same wiring fills the oracle's configuration, so both sides run the same INI setup.
+"""
 import json
 import os
 
@@ -15,69 +19,118 @@ def load_fixture(name):
     return meta, data
 
 
-def oracle_config(meta, math_mode=0, scheduler=None):
-    """Oracle configuration mirroring the tagger list the fixture script wired into the
-    reference: factors..., cell_boundary, [sampling], end_of_chain, end_of_run, start_of_run."""
+def factor_tagger(tag, kind, params, **extra):
+    t = dict(tag=tag, kind="pair_cells", potential=kind, params=list(params))
+    t.update(extra)
+    return t
+
+
+def standard_wiring(factors, chain_time, end_time, sampling_interval=0.0, first_event_time_zero=False,
+                    initial_active=0, initial_direction=0, speed=1.0):
+    """Taggers in the order the fixture script wires them into the reference:
+    factors..., cell_boundary, [sampling], end_of_chain, end_of_run, start_of_run."""
+    pc = [f["tag"] for f in factors] + ["cell_boundary"]
+    eoc = ["end_of_chain"] + pc
+    samp = ["sampling"] if sampling_interval else []
+    taggers = []
+    for f in factors:
+        t = dict(f)
+        t.setdefault("create", pc)
+        t.setdefault("trash", pc)
+        taggers.append(t)
+    taggers.append(dict(tag="cell_boundary", kind="cell_boundary", create=pc, trash=pc))
+    if samp:
+        taggers.append(dict(tag="sampling", kind="sampling", create=samp, trash=samp,
+                            sampling_interval=sampling_interval, first_event_time_zero=first_event_time_zero))
+    taggers.append(dict(tag="end_of_chain", kind="end_of_chain", create=eoc, trash=eoc, chain_time=chain_time))
+    taggers.append(dict(tag="end_of_run", kind="end_of_run", create=["end_of_run"],
+                        trash=eoc + samp + ["start_of_run", "end_of_run"], end_of_run_time=end_time))
+    taggers.append(dict(tag="start_of_run", kind="start_of_run", create=pc + samp + ["end_of_chain", "end_of_run"],
+                        trash=["start_of_run"], initial_direction_of_motion=initial_direction, speed=speed,
+                        initial_active_identifier=initial_active))
+    return taggers
+
+
+def wiring_from_meta(meta):
+    factors = []
+    for f in meta["factors"]:
+        if f["kind"] == "hard_sphere":
+            factors.append(factor_tagger(f["tag"], "hard_sphere", [f["radius"]]))
+        elif f["kind"] == "inverse_power":
+            factors.append(factor_tagger(f["tag"], "inverse_power", [f["power"], f["prefactor"]]))
+        elif f["kind"] == "inverse_power_truncated":
+            factors.append(factor_tagger(f["tag"], "inverse_power_truncated", [f["power"], f["prefactor"], f["cutoff"]]))
+        elif f["kind"] == "lennard_jones":
+            factors.append(factor_tagger(f["tag"], "lennard_jones", [f["prefactor"], f["sigma"]]))
+        else:
+            raise ValueError(f["kind"])
+    return standard_wiring(factors, meta["chain_time"], meta["end_time"], meta.get("sampling_interval", 0.0))
+
+
+_ORACLE_POT = {"hard_sphere": 0, "inverse_power": 1, "lennard_jones": 2, "inverse_power_truncated": 3,
+               "merged_image_coulomb": 4, "inverse_power_coulomb_bounding": 5}
+_ORACLE_KIND = {"pair_cells": 0, "pair_all": 1, "cell_boundary": 2, "sampling": 3, "end_of_chain": 4,
+                "end_of_run": 5, "start_of_run": 6}
+
+
+def oracle_config_from_wiring(dim, n, system_length, beta, cells_per_side, taggers, seed, scheduler="heap",
+                              math_mode=0):
     from oracle import oracle as O
     c = O.Config()
-    c.dim, c.n, c.system_length, c.beta = meta["dim"], meta["n"], meta["system_length"], meta["beta"]
+    c.dim, c.n, c.system_length, c.beta = dim, n, system_length, beta
     for d in range(3):
-        c.cells_per_side[d] = meta["cells_per_side"]
+        c.cells_per_side[d] = cells_per_side
     c.neighbor_layers = 1
     c.max_occupants = -1
-    sched = scheduler or meta["scheduler"]
-    c.scheduler = O.SCHED_LIST if sched == "list" else O.SCHED_HEAP
+    c.scheduler = O.SCHED_LIST if scheduler == "list" else O.SCHED_HEAP
     c.math_mode = math_mode
-    c.seed = meta["seed"]
-    tags = meta["taggers"]
-    idx = {t: i for i, t in enumerate(tags)}
-    nf = len(meta["factors"])
-    pc = [idx[f["tag"]] for f in meta["factors"]] + [idx["cell_boundary"]]
-    eoc = [idx["end_of_chain"]] + pc
-    samp = [idx["sampling"]] if "sampling" in idx else []
-    c.n_taggers = len(tags)
-
-    def lists(t, create, trash):
-        t.n_create = len(create)
-        for i, v in enumerate(create):
-            t.create[i] = v
-        t.n_trash = len(trash)
-        for i, v in enumerate(trash):
-            t.trash[i] = v
-
-    for k, f in enumerate(meta["factors"]):
-        t = c.taggers[idx[f["tag"]]]
-        t.kind, t.event_handler, t.n_handlers = O.TAG_PAIR_CELLS, O.EH_TWO_LEAF, meta["n"]
-        if f["kind"] == "hard_sphere":
-            t.potential = O.POT_HARD_SPHERE
-            t.par[0] = f["radius"]
-        elif f["kind"] == "inverse_power":
-            t.potential = O.POT_INVERSE_POWER
-            t.par[0], t.par[1] = f["power"], f["prefactor"]
-        elif f["kind"] == "inverse_power_truncated":
-            t.potential = O.POT_INVERSE_POWER_TRUNC
-            t.par[0], t.par[1], t.par[2] = f["power"], f["prefactor"], f["cutoff"]
-        elif f["kind"] == "lennard_jones":
-            t.potential = O.POT_LENNARD_JONES
-            t.par[0], t.par[1] = f["prefactor"], f["sigma"]
-        lists(t, pc, pc)
-    t = c.taggers[idx["cell_boundary"]]
-    t.kind, t.n_handlers = O.TAG_CELL_BOUNDARY, 1
-    lists(t, pc, pc)
-    if samp:
-        t = c.taggers[idx["sampling"]]
-        t.kind, t.n_handlers, t.value = O.TAG_SAMPLING, 1, meta["sampling_interval"]
-        lists(t, samp, samp)
-    t = c.taggers[idx["end_of_chain"]]
-    t.kind, t.n_handlers, t.value = O.TAG_END_OF_CHAIN, 1, meta["chain_time"]
-    lists(t, eoc, eoc)
-    t = c.taggers[idx["end_of_run"]]
-    t.kind, t.n_handlers, t.value = O.TAG_END_OF_RUN, 1, meta["end_time"]
-    lists(t, [idx["end_of_run"]], eoc + samp + [idx["start_of_run"], idx["end_of_run"]])
-    t = c.taggers[idx["start_of_run"]]
-    t.kind, t.n_handlers, t.value, t.ivalue, t.ivalue2 = O.TAG_START_OF_RUN, 1, 1.0, 0, 0
-    lists(t, pc + samp + [idx["end_of_chain"], idx["end_of_run"]], [idx["start_of_run"]])
+    c.seed = seed
+    idx = {t["tag"]: i for i, t in enumerate(taggers)}
+    c.n_taggers = len(taggers)
+    for i, t in enumerate(taggers):
+        g = c.taggers[i]
+        g.kind = _ORACLE_KIND[t["kind"]]
+        for j, tag in enumerate(t.get("create", [])):
+            g.create[j] = idx[tag]
+        g.n_create = len(t.get("create", []))
+        for j, tag in enumerate(t.get("trash", [])):
+            g.trash[j] = idx[tag]
+        g.n_trash = len(t.get("trash", []))
+        g.n_handlers = t.get("number_event_handlers", n)
+        if t["kind"] in ("pair_cells", "pair_all"):
+            g.event_handler = 1 if t.get("bounding_potential") else 0
+            g.potential = _ORACLE_POT[t["potential"]]
+            for j, v in enumerate(t.get("params", ())):
+                g.par[j] = v
+            if t.get("bounding_potential"):
+                g.bounding_potential = _ORACLE_POT[t["bounding_potential"]]
+                for j, v in enumerate(t.get("bounding_params", ())):
+                    g.bpar[j] = v
+            g.use_charge = 1 if t.get("charge") else 0
+        elif t["kind"] == "sampling":
+            g.value = t["sampling_interval"]
+            g.ivalue = 1 if t.get("first_event_time_zero") else 0
+        elif t["kind"] == "end_of_chain":
+            g.value = t["chain_time"]
+        elif t["kind"] == "end_of_run":
+            g.value = t["end_of_run_time"]
+        elif t["kind"] == "start_of_run":
+            g.value = t.get("speed", 1.0)
+            g.ivalue = t.get("initial_direction_of_motion", 0)
+            g.ivalue2 = t.get("initial_active_identifier", 0)
     return c
+
+
+def oracle_config(meta, math_mode=0, scheduler=None):
+    return oracle_config_from_wiring(meta["dim"], meta["n"], meta["system_length"], meta["beta"],
+                                     meta["cells_per_side"], wiring_from_meta(meta), meta["seed"],
+                                     scheduler or meta["scheduler"], math_mode)
+
+
+def device_desc(meta, scheduler=None):
+    from jellyfysh_amd.chain import make_desc
+    return make_desc(meta["dim"], meta["n"], meta["system_length"], meta["beta"], meta["cells_per_side"],
+                     wiring_from_meta(meta), meta["seed"], scheduler or meta["scheduler"])
 
 
 def fcc_lattice(cells, a, jitter, rng):
@@ -93,3 +146,38 @@ def square_lattice(m, system_length, jitter_frac, rng):
     xs = (np.arange(m) + 0.5) / m * system_length
     pts = np.array([[x, y] for y in xs for x in xs])
     return np.mod(pts + rng.uniform(-1, 1, pts.shape) * jitter_frac * system_length / m, system_length)
+
+
+def hard_disk_system(n=256, phi=0.60, seed=7, chain_time=0.5, end_time=1.0e9):
+    """Configuration 1: N hard disks in the unit square at packing fraction phi, square
+    lattice with +-1% jitter, cells of side >= the diameter."""
+    m = int(round(n ** 0.5))
+    assert m * m == n
+    radius = (phi / (n * np.pi)) ** 0.5
+    cps = int(1.0 / (2 * radius))
+    rng = np.random.default_rng(seed)
+    pos = square_lattice(m, 1.0, 0.01, rng)
+    meta = dict(dim=2, n=n, system_length=1.0, beta=1.0, cells_per_side=cps, chain_time=chain_time,
+                end_time=end_time, scheduler="list", seed=seed, sampling_interval=0.0,
+                factors=[dict(tag="nearby_sphere", kind="hard_sphere", radius=radius)])
+    return meta, pos
+
+
+def lj_system(n_cells=4, rho=0.8, seed=11, cutoff=0.0, cells_per_side=None, chain_time=1.0, end_time=1.0e9):
+    """Configuration 2 shape: LJ 12-6 as two inverse-power factors on an FCC lattice with
+    0.01 sigma jitter; cutoff > 0 selects the truncated-shifted factors."""
+    n = 4 * n_cells ** 3
+    L = (n / rho) ** (1 / 3)
+    rng = np.random.default_rng(seed)
+    pos = fcc_lattice(n_cells, L / n_cells, 0.01, rng)
+    if cells_per_side is None:
+        cells_per_side = max(3, int(L / 2.5))
+    if cutoff > 0:
+        factors = [dict(tag="lj_twelve", kind="inverse_power_truncated", power=12.0, prefactor=4.0, cutoff=cutoff),
+                   dict(tag="lj_six", kind="inverse_power_truncated", power=6.0, prefactor=-4.0, cutoff=cutoff)]
+    else:
+        factors = [dict(tag="lj_twelve", kind="inverse_power", power=12.0, prefactor=4.0),
+                   dict(tag="lj_six", kind="inverse_power", power=6.0, prefactor=-4.0)]
+    meta = dict(dim=3, n=n, system_length=L, beta=1 / 1.5, cells_per_side=cells_per_side, chain_time=chain_time,
+                end_time=end_time, scheduler="heap", seed=seed, sampling_interval=0.0, factors=factors)
+    return meta, pos
