@@ -147,6 +147,56 @@ int ecmc_system_rng_words(const ecmc_system *sys, int64_t *words);
 /* Kernel time of the last ecmc_system_run in milliseconds (HIP events on its stream). */
 int ecmc_system_last_kernel_ms(const ecmc_system *sys, double *ms);
 
+/* ---- batched independent replicas (no reference counterpart as a whole) ------------------
+ * Each replica is a reference chain (straight chains of chain_length cycling +x,+y,+z,
+ * exact pair-factor inversion, cell-boundary events of CuboidPeriodicCells, a random new
+ * active unit at each chain end) of LJ-type truncated-and-shifted inverse-power factors
+ * (powers 6 and/or 12) in a cubic box, with Philox4x32-10 draws keyed by (replica, seed)
+ * and counter (leg, unit).  Candidate displacements are bounded in FP32 first; only the
+ * few whose bound can decide the leg are inverted exactly in FP64. */
+typedef struct {
+    int32_t n_replicas;
+    int32_t n_factors;
+    double beta, cutoff, chain_length;
+    double power[2], prefactor[2];
+    uint64_t seed;
+} ecmc_batch_desc;
+
+typedef struct {
+    int32_t kind;        /* 0 pair (lift), 1 cell boundary, 2 end of chain */
+    int32_t active, target, factor;
+    double displacement; /* event displacement along the chain */
+    double x;            /* committed coordinate of the moved unit along the direction */
+} ecmc_batch_event;
+
+typedef struct ecmc_batch ecmc_batch;
+
+/* positions: replicas concatenated, n_i x 3 row-major each, inside [0, L_i). */
+int ecmc_batch_create(const ecmc_batch_desc *desc, const int32_t *n_per_replica, const double *system_length,
+                      const double *positions, ecmc_batch **out);
+void ecmc_batch_destroy(ecmc_batch *batch);
+/* Size classes: replicas with n <= class_max_n[c] (first such c) run with class_wpr[c]
+ * wavefronts per replica; one persistent kernel per class, all launched together. */
+int ecmc_batch_set_classes(ecmc_batch *batch, int32_t n_classes, const int32_t *class_max_n, const int32_t *class_wpr);
+/* 0: FP32 filter (default); 1: every candidate inverted in FP64; 2: filter plus audit of
+ * every bound against the exact value (violations counted, see ecmc_batch_audit). */
+int ecmc_batch_set_mode(ecmc_batch *batch, int32_t mode);
+/* Record the first `legs_per_replica` legs of every replica (0 = off). */
+int ecmc_batch_set_log(ecmc_batch *batch, int32_t legs_per_replica);
+/* One step: every replica advances legs_per_replica legs (synchronous). */
+int ecmc_batch_run(ecmc_batch *batch, int32_t legs_per_replica);
+/* The step captured once into a hipGraph and replayed n_steps times (synchronous). */
+int ecmc_batch_run_graph(ecmc_batch *batch, int32_t legs_per_replica, int32_t n_steps);
+int ecmc_batch_last_ms(const ecmc_batch *batch, double *ms);
+/* Per replica: active unit, direction, chain displacement left, legs done, est[8] (lift
+ * separation sum, displacement sum, pair, boundary, lifts, chain ends) and stats[4]
+ * (candidates, candidates past the geometric test, contenders, exact inversions). */
+int ecmc_batch_states(const ecmc_batch *batch, int32_t *active, int32_t *direction, double *chain_left,
+                      uint64_t *legs, double *est8, int64_t *stats4);
+int ecmc_batch_positions(const ecmc_batch *batch, double *positions);
+int ecmc_batch_log(const ecmc_batch *batch, ecmc_batch_event *host_log);
+int ecmc_batch_audit(const ecmc_batch *batch, uint64_t *violations);
+
 /* ---- batched per-pair evaluations on device memory ---------------------------------------
  * Row i uses direction dir[i] (velocity speed[i] along +dir), separation sep[3*i..3*i+dim)
  * (target minus active, minimum image), charges c1[i], c2[i] (NULL = 1.0) and potential

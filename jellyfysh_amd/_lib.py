@@ -75,6 +75,18 @@ def _signatures():
         "ecmc_math_log": (i32, [i64, vp, vp, vp]),
         "ecmc_math_pow": (i32, [i64, vp, vp, vp, vp]),
         "ecmc_mt_words": (i32, [u64, i64, vp, vp]),
+        "ecmc_batch_create": (i32, [vp, vp, pd, pd, ctypes.POINTER(vp)]),
+        "ecmc_batch_destroy": (None, [vp]),
+        "ecmc_batch_set_classes": (i32, [vp, i32, vp, vp]),
+        "ecmc_batch_set_mode": (i32, [vp, i32]),
+        "ecmc_batch_set_log": (i32, [vp, i32]),
+        "ecmc_batch_run": (i32, [vp, i32]),
+        "ecmc_batch_run_graph": (i32, [vp, i32, i32]),
+        "ecmc_batch_last_ms": (i32, [vp, pd]),
+        "ecmc_batch_states": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "ecmc_batch_positions": (i32, [vp, pd]),
+        "ecmc_batch_log": (i32, [vp, vp]),
+        "ecmc_batch_audit": (i32, [vp, ctypes.POINTER(u64)]),
     }
 
 

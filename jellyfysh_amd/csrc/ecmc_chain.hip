@@ -504,7 +504,10 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
     int64_t k = 0;
     while (k < a.max_events) {
         if (st.done || st.err) break;
-        if (!st.started) {
+        // every wave takes its branch decision before thread 0 changes the flags inside it
+        const int started = st.started, pending = st.pending_creates;
+        __syncthreads();
+        if (!started) {
             if (tid == 0) {
                 int sor = -1;
                 for (int t = 0; t < a.n_taggers; t++)
@@ -516,7 +519,7 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
                 st.chash = 0;
             }
             __syncthreads();
-        } else if (st.pending_creates) {
+        } else if (pending) {
             if (tid == 0) {
                 occupancy_update<D>(a, st);
                 st.ncand = 0;

@@ -208,6 +208,23 @@ ECMC_FN double cr_log(double x) {
     return l.h;
 }
 
+// x ** n for an integer 3 <= n <= 32 by binary powering in double-double (each product
+// carries about 2^-104 relative error), rounded once.
+ECMC_FN double dd_pow_int(double x, int n) {
+    dd base = {x, 0.0};
+    dd acc = {1.0, 0.0};
+    bool first = true;
+    while (n > 0) {
+        if (n & 1) {
+            acc = first ? base : dd_mul(acc, base);
+            first = false;
+        }
+        n >>= 1;
+        if (n) base = dd_mul(base, base);
+    }
+    return acc.h;
+}
+
 // Correctly rounded x ** y for the arguments the potentials use (x > 0).
 ECMC_FN double cr_pow(double x, double y) {
     if (y == 2.0) return x * x;
@@ -215,6 +232,7 @@ ECMC_FN double cr_pow(double x, double y) {
     if (y == 1.0) return x;
     if (y == 0.0) return 1.0;
     if (!(x > 0.0) || isinf(x) || !isfinite(y) || x < 0x1p-1022 || x == 1.0) return pow(x, y);
+    if (y >= 3.0 && y <= 32.0 && y == rint(y) && x > 0x1p-30 && x < 0x1p30) return dd_pow_int(x, (int)y);
     dd l = dd_log(x);
     dd t = dd_mul_d(l, y);
     if (fabs(t.h) > 700.0) return pow(x, y);

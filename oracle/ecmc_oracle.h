@@ -190,6 +190,30 @@ int32_t ora_engine_last_candidates(const ora_engine *e, int32_t *out, int32_t ca
 /* Number of random words consumed so far. */
 int64_t ora_engine_rng_words(const ora_engine *e);
 
+/* ---- batched replicas (the product's batched form; see ecmc_oracle_batched.c) ---------- */
+void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+typedef struct {
+    int32_t n, n_factors;
+    double system_length, beta, cutoff, chain_length;
+    double power[2], prefactor[2];
+    uint64_t seed;
+    uint32_t replica;
+    int32_t math_mode;   /* ORA_MATH_CR: what the device reproduces; ORA_MATH_LIBM: faster */
+} ora_batched_config;
+typedef struct {
+    int32_t active, direction;
+    double chain_left;
+    uint64_t leg;
+    double est[8];   /* lift separation sum, displacement sum, pair, boundary, lifts, chain ends */
+} ora_batched_state;
+typedef struct {
+    int32_t kind;     /* 0 pair, 1 cell boundary, 2 end of chain */
+    int32_t active, target, factor;
+    double displacement, x;   /* event displacement, committed coordinate along the direction */
+} ora_batched_event;
+int64_t ora_batched_run(const ora_batched_config *cfg, double *positions, ora_batched_state *st, int64_t legs,
+                        ora_batched_event *log);
+
 /* Plain reference-wiring helpers for the benchmark configurations. */
 void ora_config_hard_disks(ora_config *c, int n, double system_length, double radius, int cells_per_side,
                            double chain_time, double end_time, uint64_t seed);

@@ -244,6 +244,65 @@ def run_threads(config, positions, n_threads, events_per_replica, charges=None):
     return total, secs.value
 
 
+class BatchedConfig(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("n_factors", ctypes.c_int32), ("system_length", ctypes.c_double),
+                ("beta", ctypes.c_double), ("cutoff", ctypes.c_double), ("chain_length", ctypes.c_double),
+                ("power", ctypes.c_double * 2), ("prefactor", ctypes.c_double * 2), ("seed", ctypes.c_uint64),
+                ("replica", ctypes.c_uint32), ("math_mode", ctypes.c_int32)]
+
+
+class BatchedState(ctypes.Structure):
+    _fields_ = [("active", ctypes.c_int32), ("direction", ctypes.c_int32), ("chain_left", ctypes.c_double),
+                ("leg", ctypes.c_uint64), ("est", ctypes.c_double * 8)]
+
+
+BATCHED_EVENT_DTYPE = np.dtype([("kind", np.int32), ("active", np.int32), ("target", np.int32),
+                                ("factor", np.int32), ("displacement", np.float64), ("x", np.float64)])
+
+
+class BatchedReplica:
+    """One replica of the batched form, restated on the CPU (every candidate in FP64)."""
+
+    def __init__(self, positions, system_length, replica, seed, beta=1 / 1.5, cutoff=2.5, chain_length=1.0,
+                 factors=((12.0, 4.0), (6.0, -4.0)), math_mode=MATH_CR):
+        c = BatchedConfig()
+        c.math_mode = math_mode
+        self.pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64).reshape(-1, 3)).copy()
+        c.n = self.pos.shape[0]
+        c.n_factors = len(factors)
+        c.system_length = system_length
+        c.beta = beta
+        c.cutoff = min(cutoff, system_length / 2.0)
+        c.chain_length = chain_length
+        for i, (p, k) in enumerate(factors):
+            c.power[i], c.prefactor[i] = p, k
+        c.seed = seed
+        c.replica = replica
+        self.cfg = c
+        self.state = BatchedState()
+        self.state.chain_left = chain_length
+        L = lib()
+        L.ora_batched_run.restype = ctypes.c_int64
+        L.ora_batched_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p,
+                                      ctypes.c_int64, ctypes.c_void_p]
+
+    def run(self, legs, log=True):
+        out = np.zeros(legs, dtype=BATCHED_EVENT_DTYPE) if log else None
+        k = lib().ora_batched_run(ctypes.byref(self.cfg), _dp(self.pos), ctypes.byref(self.state), legs,
+                                  out.ctypes.data if log else None)
+        return out[:k] if log else k
+
+
+def philox(ctr, key):
+    L = lib()
+    L.ora_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32)] * 3
+    C = (ctypes.c_uint32 * 4)(*ctr)
+    K = (ctypes.c_uint32 * 2)(*key)
+    o = (ctypes.c_uint32 * 4)()
+    L.ora_philox4x32_10(C, K, o)
+    return list(o)
+
+
 def ref_lib(name):
     """The reference's own C file compiled under _ref/ (heap, mic, ipcb), or None."""
     path = os.path.join(_REF_DIR, "libref_%s.so" % name)

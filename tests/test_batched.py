@@ -1,0 +1,124 @@
+"""The batched-replica form: the oracle's restatement on the CPU (Philox KATs, invariants,
+physics sanity) and the device kernel against it (bit-exact event logs with the FP32
+filter, the all-FP64 mode and the bound audit)."""
+import numpy as np
+import pytest
+
+# Random123 known-answer vectors of Philox4x32-10
+PHILOX_KAT = [
+    ([0, 0, 0, 0], [0, 0], [0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8]),
+    ([0xffffffff] * 4, [0xffffffff] * 2, [0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd]),
+    ([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344], [0xa4093822, 0x299f31d0],
+     [0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1]),
+]
+
+
+def test_philox_known_answers(oracle):
+    for ctr, key, want in PHILOX_KAT:
+        assert oracle.philox(ctr, key) == want
+
+
+def small_batch(seed=5):
+    from jellyfysh_amd.batch import lattice_positions
+    rng = np.random.default_rng(seed)
+    ns = [64, 110, 300, 700, 1500]
+    rhos = [0.9, 0.7, 0.8, 0.85, 0.75]
+    lengths = [(n / r) ** (1 / 3) for n, r in zip(ns, rhos)]
+    pos = [lattice_positions(n, l, 0.01, rng) for n, l in zip(ns, lengths)]
+    return ns, lengths, pos
+
+
+def test_batched_oracle_invariants(oracle):
+    ns, lengths, pos = small_batch()
+    for r in (0, 2):
+        rep = oracle.BatchedReplica(pos[r], lengths[r], replica=r, seed=9)
+        log = rep.run(4000)
+        assert np.all(log["displacement"] >= 0)
+        assert np.all((rep.pos >= 0) & (rep.pos < lengths[r]))
+        lifts = log["kind"] == 0
+        assert lifts.sum() > 100
+        # a lift hands the activity on to its target
+        assert np.array_equal(log["active"][1:][lifts[:-1]], log["target"][:-1][lifts[:-1]])
+        # chains: displacement between chain ends sums to the chain length
+        ends = np.flatnonzero(log["kind"] == 2)
+        for a, b in zip(ends[:-1], ends[1:]):
+            assert abs(log["displacement"][a + 1:b + 1].sum() - 1.0) < 1e-12
+
+
+@pytest.mark.slow
+def test_batched_oracle_pressure_is_physical(oracle):
+    # LJ (rc = 2.5, shifted) at rho = 0.8, T = 1.5 is a dense liquid with beta P / rho ~ 1.8-2.3
+    from jellyfysh_amd.batch import lattice_positions
+    rng = np.random.default_rng(1)
+    n = 256
+    L = (n / 0.8) ** (1 / 3)
+    rep = oracle.BatchedReplica(lattice_positions(n, L, 0.01, rng), L, replica=0, seed=3, math_mode=oracle.MATH_LIBM)
+    rep.run(30000, log=False)
+    est0 = np.array(rep.state.est)
+    rep.run(60000, log=False)
+    est = np.array(rep.state.est) - est0
+    p = 1.0 + est[0] / est[1]
+    assert 1.0 < p < 3.5, p
+
+
+def _device_batch(classes, mode, log_legs, seed=9):
+    from jellyfysh_amd.batch import DeviceBatch
+    ns, lengths, pos = small_batch()
+    b = DeviceBatch(ns, lengths, pos, seed=seed, classes=classes)
+    b.set_mode(mode)
+    b.set_log(log_legs)
+    return b, ns, lengths, pos
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1])
+def test_gpu_batched_matches_oracle(device, oracle, mode):
+    legs = 1500
+    classes = ([128, 512, 4096], [1, 2, 4])
+    print("batched mode", mode, "legs", legs, flush=True)
+    b, ns, lengths, pos = _device_batch(classes, mode, legs)
+    b.run(legs // 2)
+    b.run(legs - legs // 2)
+    got = b.log()
+    st = b.states()
+    for r in range(len(ns)):
+        rep = oracle.BatchedReplica(pos[r], lengths[r], replica=r, seed=9)
+        want = rep.run(legs)
+        for f in ("kind", "active", "target", "factor", "displacement", "x"):
+            bad = np.flatnonzero(got[r][f] != want[f])
+            assert bad.size == 0, (r, f, int(bad[0]), got[r][bad[0]], want[bad[0]])
+        assert np.array_equal(st["est"][r][:6], np.array(rep.state.est)[:6])
+        assert st["active"][r] == rep.state.active
+    allpos = b.positions()
+    for r in range(len(ns)):
+        rep = oracle.BatchedReplica(pos[r], lengths[r], replica=r, seed=9)
+        rep.run(legs, log=False)
+        assert np.array_equal(allpos[r], rep.pos)
+
+
+@pytest.mark.gpu
+def test_gpu_batched_bounds_never_exceed_exact(device):
+    b, ns, lengths, pos = _device_batch(([128, 512, 4096], [1, 2, 4]), 2, 0)
+    print("batched audit", flush=True)
+    b.run(800)
+    assert b.audit() == 0
+    st = b.states()["stats"]
+    # the filter leaves few exact inversions per leg
+    assert st[:, 3].sum() < 0.1 * st[:, 0].sum()
+
+
+@pytest.mark.gpu
+def test_gpu_batched_graph_replay_matches_direct_launches(device):
+    classes = ([128, 512, 4096], [1, 1, 2])
+    b1, *_ = _device_batch(classes, 0, 0, seed=21)
+    b2, *_ = _device_batch(classes, 0, 0, seed=21)
+    print("batched graph replay", flush=True)
+    for _ in range(3):
+        b1.run(200)
+    b2.run_graph(200, 3)
+    s1, s2 = b1.states(), b2.states()
+    for k in ("active", "direction", "legs"):
+        assert np.array_equal(s1[k], s2[k])
+    assert np.array_equal(s1["est"], s2["est"])
+    for p1, p2 in zip(b1.positions(), b2.positions()):
+        assert np.array_equal(p1, p2)
