@@ -45,20 +45,40 @@ def test_batched_oracle_invariants(oracle):
             assert abs(log["displacement"][a + 1:b + 1].sum() - 1.0) < 1e-12
 
 
+def virial_pressure(pos, L, beta, rc):
+    """beta P / rho = 1 + beta / (3N) sum_{i<j, r<rc} r F(r) for LJ 12-6 (eps = sigma = 1);
+    the shift of the potential does not change the forces."""
+    n = len(pos)
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.round(d / L)
+    r2 = (d ** 2).sum(-1)[np.triu_indices(n, 1)]
+    r2 = r2[r2 < rc * rc]
+    s6 = r2 ** -3
+    return 1.0 + beta / (3 * n) * np.sum(48.0 * (s6 * s6 - 0.5 * s6))
+
+
 @pytest.mark.slow
-def test_batched_oracle_pressure_is_physical(oracle):
-    # LJ (rc = 2.5, shifted) at rho = 0.8, T = 1.5 is a dense liquid with beta P / rho ~ 1.8-2.3
+def test_batched_oracle_pressure_matches_virial(oracle):
+    # The ECMC estimator (lifting separations over chain displacement) and the virial of
+    # the sampled configurations are two independent routes to the same pressure.
     from jellyfysh_amd.batch import lattice_positions
     rng = np.random.default_rng(1)
-    n = 256
+    n, beta, rc = 256, 1 / 1.5, 2.5
     L = (n / 0.8) ** (1 / 3)
-    rep = oracle.BatchedReplica(lattice_positions(n, L, 0.01, rng), L, replica=0, seed=3, math_mode=oracle.MATH_LIBM)
-    rep.run(30000, log=False)
+    rep = oracle.BatchedReplica(lattice_positions(n, L, 0.01, rng), L, replica=0, seed=3, beta=beta, cutoff=rc,
+                                math_mode=oracle.MATH_LIBM)
+    rep.run(150000, log=False)
     est0 = np.array(rep.state.est)
-    rep.run(60000, log=False)
+    vir = []
+    for _ in range(120):
+        rep.run(5000, log=False)
+        vir.append(virial_pressure(rep.pos, L, beta, rc))
     est = np.array(rep.state.est) - est0
-    p = 1.0 + est[0] / est[1]
-    assert 1.0 < p < 3.5, p
+    p_ecmc = 1.0 + est[0] / est[1]
+    blocks = np.array(vir).reshape(24, 5).mean(axis=1)
+    err = blocks.std(ddof=1) / np.sqrt(len(blocks))
+    assert 2.0 < p_ecmc < 4.5
+    assert abs(p_ecmc - np.mean(vir)) < 4 * err + 0.05, (p_ecmc, np.mean(vir), err)
 
 
 def _device_batch(classes, mode, log_legs, seed=9):
