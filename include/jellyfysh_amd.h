@@ -208,6 +208,10 @@ int ecmc_pair_derivative(int32_t potential, const double *par, int32_t dim, int6
                          const double *d_speed, const double *d_sep, const double *d_c1, const double *d_c2,
                          double system_length, double *d_out, void *stream);
 
+/* Plain device-to-device copy of `bytes` (a multiple of 16).  No reference counterpart: a
+ * kernel of known traffic that scales hardware byte counters in profiles. */
+int ecmc_copy_bytes(const void *d_src, void *d_dst, int64_t bytes, void *stream);
+
 /* ---- math and random stream (device) ----------------------------------------------------- */
 int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream);
 int ecmc_math_pow(int64_t n, const double *d_x, const double *d_y, double *d_out, void *stream);

@@ -75,6 +75,7 @@ def _signatures():
         "ecmc_math_log": (i32, [i64, vp, vp, vp]),
         "ecmc_math_pow": (i32, [i64, vp, vp, vp, vp]),
         "ecmc_mt_words": (i32, [u64, i64, vp, vp]),
+        "ecmc_copy_bytes": (i32, [vp, vp, i64, vp]),
         "ecmc_batch_create": (i32, [vp, vp, pd, pd, ctypes.POINTER(vp)]),
         "ecmc_batch_destroy": (None, [vp]),
         "ecmc_batch_set_classes": (i32, [vp, i32, vp, vp]),

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/jellyfysh_amd.h"
@@ -165,6 +166,20 @@ extern "C" int ecmc_pair_derivative(int32_t potential, const double *par, int32_
                                     void *stream) {
     return launch_pair(potential, par, dim, n, d_dir, d_speed, d_sep, d_c1, d_c2, nullptr, system_length, d_out,
                        stream, 1);
+}
+
+__global__ __launch_bounds__(PT) void copy_kernel(const uint4 *src, uint4 *dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x; i < n16; i += (int64_t)gridDim.x * PT) dst[i] = src[i];
+}
+
+extern "C" int ecmc_copy_bytes(const void *d_src, void *d_dst, int64_t bytes, void *stream) {
+    if (bytes < 0 || (bytes % 16) != 0 || (bytes && (!d_src || !d_dst))) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (bytes == 0) return ECMC_OK;
+    int64_t n16 = bytes / 16;
+    int64_t blocks = std::min<int64_t>((n16 + PT - 1) / PT, 4096);
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)blocks), dim3(PT), 0, (hipStream_t)stream, (const uint4 *)d_src,
+                       (uint4 *)d_dst, n16);
+    return hipGetLastError() == hipSuccess ? ECMC_OK : ecmc_fail(ECMC_ERR_HIP, "copy kernel launch failed");
 }
 
 extern "C" int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream) {

@@ -96,6 +96,10 @@ int64_t ora_batched_run(const ora_batched_config *cfg, double *pos, ora_batched_
     bgrid_init(&g, L, cfg->cutoff);
     const int ncell = g.cps ? g.cps * g.cps * g.cps : 0;
     int *cell_of = (int *)malloc(sizeof(int) * n);
+    /* cell of every unit; only the active unit moves, so one entry changes per leg */
+    if (g.cps)
+        for (int i = 0; i < n; i++)
+            cell_of[i] = bcell1(&g, pos[3 * i]) + g.cps * (bcell1(&g, pos[3 * i + 1]) + g.cps * bcell1(&g, pos[3 * i + 2]));
     int64_t k;
     for (k = 0; k < legs; k++) {
         const int a = st->active, d = st->direction;
@@ -105,11 +109,6 @@ int64_t ora_batched_run(const ora_batched_config *cfg, double *pos, ora_batched_
         int ac[3] = {0, 0, 0};
         if (g.cps)
             for (int q = 0; q < 3; q++) ac[q] = bcell1(&g, ap[q]);
-        if (g.cps)
-            for (int i = 0; i < n; i++) {
-                int c0 = bcell1(&g, pos[3 * i]), c1 = bcell1(&g, pos[3 * i + 1]), c2 = bcell1(&g, pos[3 * i + 2]);
-                cell_of[i] = c0 + g.cps * (c1 + g.cps * c2);
-            }
         for (int j = 0; j < n; j++) {
             if (j == a) continue;
             if (g.cps) {
@@ -163,6 +162,8 @@ int64_t ora_batched_run(const ora_batched_config *cfg, double *pos, ora_batched_
             if (x >= L) x -= L;
         }
         pos[3 * a + d] = x;
+        if (g.cps)
+            cell_of[a] = bcell1(&g, pos[3 * a]) + g.cps * (bcell1(&g, pos[3 * a + 1]) + g.cps * bcell1(&g, pos[3 * a + 2]));
         st->est[1] += best.d;
         if (best.rank == 0) {
             int j = best.j;
