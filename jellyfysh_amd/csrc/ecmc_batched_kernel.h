@@ -1,0 +1,537 @@
+// ecmc_batched_kernel.h -- device code of the batched replicas (see ecmc_batched.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/jellyfysh_amd.h"
+#include "ecmc_math.h"
+#include "ecmc_potentials.h"
+
+namespace ecmc {
+
+constexpr int B_MAX_WPR = 8;
+constexpr int B_CELL_CAP = 64;       // occupants per cell
+constexpr int B_CONTENDERS = 256;    // contender list capacity per leg
+constexpr int B_MAX_CPS = 8;         // cells per side handled in LDS (512 cells)
+
+struct ReplicaDev {
+    int32_t n, cps;
+    int64_t off;           // offset of the replica's SoA rows in pos64 (3 * n doubles)
+    double L, cutoff, side;
+    int32_t bound_off, pad;
+    // state
+    int32_t active, direction;
+    double chain_left;
+    uint64_t leg;
+    double est[8];
+    int32_t err, pad2;
+    int64_t stats[4];      // candidates, geometric survivors, contenders, exact evaluations
+};
+
+struct FactorDev {
+    int32_t n_factors, pad;
+    double power[2], prefactor[2];
+};
+
+struct BatchArgs {
+    ReplicaDev *rep;
+    double *pos64;
+    const double *bounds;        // cell lower bounds, per replica cps+1 entries
+    const int32_t *order;        // replica indices of this class, largest first
+    int32_t count;               // replicas in this class
+    int32_t nmax, cellmax;       // LDS sizing of this class
+    int32_t legs;
+    int32_t mode;                // 0 filtered, 1 all candidates exact, 2 filtered + audit
+    int32_t log_legs;            // legs logged per replica (0 = none)
+    uint32_t seed_lo, pad;
+    double beta, chain_length;
+    FactorDev fac;
+    int32_t *queue;              // work counter of this class
+    ecmc_batch_event *log;       // n_replicas x log_legs
+    unsigned long long *audit;   // bound violations
+};
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) --------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
+                                              uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ __forceinline__ double unit53(uint32_t a, uint32_t b) {
+    return ((a >> 5) * 67108864.0 + (b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ double bmin_image(double s, double L) {
+    if (s >= 0.5 * L) return s - L;
+    if (s < -0.5 * L) return s + L;
+    return s;
+}
+
+__device__ __forceinline__ int bcell1(double x, double side, int cps) {
+    int c = (int)(x / side);
+    return c < cps ? c : cps - 1;
+}
+
+// ---- FP32 lower bound of the truncated inverse-power event displacement ------------------
+// Every quantity is bracketed [lo, hi] and each FP32 operation is covered by a relative
+// margin (ETA for products/quotients/roots, ETA_POW for the exp2/log2 power), far above
+// FP32 rounding (2^-24 per operation) and the documented accuracy of log2f/exp2f.
+constexpr float ETA = 0x1p-17f, ETA_POW = 0x1p-14f;
+__device__ __forceinline__ float up(float x) { return x >= 0.f ? x * (1.f + ETA) : x * (1.f - ETA); }
+__device__ __forceinline__ float dn(float x) { return x >= 0.f ? x * (1.f - ETA) : x * (1.f + ETA); }
+__device__ __forceinline__ float sq_lo(float a_abs_lo) { return dn(a_abs_lo * a_abs_lo); }
+
+// V(r2) = K / r2^(p/2) with p in {6, 12} (integer powers by multiplication), bounded up
+// (hi) or down for an r2 that is itself a lower/upper bound.
+__device__ __forceinline__ float vpow_bound(float K, float r2, int half_p, bool hi) {
+    float r3 = r2 * r2 * r2;
+    float rp = half_p == 6 ? r3 * r3 : r3;
+    float v = K / rp;
+    return hi ? v * (1.f + 8.f * 0x1p-24f) * (1.f + ETA) : v * (1.f - 8.f * 0x1p-24f) * (1.f - ETA);
+}
+
+// (K / V)^(2/p) bounded up (hi) or down.
+__device__ __forceinline__ float rstar2_bound(float K, float V, float two_over_p, bool hi) {
+    float q = K / V;
+    float r = exp2f(two_over_p * log2f(q));
+    return hi ? r * (1.f + ETA_POW) : r * (1.f - ETA_POW);
+}
+
+struct Geo {
+    float sd_lo, sd_hi, perp2_lo, perp2_hi, r2_lo, r2_hi;
+};
+
+// Lower bound of the exact displacement of one factor, or +INF when the factor surely has
+// no event, given the exponential draw's lower bound dU_lo.
+__device__ float factor_lower_bound(const Geo &g, float k, int p, float rc2_hi, float vrc_lo, float dU_lo) {
+    const float INF = __int_as_float(0x7f800000);
+    const int half_p = p / 2;
+    const float two_over_p = 2.f / (float)p;
+    if (g.perp2_lo >= rc2_hi) return INF;
+    if (k > 0.f) {
+        if (g.sd_hi <= 0.f) return INF;
+        float ueff_lo = fmaxf(vpow_bound(k, g.r2_hi, half_p, false), vrc_lo);
+        float umax_hi = vpow_bound(k, fmaxf(g.perp2_lo, 1e-30f), half_p, true);
+        float room_hi = up(umax_hi - ueff_lo);
+        if (dU_lo >= room_hi) return INF;
+        float ut_lo = dn(ueff_lo + dU_lo);
+        float rs2_hi = rstar2_bound(k, ut_lo, two_over_p, true);
+        float x_hi = up(sqrtf(fmaxf(up(rs2_hi - g.perp2_lo), 0.f)));
+        float d = dn(g.sd_lo) - x_hi;
+        d = d - fabsf(d) * 0x1p-22f;
+        return fmaxf(d, 0.f);
+    }
+    const float K = -k;
+    float x0sq_lo = g.sd_hi < 0.f ? sq_lo(-g.sd_hi) : 0.f;
+    float r02_lo = dn(g.perp2_lo + x0sq_lo);
+    if (r02_lo >= rc2_hi) return INF;
+    float vcur_hi = vpow_bound(K, fmaxf(r02_lo, 1e-30f), half_p, true);
+    float room_hi = up(vcur_hi - vrc_lo);
+    if (dU_lo >= room_hi) return INF;
+    float vt_hi = up(vcur_hi - dU_lo);
+    if (!(vt_hi > 0.f)) return 0.f;
+    float rs2_lo = rstar2_bound(K, vt_hi, two_over_p, false);
+    float y_lo = dn(sqrtf(fmaxf(dn(rs2_lo - g.perp2_hi), 0.f)));
+    float d = g.sd_lo + y_lo;
+    d = d - fabsf(d) * 0x1p-22f - 0x1p-40f;
+    return fmaxf(d, 0.f);
+}
+
+// Smallest displacement any event of the factor can have, from geometry alone: a repulsive
+// factor first has to reach the cutoff sphere, an attractive one first passes the closest
+// approach.
+__device__ __forceinline__ float factor_geometric_floor(const Geo &g, float k, float rc2_hi) {
+    if (k > 0.f) {
+        if (g.r2_lo < rc2_hi) return 0.f;
+        float entry_hi = up(sqrtf(fmaxf(up(rc2_hi - g.perp2_lo), 0.f)));
+        float d = dn(g.sd_lo) - entry_hi;
+        d = d - fabsf(d) * 0x1p-22f;
+        return fmaxf(d, 0.f);
+    }
+    return fmaxf(dn(g.sd_lo), 0.f);
+}
+
+struct BKey {
+    double d;
+    int32_t rank, j, f;
+};
+__device__ __forceinline__ bool bkey_lt(const BKey &a, const BKey &b) {
+    if (a.d != b.d) return a.d < b.d;
+    if (a.rank != b.rank) return a.rank < b.rank;
+    if (a.j != b.j) return a.j < b.j;
+    return a.f < b.f;
+}
+__device__ __forceinline__ bool bkey_eq(const BKey &a, const BKey &b) {
+    return a.d == b.d && a.rank == b.rank && a.j == b.j && a.f == b.f;
+}
+
+template <int WPR>
+__device__ void block_min(BKey &b, BKey *sk) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        BKey o;
+        o.d = __shfl_xor(b.d, off, 64);
+        o.rank = __shfl_xor(b.rank, off, 64);
+        o.j = __shfl_xor(b.j, off, 64);
+        o.f = __shfl_xor(b.f, off, 64);
+        if (bkey_lt(o, b)) b = o;
+    }
+    if (WPR > 1) {
+        if (lane == 0) sk[wave] = b;
+        __syncthreads();
+        b = sk[0];
+#pragma unroll
+        for (int w = 1; w < WPR; w++)
+            if (bkey_lt(sk[w], b)) b = sk[w];
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void bsync() { __syncthreads(); }
+
+// exact FP64 displacement of factor f of target j; the target's position is returned too
+__device__ __forceinline__ double exact_displacement(const BatchArgs &A, const ReplicaDev &R, const double *ap, int d,
+                                                     int j, int f, uint64_t leg, uint32_t replica, double *pj) {
+    double s[3];
+    const double *P = A.pos64 + R.off;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        pj[q] = P[(size_t)q * R.n + j];
+        s[q] = bmin_image(pj[q] - ap[q], R.L);
+    }
+    uint32_t w[4];
+    philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, replica, A.seed_lo, w);
+    double u = unit53(w[2 * f], w[2 * f + 1]);
+    double dU = -cr_log(1.0 - u) / A.beta;
+    return ipt_displacement<3>(d, 1.0, s, 1.0, 1.0, A.fac.power[f], A.fac.prefactor[f], R.cutoff, dU);
+}
+
+// candidate idx -> unit (binary search over the 27 prefix sums, or all units but `a`)
+__device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int *pref, const int *nearby,
+                                              const int16_t *cell_l) {
+    if (!cps) return idx < a ? idx : idx + 1;
+    int lo = 0, hi = 27;  // pref[lo] <= idx < pref[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (pref[mid] <= idx) lo = mid;
+        else hi = mid;
+    }
+    return cell_l[(size_t)nearby[lo] * B_CELL_CAP + (idx - pref[lo])];
+}
+
+template <int WPR>
+__global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NTH = 64 * WPR;
+    const int tid = threadIdx.x;
+    float *px = (float *)smem;
+    float *py = px + A.nmax;
+    float *pz = py + A.nmax;
+    int32_t *cell_n = (int32_t *)(pz + A.nmax);
+    int16_t *cell_l = (int16_t *)(cell_n + A.cellmax);
+    int32_t *cont_id = (int32_t *)(cell_l + (size_t)A.cellmax * B_CELL_CAP + (((size_t)A.cellmax * B_CELL_CAP) & 1));
+    __shared__ BKey sk[B_MAX_WPR];
+    __shared__ int s_item, s_nc, s_overflow, s_nearby[27], s_cnt[27], s_pref[28], s_M;
+    __shared__ double s_ap[3], s_newpos[3], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
+    __shared__ ReplicaDev R;
+    __shared__ unsigned long long s_stats[4];
+
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(A.queue, 1);
+        bsync();
+        const int item = s_item;
+        if (item >= A.count) break;
+        const int r = A.order[item];
+        if (tid == 0) {
+            R = A.rep[r];
+            for (int q = 0; q < 4; q++) s_stats[q] = 0;
+        }
+        bsync();
+        const int n = R.n, cps = R.cps;
+        const double L = R.L;
+        const double *P = A.pos64 + R.off;
+        for (int i = tid; i < n; i += NTH) {
+            px[i] = (float)P[i];
+            py[i] = (float)P[(size_t)n + i];
+            pz[i] = (float)P[2 * (size_t)n + i];
+        }
+        const int ncell = cps ? cps * cps * cps : 0;
+        for (int c = tid; c < ncell; c += NTH) cell_n[c] = 0;
+        if (tid < 3) s_ap[tid] = P[(size_t)tid * n + R.active];
+        if (cps && tid <= cps) s_bounds[tid] = A.bounds[R.bound_off + tid];
+        bsync();
+        if (cps) {
+            for (int i = tid; i < n; i += NTH) {
+                if (i == R.active) continue;
+                int c = bcell1(P[i], R.side, cps) + cps * (bcell1(P[(size_t)n + i], R.side, cps) +
+                                                           cps * bcell1(P[2 * (size_t)n + i], R.side, cps));
+                int slot = atomicAdd(&cell_n[c], 1);
+                if (slot < B_CELL_CAP) cell_l[(size_t)c * B_CELL_CAP + slot] = (int16_t)i;
+            }
+            bsync();
+            if (tid == 0)
+                for (int c = 0; c < ncell; c++)
+                    if (cell_n[c] > B_CELL_CAP) R.err = 1;
+        }
+        const float Lf = (float)L;
+        const float E = Lf * 0x1p-21f;
+        const double rc2 = R.cutoff * R.cutoff;
+        const float rc2_hi = (float)rc2 * (1.f + 0x1p-22f);
+        float vrc_lo[2] = {0.f, 0.f};
+        for (int f = 0; f < A.fac.n_factors; f++) {
+            double v = fabs(A.fac.prefactor[f]) / cr_pow(rc2, A.fac.power[f] / 2.0);
+            vrc_lo[f] = (float)v * (1.f - 0x1p-20f);
+        }
+        bsync();
+        for (int leg_i = 0; leg_i < A.legs && !R.err; leg_i++) {
+            const int a = R.active, d = R.direction;
+            const uint64_t leg = R.leg;
+            const double ap[3] = {s_ap[0], s_ap[1], s_ap[2]};
+            int ac[3] = {0, 0, 0};
+            if (cps) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) ac[q] = bcell1(ap[q], R.side, cps);
+                if (tid < 27) {
+                    int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
+                    int cx = (ac[0] + dx + cps) % cps, cy = (ac[1] + dy + cps) % cps, cz = (ac[2] + dz + cps) % cps;
+                    int c = cx + cps * (cy + cps * cz);
+                    s_nearby[tid] = c;
+                    s_cnt[tid] = min(cell_n[c], B_CELL_CAP);
+                }
+                bsync();
+            }
+            if (tid == 0) {
+                double h = R.chain_left, cb = INFINITY;
+                if (cps) {
+                    cb = s_bounds[ac[d] + 1] - ap[d];
+                    if (cb < h) h = cb;
+                    int acc = 0;
+                    for (int k = 0; k < 27; k++) {
+                        s_pref[k] = acc;
+                        acc += s_cnt[k];
+                    }
+                    s_pref[27] = acc;
+                    s_M = acc;
+                } else {
+                    s_M = n - 1;
+                }
+                s_h = h;
+                s_cb = cb;
+                s_nc = 0;
+                s_overflow = 0;
+            }
+            bsync();
+            const int M = s_M;
+            const double h = s_h;
+            const float apf[3] = {(float)ap[0], (float)ap[1], (float)ap[2]};
+            const float hf = (float)h * (1.f + 0x1p-22f) + 0x1p-60f;
+            BKey best = {INFINITY, 3, 0x7fffffff, 0};
+            double bestpos[3] = {0.0, 0.0, 0.0};
+            unsigned long long geo_pass = 0;
+            if (A.mode != 1) {
+                // ---- pass 1: FP32 lower bounds
+                for (int idx = tid; idx < M; idx += NTH) {
+                    const int j = candidate_unit(idx, cps, a, s_pref, s_nearby, cell_l);
+                    float sv[3] = {px[j] - apf[0], py[j] - apf[1], pz[j] - apf[2]};
+                    bool ambiguous = false;
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        if (sv[q] >= 0.5f * Lf) sv[q] -= Lf;
+                        else if (sv[q] < -0.5f * Lf) sv[q] += Lf;
+                        if (fabsf(fabsf(sv[q]) - 0.5f * Lf) <= 2.f * E) ambiguous = true;
+                    }
+                    Geo g;
+                    g.sd_lo = sv[d] - E;
+                    g.sd_hi = sv[d] + E;
+                    float p2lo = 0.f, p2hi = 0.f;
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        if (q == d) continue;
+                        float m = fabsf(sv[q]);
+                        float lo = fmaxf(m - E, 0.f), hi = m + E;
+                        p2lo += lo * lo;
+                        p2hi += hi * hi;
+                    }
+                    g.perp2_lo = dn(p2lo);
+                    g.perp2_hi = up(p2hi);
+                    float mdl = fmaxf(fabsf(sv[d]) - E, 0.f), mdh = fabsf(sv[d]) + E;
+                    g.r2_lo = dn(p2lo + mdl * mdl);
+                    g.r2_hi = up(p2hi + mdh * mdh);
+                    float tl[2] = {0.f, 0.f};
+                    bool any = false;
+                    for (int f = 0; f < A.fac.n_factors; f++) {
+                        float k = (float)A.fac.prefactor[f];
+                        float fl = ambiguous ? 0.f
+                                             : (g.perp2_lo >= rc2_hi ? __int_as_float(0x7f800000)
+                                                                      : factor_geometric_floor(g, k, rc2_hi));
+                        tl[f] = fl;
+                        if (fl <= hf) any = true;
+                    }
+                    if (!any) continue;
+                    geo_pass++;
+                    uint32_t w[4];
+                    philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, w);
+                    for (int f = 0; f < A.fac.n_factors; f++) {
+                        if (tl[f] > hf) continue;
+                        float t = 0.f;
+                        if (!ambiguous) {
+                            double u = unit53(w[2 * f], w[2 * f + 1]);
+                            // -log(1-u)/beta with a 2^-40 downward margin over the FP64 log
+                            double dU = -log(1.0 - u) / A.beta;
+                            float dU_lo = (float)(dU * (1.0 - 0x1p-40)) * (1.f - 0x1p-22f);
+                            t = factor_lower_bound(g, (float)A.fac.prefactor[f], (int)A.fac.power[f], rc2_hi, vrc_lo[f],
+                                                   dU_lo);
+                        }
+                        if (t <= hf) {
+                            int slot = atomicAdd(&s_nc, 1);
+                            if (slot < B_CONTENDERS) cont_id[slot] = (j << 1) | f;
+                            else s_overflow = 1;
+                        }
+                    }
+                }
+                bsync();
+            }
+            const bool all_exact = (A.mode == 1) || s_overflow;
+            // ---- pass 2: exact FP64 inversion of the contenders (or of every candidate)
+            const int nwork = all_exact ? M * A.fac.n_factors : min(s_nc, B_CONTENDERS);
+            for (int w = tid; w < nwork; w += NTH) {
+                int j, f;
+                if (all_exact) {
+                    f = w % A.fac.n_factors;
+                    j = candidate_unit(w / A.fac.n_factors, cps, a, s_pref, s_nearby, cell_l);
+                } else {
+                    j = cont_id[w] >> 1;
+                    f = cont_id[w] & 1;
+                }
+                double pj[3];
+                double dx = exact_displacement(A, R, ap, d, j, f, leg, (uint32_t)r, pj);
+                BKey c = {dx, 0, j, f};
+                if (bkey_lt(c, best)) {
+                    best = c;
+                    bestpos[0] = pj[0]; bestpos[1] = pj[1]; bestpos[2] = pj[2];
+                }
+            }
+            BKey mine = best;
+            if (tid == 0) {
+                BKey c1 = {s_cb, 1, 0, 0};
+                if (bkey_lt(c1, best)) best = c1;
+                BKey c2 = {R.chain_left, 2, 0, 0};
+                if (bkey_lt(c2, best)) best = c2;
+            }
+            block_min<WPR>(best, sk);
+            if (best.rank == 0 && bkey_eq(mine, best)) {
+                s_newpos[0] = bestpos[0]; s_newpos[1] = bestpos[1]; s_newpos[2] = bestpos[2];
+            }
+            if (A.mode == 2) {
+                // audit: a candidate whose exact displacement is <= h must be a contender
+                for (int idx = tid; idx < M * A.fac.n_factors; idx += NTH) {
+                    int f = idx % A.fac.n_factors;
+                    int j = candidate_unit(idx / A.fac.n_factors, cps, a, s_pref, s_nearby, cell_l);
+                    double pj[3];
+                    double dx = exact_displacement(A, R, ap, d, j, f, leg, (uint32_t)r, pj);
+                    if (dx <= h) {
+                        bool found = false;
+                        for (int q = 0; q < min(s_nc, B_CONTENDERS); q++)
+                            if (cont_id[q] == ((j << 1) | f)) found = true;
+                        if (!found && !s_overflow) atomicAdd(A.audit, 1ull);
+                    }
+                }
+            }
+            if (tid == 0) {
+                s_stats[0] += (unsigned long long)M;
+                s_stats[2] += (unsigned long long)min(s_nc, B_CONTENDERS);
+                s_stats[3] += (unsigned long long)nwork;
+            }
+            if (geo_pass) atomicAdd(&s_stats[1], geo_pass);
+            bsync();
+            // ---- commit (thread 0; the winner's position is in s_newpos)
+            if (tid == 0) {
+                double *Pw = A.pos64 + R.off;
+                double x;
+                ecmc_batch_event ev;
+                ev.kind = best.rank;
+                ev.active = a;
+                ev.target = -1;
+                ev.factor = 0;
+                ev.displacement = best.d;
+                if (best.rank == 1) {
+                    int c = ac[d];
+                    x = c + 1 < cps ? s_bounds[c + 1] : 0.0;
+                } else {
+                    x = ap[d] + best.d;
+                    if (x >= L) x -= L;
+                }
+                Pw[(size_t)d * n + a] = x;
+                float xf = (float)x;
+                if (d == 0) px[a] = xf;
+                else if (d == 1) py[a] = xf;
+                else pz[a] = xf;
+                R.est[1] += best.d;
+                int newa = a;
+                double np[3] = {ap[0], ap[1], ap[2]};
+                np[d] = x;
+                double pold[3] = {np[0], np[1], np[2]};
+                if (best.rank == 0) {
+                    const int j = best.j;
+                    double sdj = bmin_image(s_newpos[d] - ap[d], L) - best.d;
+                    R.est[0] += sdj;
+                    R.est[2] += 1.0;
+                    R.est[4] += 1.0;
+                    R.chain_left -= best.d;
+                    newa = j;
+                    np[0] = s_newpos[0]; np[1] = s_newpos[1]; np[2] = s_newpos[2];
+                    ev.target = j;
+                    ev.factor = best.f;
+                } else if (best.rank == 1) {
+                    R.est[3] += 1.0;
+                    R.chain_left -= best.d;
+                } else {
+                    uint32_t w[4];
+                    philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), 0xFFFFFFFFu, 1u, (uint32_t)r, A.seed_lo, w);
+                    newa = (int)(((uint64_t)w[0] * (uint64_t)n) >> 32);
+                    R.est[5] += 1.0;
+                    R.chain_left = A.chain_length;
+                    R.direction = (d + 1) % 3;
+                    ev.target = newa;
+                    if (newa != a) {
+                        np[0] = Pw[newa]; np[1] = Pw[(size_t)n + newa]; np[2] = Pw[2 * (size_t)n + newa];
+                    }
+                }
+                ev.x = x;
+                if (newa != a && cps) {
+                    // the old active unit joins the cell it ended in, the new one leaves its cell
+                    int ca = bcell1(pold[0], R.side, cps) + cps * (bcell1(pold[1], R.side, cps) + cps * bcell1(pold[2], R.side, cps));
+                    int na = cell_n[ca];
+                    if (na >= B_CELL_CAP) R.err = 1;
+                    else { cell_l[(size_t)ca * B_CELL_CAP + na] = (int16_t)a; cell_n[ca] = na + 1; }
+                    int cn = bcell1(np[0], R.side, cps) + cps * (bcell1(np[1], R.side, cps) + cps * bcell1(np[2], R.side, cps));
+                    int m = cell_n[cn], q = 0;
+                    while (q < m && cell_l[(size_t)cn * B_CELL_CAP + q] != newa) q++;
+                    if (q == m) R.err = 2;
+                    else { cell_l[(size_t)cn * B_CELL_CAP + q] = cell_l[(size_t)cn * B_CELL_CAP + m - 1]; cell_n[cn] = m - 1; }
+                }
+                s_ap[0] = np[0]; s_ap[1] = np[1]; s_ap[2] = np[2];
+                R.active = newa;
+                if (A.log && leg < (uint64_t)A.log_legs) A.log[(size_t)r * A.log_legs + leg] = ev;
+                R.leg = leg + 1;
+            }
+            bsync();
+        }
+        if (tid == 0) {
+            for (int q = 0; q < 4; q++) R.stats[q] += (int64_t)s_stats[q];
+            A.rep[r] = R;
+        }
+        bsync();
+    }
+}
+
+}  // namespace ecmc
