@@ -6,7 +6,7 @@ Workload of the headline number (one "step" = every replica advances --legs legs
   rc = 2.5 sigma, eps = sigma = 1, beta = 1/1.5), N drawn log-uniformly in [64, 2048],
   rho uniformly in [0.7, 0.9], chains of 1.0 sigma cycling +x,+y,+z; synthetic lattice
   starts.  Inputs are resident on the GPU before the timed region.
-Beside it, the plain form of the same potential at N = 4096, rho = 0.8 (FCC start) runs
+Beside it, the plain form of the same potential at N = 4096, rho = 0.8 (16^3 lattice start) runs
 the reference's leg loop on one workgroup, and the oracle runs a bounded sample of the
 batched workload on the host cores (the CPU baseline).
 
@@ -34,9 +34,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--legs", type=int, default=2000, help="legs per replica per step")
+    ap.add_argument("--legs", type=int, default=10000, help="legs per replica per step")
     ap.add_argument("--replicas", type=int, default=4096)
-    ap.add_argument("--plain-legs", type=int, default=20000)
+    ap.add_argument("--plain-legs", type=int, default=20000, help="legs per timed chunk of the plain chain")
+    ap.add_argument("--plain-seconds", type=float, default=30.0, help="plain-chain time for its pressure")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -48,18 +49,15 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def plain_chain_rate(legs):
+def plain_chain_rate(legs, pressure_seconds):
     """Plain form: one chain of N = 4096 LJ at rho = 0.8 with 6^3 cells, on one workgroup."""
+    from jellyfysh_amd.batch import lattice_positions
     from jellyfysh_amd.chain import DeviceChain, make_desc
-    n_cells = 16
-    n = 4 * n_cells ** 3
+    n = 4096
     L = (n / 0.8) ** (1 / 3)
-    rng = np.random.default_rng(7)
-    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
-    a = L / n_cells
-    pts = np.array([(np.array([i, j, k]) + b) * a for i in range(n_cells) for j in range(n_cells)
-                    for k in range(n_cells) for b in base])
-    pos = np.mod(pts + rng.uniform(-0.01, 0.01, pts.shape), L)
+    # N = 4096 is not 4 m^3, so the lattice start is the full 16^3 grid of spacing
+    # L / 16 = 1.07722 sigma, jittered by +-0.01 sigma
+    pos = lattice_positions(n, L, 0.01, np.random.default_rng(7))
     pc = ["lj_twelve", "lj_six", "cell_boundary"]
     taggers = [dict(tag="lj_twelve", kind="pair_cells", potential="inverse_power_truncated", params=[12.0, 4.0, 2.5],
                     create=pc, trash=pc),
@@ -75,44 +73,88 @@ def plain_chain_rate(legs):
     desc = make_desc(3, n, L, 1 / 1.5, int(L / 2.5), taggers, seed=5, scheduler="heap")
     chain = DeviceChain(desc, pos)
     chain.run(2000, log=False)                      # warm-up
+    # the timed rate: one chunk of `legs` legs
     e0 = chain.estimators()
     t0 = time.perf_counter()
     done = chain.run(legs, log=False)
     wall = time.perf_counter() - t0
     ms = chain.last_kernel_ms()
-    est = chain.estimators() - e0
-    return dict(events=int(done), kernel_ms=ms, wall_s=wall, events_per_sec=done / (ms * 1e-3),
-                pressure=1.0 + est[0] / est[1] if est[1] > 0 else None, n=n)
+    out = dict(n=n, rho=0.8, start="16^3 grid, spacing 1.07722, jitter 0.01", events=int(done), kernel_ms=ms,
+               wall_s=wall, events_per_sec=done / (ms * 1e-3))
+    # pressure: further chunks for `pressure_seconds`, batch means over the chunks after
+    # the first third (equilibration)
+    chunk_z, t_end = [], time.perf_counter() + pressure_seconds
+    while time.perf_counter() < t_end:
+        e1 = chain.estimators()
+        chain.run(legs, log=False)
+        d = chain.estimators() - e1
+        if d[1] > 0:
+            chunk_z.append(1.0 + d[0] / d[1])
+        log("plain chunk %d: Z = %.4f" % (len(chunk_z), chunk_z[-1] if chunk_z else float("nan")))
+    z = np.asarray(chunk_z[len(chunk_z) // 3:])
+    out["pressure"] = dict(mean=float(z.mean()) if len(z) else None,
+                           err=float(z.std(ddof=1) / np.sqrt(len(z))) if len(z) > 1 else None,
+                           chunks=int(len(z)), legs_per_chunk=int(legs))
+    return out
+
+
+def batched_pressure_at(rho, z, rho0=0.8, width=0.02, n=None, n_min=256):
+    """beta P / rho at rho0 from the replicas with |rho - rho0| < width (and N >= n_min):
+    least-squares line through (rho, Z), its value at rho0 and that value's standard error."""
+    sel = np.abs(rho - rho0) < width
+    if n is not None:
+        sel &= n >= n_min
+    x, y = rho[sel] - rho0, z[sel]
+    if len(x) < 4:
+        return dict(mean=None, err=None, replicas=int(len(x)))
+    A = np.stack([np.ones_like(x), x], 1)
+    coef, res, _, _ = np.linalg.lstsq(A, y, rcond=None)
+    dof = len(x) - 2
+    s2 = float(np.sum((y - A @ coef) ** 2)) / dof
+    cov = s2 * np.linalg.inv(A.T @ A)
+    return dict(mean=float(coef[0]), err=float(np.sqrt(cov[0, 0])), slope=float(coef[1]), replicas=int(len(x)),
+                window="|rho - %.2f| < %.2f, N >= %d" % (rho0, width, n_min))
 
 
 def cpu_baseline(n, lengths, pos, seconds, threads):
     """The oracle's batched restatement (FP64, glibc math, every candidate inverted) on host
-    threads, one replica per thread at a time, for about `seconds` of wall time."""
+    threads, one replica per thread at a time, for about `seconds` of wall time.  The sample
+    is the 4 x threads replicas of the workload whose density is closest to 0.8, so that
+    its pressure estimates also give the host's value at rho = 0.8."""
     from oracle import oracle as O
     O.lib()
     threads = threads or min(16, os.cpu_count() or 1)
-    order = np.argsort(-np.asarray(n))[:: max(1, len(n) // (4 * threads))][: 4 * threads]
+    n = np.asarray(n)
+    rho = n / np.asarray(lengths) ** 3
+    order = np.argsort(np.abs(rho - 0.8), kind="stable")[: 4 * threads]
+    order = order[np.argsort(-n[order], kind="stable")]          # largest first
     # calibrate legs so that the sample takes about `seconds`
     probe = O.BatchedReplica(pos[order[0]], float(lengths[order[0]]), replica=int(order[0]), seed=1,
                              math_mode=O.MATH_LIBM)
     t = time.perf_counter()
     probe.run(200, log=False)
     per_leg = (time.perf_counter() - t) / 200
-    legs = max(200, int(seconds * threads / (len(order) * per_leg * 0.7)))
+    mean_n = float(np.mean(n[order]))
+    legs = max(200, int(seconds * threads / (len(order) * per_leg * (mean_n / n[order[0]]) * 1.3)))
 
     def work(r):
         rep = O.BatchedReplica(pos[r], float(lengths[r]), replica=int(r), seed=1, math_mode=O.MATH_LIBM)
-        return rep.run(legs, log=False)
+        k = rep.run(legs, log=False)
+        return k, np.array(rep.state.est[:])
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
-        done = sum(ex.map(work, [int(r) for r in order]))
+        res = list(ex.map(work, [int(r) for r in order]))
     wall = time.perf_counter() - t0
+    done = sum(k for k, _ in res)
+    est = np.stack([e for _, e in res])
+    z = 1.0 + est[:, 0] / est[:, 1]
+    pz = batched_pressure_at(rho[order], z, width=1.0, n=n[order], n_min=0)
     return dict(value=done / wall, unit="events/s", cores=threads, kind="port",
-                sample="%d replicas of the batched workload (largest-first stride sample, N %d-%d) x %d legs, "
+                sample="%d replicas of the batched workload (density closest to 0.8, N %d-%d) x %d legs each, "
                        "oracle restatement in FP64 with glibc math, %d host threads, %.1f s"
-                       % (len(order), int(np.min(np.asarray(n)[order])), int(np.max(np.asarray(n)[order])), legs,
-                          threads, wall))
+                       % (len(order), int(n[order].min()), int(n[order].max()), legs, threads, wall),
+                pressure_rho_0_80=pz)
 
 
 def main():
@@ -129,8 +171,9 @@ def main():
     n, lengths, pos = batched_lj_workload(args.replicas, seed=2024)
     log("workload: %d replicas, %d units" % (len(n), int(np.sum(n))))
     batch = DeviceBatch(n, lengths, pos, seed=17, classes=default_classes())
-    for _ in range(args.warmup):
+    for i in range(args.warmup):
         batch.run_graph(args.legs, 1)
+        log("warm-up step %d done" % (i + 1))
     s0 = batch.states()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -147,10 +190,9 @@ def main():
     flops = cands * n_factors * FLOPS_PER_INVERSION
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     exact = float((s1["stats"][:, 3] - s0["stats"][:, 3]).sum())
-    est = s1["est"] - s0["est"]
-    p = pressure(est)
+    z = pressure(s1["est"] - s0["est"])
     rho = n / lengths ** 3
-    sel = np.abs(rho - 0.8) < 0.01
+    pz = batched_pressure_at(rho, z, n=n)
     out = {
         "metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -166,13 +208,11 @@ def main():
                              "FLOP per factor-candidate inversion (%d factors), exact FP64 inversions per "
                              "candidate = %.4f" % (FLOPS_PER_INVERSION, n_factors, exact / max(cands, 1))},
         "batched": {"kernel_ms": kernel_ms, "events": events, "candidates_per_leg": cands / max(events, 1),
-                    "pressure_rho_0.80": {"mean": float(np.mean(p[sel])) if sel.any() else None,
-                                          "err": float(np.std(p[sel], ddof=1) / np.sqrt(sel.sum()))
-                                          if sel.sum() > 1 else None, "replicas": int(sel.sum())}},
+                    "pressure_rho_0_80": pz},
     }
     if not args.no_plain:
         log("plain chain ...")
-        pl = plain_chain_rate(args.plain_legs)
+        pl = plain_chain_rate(args.plain_legs, args.plain_seconds)
         out["plain"] = pl
         out["batched_over_plain"] = value / pl["events_per_sec"]
     if not args.no_cpu:
@@ -180,6 +220,22 @@ def main():
         cb = cpu_baseline(n, lengths, pos, args.cpu_seconds, args.cpu_threads)
         out["cpu_baseline"] = cb
         out["batched_over_cpu"] = value / cb["value"]
+    # matched pressure: every pair of estimates at rho = 0.8 within two combined standard errors
+    ests = {"batched": pz}
+    if "plain" in out:
+        ests["plain"] = out["plain"]["pressure"]
+    if "cpu_baseline" in out:
+        ests["cpu"] = out["cpu_baseline"]["pressure_rho_0_80"]
+    keys, match = sorted(ests), {}
+    for i, a in enumerate(keys):
+        for b in keys[i + 1:]:
+            ea, eb = ests[a], ests[b]
+            if ea.get("mean") is None or eb.get("mean") is None:
+                continue
+            sig = float(np.hypot(ea["err"], eb["err"]))
+            match["%s_vs_%s" % (a, b)] = dict(diff=ea["mean"] - eb["mean"], sigma=sig,
+                                              within_2sigma=bool(abs(ea["mean"] - eb["mean"]) <= 2 * sig))
+    out["matched_pressure"] = match
     print(json.dumps(out), flush=True)
 
 
