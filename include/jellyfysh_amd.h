@@ -212,6 +212,33 @@ int ecmc_pair_derivative(int32_t potential, const double *par, int32_t dim, int6
  * kernel of known traffic that scales hardware byte counters in profiles. */
 int ecmc_copy_bytes(const void *d_src, void *d_dst, int64_t bytes, void *stream);
 
+/* ---- pair energy and its backward kernel (no reference counterpart as a whole) -----------
+ * U = 1/2 sum_{i != j in scope} U(s_ij) over minimum-image separations, for the inverse-power
+ * (par: power, prefactor), truncated inverse-power (power, prefactor, cutoff <= L/2),
+ * Lennard-Jones (prefactor, characteristic_length), displaced-even-power (equilibrium
+ * separation, power, prefactor) and merged-image Coulomb (alpha, fourier_cutoff,
+ * position_cutoff, prefactor) potentials.  The backward kernel gives dU/dx (n x 3: for unit
+ * i along axis d the sum over j of the reference's derivative(e_d, s_ij, c_i, c_j),
+ * potential/potential.py:153-181), dU/dpar (4, in the order of par; the Coulomb alpha and
+ * cutoffs get 0) and, for charged terms, dU/dc (n).  Positions are n x 3 row-major in
+ * [0, L); d_charge and d_molecule may be NULL when the term needs neither.  d_work is
+ * device scratch of ecmc_energy_workspace_bytes() bytes; nothing is allocated. */
+enum { ECMC_SCOPE_ALL = 0, ECMC_SCOPE_INTER = 1 /* different molecules */, ECMC_SCOPE_INTRA = 2 /* same molecule */ };
+typedef struct {
+    int32_t potential;   /* ECMC_POT_* */
+    int32_t scope;       /* ECMC_SCOPE_* */
+    int32_t use_charge;  /* multiply each pair by c_i c_j */
+    int32_t reserved;
+    double par[4];
+} ecmc_energy_term;
+int ecmc_energy_workspace_bytes(const ecmc_energy_term *term, int32_t n, int64_t *bytes);
+int ecmc_energy_forward(const ecmc_energy_term *term, int32_t n, const double *d_pos, const double *d_charge,
+                        const int32_t *d_molecule, double system_length, double *d_energy, void *d_work,
+                        void *stream);
+int ecmc_energy_backward(const ecmc_energy_term *term, int32_t n, const double *d_pos, const double *d_charge,
+                         const int32_t *d_molecule, double system_length, double *d_grad_pos, double *d_grad_par,
+                         double *d_grad_charge, void *d_work, void *stream);
+
 /* ---- math and random stream (device) ----------------------------------------------------- */
 int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream);
 int ecmc_math_pow(int64_t n, const double *d_x, const double *d_y, double *d_out, void *stream);

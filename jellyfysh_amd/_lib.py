@@ -46,6 +46,14 @@ class SystemDesc(ctypes.Structure):
                 ("seed", ctypes.c_uint64), ("n_taggers", ctypes.c_int32), ("taggers", TaggerDesc * MAX_TAGGERS)]
 
 
+SCOPE_ALL, SCOPE_INTER, SCOPE_INTRA = 0, 1, 2
+
+
+class EnergyTerm(ctypes.Structure):
+    _fields_ = [("potential", ctypes.c_int32), ("scope", ctypes.c_int32), ("use_charge", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("par", ctypes.c_double * 4)]
+
+
 EVENT_DTYPE = np.dtype([("tagger", np.int32), ("flags", np.int32), ("active", np.int32), ("target", np.int32),
                         ("tq", np.float64), ("tr", np.float64), ("pos", np.float64, (3,)), ("ncand", np.int32),
                         ("chash", np.uint32)])
@@ -88,6 +96,9 @@ def _signatures():
         "ecmc_batch_positions": (i32, [vp, pd]),
         "ecmc_batch_log": (i32, [vp, vp]),
         "ecmc_batch_audit": (i32, [vp, ctypes.POINTER(u64)]),
+        "ecmc_energy_workspace_bytes": (i32, [vp, i32, pi64]),
+        "ecmc_energy_forward": (i32, [vp, i32, vp, vp, vp, d, vp, vp, vp]),
+        "ecmc_energy_backward": (i32, [vp, i32, vp, vp, vp, d, vp, vp, vp, vp, vp]),
     }
 
 

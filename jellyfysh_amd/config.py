@@ -1,0 +1,269 @@
+"""INI front end: the reference's configuration files drive the device chain.
+
+``RunConfig.from_file(path)`` reads a JeLLyFysh INI file with the reference's factory
+conventions (base/factory.py:41-151): keys and values in snake_case, sections in
+CamelCase, ``alias (class_name)`` values whose options sit in section [Alias], lists as
+comma-separated values.  It accepts the single-process run
+(mediator/single_process_mediator.py) of a hypercubic setting with the TagActivator, the
+SingleActiveCellOccupancy over CuboidPeriodicCells, the heap or list scheduler and these
+taggers / event handlers / potentials:
+
+  taggers        excluded_cells_tagger, factor_type_map_in_state_tagger (every pair),
+                 cell_boundary_tagger, no_in_state_tagger, active_global_state_in_state_tagger
+  handlers       two_leaf_unit_event_handler, two_leaf_unit_bounding_potential_event_handler,
+                 cell_boundary_event_handler, fixed_interval_sampling_event_handler,
+                 single_independent_active_periodic_direction_end_of_chain_event_handler,
+                 final_time_end_of_run_event_handler, initial_chain_start_of_run_event_handler
+  potentials     hard_sphere_potential, inverse_power_potential, lennard_jones_potential,
+                 displaced_even_power_potential, merged_image_coulomb_potential,
+                 inverse_power_coulomb_bounding_potential, and truncated_inverse_power_potential
+                 (no reference counterpart: an inverse power truncated and shifted at a cutoff)
+
+Anything else raises ConfigurationError, as the reference's factory does for options a
+class does not take.  The initial configuration is passed in by the caller (the reference's
+input handlers are not part of this path); ``seed`` is the seed of the random stream.
+
+``SingleProcessMediator(config, positions, charges=None, seed=0).run(on_sample=None)``
+then runs the whole leg loop on the device until the end-of-run event; at each sampling
+event (fixed_interval_sampling_event_handler) it hands the positions to ``on_sample``.
+"""
+import configparser
+import re
+
+import numpy as np
+
+from .chain import DeviceChain, make_desc
+
+
+class ConfigurationError(ValueError):
+    """base.exceptions.ConfigurationError of the reference."""
+
+
+def camel(snake):
+    return "".join(w[:1].upper() + w[1:] for w in snake.split("_"))
+
+
+def split_alias(value):
+    """'alias (class_name)' -> (alias, class_name); 'class_name' -> (class_name, class_name)."""
+    m = re.fullmatch(r"\s*([A-Za-z0-9_]+)\s*(?:\(\s*([A-Za-z0-9_]+)\s*\))?\s*", value)
+    if not m:
+        raise ConfigurationError("cannot read %r as 'alias (class_name)'" % value)
+    return m.group(1), m.group(2) or m.group(1)
+
+
+def split_list(value):
+    """Comma-separated list; commas inside parentheses do not split."""
+    out, depth, cur = [], 0, ""
+    for ch in value:
+        if ch == "(":
+            depth += 1
+        elif ch == ")":
+            depth -= 1
+        if ch == "," and depth == 0:
+            out.append(cur.strip())
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        out.append(cur.strip())
+    return out
+
+
+# potential class -> (device potential, constructor arguments in the C ABI's par order,
+# their defaults)
+_POTENTIALS = {
+    "hard_sphere_potential": ("hard_sphere", [("radius", None)]),
+    "inverse_power_potential": ("inverse_power", [("power", None), ("prefactor", None)]),
+    "truncated_inverse_power_potential": ("inverse_power_truncated",
+                                          [("power", None), ("prefactor", None), ("cutoff", None)]),
+    "lennard_jones_potential": ("lennard_jones", [("prefactor", 1.0), ("characteristic_length", 1.0)]),
+    "displaced_even_power_potential": ("displaced_even_power",
+                                       [("equilibrium_separation", None), ("power", None), ("prefactor", 1.0)]),
+    "merged_image_coulomb_potential": ("merged_image_coulomb",
+                                       [("alpha", 3.45), ("fourier_cutoff", 6), ("position_cutoff", 2),
+                                        ("prefactor", 1.0)]),
+    "inverse_power_coulomb_bounding_potential": ("inverse_power_coulomb_bounding", [("prefactor", 1.5837)]),
+}
+
+_HANDLER_OPTIONS = {
+    "two_leaf_unit_event_handler": {"potential"},
+    "two_leaf_unit_bounding_potential_event_handler": {"potential", "bounding_potential", "charge"},
+    "cell_boundary_event_handler": set(),
+    "fixed_interval_sampling_event_handler": {"sampling_interval", "output_handler", "first_event_time_zero"},
+    "single_independent_active_periodic_direction_end_of_chain_event_handler": {"chain_time"},
+    "final_time_end_of_run_event_handler": {"end_of_run_time"},
+    "initial_chain_start_of_run_event_handler": {"initial_direction_of_motion", "speed",
+                                                 "initial_active_identifier"},
+}
+
+_TAGGER_OPTIONS = {"create", "trash", "event_handler", "number_event_handlers", "internal_state_label",
+                   "factor_type_maps", "tag"}
+
+
+class RunConfig:
+    """The parts of a reference INI file that the device chain runs."""
+
+    def __init__(self, parser):
+        self.cp = parser
+        run = self._section("Run", {"mediator", "setting"})
+        mediator = split_alias(run.get("mediator", "single_process_mediator"))[1]
+        if mediator != "single_process_mediator":
+            raise ConfigurationError("only the single-process mediator runs on the device (got %s)" % mediator)
+        setting = split_alias(run.get("setting", "hypercubic_setting"))[1]
+        if setting != "hypercubic_setting":
+            raise ConfigurationError("only the hypercubic setting is supported (got %s)" % setting)
+        s = self._section("HypercubicSetting", {"system_length", "beta", "dimension"})
+        self.system_length = float(s["system_length"])
+        self.beta = float(s["beta"])
+        self.dim = int(s["dimension"])
+        m = self._section("SingleProcessMediator", {"state_handler", "scheduler", "activator",
+                                                    "input_output_handler"})
+        sched = split_alias(m.get("scheduler", "heap_scheduler"))[1]
+        if sched not in ("heap_scheduler", "list_scheduler"):
+            raise ConfigurationError("unknown scheduler %s" % sched)
+        self.scheduler = "heap" if sched == "heap_scheduler" else "list"
+        act = split_alias(m.get("activator", "tag_activator"))[1]
+        if act != "tag_activator":
+            raise ConfigurationError("only the tag activator is supported (got %s)" % act)
+        ta = self._section("TagActivator", {"taggers", "internal_states"})
+        self.cells_per_side = None
+        self.neighbor_layers = 1
+        self.max_occupants = 1
+        for item in split_list(ta.get("internal_states", "")):
+            alias, cls = split_alias(item)
+            if cls != "single_active_cell_occupancy":
+                raise ConfigurationError("unsupported internal state %s" % cls)
+            occ = self._section(camel(alias), {"cells", "cell_level", "maximum_number_occupants"})
+            self.max_occupants = int(occ.get("maximum_number_occupants", 1))
+            calias, ccls = split_alias(occ.get("cells", "cuboid_periodic_cells"))
+            if ccls != "cuboid_periodic_cells":
+                raise ConfigurationError("unsupported cells %s" % ccls)
+            cs = self._section(camel(calias), {"cells_per_side", "neighbor_layers"})
+            self.cells_per_side = int(cs["cells_per_side"])
+            self.neighbor_layers = int(cs.get("neighbor_layers", 1))
+        self.taggers = []
+        for item in split_list(ta["taggers"]):
+            label, cls = split_alias(item)
+            self.taggers.append(self._tagger(label, cls))
+        labels = [t["tag"] for t in self.taggers]
+        for t in self.taggers:
+            for lst in ("create", "trash"):
+                for tag in t[lst]:
+                    if tag not in labels:
+                        raise ConfigurationError("tag %s of tagger %s is no tagger" % (tag, t["tag"]))
+        if any(t["kind"] == "pair_cells" for t in self.taggers) and self.cells_per_side is None:
+            raise ConfigurationError("excluded_cells_tagger needs a cell occupancy")
+
+    # ---- helpers ---------------------------------------------------------------------------
+    def _section(self, name, allowed):
+        if name not in self.cp:
+            return {}
+        sec = dict(self.cp[name])
+        extra = set(sec) - set(allowed)
+        if extra:
+            raise ConfigurationError("section [%s] has options %s that the device path does not take"
+                                     % (name, sorted(extra)))
+        return sec
+
+    def _potential(self, value):
+        alias, cls = split_alias(value)
+        if cls not in _POTENTIALS:
+            raise ConfigurationError("potential %s has no device implementation" % cls)
+        name, args = _POTENTIALS[cls]
+        sec = self._section(camel(alias), {a for a, _ in args})
+        params = []
+        for a, default in args:
+            if a in sec:
+                params.append(float(sec[a]))
+            elif default is None:
+                raise ConfigurationError("missing argument %s of %s in section [%s]" % (a, cls, camel(alias)))
+            else:
+                params.append(float(default))
+        return name, params
+
+    def _tagger(self, label, cls):
+        sec = self._section(camel(label), _TAGGER_OPTIONS)
+        t = {"tag": label, "create": split_list(sec.get("create", "")), "trash": split_list(sec.get("trash", ""))}
+        if "number_event_handlers" in sec:
+            t["number_event_handlers"] = int(sec["number_event_handlers"])
+        halias, hcls = split_alias(sec["event_handler"])
+        if hcls not in _HANDLER_OPTIONS:
+            raise ConfigurationError("event handler %s has no device implementation" % hcls)
+        h = self._section(camel(halias), _HANDLER_OPTIONS[hcls])
+        if hcls in ("two_leaf_unit_event_handler", "two_leaf_unit_bounding_potential_event_handler"):
+            if cls == "excluded_cells_tagger":
+                t["kind"] = "pair_cells"
+            elif cls == "factor_type_map_in_state_tagger":
+                t["kind"] = "pair_all"
+            else:
+                raise ConfigurationError("pair handlers run under excluded_cells_tagger or "
+                                         "factor_type_map_in_state_tagger (got %s)" % cls)
+            t["potential"], t["params"] = self._potential(h["potential"])
+            if hcls == "two_leaf_unit_bounding_potential_event_handler":
+                t["bounding_potential"], t["bounding_params"] = self._potential(h["bounding_potential"])
+                t["charge"] = "charge" in h
+        elif hcls == "cell_boundary_event_handler":
+            t["kind"] = "cell_boundary"
+        elif hcls == "fixed_interval_sampling_event_handler":
+            t["kind"] = "sampling"
+            t["sampling_interval"] = float(h["sampling_interval"])
+            t["first_event_time_zero"] = h.get("first_event_time_zero", "false").lower() in ("1", "yes", "true", "on")
+        elif hcls == "single_independent_active_periodic_direction_end_of_chain_event_handler":
+            t["kind"] = "end_of_chain"
+            t["chain_time"] = float(h["chain_time"])
+        elif hcls == "final_time_end_of_run_event_handler":
+            t["kind"] = "end_of_run"
+            t["end_of_run_time"] = float(h["end_of_run_time"])
+        else:
+            t["kind"] = "start_of_run"
+            t["initial_direction_of_motion"] = int(h.get("initial_direction_of_motion", 0))
+            t["speed"] = float(h.get("speed", 1.0))
+            t["initial_active_identifier"] = int(h.get("initial_active_identifier", 0))
+        return t
+
+    # ---- construction ----------------------------------------------------------------------
+    @classmethod
+    def from_string(cls, text):
+        cp = configparser.ConfigParser()
+        cp.read_string(text)
+        return cls(cp)
+
+    @classmethod
+    def from_file(cls, path):
+        cp = configparser.ConfigParser()
+        with open(path) as f:
+            cp.read_file(f)
+        return cls(cp)
+
+    def desc(self, n, seed=0):
+        cps = self.cells_per_side if self.cells_per_side is not None else 1
+        return make_desc(self.dim, n, self.system_length, self.beta, cps, self.taggers, seed,
+                         scheduler=self.scheduler, neighbor_layers=self.neighbor_layers,
+                         max_occupants=self.max_occupants)
+
+
+class SingleProcessMediator:
+    """mediator/single_process_mediator.py:84-156 with the whole leg loop on the device."""
+
+    def __init__(self, config, positions, charges=None, seed=0):
+        if isinstance(config, str):
+            config = RunConfig.from_file(config)
+        self.config = config
+        pos = np.asarray(positions, dtype=np.float64)
+        self.chain = DeviceChain(config.desc(pos.shape[0], seed), pos, charges)
+        self.sampling = [i for i, t in enumerate(config.taggers) if t["kind"] == "sampling"]
+
+    def run(self, on_sample=None, chunk=1 << 20):
+        """Run to the end-of-run event; returns the number of committed legs."""
+        mask = 0
+        if on_sample is not None:
+            for i in self.sampling:
+                mask |= 1 << i
+        self.chain.set_stop_mask(mask)
+        total = 0
+        while True:
+            total += self.chain.run(chunk, log=False)
+            if self.chain.active()[4]:      # end of run committed
+                return total
+            if on_sample is not None and mask:
+                on_sample(self.chain.positions())

@@ -50,7 +50,7 @@ __device__ void pair_row(const PairArgs &a, int64_t i, bool derivative) {
                        : ip_displacement<D>(dir, speed, s, c1, c2, a.par[0], a.par[1], dU);
         break;
     case POT_INVERSE_POWER_TRUNC:
-        r = derivative ? ip_derivative<D>(dir, speed, s, c1, c2, a.par[0], a.par[1])
+        r = derivative ? ipt_derivative<D>(dir, speed, s, c1, c2, a.par[0], a.par[1], a.par[2])
                        : ipt_displacement<D>(dir, speed, s, c1, c2, a.par[0], a.par[1], a.par[2], dU);
         break;
     case POT_LENNARD_JONES: {

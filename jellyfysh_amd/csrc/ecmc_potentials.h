@@ -110,6 +110,15 @@ ECMC_FN double ip_displacement(int dir, double speed, const double *sep, double 
     return disp / speed;
 }
 
+// Derivative of the truncated-and-shifted inverse power: the inverse power's inside the
+// cutoff sphere (|s| < rc), zero outside.
+template <int D>
+ECMC_FN double ipt_derivative(int dir, double speed, const double *s, double c1, double c2, double power,
+                              double prefactor, double cutoff) {
+    if (!(norm_sq<D>(s) < cutoff * cutoff)) return 0.0;
+    return ip_derivative<D>(dir, speed, s, c1, c2, power, prefactor);
+}
+
 // Truncated-and-shifted inverse power U(r) - U(rc) for r < rc (no reference counterpart):
 // the same inversion with the floor U(rc) in place of zero; a target whose line of motion
 // stays outside the cutoff sphere never produces an event.

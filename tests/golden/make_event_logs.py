@@ -42,6 +42,7 @@ def _reset_modules():
 def run_reference(system, n_events):
     """Build the reference object graph for `system` and record `n_events` legs."""
     _reset_modules()
+    sys.dont_write_bytecode = True      # the reference tree is read-only: no __pycache__ there
     sys.path.insert(0, REF)
     try:
         import jellyfysh.setting as setting

@@ -34,13 +34,16 @@ def test_structs_match_header_sizes(tmp_path):
         pytest.skip("no C compiler to read the header's struct sizes")
     src = tmp_path / "sizes.c"
     src.write_text('#include "jellyfysh_amd.h"\n#include <stdio.h>\n#include <stddef.h>\n'
-                   'int main(void){printf("%zu %zu %zu %zu\\n", sizeof(ecmc_tagger_desc), sizeof(ecmc_system_desc),'
-                   ' sizeof(ecmc_event), offsetof(ecmc_system_desc, taggers));return 0;}\n')
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(ecmc_tagger_desc),'
+                   ' sizeof(ecmc_system_desc), sizeof(ecmc_event), offsetof(ecmc_system_desc, taggers),'
+                   ' sizeof(ecmc_energy_term), sizeof(ecmc_batch_desc), sizeof(ecmc_batch_event));return 0;}\n')
     exe = tmp_path / "sizes"
     subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
     sizes = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    from jellyfysh_amd.batch import BATCH_EVENT_DTYPE, BatchDesc
     assert sizes == [ctypes.sizeof(_lib.TaggerDesc), ctypes.sizeof(_lib.SystemDesc), _lib.EVENT_DTYPE.itemsize,
-                     _lib.SystemDesc.taggers.offset]
+                     _lib.SystemDesc.taggers.offset, ctypes.sizeof(_lib.EnergyTerm), ctypes.sizeof(BatchDesc),
+                     BATCH_EVENT_DTYPE.itemsize]
 
 
 def test_device_count_without_gpu_is_not_an_error():
