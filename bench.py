@@ -38,26 +38,61 @@ def parse():
     ap.add_argument("--replicas", type=int, default=4096)
     ap.add_argument("--plain-legs", type=int, default=20000, help="legs per timed chunk of the plain chain")
     ap.add_argument("--plain-seconds", type=float, default=30.0, help="plain-chain time for its pressure")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--production-steps", type=int, default=4, help="untimed steps whose legs give the pressure")
+    ap.add_argument("--plain-equilibration", type=int, default=600000,
+                    help="legs of the batched kernel that equilibrate the plain chain's start")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the plain copy kernel 10 times over BYTES (scales hardware byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration(nbytes):
+    """Ten launches of a device-to-device copy of nbytes: FETCH_SIZE / WRITE_SIZE per launch
+    of this kernel read nbytes each when the counters are exact."""
+    import ctypes
+    import torch
+    from jellyfysh_amd import _lib
+    _lib.require_device()
+    nbytes -= nbytes % 16
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for _ in range(10):
+        _lib.check(_lib.lib().ecmc_copy_bytes(_lib.dptr(src), _lib.dptr(dst), nbytes, stream))
+    torch.cuda.synchronize()
+    assert torch.equal(src, dst)
+    print(json.dumps({"copy_kernel_bytes_per_launch": nbytes, "launches": 10}), flush=True)
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def plain_chain_rate(legs, pressure_seconds):
-    """Plain form: one chain of N = 4096 LJ at rho = 0.8 with 6^3 cells, on one workgroup."""
-    from jellyfysh_amd.batch import lattice_positions
+def plain_chain_rate(legs, pressure_seconds, equilibration_legs):
+    """Plain form: one chain of N = 4096 LJ at rho = 0.8 with 6^3 cells, on one workgroup.
+
+    The start is equilibrated first: the 16^3 lattice (N = 4096 is not 4 m^3, so no FCC
+    lattice fits; spacing L / 16 = 1.07722 sigma, jitter +-0.01 sigma) is run through the
+    batched kernel as a single replica of the same system for `equilibration_legs` legs,
+    then handed to the plain chain."""
+    from jellyfysh_amd.batch import DeviceBatch, lattice_positions
     from jellyfysh_amd.chain import DeviceChain, make_desc
     n = 4096
     L = (n / 0.8) ** (1 / 3)
-    # N = 4096 is not 4 m^3, so the lattice start is the full 16^3 grid of spacing
-    # L / 16 = 1.07722 sigma, jittered by +-0.01 sigma
     pos = lattice_positions(n, L, 0.01, np.random.default_rng(7))
+    eq = DeviceBatch([n], [L], [pos], seed=23, classes=([16384], [4]))
+    done_eq, t_eq = 0, time.perf_counter()
+    while done_eq < equilibration_legs:
+        eq.run(min(100000, equilibration_legs - done_eq))
+        done_eq = int(eq.states()["legs"][0])
+        log("plain start: %d equilibration legs" % done_eq)
+    pos = eq.positions()[0]
+    eq.close()
+    t_eq = time.perf_counter() - t_eq
     pc = ["lj_twelve", "lj_six", "cell_boundary"]
     taggers = [dict(tag="lj_twelve", kind="pair_cells", potential="inverse_power_truncated", params=[12.0, 4.0, 2.5],
                     create=pc, trash=pc),
@@ -74,15 +109,15 @@ def plain_chain_rate(legs, pressure_seconds):
     chain = DeviceChain(desc, pos)
     chain.run(2000, log=False)                      # warm-up
     # the timed rate: one chunk of `legs` legs
-    e0 = chain.estimators()
     t0 = time.perf_counter()
     done = chain.run(legs, log=False)
     wall = time.perf_counter() - t0
     ms = chain.last_kernel_ms()
-    out = dict(n=n, rho=0.8, start="16^3 grid, spacing 1.07722, jitter 0.01", events=int(done), kernel_ms=ms,
-               wall_s=wall, events_per_sec=done / (ms * 1e-3))
-    # pressure: further chunks for `pressure_seconds`, batch means over the chunks after
-    # the first third (equilibration)
+    out = dict(n=n, rho=0.8, start="16^3 grid (spacing 1.07722, jitter 0.01) equilibrated by %d legs of the batched "
+                                   "kernel (%.1f s)" % (done_eq, t_eq),
+               events=int(done), kernel_ms=ms, wall_s=wall, events_per_sec=done / (ms * 1e-3))
+    # pressure: chunks of `legs` legs for `pressure_seconds`; the standard error comes from
+    # 8 blocks of consecutive chunks (chunks of one chain are correlated)
     chunk_z, t_end = [], time.perf_counter() + pressure_seconds
     while time.perf_counter() < t_end:
         e1 = chain.estimators()
@@ -91,19 +126,21 @@ def plain_chain_rate(legs, pressure_seconds):
         if d[1] > 0:
             chunk_z.append(1.0 + d[0] / d[1])
         log("plain chunk %d: Z = %.4f" % (len(chunk_z), chunk_z[-1] if chunk_z else float("nan")))
-    z = np.asarray(chunk_z[len(chunk_z) // 3:])
+    z = np.asarray(chunk_z)
+    nb = min(8, len(z))
+    blocks = np.array([b.mean() for b in np.array_split(z, nb)]) if nb else np.zeros(0)
     out["pressure"] = dict(mean=float(z.mean()) if len(z) else None,
-                           err=float(z.std(ddof=1) / np.sqrt(len(z))) if len(z) > 1 else None,
-                           chunks=int(len(z)), legs_per_chunk=int(legs))
+                           err=float(blocks.std(ddof=1) / np.sqrt(nb)) if nb > 1 else None,
+                           chunks=int(len(z)), legs_per_chunk=int(legs), blocks=int(nb))
     return out
 
 
-def batched_pressure_at(rho, z, rho0=0.8, width=0.02, n=None, n_min=256):
-    """beta P / rho at rho0 from the replicas with |rho - rho0| < width (and N >= n_min):
+def batched_pressure_at(rho, z, rho0=0.8, width=0.02, n=None, n_min=256, n_max=512):
+    """beta P / rho at rho0 from the replicas with |rho - rho0| < width and n_min <= N <= n_max:
     least-squares line through (rho, Z), its value at rho0 and that value's standard error."""
     sel = np.abs(rho - rho0) < width
     if n is not None:
-        sel &= n >= n_min
+        sel &= (n >= n_min) & (n <= n_max)
     x, y = rho[sel] - rho0, z[sel]
     if len(x) < 4:
         return dict(mean=None, err=None, replicas=int(len(x)))
@@ -113,34 +150,37 @@ def batched_pressure_at(rho, z, rho0=0.8, width=0.02, n=None, n_min=256):
     s2 = float(np.sum((y - A @ coef) ** 2)) / dof
     cov = s2 * np.linalg.inv(A.T @ A)
     return dict(mean=float(coef[0]), err=float(np.sqrt(cov[0, 0])), slope=float(coef[1]), replicas=int(len(x)),
-                window="|rho - %.2f| < %.2f, N >= %d" % (rho0, width, n_min))
+                window="|rho - %.2f| < %.2f, %d <= N <= %d" % (rho0, width, n_min, n_max))
 
 
 def cpu_baseline(n, lengths, pos, seconds, threads):
     """The oracle's batched restatement (FP64, glibc math, every candidate inverted) on host
     threads, one replica per thread at a time, for about `seconds` of wall time.  The sample
-    is the 4 x threads replicas of the workload whose density is closest to 0.8, so that
-    its pressure estimates also give the host's value at rho = 0.8."""
+    is the 4 x threads replicas of the workload with 256 <= N <= 512 whose density is closest
+    to 0.8 (their candidate sets, 255 to ~370 units, bracket the workload's mean of ~335);
+    each runs its legs in two halves, and the second halves give the host's pressure."""
     from oracle import oracle as O
     O.lib()
     threads = threads or min(16, os.cpu_count() or 1)
     n = np.asarray(n)
     rho = n / np.asarray(lengths) ** 3
-    order = np.argsort(np.abs(rho - 0.8), kind="stable")[: 4 * threads]
+    pool = np.flatnonzero((n >= 256) & (n <= 512))
+    order = pool[np.argsort(np.abs(rho[pool] - 0.8), kind="stable")[: 4 * threads]]
     order = order[np.argsort(-n[order], kind="stable")]          # largest first
     # calibrate legs so that the sample takes about `seconds`
     probe = O.BatchedReplica(pos[order[0]], float(lengths[order[0]]), replica=int(order[0]), seed=1,
                              math_mode=O.MATH_LIBM)
     t = time.perf_counter()
-    probe.run(200, log=False)
-    per_leg = (time.perf_counter() - t) / 200
-    mean_n = float(np.mean(n[order]))
-    legs = max(200, int(seconds * threads / (len(order) * per_leg * (mean_n / n[order[0]]) * 1.3)))
+    probe.run(400, log=False)
+    per_leg = (time.perf_counter() - t) / 400
+    half = max(200, int(seconds * threads / (2 * len(order) * per_leg * 1.1)))
 
     def work(r):
         rep = O.BatchedReplica(pos[r], float(lengths[r]), replica=int(r), seed=1, math_mode=O.MATH_LIBM)
-        k = rep.run(legs, log=False)
-        return k, np.array(rep.state.est[:])
+        k = rep.run(half, log=False)
+        e0 = np.array(rep.state.est[:])
+        k += rep.run(half, log=False)
+        return k, np.array(rep.state.est[:]) - e0
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
@@ -149,16 +189,19 @@ def cpu_baseline(n, lengths, pos, seconds, threads):
     done = sum(k for k, _ in res)
     est = np.stack([e for _, e in res])
     z = 1.0 + est[:, 0] / est[:, 1]
-    pz = batched_pressure_at(rho[order], z, width=1.0, n=n[order], n_min=0)
+    pz = batched_pressure_at(rho[order], z, width=1.0, n=n[order])
     return dict(value=done / wall, unit="events/s", cores=threads, kind="port",
-                sample="%d replicas of the batched workload (density closest to 0.8, N %d-%d) x %d legs each, "
-                       "oracle restatement in FP64 with glibc math, %d host threads, %.1f s"
-                       % (len(order), int(n[order].min()), int(n[order].max()), legs, threads, wall),
+                sample="%d replicas of the batched workload (256 <= N <= 512, density closest to 0.8, N %d-%d) "
+                       "x %d legs each, oracle restatement in FP64 with glibc math, %d host threads, %.1f s"
+                       % (len(order), int(n[order].min()), int(n[order].max()), 2 * half, threads, wall),
                 pressure_rho_0_80=pz)
 
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: each replica runs independently, and a multi-GPU run would "
@@ -190,9 +233,14 @@ def main():
     flops = cands * n_factors * FLOPS_PER_INVERSION
     achieved = flops / (kernel_ms * 1e-3) / 1e12
     exact = float((s1["stats"][:, 3] - s0["stats"][:, 3]).sum())
-    z = pressure(s1["est"] - s0["est"])
+    # pressure from untimed production steps after the timed region (each replica then has
+    # (warmup + steps) x legs legs behind it)
+    batch.run_graph(args.legs, args.production_steps)
+    s2 = batch.states()
+    z = pressure(s2["est"] - s1["est"])
     rho = n / lengths ** 3
     pz = batched_pressure_at(rho, z, n=n)
+    pz["production_legs_per_replica"] = args.legs * args.production_steps
     out = {
         "metric": METRIC, "value": value, "unit": "events/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
@@ -212,7 +260,7 @@ def main():
     }
     if not args.no_plain:
         log("plain chain ...")
-        pl = plain_chain_rate(args.plain_legs, args.plain_seconds)
+        pl = plain_chain_rate(args.plain_legs, args.plain_seconds, args.plain_equilibration)
         out["plain"] = pl
         out["batched_over_plain"] = value / pl["events_per_sec"]
     if not args.no_cpu:

@@ -56,10 +56,15 @@ enum {
     ECMC_TAG_SAMPLING = 3,       /* NoInStateTagger + FixedIntervalSamplingEventHandler */
     ECMC_TAG_END_OF_CHAIN = 4,   /* ActiveGlobalStateInStateTagger + periodic-direction end of chain */
     ECMC_TAG_END_OF_RUN = 5,     /* NoInStateTagger + FinalTimeEndOfRunEventHandler */
-    ECMC_TAG_START_OF_RUN = 6    /* NoInStateTagger + InitialChainStartOfRunEventHandler */
+    ECMC_TAG_START_OF_RUN = 6,   /* NoInStateTagger + InitialChainStartOfRunEventHandler */
+    ECMC_TAG_SURPLUS = 7,        /* SurplusCellsTagger: the surplus units of a bounded occupancy */
+    ECMC_TAG_CELL_BOUNDING = 8   /* CellBoundingPotentialTagger: occupants of the non-nearby cells */
 };
 /* Event handlers of pair taggers */
-enum { ECMC_EH_TWO_LEAF = 0, ECMC_EH_TWO_LEAF_BOUNDING = 1 };
+/* ECMC_EH_CELL_BOUNDING: TwoLeafUnitCellBoundingPotentialEventHandler
+ * (two_leaf_unit_cell_bounding_potential_event_handler.py:113-211) with the merged-image
+ * Coulomb potential (par) and ECMC_POT_CELL_BOUNDING (bpar). */
+enum { ECMC_EH_TWO_LEAF = 0, ECMC_EH_TWO_LEAF_BOUNDING = 1, ECMC_EH_CELL_BOUNDING = 2 };
 /* Potentials (the modules of potential/) */
 enum {
     ECMC_POT_HARD_SPHERE = 0,              /* par: radius */
@@ -68,7 +73,9 @@ enum {
     ECMC_POT_INVERSE_POWER_TRUNC = 3,      /* par: power, prefactor, cutoff (no reference counterpart) */
     ECMC_POT_MERGED_IMAGE_COULOMB = 4,     /* par: alpha, fourier_cutoff, position_cutoff, prefactor */
     ECMC_POT_INVERSE_POWER_COULOMB_BOUND = 5, /* par: prefactor */
-    ECMC_POT_DISPLACED_EVEN_POWER = 6      /* par: equilibrium_separation, power, prefactor */
+    ECMC_POT_DISPLACED_EVEN_POWER = 6,     /* par: equilibrium_separation, power, prefactor */
+    ECMC_POT_CELL_BOUNDING = 7             /* CellBoundingPotential + InnerPointEstimator; par: estimator
+                                              prefactor, points_per_side, target_charge, empirical_bound */
 };
 enum { ECMC_SCHED_LIST = 0, ECMC_SCHED_HEAP = 1 };
 
@@ -242,6 +249,13 @@ int ecmc_energy_backward(const ecmc_energy_term *term, int32_t n, const double *
 /* ---- math and random stream (device) ----------------------------------------------------- */
 int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream);
 int ecmc_math_pow(int64_t n, const double *d_x, const double *d_y, double *d_out, void *stream);
+/* Largest error over every float with bit pattern in [bits_lo, bits_hi) of the device's FP32
+ * log2f (fn 0: |error| / max(|log2 x|, 1)) or exp2f (fn 1: relative error) against FP64
+ * log2 / exp2, as
+ * the bit pattern of a non-negative double combined into *d_out by atomic max (clear it
+ * first).  No reference counterpart: it measures the error the FP32 bounds of the batched
+ * kernel must cover. */
+int ecmc_fp32_math_error(int32_t fn, uint32_t bits_lo, uint32_t bits_hi, unsigned long long *d_out, void *stream);
 /* Draw n MT19937 words after seeding like random.seed(seed), on the device (one workgroup,
  * parallel twist). */
 int ecmc_mt_words(uint64_t seed, int64_t n, uint32_t *d_out, void *stream);

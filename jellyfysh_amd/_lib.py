@@ -13,10 +13,10 @@ LIB_PATH = os.path.join(_HERE, "libjellyfysh_amd.so")
 
 MAX_TAGGERS = 16
 (TAG_PAIR_CELLS, TAG_PAIR_ALL, TAG_CELL_BOUNDARY, TAG_SAMPLING, TAG_END_OF_CHAIN, TAG_END_OF_RUN,
- TAG_START_OF_RUN) = range(7)
-EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING = 0, 1
+ TAG_START_OF_RUN, TAG_SURPLUS, TAG_CELL_BOUNDING) = range(9)
+EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING, EH_CELL_BOUNDING = 0, 1, 2
 (POT_HARD_SPHERE, POT_INVERSE_POWER, POT_LENNARD_JONES, POT_INVERSE_POWER_TRUNC, POT_MERGED_IMAGE_COULOMB,
- POT_INVERSE_POWER_COULOMB_BOUND, POT_DISPLACED_EVEN_POWER) = range(7)
+ POT_INVERSE_POWER_COULOMB_BOUND, POT_DISPLACED_EVEN_POWER, POT_CELL_BOUNDING) = range(8)
 SCHED_LIST, SCHED_HEAP = 0, 1
 
 STATUS = {0: "ok", -1: "argument", -2: "hip", -3: "configuration", -4: "scheduler", -5: "tag activator",
@@ -83,6 +83,7 @@ def _signatures():
         "ecmc_math_log": (i32, [i64, vp, vp, vp]),
         "ecmc_math_pow": (i32, [i64, vp, vp, vp, vp]),
         "ecmc_mt_words": (i32, [u64, i64, vp, vp]),
+        "ecmc_fp32_math_error": (i32, [i32, ctypes.c_uint32, ctypes.c_uint32, vp, vp]),
         "ecmc_copy_bytes": (i32, [vp, vp, i64, vp]),
         "ecmc_batch_create": (i32, [vp, vp, pd, pd, ctypes.POINTER(vp)]),
         "ecmc_batch_destroy": (None, [vp]),

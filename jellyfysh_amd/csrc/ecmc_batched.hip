@@ -66,13 +66,6 @@ struct ecmc_batch {
     int32_t graph_legs = -1;
 };
 
-static size_t lds_bytes(int nmax, int cellmax) {
-    size_t b = 12ull * nmax + 4ull * cellmax + 2ull * cellmax * B_CELL_CAP;
-    b += b & 3;
-    b += 4ull * B_CONTENDERS;
-    return (b + 15) & ~15ull;
-}
-
 static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
     if (b->class_count[c] == 0) return ECMC_OK;
     BatchArgs A;
@@ -95,8 +88,8 @@ static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
     A.queue = b->d_queue + c;
     A.log = b->d_log;
     A.audit = b->d_audit;
-    size_t lds = lds_bytes(A.nmax, A.cellmax);
     int wpr = b->class_wpr[c];
+    size_t lds = batched_lds_bytes(A.nmax, A.cellmax, wpr);
     int dev = 0, cus = 0;
     hipGetDevice(&dev);
     hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);

@@ -10,7 +10,8 @@ namespace ecmc {
 
 constexpr int B_MAX_WPR = 8;
 constexpr int B_CELL_CAP = 64;       // occupants per cell
-constexpr int B_CONTENDERS = 256;    // contender list capacity per leg
+constexpr int B_CONTENDERS = 128;    // contender list capacity per leg
+constexpr int B_QUEUE = 128;         // per-wavefront queue of candidates past the geometric test
 constexpr int B_MAX_CPS = 8;         // cells per side handled in LDS (512 cells)
 
 struct ReplicaDev {
@@ -198,21 +199,97 @@ __device__ void block_min(BKey &b, BKey *sk) {
 
 __device__ __forceinline__ void bsync() { __syncthreads(); }
 
-// exact FP64 displacement of factor f of target j; the target's position is returned too
-__device__ __forceinline__ double exact_displacement(const BatchArgs &A, const ReplicaDev &R, const double *ap, int d,
-                                                     int j, int f, uint64_t leg, uint32_t replica, double *pj) {
-    double s[3];
-    const double *P = A.pos64 + R.off;
+// The device log2f is within LOG2F_ERR * max(|log2 x|, 1) of log2 x for every positive float:
+// an exhaustive scan (ecmc_fp32_math_error) measures 2^-23 and the tests require <= 2^-21.
+constexpr float LOG2F_ERR = 0x1p-20f;
+
+// FP32 lower bound of dU = -log(1 - u) / beta for u = unit53(a, b), from the first word
+// alone: u >= u24 = (a >> 8) 2^-24 exactly, -log(1 - u) >= u, and for u24 >= 2^-10 the
+// float 1 - u24 is exact and -log2f of it is within LOG2F_ERR max(|.|, 1) of
+// -log2(1 - u24).  inv_beta_lo is a lower bound of 1 / beta.  The result never exceeds the
+// exact dU.
+__device__ __forceinline__ float dU_lower(uint32_t a, float inv_beta_lo) {
+    const float u24 = (float)(a >> 8) * 0x1p-24f;
+    float l;
+    if (u24 < 0x1p-10f) {
+        l = u24;                                           // -ln(1 - u) >= u >= u24
+    } else {
+        const float ml2 = -log2f(1.f - u24);
+        const float ml2_lo = ml2 - LOG2F_ERR * fmaxf(ml2, 1.f);  // <= -log2(1 - u24) <= -log2(1 - u)
+        l = ml2_lo * 0x1.62e42ep-1f;                       // ln 2 rounded down
+    }
+    return fmaxf(l * inv_beta_lo * (1.f - 0x1p-20f), 0.f);
+}
+
+// Exact FP64 event displacement of one truncated-and-shifted inverse-power factor
+// (k / r^p - k / rc^p inside rc) for separation s (target minus active, minimum image)
+// along +d with unit speed: the same operations, in the same order, as ipt_displacement
+// (and so the oracle), arranged so that lanes of the repulsive and of the attractive
+// factor share the calls of the correctly rounded pow.  urc = k / rc^p (FP64, as computed
+// by ipt_displacement).
+__device__ __forceinline__ double exact_ipt(double s0, double s1, double s2, int d, double power, double prefactor,
+                                            double rc2, double urc, double dU) {
+    double s[3] = {s0, s1, s2};
+    const double perp2 = others_sq<3>(s, d);
+    if (perp2 >= rc2) return INFINITY;
+    const bool rep = prefactor > 0.0;
+    double disp = 0.0;
+    bool live = true;
+    if (rep) {
+        if (s[d] <= 0.0) live = false;
+    } else if (s[d] > 0.0) {
+        disp += s[d];
+        s[d] = 0.0;
+    }
+    const double sn2 = norm_sq<3>(s);
+    if (!rep && sn2 >= rc2) live = false;
+    if (!live) return INFINITY;
+    double cur = urc;
+    if (!rep || sn2 < rc2) cur = prefactor / cr_pow(sn2, power / 2.0);  // ip_potential with cp = 1
+    if (rep) {
+        double smax[3] = {s[0], s[1], s[2]};
+        smax[d] = 0.0;
+        const double maxp = prefactor / cr_pow(norm_sq<3>(smax), power / 2.0);
+        live = dU < maxp - cur;
+    } else {
+        live = !(cur + dU >= urc);
+    }
+    if (!live) return INFINITY;
+    const double nsq = cr_pow(prefactor / (cur + dU), 2.0 / power);
+    const double a = nsq - perp2;
+    if (a < 0.0) return NAN;
+    return rep ? s[d] - sqrt(a) : disp + (s[d] + sqrt(a));
+}
+
+// FP32 geometry of a candidate: separation bounds with the position slack E, and whether
+// the minimum image is ambiguous (then the candidate is forced into the contender list).
+__device__ __forceinline__ bool cand_geometry(float dx, float dy, float dz, int d, float Lf, float E, Geo &g) {
+    float sv[3] = {dx, dy, dz};
+    bool ambiguous = false;
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        pj[q] = P[(size_t)q * R.n + j];
-        s[q] = bmin_image(pj[q] - ap[q], R.L);
+        if (sv[q] >= 0.5f * Lf) sv[q] -= Lf;
+        else if (sv[q] < -0.5f * Lf) sv[q] += Lf;
+        if (fabsf(fabsf(sv[q]) - 0.5f * Lf) <= 2.f * E) ambiguous = true;
     }
-    uint32_t w[4];
-    philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, replica, A.seed_lo, w);
-    double u = unit53(w[2 * f], w[2 * f + 1]);
-    double dU = -cr_log(1.0 - u) / A.beta;
-    return ipt_displacement<3>(d, 1.0, s, 1.0, 1.0, A.fac.power[f], A.fac.prefactor[f], R.cutoff, dU);
+    const float sd = d == 0 ? sv[0] : (d == 1 ? sv[1] : sv[2]);
+    g.sd_lo = sd - E;
+    g.sd_hi = sd + E;
+    float p2lo = 0.f, p2hi = 0.f;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        if (q == d) continue;
+        const float m = fabsf(sv[q]);
+        const float lo = fmaxf(m - E, 0.f), hi = m + E;
+        p2lo += lo * lo;
+        p2hi += hi * hi;
+    }
+    g.perp2_lo = dn(p2lo);
+    g.perp2_hi = up(p2hi);
+    const float mdl = fmaxf(fabsf(sd) - E, 0.f), mdh = fabsf(sd) + E;
+    g.r2_lo = dn(p2lo + mdl * mdl);
+    g.r2_hi = up(p2hi + mdh * mdh);
+    return ambiguous;
 }
 
 // candidate idx -> unit (binary search over the 27 prefix sums, or all units but `a`)
@@ -228,17 +305,43 @@ __device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int
     return cell_l[(size_t)nearby[lo] * B_CELL_CAP + (idx - pref[lo])];
 }
 
+// LDS bytes of the dynamic part for a class (FP32 positions, cell lists, contenders and
+// the per-wavefront candidate queues); the host sizes launches with the same function.
+__host__ __device__ inline size_t batched_lds_bytes(int nmax, int cellmax, int wpr) {
+    size_t b = 12ull * nmax + 4ull * cellmax + 2ull * cellmax * B_CELL_CAP;
+    b = (b + 7) & ~7ull;
+    b += (24ull + 8ull + 4ull) * B_CONTENDERS;     // positions, random words, identifiers
+    b += 4ull * B_QUEUE * wpr;
+    return (b + 15) & ~15ull;
+}
+
+// One leg of a replica, for the workgroup that owns it:
+//   1. horizon h = min(distance to the next cell wall, rest of the chain) (FP64);
+//   2. pass 1a over every candidate: FP32 geometry and the geometric floors of its
+//      factors; candidates whose floors are all beyond h are done, the others are appended
+//      to the wavefront's queue (ballot compaction);
+//   3. pass 1b over each full queue: Philox draws, FP32 lower bounds of the event
+//      displacements (FP32 dU bound from the draw's leading bits); a factor whose bound is
+//      <= h is a contender and its FP64 target position is loaded right away (the load's
+//      latency overlaps the rest of the pass);
+//   4. pass 2, one contender per thread: exact FP64 inversion with correctly rounded
+//      pow/log, workgroup min-reduce over (displacement, kind, target, factor);
+//   5. thread 0 commits.
 template <int WPR>
 __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NTH = 64 * WPR;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     float *px = (float *)smem;
     float *py = px + A.nmax;
     float *pz = py + A.nmax;
     int32_t *cell_n = (int32_t *)(pz + A.nmax);
     int16_t *cell_l = (int16_t *)(cell_n + A.cellmax);
-    int32_t *cont_id = (int32_t *)(cell_l + (size_t)A.cellmax * B_CELL_CAP + (((size_t)A.cellmax * B_CELL_CAP) & 1));
+    size_t off = (12ull * A.nmax + 4ull * A.cellmax + 2ull * A.cellmax * B_CELL_CAP + 7) & ~7ull;
+    double *cont_pos = (double *)(smem + off);                       // B_CONTENDERS x 3
+    uint32_t *cont_w = (uint32_t *)(cont_pos + 3 * B_CONTENDERS);    // B_CONTENDERS x 2
+    int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
+    int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
     __shared__ int s_item, s_nc, s_overflow, s_nearby[27], s_cnt[27], s_pref[28], s_M;
     __shared__ double s_ap[3], s_newpos[3], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
@@ -286,11 +389,17 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
         const float E = Lf * 0x1p-21f;
         const double rc2 = R.cutoff * R.cutoff;
         const float rc2_hi = (float)rc2 * (1.f + 0x1p-22f);
-        float vrc_lo[2] = {0.f, 0.f};
-        for (int f = 0; f < A.fac.n_factors; f++) {
-            double v = fabs(A.fac.prefactor[f]) / cr_pow(rc2, A.fac.power[f] / 2.0);
-            vrc_lo[f] = (float)v * (1.f - 0x1p-20f);
+        const int nf = A.fac.n_factors;
+        float vrc_lo[2] = {0.f, 0.f}, kf[2] = {0.f, 0.f};
+        int pf[2] = {12, 12};
+        double urc[2] = {0.0, 0.0};
+        for (int f = 0; f < nf; f++) {
+            urc[f] = A.fac.prefactor[f] / cr_pow(rc2, A.fac.power[f] / 2.0);
+            vrc_lo[f] = (float)fabs(urc[f]) * (1.f - 0x1p-20f);
+            kf[f] = (float)A.fac.prefactor[f];
+            pf[f] = (int)A.fac.power[f];
         }
+        const float inv_beta_lo = (float)(1.0 / A.beta) * (1.f - 0x1p-22f);
         bsync();
         for (int leg_i = 0; leg_i < A.legs && !R.err; leg_i++) {
             const int a = R.active, d = R.direction;
@@ -337,83 +446,111 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
             BKey best = {INFINITY, 3, 0x7fffffff, 0};
             double bestpos[3] = {0.0, 0.0, 0.0};
             unsigned long long geo_pass = 0;
+            // contenders found by this thread whose FP64 position is held in registers
+            int c0 = -1, c1 = -1;
+            double c0x = 0, c0y = 0, c0z = 0, c1x = 0, c1y = 0, c1z = 0;
             if (A.mode != 1) {
-                // ---- pass 1: FP32 lower bounds
-                for (int idx = tid; idx < M; idx += NTH) {
-                    const int j = candidate_unit(idx, cps, a, s_pref, s_nearby, cell_l);
-                    float sv[3] = {px[j] - apf[0], py[j] - apf[1], pz[j] - apf[2]};
-                    bool ambiguous = false;
-#pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        if (sv[q] >= 0.5f * Lf) sv[q] -= Lf;
-                        else if (sv[q] < -0.5f * Lf) sv[q] += Lf;
-                        if (fabsf(fabsf(sv[q]) - 0.5f * Lf) <= 2.f * E) ambiguous = true;
-                    }
+                // ---- pass 1b: bounds of the queued candidates of this wavefront
+                auto bound_queue = [&](int cnt) {
+                    if (lane >= cnt) return;
+                    const int j = queue[lane];
                     Geo g;
-                    g.sd_lo = sv[d] - E;
-                    g.sd_hi = sv[d] + E;
-                    float p2lo = 0.f, p2hi = 0.f;
-#pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        if (q == d) continue;
-                        float m = fabsf(sv[q]);
-                        float lo = fmaxf(m - E, 0.f), hi = m + E;
-                        p2lo += lo * lo;
-                        p2hi += hi * hi;
-                    }
-                    g.perp2_lo = dn(p2lo);
-                    g.perp2_hi = up(p2hi);
-                    float mdl = fmaxf(fabsf(sv[d]) - E, 0.f), mdh = fabsf(sv[d]) + E;
-                    g.r2_lo = dn(p2lo + mdl * mdl);
-                    g.r2_hi = up(p2hi + mdh * mdh);
-                    float tl[2] = {0.f, 0.f};
-                    bool any = false;
-                    for (int f = 0; f < A.fac.n_factors; f++) {
-                        float k = (float)A.fac.prefactor[f];
-                        float fl = ambiguous ? 0.f
-                                             : (g.perp2_lo >= rc2_hi ? __int_as_float(0x7f800000)
-                                                                      : factor_geometric_floor(g, k, rc2_hi));
-                        tl[f] = fl;
-                        if (fl <= hf) any = true;
-                    }
-                    if (!any) continue;
-                    geo_pass++;
+                    const bool amb = cand_geometry(px[j] - apf[0], py[j] - apf[1], pz[j] - apf[2], d, Lf, E, g);
                     uint32_t w[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, w);
-                    for (int f = 0; f < A.fac.n_factors; f++) {
-                        if (tl[f] > hf) continue;
+                    for (int f = 0; f < nf; f++) {
                         float t = 0.f;
-                        if (!ambiguous) {
-                            double u = unit53(w[2 * f], w[2 * f + 1]);
-                            // -log(1-u)/beta with a 2^-40 downward margin over the FP64 log
-                            double dU = -log(1.0 - u) / A.beta;
-                            float dU_lo = (float)(dU * (1.0 - 0x1p-40)) * (1.f - 0x1p-22f);
-                            t = factor_lower_bound(g, (float)A.fac.prefactor[f], (int)A.fac.power[f], rc2_hi, vrc_lo[f],
-                                                   dU_lo);
+                        if (!amb) {
+                            if (g.perp2_lo >= rc2_hi) continue;
+                            if (factor_geometric_floor(g, kf[f], rc2_hi) > hf) continue;
+                            t = factor_lower_bound(g, kf[f], pf[f], rc2_hi, vrc_lo[f], dU_lower(w[2 * f], inv_beta_lo));
                         }
                         if (t <= hf) {
-                            int slot = atomicAdd(&s_nc, 1);
-                            if (slot < B_CONTENDERS) cont_id[slot] = (j << 1) | f;
-                            else s_overflow = 1;
+                            const int slot = atomicAdd(&s_nc, 1);
+                            if (slot >= B_CONTENDERS) {
+                                s_overflow = 1;
+                                continue;
+                            }
+                            cont_id[slot] = (j << 1) | f;
+                            cont_w[2 * slot] = w[2 * f];
+                            cont_w[2 * slot + 1] = w[2 * f + 1];
+                            if (c0 < 0) {
+                                c0 = slot;
+                                c0x = P[j]; c0y = P[(size_t)n + j]; c0z = P[2 * (size_t)n + j];
+                            } else if (c1 < 0) {
+                                c1 = slot;
+                                c1x = P[j]; c1y = P[(size_t)n + j]; c1z = P[2 * (size_t)n + j];
+                            } else {
+                                cont_pos[3 * slot] = P[j];
+                                cont_pos[3 * slot + 1] = P[(size_t)n + j];
+                                cont_pos[3 * slot + 2] = P[2 * (size_t)n + j];
+                            }
                         }
                     }
+                };
+                // ---- pass 1a: geometric test of every candidate, survivors queued
+                int qn = 0;
+                for (int base = wave * 64; base < M; base += NTH) {
+                    const int idx = base + lane;
+                    bool surv = false;
+                    int j = 0;
+                    if (idx < M) {
+                        j = candidate_unit(idx, cps, a, s_pref, s_nearby, cell_l);
+                        Geo g;
+                        const bool amb = cand_geometry(px[j] - apf[0], py[j] - apf[1], pz[j] - apf[2], d, Lf, E, g);
+                        if (amb) {
+                            surv = true;
+                        } else if (g.perp2_lo < rc2_hi) {
+                            for (int f = 0; f < nf; f++)
+                                if (factor_geometric_floor(g, kf[f], rc2_hi) <= hf) surv = true;
+                        }
+                    }
+                    const unsigned long long m = __ballot(surv);
+                    const int before = __popcll(m & ((1ull << lane) - 1ull));
+                    if (surv) queue[qn + before] = j;
+                    qn += __popcll(m);
+                    if (qn >= 64) {
+                        bound_queue(64);
+                        const int rem = qn - 64;
+                        const int v = lane < rem ? queue[64 + lane] : 0;
+                        if (lane < rem) queue[lane] = v;
+                        qn = rem;
+                    }
+                    geo_pass += surv ? 1ull : 0ull;
                 }
+                if (qn > 0) bound_queue(qn);
+                // held positions to LDS, for the thread that evaluates each contender
+                if (c0 >= 0) { cont_pos[3 * c0] = c0x; cont_pos[3 * c0 + 1] = c0y; cont_pos[3 * c0 + 2] = c0z; }
+                if (c1 >= 0) { cont_pos[3 * c1] = c1x; cont_pos[3 * c1 + 1] = c1y; cont_pos[3 * c1 + 2] = c1z; }
                 bsync();
             }
             const bool all_exact = (A.mode == 1) || s_overflow;
             // ---- pass 2: exact FP64 inversion of the contenders (or of every candidate)
-            const int nwork = all_exact ? M * A.fac.n_factors : min(s_nc, B_CONTENDERS);
+            const int nwork = all_exact ? M * nf : min(s_nc, B_CONTENDERS);
             for (int w = tid; w < nwork; w += NTH) {
                 int j, f;
+                double pj[3];
+                uint32_t wa, wb;
                 if (all_exact) {
-                    f = w % A.fac.n_factors;
-                    j = candidate_unit(w / A.fac.n_factors, cps, a, s_pref, s_nearby, cell_l);
+                    f = w % nf;
+                    j = candidate_unit(w / nf, cps, a, s_pref, s_nearby, cell_l);
+                    pj[0] = P[j]; pj[1] = P[(size_t)n + j]; pj[2] = P[2 * (size_t)n + j];
+                    uint32_t ww[4];
+                    philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
+                    wa = ww[2 * f];
+                    wb = ww[2 * f + 1];
                 } else {
                     j = cont_id[w] >> 1;
                     f = cont_id[w] & 1;
+                    pj[0] = cont_pos[3 * w]; pj[1] = cont_pos[3 * w + 1]; pj[2] = cont_pos[3 * w + 2];
+                    wa = cont_w[2 * w];
+                    wb = cont_w[2 * w + 1];
                 }
-                double pj[3];
-                double dx = exact_displacement(A, R, ap, d, j, f, leg, (uint32_t)r, pj);
+                const double u = unit53(wa, wb);
+                const double dU = -cr_log(1.0 - u) / A.beta;
+                const double dx = exact_ipt(bmin_image(pj[0] - ap[0], L), bmin_image(pj[1] - ap[1], L),
+                                            bmin_image(pj[2] - ap[2], L), d, A.fac.power[f], A.fac.prefactor[f],
+                                            rc2, urc[f], dU);
                 BKey c = {dx, 0, j, f};
                 if (bkey_lt(c, best)) {
                     best = c;
@@ -422,10 +559,10 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
             }
             BKey mine = best;
             if (tid == 0) {
-                BKey c1 = {s_cb, 1, 0, 0};
-                if (bkey_lt(c1, best)) best = c1;
-                BKey c2 = {R.chain_left, 2, 0, 0};
-                if (bkey_lt(c2, best)) best = c2;
+                BKey c1k = {s_cb, 1, 0, 0};
+                if (bkey_lt(c1k, best)) best = c1k;
+                BKey c2k = {R.chain_left, 2, 0, 0};
+                if (bkey_lt(c2k, best)) best = c2k;
             }
             block_min<WPR>(best, sk);
             if (best.rank == 0 && bkey_eq(mine, best)) {
@@ -433,11 +570,15 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
             }
             if (A.mode == 2) {
                 // audit: a candidate whose exact displacement is <= h must be a contender
-                for (int idx = tid; idx < M * A.fac.n_factors; idx += NTH) {
-                    int f = idx % A.fac.n_factors;
-                    int j = candidate_unit(idx / A.fac.n_factors, cps, a, s_pref, s_nearby, cell_l);
-                    double pj[3];
-                    double dx = exact_displacement(A, R, ap, d, j, f, leg, (uint32_t)r, pj);
+                for (int idx = tid; idx < M * nf; idx += NTH) {
+                    const int f = idx % nf;
+                    const int j = candidate_unit(idx / nf, cps, a, s_pref, s_nearby, cell_l);
+                    uint32_t ww[4];
+                    philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
+                    const double dU = -cr_log(1.0 - unit53(ww[2 * f], ww[2 * f + 1])) / A.beta;
+                    const double dx = exact_ipt(bmin_image(P[j] - ap[0], L), bmin_image(P[(size_t)n + j] - ap[1], L),
+                                                bmin_image(P[2 * (size_t)n + j] - ap[2], L), d, A.fac.power[f],
+                                                A.fac.prefactor[f], rc2, urc[f], dU);
                     if (dx <= h) {
                         bool found = false;
                         for (int q = 0; q < min(s_nc, B_CONTENDERS); q++)

@@ -182,6 +182,47 @@ extern "C" int ecmc_copy_bytes(const void *d_src, void *d_dst, int64_t bytes, vo
     return hipGetLastError() == hipSuccess ? ECMC_OK : ecmc_fail(ECMC_ERR_HIP, "copy kernel launch failed");
 }
 
+// Largest error of the FP32 log2f (absolute below 1, relative above: |err| / max(|log2 x|, 1))
+// or exp2f (relative) that the batched kernel's FP32 bounds rely on, against FP64 log2 /
+// exp2, over every float whose bit pattern lies in [lo, hi).  The result is a non-negative double, so its bit pattern orders like an unsigned
+// integer and one atomicMax per wavefront combines the lanes.
+__global__ __launch_bounds__(PT) void fp32_math_error_kernel(int fn, uint32_t lo, uint32_t hi,
+                                                             unsigned long long *out) {
+    double worst = 0.0;
+    for (uint64_t b = (uint64_t)lo + (uint64_t)blockIdx.x * PT + threadIdx.x; b < hi;
+         b += (uint64_t)gridDim.x * PT) {
+        const float x = __uint_as_float((uint32_t)b);
+        double err;
+        if (fn == 0) {
+            const double got = (double)log2f(x), ref = log2((double)x);
+            err = fabs(got - ref) / fmax(fabs(ref), 1.0);
+        } else {
+            const double got = (double)exp2f(x), ref = exp2((double)x);
+            err = fabs(got - ref) / ref;
+        }
+        if (!(err <= worst)) worst = err;      // NaN propagates as the worst case
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(worst, off, 64);
+        if (!(o <= worst)) worst = o;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        unsigned long long bits = __double_as_longlong(worst);
+        if (worst != worst) bits = 0x7ff8000000000000ull;
+        atomicMax(out, bits);
+    }
+}
+
+extern "C" int ecmc_fp32_math_error(int32_t fn, uint32_t bits_lo, uint32_t bits_hi, unsigned long long *d_out,
+                                    void *stream) {
+    if ((fn != 0 && fn != 1) || bits_hi < bits_lo || !d_out) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (bits_hi == bits_lo) return ECMC_OK;
+    hipLaunchKernelGGL(fp32_math_error_kernel, dim3(4096), dim3(PT), 0, (hipStream_t)stream, fn, bits_lo, bits_hi,
+                       d_out);
+    return hipGetLastError() == hipSuccess ? ECMC_OK : ecmc_fail(ECMC_ERR_HIP, "math check launch failed");
+}
+
 extern "C" int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream) {
     if (n < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
     if (n == 0) return ECMC_OK;
