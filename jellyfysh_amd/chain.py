@@ -13,12 +13,16 @@ from . import _lib as L
 
 _KINDS = {"pair_cells": L.TAG_PAIR_CELLS, "pair_all": L.TAG_PAIR_ALL, "cell_boundary": L.TAG_CELL_BOUNDARY,
           "sampling": L.TAG_SAMPLING, "end_of_chain": L.TAG_END_OF_CHAIN, "end_of_run": L.TAG_END_OF_RUN,
-          "start_of_run": L.TAG_START_OF_RUN}
+          "start_of_run": L.TAG_START_OF_RUN, "surplus": L.TAG_SURPLUS, "cell_bounding": L.TAG_CELL_BOUNDING}
+_PAIR_KINDS = ("pair_cells", "pair_all", "surplus", "cell_bounding")
+# InnerPointEstimator defaults (inner_point_estimator.py:45-46): prefactor, points_per_side,
+# target_charge, empirical_bound
+CELL_BOUNDING_DEFAULTS = (1.5, 10.0, 1.0, float("inf"))
 _POTENTIALS = {"hard_sphere": L.POT_HARD_SPHERE, "inverse_power": L.POT_INVERSE_POWER,
                "lennard_jones": L.POT_LENNARD_JONES, "inverse_power_truncated": L.POT_INVERSE_POWER_TRUNC,
                "merged_image_coulomb": L.POT_MERGED_IMAGE_COULOMB,
                "inverse_power_coulomb_bounding": L.POT_INVERSE_POWER_COULOMB_BOUND,
-               "displaced_even_power": L.POT_DISPLACED_EVEN_POWER}
+               "displaced_even_power": L.POT_DISPLACED_EVEN_POWER, "cell_bounding": L.POT_CELL_BOUNDING}
 
 
 def potential_code(name):
@@ -63,12 +67,21 @@ def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, schedu
                 field[j] = index[tag]
             setattr(g, count, len(lst))
         g.n_handlers = int(t.get("number_event_handlers", n))
-        if t["kind"] in ("pair_cells", "pair_all"):
-            g.event_handler = L.EH_TWO_LEAF_BOUNDING if t.get("bounding_potential") else L.EH_TWO_LEAF
+        if t["kind"] in _PAIR_KINDS:
+            if t["kind"] == "cell_bounding":
+                g.event_handler = L.EH_CELL_BOUNDING
+            else:
+                g.event_handler = L.EH_TWO_LEAF_BOUNDING if t.get("bounding_potential") else L.EH_TWO_LEAF
             g.potential = potential_code(t["potential"])
             for j, v in enumerate(t.get("params", ())):
                 g.par[j] = float(v)
-            if t.get("bounding_potential"):
+            if t["kind"] == "cell_bounding":
+                g.bounding_potential = L.POT_CELL_BOUNDING
+                bp = list(t.get("bounding_params", ()))
+                bp = bp + list(CELL_BOUNDING_DEFAULTS[len(bp):])
+                for j, v in enumerate(bp):
+                    g.bpar[j] = float(v)
+            elif t.get("bounding_potential"):
                 g.bounding_potential = potential_code(t["bounding_potential"])
                 for j, v in enumerate(t.get("bounding_params", ())):
                     g.bpar[j] = float(v)

@@ -9,9 +9,12 @@ SingleActiveCellOccupancy over CuboidPeriodicCells, the heap or list scheduler a
 taggers / event handlers / potentials:
 
   taggers        excluded_cells_tagger, factor_type_map_in_state_tagger (every pair),
+                 surplus_cells_tagger, cell_bounding_potential_tagger,
                  cell_boundary_tagger, no_in_state_tagger, active_global_state_in_state_tagger
   handlers       two_leaf_unit_event_handler, two_leaf_unit_bounding_potential_event_handler,
-                 cell_boundary_event_handler, fixed_interval_sampling_event_handler,
+                 two_leaf_unit_cell_bounding_potential_event_handler (cell_bounding_potential
+                 with an inner_point_estimator), cell_boundary_event_handler,
+                 fixed_interval_sampling_event_handler,
                  single_independent_active_periodic_direction_end_of_chain_event_handler,
                  final_time_end_of_run_event_handler, initial_chain_start_of_run_event_handler
   potentials     hard_sphere_potential, inverse_power_potential, lennard_jones_potential,
@@ -88,6 +91,7 @@ _POTENTIALS = {
 _HANDLER_OPTIONS = {
     "two_leaf_unit_event_handler": {"potential"},
     "two_leaf_unit_bounding_potential_event_handler": {"potential", "bounding_potential", "charge"},
+    "two_leaf_unit_cell_bounding_potential_event_handler": {"potential", "bounding_potential", "charge"},
     "cell_boundary_event_handler": set(),
     "fixed_interval_sampling_event_handler": {"sampling_interval", "output_handler", "first_event_time_zero"},
     "single_independent_active_periodic_direction_end_of_chain_event_handler": {"chain_time"},
@@ -139,7 +143,8 @@ class RunConfig:
             if ccls != "cuboid_periodic_cells":
                 raise ConfigurationError("unsupported cells %s" % ccls)
             cs = self._section(camel(calias), {"cells_per_side", "neighbor_layers"})
-            self.cells_per_side = int(cs["cells_per_side"])
+            cps = [int(v) for v in split_list(cs["cells_per_side"])]
+            self.cells_per_side = cps[0] if len(cps) == 1 else cps
             self.neighbor_layers = int(cs.get("neighbor_layers", 1))
         self.taggers = []
         for item in split_list(ta["taggers"]):
@@ -181,6 +186,23 @@ class RunConfig:
                 params.append(float(default))
         return name, params
 
+    def _cell_bounding(self, value):
+        """cell_bounding_potential (estimator = inner_point_estimator) -> the C ABI's bpar:
+        estimator prefactor, points_per_side, target_charge, empirical_bound."""
+        alias, cls = split_alias(value)
+        if cls != "cell_bounding_potential":
+            raise ConfigurationError("the cell-bounding handler needs cell_bounding_potential (got %s)" % cls)
+        sec = self._section(camel(alias), {"estimator"})
+        ealias, ecls = split_alias(sec.get("estimator", "inner_point_estimator"))
+        if ecls != "inner_point_estimator":
+            raise ConfigurationError("only the inner_point_estimator is supported (got %s)" % ecls)
+        est = self._section(camel(ealias), {"prefactor", "potential", "target_charge", "points_per_side",
+                                            "empirical_bound", "periodic_boundaries"})
+        if est.get("periodic_boundaries", "true").lower() not in ("1", "yes", "true", "on"):
+            raise ConfigurationError("the estimator corrects separations for periodic boundaries here")
+        return [float(est.get("prefactor", 1.5)), float(est.get("points_per_side", 10)),
+                float(est.get("target_charge", 1.0)), float(est.get("empirical_bound", "inf"))]
+
     def _tagger(self, label, cls):
         sec = self._section(camel(label), _TAGGER_OPTIONS)
         t = {"tag": label, "create": split_list(sec.get("create", "")), "trash": split_list(sec.get("trash", ""))}
@@ -190,13 +212,22 @@ class RunConfig:
         if hcls not in _HANDLER_OPTIONS:
             raise ConfigurationError("event handler %s has no device implementation" % hcls)
         h = self._section(camel(halias), _HANDLER_OPTIONS[hcls])
-        if hcls in ("two_leaf_unit_event_handler", "two_leaf_unit_bounding_potential_event_handler"):
+        if hcls == "two_leaf_unit_cell_bounding_potential_event_handler":
+            if cls != "cell_bounding_potential_tagger":
+                raise ConfigurationError("the cell-bounding handler runs under cell_bounding_potential_tagger")
+            t["kind"] = "cell_bounding"
+            t["potential"], t["params"] = self._potential(h["potential"])
+            t["bounding_params"] = self._cell_bounding(h["bounding_potential"])
+            t["charge"] = "charge" in h
+        elif hcls in ("two_leaf_unit_event_handler", "two_leaf_unit_bounding_potential_event_handler"):
             if cls == "excluded_cells_tagger":
                 t["kind"] = "pair_cells"
             elif cls == "factor_type_map_in_state_tagger":
                 t["kind"] = "pair_all"
+            elif cls == "surplus_cells_tagger":
+                t["kind"] = "surplus"
             else:
-                raise ConfigurationError("pair handlers run under excluded_cells_tagger or "
+                raise ConfigurationError("pair handlers run under excluded_cells_tagger, surplus_cells_tagger or "
                                          "factor_type_map_in_state_tagger (got %s)" % cls)
             t["potential"], t["params"] = self._potential(h["potential"])
             if hcls == "two_leaf_unit_bounding_potential_event_handler":

@@ -50,6 +50,7 @@ struct Slot {
     double cb_boundary;
     int32_t cb_dir, adir;
     double aspeed;
+    int32_t cb_cell, pad_cb;   // active cell of a cell-bounding candidate
 };
 
 struct ChainState {
@@ -65,6 +66,7 @@ struct ChainState {
     double est[8];
     int32_t ncand;
     uint32_t chash;
+    int32_t nsur, pad_s;       // surplus units of a bounded occupancy
     Slot slot[ECMC_MAX_TAGGERS];
 };
 
@@ -89,6 +91,11 @@ struct ChainArgs {
     ecmc_event *log;
     int64_t max_events;
     int64_t *n_out;
+    // bounded occupancy (single_active_cell_occupancy.py:62) and cell bounding
+    int32_t max_occ, sur_cap, layers, pad2;
+    int32_t *sur_cell, *sur_id;   // surplus units in the order the reference's dict yields them
+    int32_t *cand;                // candidate list of the cell-bounding tagger
+    const double *cbt;            // cell-bound table: (cell offset, direction) -> upper, lower
 };
 
 __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
@@ -201,6 +208,39 @@ __device__ __forceinline__ bool pot_draws(const ecmc_tagger_desc &tg) {
     return tg.potential != ECMC_POT_HARD_SPHERE;
 }
 
+// Bounding event rate of a cell-bounding candidate: the InnerPointEstimator bound of the
+// target cell's offset from the active cell along dir, upper or lower by the sign of the
+// charge correction factor (cell_bounding_potential.py:161-201).
+template <int D>
+__device__ double cell_rate(const ChainArgs &a, const ecmc_tagger_desc &tg, int acell, int target, int dir, double ca,
+                            double ct) {
+    double tp[D];
+    load_pos<D>(a, target, tp);
+    const int tcell = pos_to_cell<D>(a, tp);
+    int rel = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const int ti = (tcell / a.cum[d]) % a.cps[d], ai = (acell / a.cum[d]) % a.cps[d];
+        rel += ((ti - ai) % a.cps[d] + a.cps[d]) % a.cps[d] * a.cum[d];
+    }
+    const double *row = a.cbt + (size_t)(rel * 3 + dir) * 2;
+    if (tg.use_charge) {
+        const double cf = ca * ct / tg.bpar[2];
+        return cf > 0.0 ? row[0] * cf : row[1] * cf;
+    }
+    return row[0];
+}
+
+// Is cell c among the nearby cells of cell ac (within `layers` per axis, periodic)?
+__device__ __forceinline__ bool is_nearby(const ChainArgs &a, int dim, int c, int ac) {
+    for (int d = 0; d < dim; d++) {
+        const int ci = (c / a.cum[d]) % a.cps[d], ai = (ac / a.cum[d]) % a.cps[d];
+        const int off = ((ci - ai) % a.cps[d] + a.cps[d]) % a.cps[d];
+        if (off > a.layers && off < a.cps[d] - a.layers) return false;
+    }
+    return true;
+}
+
 // Candidate time displacement of one pair (send_event_time of the pair event handlers).
 template <int D>
 __device__ double pair_dt(const ChainArgs &a, const ecmc_tagger_desc &tg, int dir, double speed, const double *s,
@@ -270,7 +310,7 @@ __device__ void block_reduce(Best &b, uint32_t &h, Best *sb, uint32_t *sh) {
 
 template <int D>
 __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &tg, int tix,
-                            int *nb_cell, int *nb_pref, Best *sb, uint32_t *sh) {
+                            int *nb_cell, int *nb_pref, Best *sb, uint32_t *sh, int *scan) {
     const int tid = threadIdx.x;
     if (tid == 0) {
         int nc = 0;
@@ -294,7 +334,41 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
     __syncthreads();
     const int nc = nb_cell[31];
     const bool all = tg.kind == ECMC_TAG_PAIR_ALL;
-    const int M = all ? a.n - 1 : nb_pref[nc];
+    const bool sur = tg.kind == ECMC_TAG_SURPLUS;
+    const bool far = tg.kind == ECMC_TAG_CELL_BOUNDING;
+    if (far) {
+        // cell_bounding_potential_tagger.py:150-155: the occupied cells that are not nearby the
+        // active cell, in cell order; each threads scans a contiguous run of cells, an
+        // exclusive scan of the counts places its candidates
+        const int ac = st.occ_cell;
+        const int chunk = (a.ncells + NT - 1) / NT;
+        const int lo = min(tid * chunk, a.ncells), hi = min(lo + chunk, a.ncells);
+        int cnt = 0;
+        for (int c = lo; c < hi; c++)
+            if (a.occ_n[c] > 0 && !is_nearby(a, D, c, ac)) cnt++;
+        scan[tid] = cnt;
+        __syncthreads();
+        if (tid == 0) {
+            int acc = 0;
+            for (int i = 0; i < NT; i++) {
+                const int v = scan[i];
+                scan[i] = acc;
+                acc += v;
+            }
+            scan[NT] = acc;
+        }
+        __syncthreads();
+        int k = scan[tid];
+        bool multi = false;
+        for (int c = lo; c < hi; c++)
+            if (a.occ_n[c] > 0 && !is_nearby(a, D, c, ac)) {
+                if (a.occ_n[c] != 1) multi = true;
+                a.cand[k++] = a.occ[(size_t)c * a.occ_cap];
+            }
+        if (multi) st.err = ECMC_ERR_STATE;   // the handler takes exactly one target unit
+        __syncthreads();
+    }
+    const int M = all ? a.n - 1 : (sur ? st.nsur : (far ? scan[NT] : nb_pref[nc]));
     const bool draws = pot_draws(tg);
     ring_ensure(st, mt, a, draws ? 2ull * (uint64_t)M : 0ull);
     const uint64_t base = st.cursor;
@@ -311,6 +385,10 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
         int target;
         if (all) {
             target = j < act ? j : j + 1;
+        } else if (sur) {
+            target = a.sur_id[j];
+        } else if (far) {
+            target = a.cand[j];
         } else {
             int c = 0;
             while (nb_pref[c + 1] <= j) c++;
@@ -325,7 +403,14 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
             dU = -cr_log(1.0 - u) / a.beta;
         }
         const double ct = tg.use_charge ? a.charge[target] : 1.0;
-        double dt = pair_dt<D>(a, tg, dir, speed, s, ca, ct, dU);
+        double dt;
+        if (tg.event_handler == ECMC_EH_CELL_BOUNDING) {
+            // two_leaf_unit_cell_bounding_potential_event_handler.py:171-177
+            const double rate = cell_rate<D>(a, tg, st.occ_cell, target, dir, ca, ct);
+            dt = (rate > 0 ? dU / rate : INFINITY) / speed;
+        } else {
+            dt = pair_dt<D>(a, tg, dir, speed, s, ca, ct, dU);
+        }
         Time t = time_add(at, dt);
         Best c = {t.q, t.r, j};
         if (isnan(dt)) c.q = -INFINITY;  // flags a ValueError of the reference code
@@ -346,12 +431,17 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
             int j = best.j, target;
             if (all) {
                 target = j < act ? j : j + 1;
+            } else if (sur) {
+                target = a.sur_id[j];
+            } else if (far) {
+                target = a.cand[j];
             } else {
                 int c = 0;
                 while (nb_pref[c + 1] <= j) c++;
                 target = a.occ[(size_t)nb_cell[c] * a.occ_cap + (j - nb_pref[c])];
             }
             sl.valid = 1;
+            sl.cb_cell = st.occ_cell;
             sl.t.q = best.q;
             sl.t.r = best.r;
             sl.seq = st.seq + j;
@@ -373,9 +463,35 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
     __syncthreads();
 }
 
+// self._surplus.setdefault(cell, []).append(id) on the flat ordered list: join the cell's
+// group or open a new group at the end (dict insertion order; empty groups are deleted).
+__device__ bool surplus_append(const ChainArgs &a, ChainState &st, int cell, int id) {
+    if (st.nsur >= a.sur_cap) return false;
+    int at = st.nsur;
+    for (int i = st.nsur - 1; i >= 0; i--)
+        if (a.sur_cell[i] == cell) { at = i + 1; break; }
+    for (int i = st.nsur; i > at; i--) { a.sur_cell[i] = a.sur_cell[i - 1]; a.sur_id[i] = a.sur_id[i - 1]; }
+    a.sur_cell[at] = cell;
+    a.sur_id[at] = id;
+    st.nsur++;
+    return true;
+}
+__device__ bool surplus_remove(const ChainArgs &a, ChainState &st, int cell, int id) {
+    for (int i = 0; i < st.nsur; i++)
+        if (a.sur_cell[i] == cell && a.sur_id[i] == id) {
+            for (int k = i; k + 1 < st.nsur; k++) { a.sur_cell[k] = a.sur_cell[k + 1]; a.sur_id[k] = a.sur_id[k + 1]; }
+            st.nsur--;
+            return true;
+        }
+    return false;
+}
+
 template <int D>
 __device__ void occupancy_update(const ChainArgs &a, ChainState &st) {
-    // single_active_cell_occupancy.py:149-203 (unbounded occupant lists)
+    // single_active_cell_occupancy.py:149-203: the unit that stops being active joins its
+    // cell's occupants, or the surplus when the cell is full; the new active unit leaves the
+    // occupants or the surplus (the surplus never moves into the occupants: that branch of
+    // the reference needs an empty surplus list, which it deletes right away)
     int id = st.active;
     double p[D];
     load_pos<D>(a, id, p);
@@ -383,9 +499,14 @@ __device__ void occupancy_update(const ChainArgs &a, ChainState &st) {
         if (st.occ_id >= 0) {
             int c = st.occ_cell;
             int n = a.occ_n[c];
-            if (n >= a.occ_cap) { st.err = ECMC_ERR_STATE; return; }
-            a.occ[(size_t)c * a.occ_cap + n] = st.occ_id;
-            a.occ_n[c] = n + 1;
+            if (a.max_occ <= 0 || n < a.max_occ) {
+                if (n >= a.occ_cap) { st.err = ECMC_ERR_STATE; return; }
+                a.occ[(size_t)c * a.occ_cap + n] = st.occ_id;
+                a.occ_n[c] = n + 1;
+            } else if (!surplus_append(a, st, c, st.occ_id)) {
+                st.err = ECMC_ERR_STATE;
+                return;
+            }
         }
         int c = pos_to_cell<D>(a, p);
         if (c < 0) { st.err = ECMC_ERR_STATE; return; }
@@ -395,7 +516,10 @@ __device__ void occupancy_update(const ChainArgs &a, ChainState &st) {
         int32_t *L = a.occ + (size_t)c * a.occ_cap;
         int i = 0;
         while (i < n && L[i] != id) i++;
-        if (i == n) { st.err = ECMC_ERR_STATE; return; }
+        if (i == n) {
+            if (!surplus_remove(a, st, c, id)) st.err = ECMC_ERR_STATE;
+            return;
+        }
         for (; i + 1 < n; i++) L[i] = L[i + 1];
         a.occ_n[c] = n - 1;
     } else {
@@ -495,6 +619,7 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
     __shared__ int nb_pref[33];
     __shared__ Best sb[NT / 64];
     __shared__ uint32_t sh[NT / 64];
+    __shared__ int scan[NT + 1];
     __shared__ int flag;
     const int tid = threadIdx.x;
     if (tid == 0) st = *a.st;
@@ -530,8 +655,9 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
             for (int ci = 0; ci < w.n_create && !st.err; ci++) {
                 const int t = w.create[ci];
                 const ecmc_tagger_desc &c = tg[t];
-                if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL) {
-                    create_pair<D>(a, st, mt, c, t, nb_cell, nb_pref, sb, sh);
+                if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL || c.kind == ECMC_TAG_SURPLUS ||
+                    c.kind == ECMC_TAG_CELL_BOUNDING) {
+                    create_pair<D>(a, st, mt, c, t, nb_cell, nb_pref, sb, sh, scan);
                 } else if (c.kind == ECMC_TAG_END_OF_CHAIN) {
                     if (tid == 0) st.slot[t].valid = 0;
                     __syncthreads();
@@ -572,8 +698,10 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
         __syncthreads();
         if (st.err) break;
         const int w = flag;
-        const bool thinning = (tg[w].kind == ECMC_TAG_PAIR_CELLS || tg[w].kind == ECMC_TAG_PAIR_ALL) &&
-                              tg[w].event_handler == ECMC_EH_TWO_LEAF_BOUNDING;
+        const bool thinning = (tg[w].kind == ECMC_TAG_PAIR_CELLS || tg[w].kind == ECMC_TAG_PAIR_ALL ||
+                               tg[w].kind == ECMC_TAG_SURPLUS || tg[w].kind == ECMC_TAG_CELL_BOUNDING) &&
+                              (tg[w].event_handler == ECMC_EH_TWO_LEAF_BOUNDING ||
+                               tg[w].event_handler == ECMC_EH_CELL_BOUNDING);
         if (thinning) ring_ensure(st, mt, a, 2);
         if (tid == 0) {
             Slot &sl = st.slot[w];
@@ -591,12 +719,32 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
             double p[D];
             switch (T.kind) {
             case ECMC_TAG_PAIR_CELLS:
-            case ECMC_TAG_PAIR_ALL: {
+            case ECMC_TAG_PAIR_ALL:
+            case ECMC_TAG_SURPLUS:
+            case ECMC_TAG_CELL_BOUNDING: {
                 ev.target = sl.target;
                 bool lift = true;
                 double s[D];
                 separation_vector<D>(a, sl.apos, sl.tpos, s);
-                if (T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
+                if (T.event_handler == ECMC_EH_CELL_BOUNDING) {
+                    // two_leaf_unit_cell_bounding_potential_event_handler.py:181-211: the active
+                    // unit is still in its cell; bound = the candidate's rate; thinning as below
+                    if (pos_to_cell<D>(a, sl.apos) != sl.cb_cell) { st.err = ECMC_ERR_STATE; break; }
+                    double q3[3] = {0, 0, 0};
+                    for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
+                    const double c1 = T.use_charge ? a.charge[act] : 1.0;
+                    const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
+                    double bound = cell_rate<D>(a, T, sl.cb_cell, sl.target, sl.adir, c1, c2) * sl.aspeed;
+                    double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+                    lift = false;
+                    if (real > 0) {
+                        double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
+                        st.cursor += 2;
+                        u = 0 + (bound - 0) * u;
+                        if (u < real) lift = true;
+                    }
+                    if (!lift) st.est[6] += 1.0;
+                } else if (T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
                     // event_handler_with_bounding_potential.py:75-101
                     double q3[3] = {0, 0, 0};
                     for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
@@ -693,6 +841,56 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
     for (int i = tid; i < 624; i += NT) a.mt[i] = mt[i];
 }
 
+// InnerPointEstimator.derivative_bound (inner_point_estimator.py:108-171) for the cell
+// separations of cell_bounding_potential.py:100-126: one workgroup per (cell, direction)
+// evaluates the merged-image Coulomb derivative on the (pps + 1)^3 grid spanning the corners
+// (corners[6c .. 6c+6] = lower[3], upper[3]; NaN marks a nearby separation).
+__global__ __launch_bounds__(NT) void cell_bound_kernel(Ewald ew, const double *corners, double L, double pot_pref,
+                                                        double target_charge, int pps, double est_pref, double emp,
+                                                        double *out) {
+    __shared__ double smax[NT], smin[NT];
+    const int c = blockIdx.x / 3, dir = blockIdx.x % 3, t = threadIdx.x;
+    const double *cr = corners + (size_t)c * 6;
+    double *o = out + (size_t)blockIdx.x * 2;
+    if (isnan(cr[0])) {
+        if (t == 0) { o[0] = NAN; o[1] = NAN; }
+        return;
+    }
+    const int np = pps + 1, total = np * np * np;
+    const double half = L / 2.0;
+    double mx = -INFINITY, mn = INFINITY;
+    for (int k = t; k < total; k += NT) {
+        const int ix = k / (np * np), iy = (k / np) % np, iz = k % np;
+        double sep[3];
+        sep[0] = cr[0] + (cr[3] - cr[0]) * ix / pps;
+        sep[1] = cr[1] + (cr[4] - cr[1]) * iy / pps;
+        sep[2] = cr[2] + (cr[5] - cr[2]) * iz / pps;
+#pragma unroll
+        for (int d = 0; d < 3; d++) sep[d] = py_mod(sep[d] + half, L) - half;   // correct_separation
+        double der = pot_pref * 1.0 * target_charge * ewald_derivative(ew, sep[dir], sep[(dir + 1) % 3], sep[(dir + 2) % 3]);
+        der = der * 1.0;
+        if (der > mx) mx = der;
+        if (der < mn) mn = der;
+    }
+    smax[t] = mx;
+    smin[t] = mn;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            if (smax[t + w] > smax[t]) smax[t] = smax[t + w];
+            if (smin[t + w] < smin[t]) smin[t] = smin[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        double upper = smax[0], lower = smin[0];
+        upper = upper > 0.0 ? upper * est_pref : upper / est_pref;
+        lower = lower > 0.0 ? lower / est_pref : lower * est_pref;
+        o[0] = upper < emp ? upper : emp;
+        o[1] = lower > -emp ? lower : -emp;
+    }
+}
+
 // ---- host side --------------------------------------------------------------------------
 static void mt_seed_host(uint64_t seed, uint32_t *mt) {
     uint32_t key[2] = {(uint32_t)(seed & 0xffffffffu), (uint32_t)(seed >> 32)};
@@ -724,6 +922,8 @@ struct ecmc_system {
     std::vector<double> cmin;
     double *d_pos = nullptr, *d_charge = nullptr, *d_cmin = nullptr, *d_ewald = nullptr;
     int32_t *d_nearby = nullptr, *d_nearby_n = nullptr, *d_occ = nullptr, *d_occ_n = nullptr;
+    int32_t *d_sur_cell = nullptr, *d_sur_id = nullptr, *d_cand = nullptr;
+    double *d_cbt = nullptr, *d_corners = nullptr;
     uint32_t *d_mt = nullptr, *d_ring = nullptr;
     ChainState *d_st = nullptr;
     ecmc_tagger_desc *d_tg = nullptr;
@@ -752,14 +952,37 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     if (n < 2) return ecmc_fail(ECMC_ERR_CONFIGURATION, "at least two units are required");
     if (desc->n_taggers < 1 || desc->n_taggers > ECMC_MAX_TAGGERS)
         return ecmc_fail(ECMC_ERR_CONFIGURATION, "number of taggers out of range");
-    if (desc->max_occupants > 0)
-        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the device chain supports unbounded cell occupancy only");
     if (desc->neighbor_layers < 0 || desc->neighbor_layers > 1)
         return ecmc_fail(ECMC_ERR_CONFIGURATION, "neighbor_layers must be 0 or 1");
-    int ewald_tagger = -1;
+    int ewald_tagger = -1, cb_tagger = -1;
     for (int t = 0; t < desc->n_taggers; t++) {
         const ecmc_tagger_desc &tg = desc->taggers[t];
-        if (tg.kind < 0 || tg.kind > ECMC_TAG_START_OF_RUN) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
+        if (tg.kind < 0 || tg.kind > ECMC_TAG_CELL_BOUNDING) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
+        if (tg.kind == ECMC_TAG_SURPLUS && desc->max_occupants <= 0)
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "the surplus tagger needs a bounded cell occupancy");
+        if ((tg.kind == ECMC_TAG_CELL_BOUNDING) != (tg.event_handler == ECMC_EH_CELL_BOUNDING))
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "the cell-bounding handler runs under the cell-bounding tagger only");
+        if (tg.kind == ECMC_TAG_CELL_BOUNDING) {
+            if (tg.potential != ECMC_POT_MERGED_IMAGE_COULOMB || tg.bounding_potential != ECMC_POT_CELL_BOUNDING || dim != 3)
+                return ecmc_fail(ECMC_ERR_CONFIGURATION, "cell bounding needs merged-image Coulomb with the cell bounding potential in 3D");
+            if (desc->max_occupants != 1)
+                return ecmc_fail(ECMC_ERR_CONFIGURATION, "cell bounding needs maximum_number_occupants = 1");
+            if (!(tg.bpar[0] > 0.0) || tg.bpar[1] < 1.0 || tg.bpar[1] > 64.0 || (tg.use_charge && tg.bpar[2] == 0.0))
+                return ecmc_fail(ECMC_ERR_CONFIGURATION, "estimator prefactor, points_per_side or target_charge out of range");
+            if (cb_tagger >= 0) {
+                const ecmc_tagger_desc &o = desc->taggers[cb_tagger];
+                for (int q = 0; q < 4; q++)
+                    if (o.bpar[q] != tg.bpar[q] || o.par[q] != tg.par[q])
+                        return ecmc_fail(ECMC_ERR_CONFIGURATION, "one cell-bounding parameter set per system");
+            }
+            cb_tagger = t;
+            if (ewald_tagger >= 0) {
+                const ecmc_tagger_desc &o = desc->taggers[ewald_tagger];
+                if (o.par[0] != tg.par[0] || o.par[1] != tg.par[1] || o.par[2] != tg.par[2])
+                    return ecmc_fail(ECMC_ERR_CONFIGURATION, "one merged-image Coulomb parameter set per system");
+            }
+            ewald_tagger = t;
+        }
         if (tg.n_create < 0 || tg.n_create > ECMC_MAX_TAGGERS || tg.n_trash < 0 || tg.n_trash > ECMC_MAX_TAGGERS)
             return ecmc_fail(ECMC_ERR_CONFIGURATION, "create/trash list length out of range");
         if (tg.kind == ECMC_TAG_START_OF_RUN &&
@@ -773,7 +996,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
             if (tg.create[i] < 0 || tg.create[i] >= desc->n_taggers) return ecmc_fail(ECMC_ERR_CONFIGURATION, "create tag out of range");
         for (int i = 0; i < tg.n_trash; i++)
             if (tg.trash[i] < 0 || tg.trash[i] >= desc->n_taggers) return ecmc_fail(ECMC_ERR_CONFIGURATION, "trash tag out of range");
-        if (tg.kind == ECMC_TAG_PAIR_CELLS || tg.kind == ECMC_TAG_PAIR_ALL) {
+        if (tg.kind == ECMC_TAG_PAIR_CELLS || tg.kind == ECMC_TAG_PAIR_ALL || tg.kind == ECMC_TAG_SURPLUS) {
             if (tg.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
                 if (tg.potential != ECMC_POT_MERGED_IMAGE_COULOMB || tg.bounding_potential != ECMC_POT_INVERSE_POWER_COULOMB_BOUND || dim != 3)
                     return ecmc_fail(ECMC_ERR_CONFIGURATION, "bounding handler needs merged-image Coulomb with the inverse-power Coulomb bound in 3D");
@@ -804,10 +1027,19 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     s->nb_stride = grid.nb_stride;
     // initial occupancy in global-state order (single_active_cell_occupancy.py:111-121)
     std::vector<std::vector<int32_t>> lists(grid.count);
+    std::vector<int32_t> sur_cell, sur_id;     // the surplus dict, flattened in its yield order
     for (int i = 0; i < n; i++) {
         int c = grid.position_to_cell(positions + (size_t)i * dim);
         if (c < 0) { delete s; return ecmc_fail(ECMC_ERR_STATE, "initial position outside of the cell system"); }
-        lists[c].push_back(i);
+        if (desc->max_occupants <= 0 || (int)lists[c].size() < desc->max_occupants) {
+            lists[c].push_back(i);
+        } else {
+            size_t at = sur_cell.size();
+            for (size_t k = sur_cell.size(); k-- > 0;)
+                if (sur_cell[k] == c) { at = k + 1; break; }
+            sur_cell.insert(sur_cell.begin() + at, c);
+            sur_id.insert(sur_id.begin() + at, i);
+        }
     }
     size_t maxocc = 0;
     for (auto &l : lists) maxocc = l.size() > maxocc ? l.size() : maxocc;
@@ -839,6 +1071,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     st.occ_cell = -1;
     st.last.q = -INFINITY;
     st.last.r = -INFINITY;
+    st.nsur = (int32_t)sur_cell.size();
     for (int t = 0; t < desc->n_taggers; t++) {
         st.slot[t].valid = 0;
         const ecmc_tagger_desc &tg = desc->taggers[t];
@@ -880,6 +1113,13 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     ECMC_ALLOC(s->d_tg, sizeof(ecmc_tagger_desc) * ECMC_MAX_TAGGERS);
     ECMC_ALLOC(s->d_nout, sizeof(int64_t));
     if (!ewald.empty()) { ECMC_ALLOC(s->d_ewald, sizeof(double) * ewald.size()); }
+    ECMC_ALLOC(s->d_sur_cell, sizeof(int32_t) * n);
+    ECMC_ALLOC(s->d_sur_id, sizeof(int32_t) * n);
+    ECMC_ALLOC(s->d_cand, sizeof(int32_t) * n);
+    if (!sur_cell.empty()) {
+        ECMC_UP(s->d_sur_cell, sur_cell.data(), sizeof(int32_t) * sur_cell.size());
+        ECMC_UP(s->d_sur_id, sur_id.data(), sizeof(int32_t) * sur_id.size());
+    }
     ECMC_UP(s->d_pos, pos_soa.data(), sizeof(double) * dim * n);
     ECMC_UP(s->d_charge, ch.data(), sizeof(double) * n);
     ECMC_UP(s->d_cmin, grid.cmin.data(), sizeof(double) * grid.cmin.size());
@@ -925,6 +1165,40 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     a.tg = s->d_tg;
     a.ewald.f = s->d_ewald;
     a.n_out = s->d_nout;
+    a.max_occ = desc->max_occupants;
+    a.sur_cap = n;
+    a.layers = desc->neighbor_layers;
+    a.sur_cell = s->d_sur_cell;
+    a.sur_id = s->d_sur_id;
+    a.cand = s->d_cand;
+    if (cb_tagger >= 0) {
+        // cell-bound tables on the device (cell_bounding_potential.py:100-126)
+        const ecmc_tagger_desc &tg = desc->taggers[cb_tagger];
+        std::vector<double> corners((size_t)grid.count * 6);
+        const int32_t *near0 = grid.nearby.data();
+        for (int c = 0; c < grid.count; c++) {
+            bool is_near = false;
+            for (int k = 0; k < grid.nearby_n[0]; k++) if (near0[k] == c) is_near = true;
+            for (int d = 0; d < 3; d++) {
+                corners[(size_t)c * 6 + d] = is_near ? NAN : grid.cmin[(size_t)c * dim + d] - grid.cmax[d];
+                corners[(size_t)c * 6 + 3 + d] = is_near ? NAN : grid.cmax[(size_t)c * dim + d] - grid.cmin[d];
+            }
+        }
+        if (hipMalloc((void **)&s->d_corners, sizeof(double) * corners.size()) != hipSuccess ||
+            hipMalloc((void **)&s->d_cbt, sizeof(double) * grid.count * 6) != hipSuccess ||
+            hipMemcpy(s->d_corners, corners.data(), sizeof(double) * corners.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            ecmc_system_destroy(s);
+            return ecmc_fail(ECMC_ERR_HIP, "cell-bound table allocation failed");
+        }
+        hipLaunchKernelGGL(cell_bound_kernel, dim3(grid.count * 3), dim3(NT), 0, s->stream, a.ewald, s->d_corners,
+                           desc->system_length, tg.par[3], tg.bpar[2], (int)tg.bpar[1], tg.bpar[0], tg.bpar[3],
+                           s->d_cbt);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
+            ecmc_system_destroy(s);
+            return ecmc_fail(ECMC_ERR_HIP, "cell-bound table kernel failed");
+        }
+        a.cbt = s->d_cbt;
+    }
     *out = s;
     return ECMC_OK;
 }
@@ -933,6 +1207,7 @@ extern "C" void ecmc_system_destroy(ecmc_system *s) {
     if (!s) return;
     hipFree(s->d_pos); hipFree(s->d_charge); hipFree(s->d_cmin); hipFree(s->d_nearby); hipFree(s->d_nearby_n);
     hipFree(s->d_occ); hipFree(s->d_occ_n); hipFree(s->d_mt); hipFree(s->d_ring); hipFree(s->d_st);
+    hipFree(s->d_sur_cell); hipFree(s->d_sur_id); hipFree(s->d_cand); hipFree(s->d_cbt); hipFree(s->d_corners);
     hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);

@@ -110,6 +110,18 @@ int ora_cells_position_to_cell(const ora_cells *c, const double *position);
 int ora_cells_nearby(const ora_cells *c, int cell, int *out); /* canonical order, returns count */
 int ora_cells_neighbor(const ora_cells *c, int cell, int direction, int positive);
 
+/* ---- cell bounding potential (cell_bounding_potential.py:70-126 with
+ * inner_point_estimator.py:108-171) ---------------------------------------------------------
+ * For every cell c not among the nearby cells of the zero cell and every direction d:
+ * out[(c * 3 + d) * 2] = upper and out[(c * 3 + d) * 2 + 1] = lower bound of the derivative
+ * prefactor * 1.0 * target_charge * ewald_derivative over the (pps + 1)^3 grid spanning
+ * [c.min - zero.max, c.max - zero.min] (separations corrected for periodic boundaries),
+ * scaled by the estimator prefactor and clipped by the empirical bound; NaN for nearby
+ * cells.  3-D only. */
+void ora_cell_bound_table(const ora_cells *cells, const ora_ewald *ewald, double potential_prefactor,
+                          double target_charge, int points_per_side, double estimator_prefactor,
+                          double empirical_bound, double system_length, double *out);
+
 /* ---- the chain engine ---------------------------------------------------------------- */
 enum {
     ORA_TAG_PAIR_CELLS = 0,   /* ExcludedCellsTagger -> pair event handler */
@@ -118,9 +130,14 @@ enum {
     ORA_TAG_SAMPLING = 3,
     ORA_TAG_END_OF_CHAIN = 4,
     ORA_TAG_END_OF_RUN = 5,
-    ORA_TAG_START_OF_RUN = 6
+    ORA_TAG_START_OF_RUN = 6,
+    ORA_TAG_SURPLUS = 7,      /* SurplusCellsTagger: the surplus units in dict order -> pair handler */
+    ORA_TAG_CELL_BOUNDING = 8 /* CellBoundingPotentialTagger: occupants of the non-nearby cells */
 };
-enum { ORA_EH_TWO_LEAF = 0, ORA_EH_TWO_LEAF_BOUNDING = 1 };
+/* ORA_EH_CELL_BOUNDING: TwoLeafUnitCellBoundingPotentialEventHandler with a merged-image
+ * Coulomb potential (par) and a CellBoundingPotential from an InnerPointEstimator
+ * (bpar: estimator prefactor, points_per_side, target_charge, empirical_bound). */
+enum { ORA_EH_TWO_LEAF = 0, ORA_EH_TWO_LEAF_BOUNDING = 1, ORA_EH_CELL_BOUNDING = 2 };
 enum {
     ORA_POT_HARD_SPHERE = 0,      /* par: radius */
     ORA_POT_INVERSE_POWER = 1,    /* par: power, prefactor */

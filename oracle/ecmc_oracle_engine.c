@@ -108,6 +108,63 @@ int ora_cells_neighbor(const ora_cells *c, int cell, int direction, int positive
     return (int)(cell + (long)(nid - id) * c->cum[direction]);
 }
 
+/* cuboid_periodic_cells.py:156-180: the cell whose offset from the zero cell equals that of
+ * `cell` from `ref` (cell centres, periodic), i.e. the per-axis identifier difference. */
+static int cells_relative(const ora_cells *c, int cell, int ref) {
+    int out = 0;
+    for (int d = 0; d < c->dim; d++) {
+        int a = (int)((cell / c->cum[d]) % c->cps[d]), b = (int)((ref / c->cum[d]) % c->cps[d]);
+        out += (int)((((a - b) % c->cps[d]) + c->cps[d]) % c->cps[d] * c->cum[d]);
+    }
+    return out;
+}
+
+/* InnerPointEstimator.derivative_bound (inner_point_estimator.py:108-171) for every cell
+ * separation that is not nearby the zero cell (cell_bounding_potential.py:100-126). */
+void ora_cell_bound_table(const ora_cells *cells, const ora_ewald *ewald, double potential_prefactor,
+                          double target_charge, int pps, double est_prefactor, double empirical_bound,
+                          double system_length, double *out) {
+    const int nc = cells->count, dim = cells->dim;
+    const double half = system_length / 2.0;
+    int near[27 * 27];
+    const int nn = ora_cells_nearby(cells, 0, near);
+    for (int c = 0; c < nc; c++) {
+        int is_near = 0;
+        for (int k = 0; k < nn; k++) if (near[k] == c) is_near = 1;
+        for (int d = 0; d < 3; d++) { out[(c * 3 + d) * 2] = NAN; out[(c * 3 + d) * 2 + 1] = NAN; }
+        if (is_near) continue;
+        double lo[3], hi[3];
+        for (int d = 0; d < dim; d++) {
+            lo[d] = cells->cmin[(size_t)c * dim + d] - cells->cmax[d];
+            hi[d] = cells->cmax[(size_t)c * dim + d] - cells->cmin[d];
+        }
+        for (int dir = 0; dir < dim; dir++) {
+            double upper = -INFINITY, lower = INFINITY;
+            for (int ix = 0; ix <= pps; ix++) {
+                double px = lo[0] + (hi[0] - lo[0]) * ix / pps;
+                for (int iy = 0; iy <= pps; iy++) {
+                    double py = lo[1] + (hi[1] - lo[1]) * iy / pps;
+                    for (int iz = 0; iz <= pps; iz++) {
+                        double pz = lo[2] + (hi[2] - lo[2]) * iz / pps;
+                        double sep[3] = {px, py, pz};
+                        for (int d = 0; d < 3; d++) sep[d] = ora_py_mod(sep[d] + half, system_length) - half;
+                        double p[3];
+                        for (int d = 0; d < 3; d++) p[d] = sep[(dir + d) % 3];       /* permutation_3d */
+                        double der = potential_prefactor * 1.0 * target_charge * ora_ewald_derivative(ewald, p[0], p[1], p[2]);
+                        der = der * 1.0;                                                  /* times the speed */
+                        if (der > upper) upper = der;
+                        if (der < lower) lower = der;
+                    }
+                }
+            }
+            upper = upper > 0.0 ? upper * est_prefactor : upper / est_prefactor;
+            lower = lower > 0.0 ? lower / est_prefactor : lower * est_prefactor;
+            out[(c * 3 + dir) * 2] = upper < empirical_bound ? upper : empirical_bound;
+            out[(c * 3 + dir) * 2 + 1] = lower > -empirical_bound ? lower : -empirical_bound;
+        }
+    }
+}
+
 /* ===== small growable int list ========================================================= */
 typedef struct { int *v; int n, cap; } ilist;
 static void il_push(ilist *l, int x) {
@@ -123,6 +180,8 @@ static int il_remove(ilist *l, int x) { /* list.remove: first occurrence */
 /* ===== handlers ======================================================================== */
 typedef struct {
     int tagger;
+    double cb_rate;         /* bounding event rate of a cell-bounding candidate */
+    int cb_cell;            /* active cell when that candidate was created */
     int pending;           /* event currently stored in the scheduler */
     double tq, tr;
     int target;            /* pair target or new active (end of chain) */
@@ -147,7 +206,9 @@ struct ora_engine {
     int active; int adir; double aspeed; double aq, ar;   /* lifting state */
     ora_cells *cells;
     ilist *occ;             /* per cell */
-    ilist *surplus;         /* per cell */
+    int *sur_cell, *sur_id; /* surplus units in the order the reference's dict yields them */
+    int nsur, capsur;
+    double *cbt[ORA_MAX_TAGGERS]; /* cell-bound tables of cell-bounding taggers */
     int occ_active_id, occ_active_cell;
     ora_mt rng;
     handler *h; int nh;
@@ -249,9 +310,33 @@ static void sched_trash(ora_engine *e, int hidx) {
 
 /* --- occupancy (single_active_cell_occupancy.py:95-203) ------------------------------- */
 static int occ_bounded(const ora_engine *e) { return e->cfg.max_occupants > 0; }
+/* self._surplus.setdefault(cell, []).append(id): join the cell's group, or open a new group
+ * at the end (dict insertion order; a group that empties is deleted). */
+static void sur_append(ora_engine *e, int cell, int id) {
+    if (e->nsur == e->capsur) {
+        e->capsur = e->capsur ? 2 * e->capsur : 64;
+        e->sur_cell = (int *)realloc(e->sur_cell, sizeof(int) * e->capsur);
+        e->sur_id = (int *)realloc(e->sur_id, sizeof(int) * e->capsur);
+    }
+    int at = e->nsur;
+    for (int i = e->nsur - 1; i >= 0; i--) if (e->sur_cell[i] == cell) { at = i + 1; break; }
+    memmove(e->sur_cell + at + 1, e->sur_cell + at, sizeof(int) * (e->nsur - at));
+    memmove(e->sur_id + at + 1, e->sur_id + at, sizeof(int) * (e->nsur - at));
+    e->sur_cell[at] = cell; e->sur_id[at] = id; e->nsur++;
+}
+static int sur_remove(ora_engine *e, int cell, int id) {
+    for (int i = 0; i < e->nsur; i++)
+        if (e->sur_cell[i] == cell && e->sur_id[i] == id) {
+            memmove(e->sur_cell + i, e->sur_cell + i + 1, sizeof(int) * (e->nsur - i - 1));
+            memmove(e->sur_id + i, e->sur_id + i + 1, sizeof(int) * (e->nsur - i - 1));
+            e->nsur--;
+            return 1;
+        }
+    return 0;
+}
 static void occ_insert(ora_engine *e, int cell, int id) {
     if (!occ_bounded(e) || e->occ[cell].n < e->cfg.max_occupants) il_push(&e->occ[cell], id);
-    else il_push(&e->surplus[cell], id);
+    else sur_append(e, cell, id);
 }
 static void occ_update(ora_engine *e) {
     int id = e->active;
@@ -263,7 +348,7 @@ static void occ_update(ora_engine *e) {
         if (!il_remove(&e->occ[e->occ_active_cell], id)) {
             /* the move from the surplus list into the occupant list never triggers in the
              * reference (the surplus dict never holds an empty list), so only remove */
-            if (!il_remove(&e->surplus[e->occ_active_cell], id)) eng_fail(e, "active unit not found in occupancy");
+            if (!sur_remove(e, e->occ_active_cell, id)) eng_fail(e, "active unit not found in occupancy");
         }
     } else {
         e->occ_active_cell = ora_cells_position_to_cell(e->cells, e->pos + (size_t)id * e->dim);
@@ -313,6 +398,26 @@ static double pair_displacement(ora_engine *e, const ora_tagger *tg, int tidx, c
         default: eng_fail(e, "unsupported potential for TwoLeafUnitEventHandler"); return INFINITY;
         }
     }
+    if (tg->event_handler == ORA_EH_CELL_BOUNDING) {
+        /* two_leaf_unit_cell_bounding_potential_event_handler.py:159-179 with
+         * cell_bounding_potential.py:161-221 */
+        double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
+        handler *hd = &e->h[tidx];
+        int tcell = ora_cells_position_to_cell(e->cells, e->pos + (size_t)t * e->dim);
+        int rel = cells_relative(e->cells, tcell, e->occ_active_cell);
+        const double *row = e->cbt[hd->tagger] + (size_t)(rel * 3 + dir) * 2;
+        double rate;
+        if (tg->use_charge) {
+            double cf = c1 * c2 / tg->bpar[2];                  /* charge_correction_factor */
+            rate = cf > 0.0 ? row[0] * cf : row[1] * cf;
+        } else {
+            rate = row[0];
+        }
+        hd->cb_rate = rate;
+        hd->cb_cell = e->occ_active_cell;
+        double disp = rate > 0 ? dU / rate : INFINITY;
+        return disp / speed;
+    }
     if (tg->event_handler == ORA_EH_TWO_LEAF_BOUNDING && tg->bounding_potential == ORA_POT_IPCB) {
         /* two_leaf_unit_bounding_potential_event_handler.py:138-143 */
         double p[3];
@@ -321,7 +426,6 @@ static double pair_displacement(ora_engine *e, const ora_tagger *tg, int tidx, c
         double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
         return ora_ipcb_displacement(pp, p[0], p[1], p[2], dU, e->L) / speed;
     }
-    (void)tidx;
     eng_fail(e, "unsupported event handler");
     return INFINITY;
 }
@@ -340,46 +444,65 @@ static void create_tagger(ora_engine *e, int t) {
     int a = e->active;
     switch (tg->kind) {
     case ORA_TAG_PAIR_CELLS:
-    case ORA_TAG_PAIR_ALL: {
+    case ORA_TAG_PAIR_ALL:
+    case ORA_TAG_SURPLUS:
+    case ORA_TAG_CELL_BOUNDING: {
         if (a < 0) return;
+        /* the candidate targets in the order the tagger yields them:
+         * excluded_cells_tagger.py:129-132 (nearby cells, occupants in list order),
+         * surplus_cells_tagger.py:128-131 (the surplus dict's order),
+         * cell_bounding_potential_tagger.py:150-155 (cells in yield_cells() order that are
+         * occupied and not nearby), or every other unit in identifier order */
+        int *targets = (int *)malloc(sizeof(int) * (size_t)(e->n + 1));
+        int nt = 0;
         int cells[27 * 27];
         int ncells = 0;
-        if (tg->kind == ORA_TAG_PAIR_CELLS) {
-            if (e->occ_active_cell < 0) return;
+        if (tg->kind == ORA_TAG_PAIR_CELLS || tg->kind == ORA_TAG_CELL_BOUNDING) {
+            if (e->occ_active_cell < 0) { free(targets); return; }
             ncells = ora_cells_nearby(e->cells, e->occ_active_cell, cells);
         }
-        int total = tg->kind == ORA_TAG_PAIR_CELLS ? 0 : e->n - 1;
-        if (tg->kind == ORA_TAG_PAIR_CELLS)
-            for (int c = 0; c < ncells; c++) total += e->occ[cells[c]].n;
-        int k = 0;
-        for (int c = 0; (tg->kind == ORA_TAG_PAIR_CELLS ? c < ncells : c < 1); c++) {
-            int cnt = tg->kind == ORA_TAG_PAIR_CELLS ? e->occ[cells[c]].n : e->n;
-            for (int o = 0; o < cnt; o++) {
-                int tid = tg->kind == ORA_TAG_PAIR_CELLS ? e->occ[cells[c]].v[o] : o;
-                if (tid == a) continue;
-                int hidx = pool_pop(e, t);
-                if (hidx < 0) return;
-                handler *hd = &e->h[hidx];
-                hd->aid = a; hd->adir = e->adir; hd->aspeed = e->aspeed; hd->aq = e->aq; hd->ar = e->ar;
-                memcpy(hd->apos, e->pos + (size_t)a * e->dim, sizeof(double) * e->dim);
-                memcpy(hd->tpos, e->pos + (size_t)tid * e->dim, sizeof(double) * e->dim);
-                hd->target = tid;
-                double s[3];
-                separation_vector(e, hd->apos, hd->tpos, s);
-                double dt = pair_displacement(e, tg, t, s, a, tid);
-                double q, r;
-                ora_time_add(e->aq, e->ar, dt, &q, &r);
-                /* send_event_time time-slices the in-state copies to the event time */
-                if (!isinf(q)) slice_pos(e, hd->apos, e->adir, e->aspeed, e->aq, e->ar, q, r);
-                hd->aq = q; hd->ar = r;
-                sched_push(e, hidx, q, r);
-                if (e->ncand == e->capcand) { e->capcand = e->capcand ? 2 * e->capcand : 1024; e->cand = (int32_t *)realloc(e->cand, sizeof(int32_t) * e->capcand); }
-                e->chash += cand_mix((uint32_t)e->ncand, (uint32_t)tid);
-                e->cand[e->ncand++] = tid;
-                k++;
+        if (tg->kind == ORA_TAG_PAIR_CELLS) {
+            for (int c = 0; c < ncells; c++)
+                for (int o = 0; o < e->occ[cells[c]].n; o++) targets[nt++] = e->occ[cells[c]].v[o];
+        } else if (tg->kind == ORA_TAG_PAIR_ALL) {
+            for (int o = 0; o < e->n; o++) targets[nt++] = o;
+        } else if (tg->kind == ORA_TAG_SURPLUS) {
+            for (int o = 0; o < e->nsur; o++) targets[nt++] = e->sur_id[o];
+        } else {
+            int nc = ora_cells_count(e->cells);
+            for (int c = 0; c < nc; c++) {
+                if (e->occ[c].n == 0) continue;
+                int near = 0;
+                for (int k = 0; k < ncells; k++) if (cells[k] == c) near = 1;
+                if (near) continue;
+                if (e->occ[c].n != 1) { free(targets); eng_fail(e, "cell bounding needs at most one occupant per cell"); return; }
+                targets[nt++] = e->occ[c].v[0];
             }
         }
-        (void)total;
+        for (int k = 0; k < nt; k++) {
+            int tid = targets[k];
+            if (tid == a) continue;
+            int hidx = pool_pop(e, t);
+            if (hidx < 0) { free(targets); return; }
+            handler *hd = &e->h[hidx];
+            hd->aid = a; hd->adir = e->adir; hd->aspeed = e->aspeed; hd->aq = e->aq; hd->ar = e->ar;
+            memcpy(hd->apos, e->pos + (size_t)a * e->dim, sizeof(double) * e->dim);
+            memcpy(hd->tpos, e->pos + (size_t)tid * e->dim, sizeof(double) * e->dim);
+            hd->target = tid;
+            double s[3];
+            separation_vector(e, hd->apos, hd->tpos, s);
+            double dt = pair_displacement(e, tg, hidx, s, a, tid);
+            double q, r;
+            ora_time_add(e->aq, e->ar, dt, &q, &r);
+            /* send_event_time time-slices the in-state copies to the event time */
+            if (!isinf(q)) slice_pos(e, hd->apos, e->adir, e->aspeed, e->aq, e->ar, q, r);
+            hd->aq = q; hd->ar = r;
+            sched_push(e, hidx, q, r);
+            if (e->ncand == e->capcand) { e->capcand = e->capcand ? 2 * e->capcand : 1024; e->cand = (int32_t *)realloc(e->cand, sizeof(int32_t) * e->capcand); }
+            e->chash += cand_mix((uint32_t)e->ncand, (uint32_t)tid);
+            e->cand[e->ncand++] = tid;
+        }
+        free(targets);
         break;
     }
     case ORA_TAG_CELL_BOUNDARY: {
@@ -461,10 +584,31 @@ static int out_state(ora_engine *e, int hidx, ora_event *ev) {
     int a = e->active;
     switch (tg->kind) {
     case ORA_TAG_PAIR_CELLS:
-    case ORA_TAG_PAIR_ALL: {
+    case ORA_TAG_PAIR_ALL:
+    case ORA_TAG_SURPLUS:
+    case ORA_TAG_CELL_BOUNDING: {
         ev->target = hd->target;
         int lift = 1;
-        if (tg->event_handler == ORA_EH_TWO_LEAF_BOUNDING) {
+        if (tg->event_handler == ORA_EH_CELL_BOUNDING) {
+            /* two_leaf_unit_cell_bounding_potential_event_handler.py:181-211: confirm the
+             * active cell, bound = the candidate's rate, then thin against the real derivative */
+            if (ora_cells_position_to_cell(e->cells, hd->apos) != hd->cb_cell) {
+                eng_fail(e, "cell-bounding event after the active unit left its cell");
+                return -1;
+            }
+            double s[3], p[3];
+            separation_vector(e, hd->apos, hd->tpos, s);
+            for (int d = 0; d < 3; d++) p[d] = s[(hd->adir + d) % 3];
+            double c1 = tg->use_charge ? e->charge[hd->aid] : 1.0;
+            double c2 = tg->use_charge ? e->charge[hd->target] : 1.0;
+            double bound = hd->cb_rate * hd->aspeed;
+            double real = tg->par[3] * c1 * c2 * ora_ewald_derivative(e->ewald[hd->tagger], p[0], p[1], p[2]) * hd->aspeed;
+            lift = 0;
+            if (real > 0) {
+                double u = ora_uniform(&e->rng, 0, bound);
+                if (u < real) lift = 1;
+            }
+        } else if (tg->event_handler == ORA_EH_TWO_LEAF_BOUNDING) {
             /* two_leaf_unit_bounding_potential_event_handler.py:148-168 and
              * event_handler_with_bounding_potential.py:75-101 */
             double s[3], p[3];
@@ -537,6 +681,11 @@ static int out_state(ora_engine *e, int hidx, ora_event *ev) {
 ora_engine *ora_engine_create(const ora_config *cfg, const double *positions, const double *charges) {
     ora_engine *e = (ora_engine *)calloc(1, sizeof(ora_engine));
     e->cfg = *cfg;
+    if (cfg->n_taggers < 1 || cfg->n_taggers > ORA_MAX_TAGGERS) {
+        e->cfg.n_taggers = 0;
+        eng_fail(e, "number of taggers out of range");
+        return e;
+    }
     e->dim = cfg->dim; e->n = cfg->n; e->L = cfg->system_length; e->half_L = cfg->system_length / 2.0;
     e->pos = (double *)malloc(sizeof(double) * e->n * e->dim);
     memcpy(e->pos, positions, sizeof(double) * e->n * e->dim);
@@ -549,7 +698,6 @@ ora_engine *ora_engine_create(const ora_config *cfg, const double *positions, co
     e->cells = ora_cells_create(e->dim, e->L, cps, cfg->neighbor_layers);
     int nc = ora_cells_count(e->cells);
     e->occ = (ilist *)calloc(nc, sizeof(ilist));
-    e->surplus = (ilist *)calloc(nc, sizeof(ilist));
     e->occ_active_id = -1; e->occ_active_cell = -1;
     for (int i = 0; i < e->n; i++) {
         int c = ora_cells_position_to_cell(e->cells, e->pos + (size_t)i * e->dim);
@@ -575,6 +723,16 @@ ora_engine *ora_engine_create(const ora_config *cfg, const double *positions, co
         if (cfg->taggers[t].potential == ORA_POT_MERGED_IMAGE_COULOMB)
             e->ewald[t] = ora_ewald_create((int)cfg->taggers[t].par[1], (int)cfg->taggers[t].par[2],
                                            cfg->taggers[t].par[0], e->L);
+        if (cfg->taggers[t].event_handler == ORA_EH_CELL_BOUNDING) {
+            const ora_tagger *tg = &cfg->taggers[t];
+            if (e->dim != 3 || !e->ewald[t] || cfg->max_occupants != 1) {
+                eng_fail(e, "cell bounding needs 3-D merged-image Coulomb and one occupant per cell");
+                return e;
+            }
+            e->cbt[t] = (double *)malloc(sizeof(double) * (size_t)nc * 6);
+            ora_cell_bound_table(e->cells, e->ewald[t], tg->par[3], tg->bpar[2], (int)tg->bpar[1], tg->bpar[0],
+                                 tg->bpar[3], e->L, e->cbt[t]);
+        }
     }
     return e;
 }
@@ -582,9 +740,11 @@ ora_engine *ora_engine_create(const ora_config *cfg, const double *positions, co
 void ora_engine_destroy(ora_engine *e) {
     if (!e) return;
     int nc = ora_cells_count(e->cells);
-    for (int c = 0; c < nc; c++) { free(e->occ[c].v); free(e->surplus[c].v); }
-    for (int t = 0; t < e->cfg.n_taggers; t++) { free(e->running[t].v); free(e->idle[t].v); ora_ewald_destroy(e->ewald[t]); }
-    free(e->occ); free(e->surplus); free(e->running); free(e->idle);
+    for (int c = 0; c < nc; c++) free(e->occ[c].v);
+    for (int t = 0; t < e->cfg.n_taggers; t++) {
+        free(e->running[t].v); free(e->idle[t].v); ora_ewald_destroy(e->ewald[t]); free(e->cbt[t]);
+    }
+    free(e->occ); free(e->sur_cell); free(e->sur_id); free(e->running); free(e->idle);
     ora_cells_destroy(e->cells);
     free(e->pos); free(e->charge); free(e->h); free(e->lst); free(e->heap); free(e->cand);
     free(e);

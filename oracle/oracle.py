@@ -18,7 +18,8 @@ _REF_DIR = os.path.join(_HERE, "_ref")
 MATH_LIBM, MATH_CR = 0, 1
 TAG_PAIR_CELLS, TAG_PAIR_ALL, TAG_CELL_BOUNDARY, TAG_SAMPLING, TAG_END_OF_CHAIN, TAG_END_OF_RUN, \
     TAG_START_OF_RUN = range(7)
-EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING = 0, 1
+EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING, EH_CELL_BOUNDING = 0, 1, 2
+TAG_SURPLUS, TAG_CELL_BOUNDING = 7, 8
 POT_HARD_SPHERE, POT_INVERSE_POWER, POT_LENNARD_JONES, POT_INVERSE_POWER_TRUNC, POT_MERGED_IMAGE_COULOMB, \
     POT_IPCB = range(6)
 SCHED_LIST, SCHED_HEAP = 0, 1
@@ -83,6 +84,7 @@ def lib():
             "ora_cells_count": (i32, [vp]), "ora_cells_bounds": (None, [vp, i32, pd, pd]),
             "ora_cells_position_to_cell": (i32, [vp, pd]), "ora_cells_nearby": (i32, [vp, i32, ctypes.POINTER(i32)]),
             "ora_cells_neighbor": (i32, [vp, i32, i32, i32]),
+            "ora_cell_bound_table": (None, [vp, vp, d, d, i32, d, d, d, pd]),
             "ora_engine_create": (vp, [vp, pd, pd]), "ora_engine_destroy": (None, [vp]),
             "ora_engine_run": (i64, [vp, i64, vp]), "ora_engine_done": (i32, [vp]),
             "ora_engine_error": (i32, [vp, ctypes.c_char_p, i32]), "ora_engine_positions": (None, [vp, pd]),
@@ -162,6 +164,20 @@ class Cells:
 
     def neighbor(self, cell, direction, positive):
         return lib().ora_cells_neighbor(self._h, cell, direction, int(positive))
+
+    def cell_bound_table(self, system_length, alpha, fourier_cutoff, position_cutoff, prefactor=1.0,
+                         target_charge=1.0, points_per_side=10, estimator_prefactor=1.5,
+                         empirical_bound=float("inf")):
+        """InnerPointEstimator bounds of the merged-image Coulomb derivative for every cell
+        separation: array (cells, 3 directions, [upper, lower]), NaN for nearby separations."""
+        h = lib().ora_ewald_create(int(fourier_cutoff), int(position_cutoff), alpha, system_length)
+        try:
+            out = np.zeros(self.count() * 6)
+            lib().ora_cell_bound_table(self._h, h, prefactor, target_charge, int(points_per_side),
+                                       estimator_prefactor, empirical_bound, system_length, _dp(out))
+        finally:
+            lib().ora_ewald_destroy(h)
+        return out.reshape(-1, 3, 2)
 
 
 class Engine:
