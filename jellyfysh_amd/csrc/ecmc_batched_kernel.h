@@ -56,10 +56,9 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
                                               uint32_t k1, uint32_t out[4]) {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
-        uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
-        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c0 = n0; c1 = (uint32_t)p1; c2 = n2; c3 = (uint32_t)p0;
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
     }
@@ -199,6 +198,11 @@ __device__ void block_min(BKey &b, BKey *sk) {
 
 __device__ __forceinline__ void bsync() { __syncthreads(); }
 
+// Factor-indexed values held in registers (a runtime index into a private array would put
+// the array in scratch memory).
+template <class T>
+__device__ __forceinline__ T pick(int f, T v0, T v1) { return f ? v1 : v0; }
+
 // The device log2f is within LOG2F_ERR * max(|log2 x|, 1) of log2 x for every positive float:
 // an exhaustive scan (ecmc_fp32_math_error) measures 2^-23 and the tests require <= 2^-21.
 constexpr float LOG2F_ERR = 0x1p-20f;
@@ -227,7 +231,13 @@ __device__ __forceinline__ float dU_lower(uint32_t a, float inv_beta_lo) {
 // (and so the oracle), arranged so that lanes of the repulsive and of the attractive
 // factor share the calls of the correctly rounded pow.  urc = k / rc^p (FP64, as computed
 // by ipt_displacement).
-__device__ __forceinline__ double exact_ipt(double s0, double s1, double s2, int d, double power, double prefactor,
+#ifndef ECMC_EXACT_INLINE
+#define ECMC_EXACT_INLINE __forceinline__
+#endif
+#ifndef ECMC_BATCHED_WAVES_PER_SIMD
+#define ECMC_BATCHED_WAVES_PER_SIMD 1
+#endif
+__device__ ECMC_EXACT_INLINE double exact_ipt(double s0, double s1, double s2, int d, double power, double prefactor,
                                             double rc2, double urc, double dU) {
     double s[3] = {s0, s1, s2};
     const double perp2 = others_sq<3>(s, d);
@@ -328,7 +338,7 @@ __host__ __device__ inline size_t batched_lds_bytes(int nmax, int cellmax, int w
 //      pow/log, workgroup min-reduce over (displacement, kind, target, factor);
 //   5. thread 0 commits.
 template <int WPR>
-__global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
+__global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched_kernel(BatchArgs A) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NTH = 64 * WPR;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -343,7 +353,7 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
     int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
     int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
-    __shared__ int s_item, s_nc, s_overflow, s_nearby[27], s_cnt[27], s_pref[28], s_M;
+    __shared__ int s_item, s_nc, s_overflow, s_nearby[27], s_pref[28], s_M;
     __shared__ double s_ap[3], s_newpos[3], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
     __shared__ ReplicaDev R;
     __shared__ unsigned long long s_stats[4];
@@ -390,15 +400,13 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
         const double rc2 = R.cutoff * R.cutoff;
         const float rc2_hi = (float)rc2 * (1.f + 0x1p-22f);
         const int nf = A.fac.n_factors;
-        float vrc_lo[2] = {0.f, 0.f}, kf[2] = {0.f, 0.f};
-        int pf[2] = {12, 12};
-        double urc[2] = {0.0, 0.0};
-        for (int f = 0; f < nf; f++) {
-            urc[f] = A.fac.prefactor[f] / cr_pow(rc2, A.fac.power[f] / 2.0);
-            vrc_lo[f] = (float)fabs(urc[f]) * (1.f - 0x1p-20f);
-            kf[f] = (float)A.fac.prefactor[f];
-            pf[f] = (int)A.fac.power[f];
-        }
+        // per-factor constants (the second factor's equal the first's when there is one factor)
+        const double pw0 = A.fac.power[0], pw1 = nf > 1 ? A.fac.power[1] : pw0;
+        const double pk0 = A.fac.prefactor[0], pk1 = nf > 1 ? A.fac.prefactor[1] : pk0;
+        const double urc0 = pk0 / cr_pow(rc2, pw0 / 2.0), urc1 = pk1 / cr_pow(rc2, pw1 / 2.0);
+        const float vrc0 = (float)fabs(urc0) * (1.f - 0x1p-20f), vrc1 = (float)fabs(urc1) * (1.f - 0x1p-20f);
+        const float kf0 = (float)pk0, kf1 = (float)pk1;
+        const int pf0 = (int)pw0, pf1 = (int)pw1;
         const float inv_beta_lo = (float)(1.0 / A.beta) * (1.f - 0x1p-22f);
         bsync();
         for (int leg_i = 0; leg_i < A.legs && !R.err; leg_i++) {
@@ -409,27 +417,32 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
             if (cps) {
 #pragma unroll
                 for (int q = 0; q < 3; q++) ac[q] = bcell1(ap[q], R.side, cps);
-                if (tid < 27) {
-                    int dx = tid % 3 - 1, dy = (tid / 3) % 3 - 1, dz = tid / 9 - 1;
-                    int cx = (ac[0] + dx + cps) % cps, cy = (ac[1] + dy + cps) % cps, cz = (ac[2] + dz + cps) % cps;
-                    int c = cx + cps * (cy + cps * cz);
-                    s_nearby[tid] = c;
-                    s_cnt[tid] = min(cell_n[c], B_CELL_CAP);
+                if (wave == 0) {
+                    // the 27 nearby cells and an inclusive scan of their counts across lanes
+                    int cnt = 0;
+                    if (lane < 27) {
+                        int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
+                        int cx = (ac[0] + dx + cps) % cps, cy = (ac[1] + dy + cps) % cps, cz = (ac[2] + dz + cps) % cps;
+                        int c = cx + cps * (cy + cps * cz);
+                        s_nearby[lane] = c;
+                        cnt = min(cell_n[c], B_CELL_CAP);
+                    }
+                    int incl = cnt;
+#pragma unroll
+                    for (int off = 1; off < 32; off <<= 1) {
+                        const int o = __shfl_up(incl, off, 64);
+                        if (lane >= off) incl += o;
+                    }
+                    if (lane < 27) s_pref[lane + 1] = incl;
+                    if (lane == 0) s_pref[0] = 0;
+                    if (lane == 26) s_M = incl;
                 }
-                bsync();
             }
             if (tid == 0) {
                 double h = R.chain_left, cb = INFINITY;
                 if (cps) {
                     cb = s_bounds[ac[d] + 1] - ap[d];
                     if (cb < h) h = cb;
-                    int acc = 0;
-                    for (int k = 0; k < 27; k++) {
-                        s_pref[k] = acc;
-                        acc += s_cnt[k];
-                    }
-                    s_pref[27] = acc;
-                    s_M = acc;
                 } else {
                     s_M = n - 1;
                 }
@@ -462,8 +475,9 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
                         float t = 0.f;
                         if (!amb) {
                             if (g.perp2_lo >= rc2_hi) continue;
-                            if (factor_geometric_floor(g, kf[f], rc2_hi) > hf) continue;
-                            t = factor_lower_bound(g, kf[f], pf[f], rc2_hi, vrc_lo[f], dU_lower(w[2 * f], inv_beta_lo));
+                            if (factor_geometric_floor(g, pick(f, kf0, kf1), rc2_hi) > hf) continue;
+                            t = factor_lower_bound(g, pick(f, kf0, kf1), pick(f, pf0, pf1), rc2_hi, pick(f, vrc0, vrc1),
+                                                   dU_lower(pick(f, w[0], w[2]), inv_beta_lo));
                         }
                         if (t <= hf) {
                             const int slot = atomicAdd(&s_nc, 1);
@@ -472,8 +486,8 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
                                 continue;
                             }
                             cont_id[slot] = (j << 1) | f;
-                            cont_w[2 * slot] = w[2 * f];
-                            cont_w[2 * slot + 1] = w[2 * f + 1];
+                            cont_w[2 * slot] = pick(f, w[0], w[2]);
+                            cont_w[2 * slot + 1] = pick(f, w[1], w[3]);
                             if (c0 < 0) {
                                 c0 = slot;
                                 c0x = P[j]; c0y = P[(size_t)n + j]; c0z = P[2 * (size_t)n + j];
@@ -502,7 +516,7 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
                             surv = true;
                         } else if (g.perp2_lo < rc2_hi) {
                             for (int f = 0; f < nf; f++)
-                                if (factor_geometric_floor(g, kf[f], rc2_hi) <= hf) surv = true;
+                                if (factor_geometric_floor(g, pick(f, kf0, kf1), rc2_hi) <= hf) surv = true;
                         }
                     }
                     const unsigned long long m = __ballot(surv);
@@ -537,8 +551,8 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
                     pj[0] = P[j]; pj[1] = P[(size_t)n + j]; pj[2] = P[2 * (size_t)n + j];
                     uint32_t ww[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
-                    wa = ww[2 * f];
-                    wb = ww[2 * f + 1];
+                    wa = pick(f, ww[0], ww[2]);
+                    wb = pick(f, ww[1], ww[3]);
                 } else {
                     j = cont_id[w] >> 1;
                     f = cont_id[w] & 1;
@@ -549,8 +563,8 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
                 const double u = unit53(wa, wb);
                 const double dU = -cr_log(1.0 - u) / A.beta;
                 const double dx = exact_ipt(bmin_image(pj[0] - ap[0], L), bmin_image(pj[1] - ap[1], L),
-                                            bmin_image(pj[2] - ap[2], L), d, A.fac.power[f], A.fac.prefactor[f],
-                                            rc2, urc[f], dU);
+                                            bmin_image(pj[2] - ap[2], L), d, pick(f, pw0, pw1), pick(f, pk0, pk1),
+                                            rc2, pick(f, urc0, urc1), dU);
                 BKey c = {dx, 0, j, f};
                 if (bkey_lt(c, best)) {
                     best = c;
@@ -575,10 +589,10 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
                     const int j = candidate_unit(idx / nf, cps, a, s_pref, s_nearby, cell_l);
                     uint32_t ww[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
-                    const double dU = -cr_log(1.0 - unit53(ww[2 * f], ww[2 * f + 1])) / A.beta;
+                    const double dU = -cr_log(1.0 - unit53(pick(f, ww[0], ww[2]), pick(f, ww[1], ww[3]))) / A.beta;
                     const double dx = exact_ipt(bmin_image(P[j] - ap[0], L), bmin_image(P[(size_t)n + j] - ap[1], L),
-                                                bmin_image(P[2 * (size_t)n + j] - ap[2], L), d, A.fac.power[f],
-                                                A.fac.prefactor[f], rc2, urc[f], dU);
+                                                bmin_image(P[2 * (size_t)n + j] - ap[2], L), d, pick(f, pw0, pw1),
+                                                pick(f, pk0, pk1), rc2, pick(f, urc0, urc1), dU);
                     if (dx <= h) {
                         bool found = false;
                         for (int q = 0; q < min(s_nc, B_CONTENDERS); q++)
@@ -594,16 +608,12 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
             }
             if (geo_pass) atomicAdd(&s_stats[1], geo_pass);
             bsync();
-            // ---- commit (thread 0; the winner's position is in s_newpos)
-            if (tid == 0) {
+            // ---- commit (wavefront 0; the winner's position is in s_newpos): the values below
+            // are uniform across the wavefront, a ballot finds the new active unit's slot in its
+            // cell list, lane 0 writes
+            if (wave == 0) {
                 double *Pw = A.pos64 + R.off;
                 double x;
-                ecmc_batch_event ev;
-                ev.kind = best.rank;
-                ev.active = a;
-                ev.target = -1;
-                ev.factor = 0;
-                ev.displacement = best.d;
                 if (best.rank == 1) {
                     int c = ac[d];
                     x = c + 1 < cps ? s_bounds[c + 1] : 0.0;
@@ -611,59 +621,80 @@ __global__ __launch_bounds__(64 * WPR) void batched_kernel(BatchArgs A) {
                     x = ap[d] + best.d;
                     if (x >= L) x -= L;
                 }
-                Pw[(size_t)d * n + a] = x;
-                float xf = (float)x;
-                if (d == 0) px[a] = xf;
-                else if (d == 1) py[a] = xf;
-                else pz[a] = xf;
-                R.est[1] += best.d;
                 int newa = a;
                 double np[3] = {ap[0], ap[1], ap[2]};
                 np[d] = x;
-                double pold[3] = {np[0], np[1], np[2]};
+                const double pold[3] = {np[0], np[1], np[2]};
                 if (best.rank == 0) {
-                    const int j = best.j;
-                    double sdj = bmin_image(s_newpos[d] - ap[d], L) - best.d;
-                    R.est[0] += sdj;
-                    R.est[2] += 1.0;
-                    R.est[4] += 1.0;
-                    R.chain_left -= best.d;
-                    newa = j;
+                    newa = best.j;
                     np[0] = s_newpos[0]; np[1] = s_newpos[1]; np[2] = s_newpos[2];
-                    ev.target = j;
-                    ev.factor = best.f;
-                } else if (best.rank == 1) {
-                    R.est[3] += 1.0;
-                    R.chain_left -= best.d;
-                } else {
+                } else if (best.rank == 2) {
                     uint32_t w[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), 0xFFFFFFFFu, 1u, (uint32_t)r, A.seed_lo, w);
                     newa = (int)(((uint64_t)w[0] * (uint64_t)n) >> 32);
-                    R.est[5] += 1.0;
-                    R.chain_left = A.chain_length;
-                    R.direction = (d + 1) % 3;
-                    ev.target = newa;
                     if (newa != a) {
                         np[0] = Pw[newa]; np[1] = Pw[(size_t)n + newa]; np[2] = Pw[2 * (size_t)n + newa];
                     }
                 }
-                ev.x = x;
+                int ca = 0, cn = 0, q = -1, m = 0;
                 if (newa != a && cps) {
-                    // the old active unit joins the cell it ended in, the new one leaves its cell
-                    int ca = bcell1(pold[0], R.side, cps) + cps * (bcell1(pold[1], R.side, cps) + cps * bcell1(pold[2], R.side, cps));
-                    int na = cell_n[ca];
-                    if (na >= B_CELL_CAP) R.err = 1;
-                    else { cell_l[(size_t)ca * B_CELL_CAP + na] = (int16_t)a; cell_n[ca] = na + 1; }
-                    int cn = bcell1(np[0], R.side, cps) + cps * (bcell1(np[1], R.side, cps) + cps * bcell1(np[2], R.side, cps));
-                    int m = cell_n[cn], q = 0;
-                    while (q < m && cell_l[(size_t)cn * B_CELL_CAP + q] != newa) q++;
-                    if (q == m) R.err = 2;
-                    else { cell_l[(size_t)cn * B_CELL_CAP + q] = cell_l[(size_t)cn * B_CELL_CAP + m - 1]; cell_n[cn] = m - 1; }
+                    ca = bcell1(pold[0], R.side, cps) + cps * (bcell1(pold[1], R.side, cps) + cps * bcell1(pold[2], R.side, cps));
+                    cn = bcell1(np[0], R.side, cps) + cps * (bcell1(np[1], R.side, cps) + cps * bcell1(np[2], R.side, cps));
+                    m = cell_n[cn];
+                    const bool hit = lane < m && cell_l[(size_t)cn * B_CELL_CAP + lane] == newa;
+                    const unsigned long long mask = __ballot(hit);
+                    q = mask ? __ffsll((long long)mask) - 1 : -1;
                 }
-                s_ap[0] = np[0]; s_ap[1] = np[1]; s_ap[2] = np[2];
-                R.active = newa;
-                if (A.log && leg < (uint64_t)A.log_legs) A.log[(size_t)r * A.log_legs + leg] = ev;
-                R.leg = leg + 1;
+                if (lane == 0) {
+                    ecmc_batch_event ev;
+                    ev.kind = best.rank;
+                    ev.active = a;
+                    ev.target = -1;
+                    ev.factor = 0;
+                    ev.displacement = best.d;
+                    Pw[(size_t)d * n + a] = x;
+                    float xf = (float)x;
+                    if (d == 0) px[a] = xf;
+                    else if (d == 1) py[a] = xf;
+                    else pz[a] = xf;
+                    R.est[1] += best.d;
+                    if (best.rank == 0) {
+                        double sdj = bmin_image(s_newpos[d] - ap[d], L) - best.d;
+                        R.est[0] += sdj;
+                        R.est[2] += 1.0;
+                        R.est[4] += 1.0;
+                        R.chain_left -= best.d;
+                        ev.target = best.j;
+                        ev.factor = best.f;
+                    } else if (best.rank == 1) {
+                        R.est[3] += 1.0;
+                        R.chain_left -= best.d;
+                    } else {
+                        R.est[5] += 1.0;
+                        R.chain_left = A.chain_length;
+                        R.direction = (d + 1) % 3;
+                        ev.target = newa;
+                    }
+                    ev.x = x;
+                    if (newa != a && cps) {
+                        // the old active unit joins the cell it ended in, the new one leaves its
+                        // cell (its slot was found before the append, so q < m)
+                        int na = cell_n[ca];
+                        if (na >= B_CELL_CAP) R.err = 1;
+                        else { cell_l[(size_t)ca * B_CELL_CAP + na] = (int16_t)a; cell_n[ca] = na + 1; }
+                        if (q < 0) {
+                            R.err = 2;
+                        } else {
+                            const int last = cell_n[cn] - 1;
+                            cell_l[(size_t)cn * B_CELL_CAP + q] = cell_l[(size_t)cn * B_CELL_CAP + last];
+                            cell_n[cn] = last;
+                        }
+                    }
+                    s_ap[0] = np[0]; s_ap[1] = np[1]; s_ap[2] = np[2];
+                    R.active = newa;
+                    if (A.log && leg < (uint64_t)A.log_legs) A.log[(size_t)r * A.log_legs + leg] = ev;
+                    R.leg = leg + 1;
+                }
             }
             bsync();
         }

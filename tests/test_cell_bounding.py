@@ -112,9 +112,14 @@ def test_gpu_cell_bounded_matches_oracle(device, oracle, n):
     got = DeviceChain(desc, pos, charges).run(legs)
     want = _oracle_engine(oracle, meta, taggers, pos, charges, oracle.MATH_CR).run(legs)
     assert len(got) == len(want) == legs
-    for f in ("tagger", "flags", "target", "tq", "tr", "pos", "ncand", "chash"):
-        eq = got[f] == want[f]
-        if eq.ndim > 1:
-            eq = eq.all(axis=1)
-        bad = np.flatnonzero(~eq)
+    # The event sequence is exact.  The bound tables come from erfc/exp of the device's math
+    # library on one side and of glibc on the other (neither correctly rounded), so event times
+    # and positions agree to rounding, within 1e-12 relative.
+    for f in ("tagger", "flags", "target", "ncand", "chash"):
+        bad = np.flatnonzero(got[f] != want[f])
         assert bad.size == 0, "field %s differs first at leg %d" % (f, bad[0])
+    t_got, t_want = got["tq"] + got["tr"], want["tq"] + want["tr"]
+    rel_t = np.max(np.abs(t_got - t_want) / np.maximum(np.abs(t_want), 1.0))
+    rel_p = np.max(np.abs(got["pos"] - want["pos"])) / meta["system_length"]
+    print("largest relative deviation: time %.2e, position %.2e" % (rel_t, rel_p), flush=True)
+    assert rel_t < 1e-12 and rel_p < 1e-12
