@@ -68,6 +68,21 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_kernel_bytes_per_launch": nbytes, "launches": 10}), flush=True)
 
 
+def profiled_traffic(kernel, legs):
+    """HBM bytes per launch of `kernel` at `legs` legs per replica, from the committed counter
+    profile of this workload (FETCH_SIZE scaled by the copy calibration's factor, plus
+    WRITE_SIZE); None when the profile has no line for it."""
+    path = os.path.join(ROOT, "profiles", "batched_kernel_lj_4096_replicas_counters.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        # traffic <kernel> legs=<L> bytes_per_launch=<B>
+        if len(f) == 4 and f[0] == "traffic" and f[1] == kernel and f[2] == "legs=%d" % legs:
+            return float(f[3].split("=")[1])
+    return None
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
@@ -218,21 +233,40 @@ def main():
         batch.run_graph(args.legs, 1)
         log("warm-up step %d done" % (i + 1))
     s0 = batch.states()
+    # timed steps: direct launches with events on every class's stream (a step of this size is
+    # hundreds of milliseconds, so a graph replay would save nothing measurable)
+    batch.set_class_timing(True)
+    cls_ms = [[0.0, 0] for _ in batch.classes[0]]
+    kernel_ms = 0.0
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    kernel_ms = batch.run_graph(args.legs, args.steps)
+    for _ in range(args.steps):
+        kernel_ms += batch.run(args.legs)
+        for c, (ms, k) in enumerate(batch.class_ms()):
+            cls_ms[c][0] += ms
+            cls_ms[c][1] += k
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    batch.set_class_timing(False)
     s1 = batch.states()
     events = int((s1["legs"] - s0["legs"]).sum())
     value = events / wall
-    log("batched: %d events in %.3f s (kernel %.3f ms)" % (events, wall, kernel_ms))
-    # algorithmic work: every factor candidate of every leg inverted once
-    cands = float((s1["stats"][:, 0] - s0["stats"][:, 0]).sum())
+    log("batched: %d events in %.3f s (steps %.3f ms)" % (events, wall, kernel_ms))
+    # Roofline of the dominant kernel (the size class with the longest summed kernel time):
+    # algorithmic work = every factor candidate of its replicas' legs inverted once, per launch,
+    # over its average launch duration from the class's own stream events.
+    cls_of = np.searchsorted(np.asarray(batch.classes[0]), n, side="left")
+    dom = int(np.argmax([ms for ms, _ in cls_ms]))
+    dom_ms, dom_launches = cls_ms[dom]
+    cands_r = (s1["stats"][:, 0] - s0["stats"][:, 0]).astype(np.float64)
+    cands = float(cands_r.sum())
     n_factors = 2
-    flops = cands * n_factors * FLOPS_PER_INVERSION
-    achieved = flops / (kernel_ms * 1e-3) / 1e12
+    flops_per_launch = float(cands_r[cls_of == dom].sum()) * n_factors * FLOPS_PER_INVERSION / max(dom_launches, 1)
+    launch_ms = dom_ms / max(dom_launches, 1)
+    achieved = flops_per_launch / (launch_ms * 1e-3) / 1e12
     exact = float((s1["stats"][:, 3] - s0["stats"][:, 3]).sum())
+    dom_name = "ecmc::batched_kernel<%d>" % batch.classes[1][dom]
+    traffic = profiled_traffic(dom_name, args.legs)
     # pressure from untimed production steps after the timed region (each replica then has
     # (warmup + steps) x legs legs behind it)
     batch.run_graph(args.legs, args.production_steps)
@@ -251,12 +285,16 @@ def main():
                    "classes": {"max_n": [int(v) for v in batch.classes[0]],
                                "wavefronts_per_replica": [int(v) for v in batch.classes[1]]}},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS, "traffic": None,
-                     "note": "compute-bound VALU kernel priced against the FP32 peak; algorithmic work = %g "
+                     "frac": achieved / PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": dom_name, "launch_ms": launch_ms, "launches": dom_launches,
+                     "flop_per_launch": flops_per_launch,
+                     "note": "VALU kernel priced against the FP32 vector peak; algorithmic work = %g "
                              "FLOP per factor-candidate inversion (%d factors), exact FP64 inversions per "
-                             "candidate = %.4f" % (FLOPS_PER_INVERSION, n_factors, exact / max(cands, 1))},
-        "batched": {"kernel_ms": kernel_ms, "events": events, "candidates_per_leg": cands / max(events, 1),
-                    "pressure_rho_0_80": pz},
+                             "candidate = %.4f; traffic = FETCH_SIZE x 2 + WRITE_SIZE per launch from the "
+                             "committed counter profile of this workload" % (FLOPS_PER_INVERSION, n_factors,
+                                                                            exact / max(cands, 1))},
+        "batched": {"step_ms_sum": kernel_ms, "events": events, "candidates_per_leg": cands / max(events, 1),
+                    "class_kernel_ms": [ms for ms, _ in cls_ms], "pressure_rho_0_80": pz},
     }
     if not args.no_plain:
         log("plain chain ...")

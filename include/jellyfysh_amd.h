@@ -195,6 +195,12 @@ int ecmc_batch_run(ecmc_batch *batch, int32_t legs_per_replica);
 /* The step captured once into a hipGraph and replayed n_steps times (synchronous). */
 int ecmc_batch_run_graph(ecmc_batch *batch, int32_t legs_per_replica, int32_t n_steps);
 int ecmc_batch_last_ms(const ecmc_batch *batch, double *ms);
+/* Class timing: when on, ecmc_batch_run times every size class's kernel with events on the
+ * class's own stream, and ecmc_batch_class_ms returns its kernel milliseconds and launches (1,
+ * or 0 for an empty class) of the last run call; graph replays are not class-timed (launches
+ * 0).  No reference counterpart: the reference has no batched form. */
+int ecmc_batch_set_class_timing(ecmc_batch *batch, int32_t on);
+int ecmc_batch_class_ms(const ecmc_batch *batch, int32_t cls, double *ms, int32_t *launches);
 /* Per replica: active unit, direction, chain displacement left, legs done, est[8] (lift
  * separation sum, displacement sum, pair, boundary, lifts, chain ends) and stats[4]
  * (candidates, candidates past the geometric test, contenders, exact inversions). */

@@ -125,6 +125,19 @@ class DeviceBatch:
         L.check(self._L.ecmc_batch_run_graph(self._h, int(legs_per_replica), int(n_steps)))
         return self.last_ms()
 
+    def set_class_timing(self, on=True):
+        L.check(self._L.ecmc_batch_set_class_timing(self._h, int(bool(on))))
+
+    def class_ms(self):
+        """Per size class: (kernel ms, launches) of the last run() call, timed with events on the
+        class's own stream while class timing is on (graph replays are not class-timed)."""
+        out = []
+        for c in range(len(self.classes[0])):
+            ms, k = ctypes.c_double(), ctypes.c_int32()
+            L.check(self._L.ecmc_batch_class_ms(self._h, c, ctypes.byref(ms), ctypes.byref(k)))
+            out.append((ms.value, k.value))
+        return out
+
     def last_ms(self):
         ms = ctypes.c_double()
         L.check(self._L.ecmc_batch_last_ms(self._h, ctypes.byref(ms)))

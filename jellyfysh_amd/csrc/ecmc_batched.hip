@@ -57,6 +57,12 @@ struct ecmc_batch {
     hipStream_t main_stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<hipEvent_t> fork;
+    // per class: timing events around its kernel on the class's stream (direct launches only:
+    // events recorded inside a captured graph carry no timestamps) and its kernel time of the
+    // last ecmc_batch_run call
+    std::vector<hipEvent_t> cls_ev0, cls_ev1;
+    std::vector<double> cls_ms;
+    std::vector<int32_t> cls_launches;
     ecmc_batch_event *d_log = nullptr;
     int32_t log_legs = 0;
     unsigned long long *d_audit = nullptr;
@@ -64,7 +70,21 @@ struct ecmc_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int32_t graph_legs = -1;
+    bool class_timing = false;
 };
+
+// Add the kernel time of every class of the step that just completed.
+static int add_class_times(ecmc_batch *b) {
+    for (int c = 0; c < b->n_classes; c++) {
+        if (b->class_count[c] == 0) continue;
+        float f = 0.f;
+        if (hipEventElapsedTime(&f, b->cls_ev0[c], b->cls_ev1[c]) != hipSuccess)
+            return ecmc_fail(ECMC_ERR_HIP, "class event timing failed");
+        b->cls_ms[c] += f;
+        b->cls_launches[c] += 1;
+    }
+    return ECMC_OK;
+}
 
 static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
     if (b->class_count[c] == 0) return ECMC_OK;
@@ -114,13 +134,15 @@ static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
 }
 
 // One step: reset the work queues, then every size class on its own stream.
-static int enqueue_step(ecmc_batch *b, int legs, hipStream_t main) {
+static int enqueue_step(ecmc_batch *b, int legs, hipStream_t main, bool timed) {
     hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, main, b->d_queue, b->n_classes);
     hipEventRecord(b->fork[0], main);
     for (int c = 0; c < b->n_classes; c++) {
         hipStreamWaitEvent(b->streams[c], b->fork[0], 0);
+        if (timed) hipEventRecord(b->cls_ev0[c], b->streams[c]);
         int rc = launch_class(b, c, legs, b->streams[c]);
         if (rc) return rc;
+        if (timed) hipEventRecord(b->cls_ev1[c], b->streams[c]);
         hipEventRecord(b->fork[c + 1], b->streams[c]);
         hipStreamWaitEvent(main, b->fork[c + 1], 0);
     }
@@ -235,7 +257,9 @@ extern "C" int ecmc_batch_set_classes(ecmc_batch *b, int32_t n_classes, const in
     for (auto p : b->d_order) hipFree(p);
     for (auto s : b->streams) hipStreamDestroy(s);
     for (auto e : b->fork) hipEventDestroy(e);
-    b->d_order.clear(); b->streams.clear(); b->fork.clear();
+    for (auto e : b->cls_ev0) hipEventDestroy(e);
+    for (auto e : b->cls_ev1) hipEventDestroy(e);
+    b->d_order.clear(); b->streams.clear(); b->fork.clear(); b->cls_ev0.clear(); b->cls_ev1.clear();
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
     b->n_classes = n_classes;
@@ -273,6 +297,15 @@ extern "C" int ecmc_batch_set_classes(ecmc_batch *b, int32_t n_classes, const in
         hipEventCreateWithFlags(&e, hipEventDisableTiming);
         b->fork.push_back(e);
     }
+    for (int c = 0; c < n_classes; c++) {
+        hipEvent_t e0, e1;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        b->cls_ev0.push_back(e0);
+        b->cls_ev1.push_back(e1);
+    }
+    b->cls_ms.assign(n_classes, 0.0);
+    b->cls_launches.assign(n_classes, 0);
     return ECMC_OK;
 }
 
@@ -283,6 +316,8 @@ extern "C" void ecmc_batch_destroy(ecmc_batch *b) {
     for (auto p : b->d_order) hipFree(p);
     for (auto s : b->streams) hipStreamDestroy(s);
     for (auto e : b->fork) hipEventDestroy(e);
+    for (auto e : b->cls_ev0) hipEventDestroy(e);
+    for (auto e : b->cls_ev1) hipEventDestroy(e);
     hipFree(b->d_rep); hipFree(b->d_pos); hipFree(b->d_bounds); hipFree(b->d_queue); hipFree(b->d_log);
     hipFree(b->d_audit);
     if (b->ev0) hipEventDestroy(b->ev0);
@@ -315,11 +350,13 @@ extern "C" int ecmc_batch_set_log(ecmc_batch *b, int32_t legs_per_replica) {
 extern "C" int ecmc_batch_run(ecmc_batch *b, int32_t legs_per_replica) {
     if (!b || legs_per_replica < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
     hipEventRecord(b->ev0, b->main_stream);
-    int rc = enqueue_step(b, legs_per_replica, b->main_stream);
+    int rc = enqueue_step(b, legs_per_replica, b->main_stream, b->class_timing);
     if (rc) return rc;
     hipEventRecord(b->ev1, b->main_stream);
     if (hipStreamSynchronize(b->main_stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "batched kernel failed");
-    return ECMC_OK;
+    std::fill(b->cls_ms.begin(), b->cls_ms.end(), 0.0);
+    std::fill(b->cls_launches.begin(), b->cls_launches.end(), 0);
+    return b->class_timing ? add_class_times(b) : ECMC_OK;
 }
 
 // Capture one step (queue reset + every class) into a hipGraph, then replay it n_steps times.
@@ -330,7 +367,7 @@ extern "C" int ecmc_batch_run_graph(ecmc_batch *b, int32_t legs_per_replica, int
         if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
         if (hipStreamBeginCapture(b->main_stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
             return ecmc_fail(ECMC_ERR_HIP, "stream capture failed");
-        int rc = enqueue_step(b, legs_per_replica, b->main_stream);
+        int rc = enqueue_step(b, legs_per_replica, b->main_stream, false);
         hipGraph_t g = nullptr;
         hipError_t e = hipStreamEndCapture(b->main_stream, &g);
         if (rc) { if (g) hipGraphDestroy(g); return rc; }
@@ -340,11 +377,27 @@ extern "C" int ecmc_batch_run_graph(ecmc_batch *b, int32_t legs_per_replica, int
             return ecmc_fail(ECMC_ERR_HIP, "graph instantiation failed");
         b->graph_legs = legs_per_replica;
     }
+    std::fill(b->cls_ms.begin(), b->cls_ms.end(), 0.0);
+    std::fill(b->cls_launches.begin(), b->cls_launches.end(), 0);   // replays are not class-timed
     hipEventRecord(b->ev0, b->main_stream);
     for (int s = 0; s < n_steps; s++)
         if (hipGraphLaunch(b->graph_exec, b->main_stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "graph launch failed");
     hipEventRecord(b->ev1, b->main_stream);
     if (hipStreamSynchronize(b->main_stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "batched graph failed");
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_batch_set_class_timing(ecmc_batch *b, int32_t on) {
+    if (!b) return ecmc_fail(ECMC_ERR_ARGUMENT, "null batch");
+    b->class_timing = on != 0;
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_batch_class_ms(const ecmc_batch *b, int32_t cls, double *ms, int32_t *launches) {
+    if (!b || !ms || !launches) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    if (cls < 0 || cls >= b->n_classes) return ecmc_fail(ECMC_ERR_ARGUMENT, "no such class");
+    *ms = b->cls_ms[cls];
+    *launches = b->cls_launches[cls];
     return ECMC_OK;
 }
 

@@ -142,3 +142,25 @@ def test_gpu_batched_graph_replay_matches_direct_launches(device):
     assert np.array_equal(s1["est"], s2["est"])
     for p1, p2 in zip(b1.positions(), b2.positions()):
         assert np.array_equal(p1, p2)
+
+
+@pytest.mark.gpu
+def test_gpu_batched_class_timing(device):
+    # per-class events on the class streams time direct launches without changing the result;
+    # graph replays are not class-timed
+    classes = ([128, 512, 4096], [1, 1, 2])
+    b1, *_ = _device_batch(classes, 0, 0, seed=23)
+    b2, *_ = _device_batch(classes, 0, 0, seed=23)
+    print("batched class timing", flush=True)
+    b1.run_graph(150, 3)
+    b2.set_class_timing(True)
+    counts = np.bincount(np.searchsorted(np.asarray(b2.classes[0]), b2.n, side="left"), minlength=3)
+    for _ in range(3):
+        total = b2.run(150)
+        for (ms, k), cnt in zip(b2.class_ms(), counts):
+            assert k == (1 if cnt else 0)
+            assert (ms > 0) == bool(cnt)
+            assert ms <= total * 1.01
+    assert np.array_equal(b1.states()["est"], b2.states()["est"])
+    b2.run_graph(150, 1)
+    assert all(k == 0 for _, k in b2.class_ms())
