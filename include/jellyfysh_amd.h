@@ -195,6 +195,10 @@ int ecmc_batch_run(ecmc_batch *batch, int32_t legs_per_replica);
 /* The step captured once into a hipGraph and replayed n_steps times (synchronous). */
 int ecmc_batch_run_graph(ecmc_batch *batch, int32_t legs_per_replica, int32_t n_steps);
 int ecmc_batch_last_ms(const ecmc_batch *batch, double *ms);
+/* Limit every size class's persistent grid to max_blocks workgroups (0 = as many as fit), e.g.
+ * to leave room for other work on the device.  A replica's legs run in chunks, and any block
+ * may take the next chunk, so results do not depend on the limit.  No reference counterpart. */
+int ecmc_batch_set_max_blocks(ecmc_batch *batch, int32_t max_blocks);
 /* Class timing: when on, ecmc_batch_run times every size class's kernel with events on the
  * class's own stream, and ecmc_batch_class_ms returns its kernel milliseconds and launches (1,
  * or 0 for an empty class) of the last run call; graph replays are not class-timed (launches

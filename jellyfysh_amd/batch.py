@@ -125,6 +125,10 @@ class DeviceBatch:
         L.check(self._L.ecmc_batch_run_graph(self._h, int(legs_per_replica), int(n_steps)))
         return self.last_ms()
 
+    def set_max_blocks(self, max_blocks):
+        """Limit each size class's persistent grid (0 = as many workgroups as fit)."""
+        L.check(self._L.ecmc_batch_set_max_blocks(self._h, int(max_blocks)))
+
     def set_class_timing(self, on=True):
         L.check(self._L.ecmc_batch_set_class_timing(self._h, int(bool(on))))
 

@@ -31,9 +31,14 @@
 
 namespace ecmc {
 
-__global__ void zero_counters(int32_t *q, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) q[i] = 0;
+// Work items per replica and step (the step's legs split into this many chunks at most).
+constexpr int WORK_CHUNKS = 16;
+
+// Start of a step for one class: every member free with no chunk done, claim cursor at 0.
+__global__ void reset_work(int32_t *state, int count, int32_t *q) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) state[i] = 0;
+    if (i == 0) q[0] = 0;
 }
 
 }  // namespace ecmc
@@ -51,7 +56,7 @@ struct ecmc_batch {
     // size classes
     int n_classes = 0;
     std::vector<int32_t> class_max_n, class_wpr, class_count, class_nmax, class_cellmax;
-    std::vector<int32_t *> d_order;
+    std::vector<int32_t *> d_order, d_state;
     int32_t *d_queue = nullptr;
     std::vector<hipStream_t> streams;
     hipStream_t main_stream = nullptr;
@@ -71,6 +76,7 @@ struct ecmc_batch {
     hipGraphExec_t graph_exec = nullptr;
     int32_t graph_legs = -1;
     bool class_timing = false;
+    int32_t max_blocks = 0;   // limit of every class's persistent grid (0 = as many as fit)
 };
 
 // Add the kernel time of every class of the step that just completed.
@@ -94,7 +100,13 @@ static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
     A.pos64 = b->d_pos;
     A.bounds = b->d_bounds;
     A.order = b->d_order[c];
+    A.state = b->d_state[c];
     A.count = b->class_count[c];
+    // the step's legs in at most WORK_CHUNKS chunks per replica: a replica is free again after
+    // each chunk, so blocks keep busy to the end of the step whatever the replica count
+    const int nch = std::max(1, std::min(WORK_CHUNKS, legs));
+    A.chunk_legs = std::max(1, (legs + nch - 1) / nch);
+    A.nchunks = legs > 0 ? (legs + A.chunk_legs - 1) / A.chunk_legs : 0;
     A.nmax = b->class_nmax[c];
     A.cellmax = b->class_cellmax[c];
     A.legs = legs;
@@ -105,7 +117,8 @@ static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
     A.chain_length = b->desc.chain_length;
     A.fac.n_factors = b->desc.n_factors;
     for (int f = 0; f < 2; f++) { A.fac.power[f] = b->desc.power[f]; A.fac.prefactor[f] = b->desc.prefactor[f]; }
-    A.queue = b->d_queue + c;
+    A.queue = b->d_queue + 4 * c;
+    hipLaunchKernelGGL(reset_work, dim3((A.count + 255) / 256), dim3(256), 0, st, A.state, A.count, A.queue);
     A.log = b->d_log;
     A.audit = b->d_audit;
     int wpr = b->class_wpr[c];
@@ -119,6 +132,7 @@ static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batched_kernel<W>, 64 * W, lds);              \
         if (per_cu < 1) return ecmc_fail(ECMC_ERR_CONFIGURATION, "replica does not fit in LDS");           \
         int grid = std::min(A.count, per_cu * std::max(cus, 1));                                           \
+        if (b->max_blocks > 0) grid = std::min(grid, b->max_blocks);                                       \
         hipLaunchKernelGGL(batched_kernel<W>, dim3(grid), dim3(64 * W), lds, st, A);                       \
     }
     switch (wpr) {
@@ -135,7 +149,6 @@ static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
 
 // One step: reset the work queues, then every size class on its own stream.
 static int enqueue_step(ecmc_batch *b, int legs, hipStream_t main, bool timed) {
-    hipLaunchKernelGGL(zero_counters, dim3(1), dim3(64), 0, main, b->d_queue, b->n_classes);
     hipEventRecord(b->fork[0], main);
     for (int c = 0; c < b->n_classes; c++) {
         hipStreamWaitEvent(b->streams[c], b->fork[0], 0);
@@ -225,7 +238,7 @@ extern "C" int ecmc_batch_create(const ecmc_batch_desc *desc, const int32_t *n_p
     if (hipMalloc((void **)&b->d_rep, sizeof(ReplicaDev) * b->n_rep) != hipSuccess ||
         hipMalloc((void **)&b->d_pos, sizeof(double) * soa.size()) != hipSuccess ||
         hipMalloc((void **)&b->d_bounds, sizeof(double) * bounds.size()) != hipSuccess ||
-        hipMalloc((void **)&b->d_queue, sizeof(int32_t) * 16) != hipSuccess ||
+        hipMalloc((void **)&b->d_queue, sizeof(int32_t) * 4 * 8) != hipSuccess ||
         hipMalloc((void **)&b->d_audit, sizeof(unsigned long long)) != hipSuccess) {
         ecmc_batch_destroy(b);
         return ecmc_fail(ECMC_ERR_HIP, "hipMalloc failed");
@@ -233,6 +246,8 @@ extern "C" int ecmc_batch_create(const ecmc_batch_desc *desc, const int32_t *n_p
     hipMemcpy(b->d_rep, b->rep_host.data(), sizeof(ReplicaDev) * b->n_rep, hipMemcpyHostToDevice);
     hipMemcpy(b->d_pos, soa.data(), sizeof(double) * soa.size(), hipMemcpyHostToDevice);
     hipMemcpy(b->d_bounds, bounds.data(), sizeof(double) * bounds.size(), hipMemcpyHostToDevice);
+    int32_t qzero[4 * 8] = {0};
+    hipMemcpy(b->d_queue, qzero, sizeof(qzero), hipMemcpyHostToDevice);
     unsigned long long zero = 0;
     hipMemcpy(b->d_audit, &zero, sizeof(zero), hipMemcpyHostToDevice);
     hipStreamCreate(&b->main_stream);
@@ -255,10 +270,12 @@ extern "C" int ecmc_batch_set_classes(ecmc_batch *b, int32_t n_classes, const in
     if (class_max_n[n_classes - 1] < *std::max_element(b->n.begin(), b->n.end()))
         return ecmc_fail(ECMC_ERR_ARGUMENT, "the last class must hold the largest replica");
     for (auto p : b->d_order) hipFree(p);
+    for (auto p : b->d_state) hipFree(p);
     for (auto s : b->streams) hipStreamDestroy(s);
     for (auto e : b->fork) hipEventDestroy(e);
     for (auto e : b->cls_ev0) hipEventDestroy(e);
     for (auto e : b->cls_ev1) hipEventDestroy(e);
+    b->d_state.clear();
     b->d_order.clear(); b->streams.clear(); b->fork.clear(); b->cls_ev0.clear(); b->cls_ev1.clear();
     if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
     if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
@@ -288,6 +305,10 @@ extern "C" int ecmc_batch_set_classes(ecmc_batch *b, int32_t n_classes, const in
             return ecmc_fail(ECMC_ERR_HIP, "hipMalloc failed");
         if (!m.empty()) hipMemcpy(d, m.data(), sizeof(int32_t) * m.size(), hipMemcpyHostToDevice);
         b->d_order.push_back(d);
+        int32_t *state = nullptr;
+        if (hipMalloc((void **)&state, sizeof(int32_t) * std::max<size_t>(m.size(), 1)) != hipSuccess)
+            return ecmc_fail(ECMC_ERR_HIP, "hipMalloc failed");
+        b->d_state.push_back(state);
         hipStream_t s;
         hipStreamCreate(&s);
         b->streams.push_back(s);
@@ -314,6 +335,7 @@ extern "C" void ecmc_batch_destroy(ecmc_batch *b) {
     if (b->graph_exec) hipGraphExecDestroy(b->graph_exec);
     if (b->graph) hipGraphDestroy(b->graph);
     for (auto p : b->d_order) hipFree(p);
+    for (auto p : b->d_state) hipFree(p);
     for (auto s : b->streams) hipStreamDestroy(s);
     for (auto e : b->fork) hipEventDestroy(e);
     for (auto e : b->cls_ev0) hipEventDestroy(e);
@@ -387,6 +409,14 @@ extern "C" int ecmc_batch_run_graph(ecmc_batch *b, int32_t legs_per_replica, int
     return ECMC_OK;
 }
 
+extern "C" int ecmc_batch_set_max_blocks(ecmc_batch *b, int32_t max_blocks) {
+    if (!b || max_blocks < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    b->max_blocks = max_blocks;
+    if (b->graph_exec) { hipGraphExecDestroy(b->graph_exec); b->graph_exec = nullptr; }
+    if (b->graph) { hipGraphDestroy(b->graph); b->graph = nullptr; }
+    return ECMC_OK;
+}
+
 extern "C" int ecmc_batch_set_class_timing(ecmc_batch *b, int32_t on) {
     if (!b) return ecmc_fail(ECMC_ERR_ARGUMENT, "null batch");
     b->class_timing = on != 0;
@@ -414,7 +444,9 @@ static int pull_replicas(const ecmc_batch *b, std::vector<ReplicaDev> &rep) {
     if (hipMemcpy(rep.data(), b->d_rep, sizeof(ReplicaDev) * b->n_rep, hipMemcpyDeviceToHost) != hipSuccess)
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
     for (int r = 0; r < b->n_rep; r++)
-        if (rep[r].err) return ecmc_fail(ECMC_ERR_STATE, "a cell list overflowed (more than 64 units in one cell)");
+        if (rep[r].err == 3) return ecmc_fail(ECMC_ERR_STATE, "a replica's chunk was held by two workgroups");
+        else if (rep[r].err) return ecmc_fail(ECMC_ERR_STATE, "a cell list overflowed (more than 64 units in one cell)");
+
     return ECMC_OK;
 }
 

@@ -38,15 +38,17 @@ struct BatchArgs {
     double *pos64;
     const double *bounds;        // cell lower bounds, per replica cps+1 entries
     const int32_t *order;        // replica indices of this class, largest first
+    int32_t *state;              // per class member: (chunks done << 1) | held
     int32_t count;               // replicas in this class
     int32_t nmax, cellmax;       // LDS sizing of this class
-    int32_t legs;
+    int32_t legs;                // legs per replica of this step
+    int32_t chunk_legs, nchunks; // a work item is one chunk of legs of one replica
     int32_t mode;                // 0 filtered, 1 all candidates exact, 2 filtered + audit
     int32_t log_legs;            // legs logged per replica (0 = none)
     uint32_t seed_lo, pad;
     double beta, chain_length;
     FactorDev fac;
-    int32_t *queue;              // work counter of this class
+    int32_t *queue;              // [0] claim cursor
     ecmc_batch_event *log;       // n_replicas x log_legs
     unsigned long long *audit;   // bound violations
 };
@@ -353,22 +355,57 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
     int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
-    __shared__ int s_item, s_nc, s_overflow, s_nearby[27], s_pref[28], s_M;
+    __shared__ int s_item, s_chunk, s_nc, s_overflow, s_nearby[27], s_pref[28], s_M;
     __shared__ double s_ap[3], s_newpos[3], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
     __shared__ ReplicaDev R;
     __shared__ unsigned long long s_stats[4];
 
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(A.queue, 1);
+        // Claim a work item: a class member that no block holds and that has chunks left.  Wave
+        // 0 probes the members 64 at a time from a rotating cursor and takes one with a
+        // compare-and-swap; when none is free the block ends (the members left are held by
+        // running blocks, which take them on).  Every probe is bounded, so the grid drains.
+        if (wave == 0) {
+            int got = -1, chunk = 0;
+            const uint32_t start = (uint32_t)__shfl(lane == 0 ? atomicAdd(A.queue, 1) : 0, 0);
+            for (int base = 0; base < A.count && got < 0; base += 64) {
+                const int m = (int)((start + (uint32_t)(base + lane)) % (uint32_t)A.count);
+                const bool in = base + lane < A.count;
+                const int st = in ? __hip_atomic_load(A.state + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 1;
+                uint64_t free_mask = __ballot(in && !(st & 1) && (st >> 1) < A.nchunks);
+                while (free_mask && got < 0) {
+                    const int l = __ffsll((unsigned long long)free_mask) - 1;
+                    int won = 0;
+                    if (lane == l) {
+                        int expect = st;
+                        won = __hip_atomic_compare_exchange_strong(A.state + m, &expect, st | 1, __ATOMIC_RELAXED,
+                                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    if (__shfl(won, l)) {
+                        got = __shfl(m, l);
+                        chunk = __shfl(st >> 1, l);
+                    }
+                    free_mask &= free_mask - 1;
+                }
+            }
+            if (lane == 0) {
+                s_item = got;
+                s_chunk = chunk;
+                // acquire: the previous holder's stores (any XCD) are visible from here on
+                if (got >= 0) __threadfence();
+            }
+        }
         bsync();
-        const int item = s_item;
-        if (item >= A.count) break;
-        const int r = A.order[item];
+        const int member = s_item;
+        if (member < 0) break;
+        const int r = A.order[member];
         if (tid == 0) {
             R = A.rep[r];
             for (int q = 0; q < 4; q++) s_stats[q] = 0;
         }
         bsync();
+        const int chunk = s_chunk;
+        const int legs_now = min(A.chunk_legs, A.legs - chunk * A.chunk_legs);
         const int n = R.n, cps = R.cps;
         const double L = R.L;
         const double *P = A.pos64 + R.off;
@@ -409,7 +446,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const int pf0 = (int)pw0, pf1 = (int)pw1;
         const float inv_beta_lo = (float)(1.0 / A.beta) * (1.f - 0x1p-22f);
         bsync();
-        for (int leg_i = 0; leg_i < A.legs && !R.err; leg_i++) {
+        for (int leg_i = 0; leg_i < legs_now && !R.err; leg_i++) {
             const int a = R.active, d = R.direction;
             const uint64_t leg = R.leg;
             const double ap[3] = {s_ap[0], s_ap[1], s_ap[2]};
@@ -702,7 +739,16 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             for (int q = 0; q < 4; q++) R.stats[q] += (int64_t)s_stats[q];
             A.rep[r] = R;
         }
-        bsync();
+        bsync();   // every store of this chunk (positions, log, state) has completed
+        if (tid == 0) {
+            // release: make this block's stores visible device-wide, then free the member with
+            // one more chunk done
+            // (an exchange, whose return is awaited, so this block's next probe sees the update)
+            __threadfence();
+            const int prev = __hip_atomic_exchange(A.state + member, (chunk + 1) << 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+            if (prev != ((chunk << 1) | 1)) A.rep[r].err = 3;   // the claim protocol was broken
+        }
     }
 }
 

@@ -164,3 +164,23 @@ def test_gpu_batched_class_timing(device):
     assert np.array_equal(b1.states()["est"], b2.states()["est"])
     b2.run_graph(150, 1)
     assert all(k == 0 for _, k in b2.class_ms())
+
+
+@pytest.mark.gpu
+def test_gpu_batched_chunks_migrate_between_blocks(device):
+    # two workgroups share five replicas: after every chunk a replica may continue on another
+    # workgroup (another XCD), and the result equals one workgroup per replica, bit for bit
+    classes = ([4096], [1])
+    b1, *_ = _device_batch(classes, 0, 0, seed=29)
+    b2, *_ = _device_batch(classes, 0, 0, seed=29)
+    b2.set_max_blocks(2)
+    print("batched chunk migration", flush=True)
+    for b in (b1, b2):
+        b.run(700)
+        b.run_graph(300, 2)
+    s1, s2 = b1.states(), b2.states()
+    for k in ("active", "direction", "legs", "stats"):
+        assert np.array_equal(s1[k], s2[k])
+    assert np.array_equal(s1["est"], s2["est"])
+    for p1, p2 in zip(b1.positions(), b2.positions()):
+        assert np.array_equal(p1, p2)
