@@ -322,7 +322,7 @@ __device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int
 __host__ __device__ inline size_t batched_lds_bytes(int nmax, int cellmax, int wpr) {
     size_t b = 12ull * nmax + 4ull * cellmax + 2ull * cellmax * B_CELL_CAP;
     b = (b + 7) & ~7ull;
-    b += (24ull + 8ull + 4ull) * B_CONTENDERS;     // positions, random words, identifiers
+    b += (8ull + 4ull) * B_CONTENDERS;             // random words, identifiers
     b += 4ull * B_QUEUE * wpr;
     return (b + 15) & ~15ull;
 }
@@ -350,8 +350,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     int32_t *cell_n = (int32_t *)(pz + A.nmax);
     int16_t *cell_l = (int16_t *)(cell_n + A.cellmax);
     size_t off = (12ull * A.nmax + 4ull * A.cellmax + 2ull * A.cellmax * B_CELL_CAP + 7) & ~7ull;
-    double *cont_pos = (double *)(smem + off);                       // B_CONTENDERS x 3
-    uint32_t *cont_w = (uint32_t *)(cont_pos + 3 * B_CONTENDERS);    // B_CONTENDERS x 2
+    uint32_t *cont_w = (uint32_t *)(smem + off);                     // B_CONTENDERS x 2
     int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
     int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
@@ -497,8 +496,6 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             double bestpos[3] = {0.0, 0.0, 0.0};
             unsigned long long geo_pass = 0;
             // contenders found by this thread whose FP64 position is held in registers
-            int c0 = -1, c1 = -1;
-            double c0x = 0, c0y = 0, c0z = 0, c1x = 0, c1y = 0, c1z = 0;
             if (A.mode != 1) {
                 // ---- pass 1b: bounds of the queued candidates of this wavefront
                 auto bound_queue = [&](int cnt) {
@@ -525,17 +522,6 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                             cont_id[slot] = (j << 1) | f;
                             cont_w[2 * slot] = pick(f, w[0], w[2]);
                             cont_w[2 * slot + 1] = pick(f, w[1], w[3]);
-                            if (c0 < 0) {
-                                c0 = slot;
-                                c0x = P[j]; c0y = P[(size_t)n + j]; c0z = P[2 * (size_t)n + j];
-                            } else if (c1 < 0) {
-                                c1 = slot;
-                                c1x = P[j]; c1y = P[(size_t)n + j]; c1z = P[2 * (size_t)n + j];
-                            } else {
-                                cont_pos[3 * slot] = P[j];
-                                cont_pos[3 * slot + 1] = P[(size_t)n + j];
-                                cont_pos[3 * slot + 2] = P[2 * (size_t)n + j];
-                            }
                         }
                     }
                 };
@@ -570,9 +556,6 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     geo_pass += surv ? 1ull : 0ull;
                 }
                 if (qn > 0) bound_queue(qn);
-                // held positions to LDS, for the thread that evaluates each contender
-                if (c0 >= 0) { cont_pos[3 * c0] = c0x; cont_pos[3 * c0 + 1] = c0y; cont_pos[3 * c0 + 2] = c0z; }
-                if (c1 >= 0) { cont_pos[3 * c1] = c1x; cont_pos[3 * c1 + 1] = c1y; cont_pos[3 * c1 + 2] = c1z; }
                 bsync();
             }
             const bool all_exact = (A.mode == 1) || s_overflow;
@@ -593,7 +576,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 } else {
                     j = cont_id[w] >> 1;
                     f = cont_id[w] & 1;
-                    pj[0] = cont_pos[3 * w]; pj[1] = cont_pos[3 * w + 1]; pj[2] = cont_pos[3 * w + 2];
+                    // FP64 position straight from memory: its latency overlaps the logarithm below
+                    pj[0] = P[j]; pj[1] = P[(size_t)n + j]; pj[2] = P[2 * (size_t)n + j];
                     wa = cont_w[2 * w];
                     wb = cont_w[2 * w + 1];
                 }

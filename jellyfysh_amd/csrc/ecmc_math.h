@@ -200,12 +200,83 @@ ECMC_FN dd dd_exp(dd t) {
     return res;
 }
 
-// Correctly rounded natural logarithm.
+// True when every value within hi + lo +- rel * |hi| rounds to the same double as hi + lo:
+// the correctly rounded result of a quantity known to that accuracy is then hi + lo.
+ECMC_FN bool rounding_settled(double hi, double lo, double rel) {
+    const double e = rel * fabs(hi);
+    return hi + (lo - e) == hi + (lo + e);
+}
+
+// log x as hi + lo with relative error below 2^-64 (2^-68.5 the largest seen against quad
+// precision), for a positive normal x (the fast path of cr_log).  Same reduction as dd_log, x = 2^e m, z = m r_i - 1 = zh + zl exactly, |zh| < 2^-7;
+// log1p(z) = z - z^2/2 + sum_{k>=3} (-1)^(k+1) z^k / k, whose first two terms are summed
+// exactly and whose tail zh^3 (1/3 - zh/4 + ... - zh^7/10) (truncation below zh^11/11) and the
+// first-order zl terms are taken in double.  Relative to |log x| >= 2^-8 (i != 0 or e != 0),
+// or to log1p(zh) when i = e = 0 (zl = 0 there), the double parts contribute below 2^-67.
+ECMC_FN double log_fast(double x, double &lo) {
+    int e;
+    double m = frexp(x, &e);
+    m = m * 2.0;
+    e -= 1;
+    if (m > 0x1.6a09e667f3bcdp+0) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const int ti = (int)rint((m - 1.0) * 128.0) - ECMC_LOG_TMIN;
+    const dd p = two_prod(m, ecmc_log_r[ti]);
+    const double zh = p.h - 1.0, zl = p.l;      // p.h - 1 is exact
+    const dd z2 = two_prod(zh, zh);
+    double q = -1.0 / 10.0;
+    q = fma(q, zh, 1.0 / 9.0);
+    q = fma(q, zh, -1.0 / 8.0);
+    q = fma(q, zh, 1.0 / 7.0);
+    q = fma(q, zh, -1.0 / 6.0);
+    q = fma(q, zh, 1.0 / 5.0);
+    q = fma(q, zh, -1.0 / 4.0);
+    q = fma(q, zh, 1.0 / 3.0);
+    const double tail = (zh * z2.h) * q + zl * (z2.h * (1.0 - zh) - zh);
+    const double ed = (double)e;
+    const dd el = two_prod(ed, ECMC_LN2_H);
+    const dd s1 = two_sum(el.h, ecmc_log_th[ti]);
+    const dd s2 = two_sum(s1.h, zh);
+    const dd s3 = two_sum(s2.h, -0.5 * z2.h);
+    const double l = ((s1.l + s2.l) + s3.l) +
+                     (((el.l + ed * ECMC_LN2_L) + ecmc_log_tl[ti]) + ((zl - 0.5 * z2.l) + tail));
+    const dd r = quick_two_sum(s3.h, l);
+    lo = r.l;
+    return r.h;
+}
+
+// Correctly rounded natural logarithm: the fast path when its rounding is settled (all but
+// about one call in 500), else the double-double path.
 ECMC_FN double cr_log(double x) {
     if (!(x > 0.0) || isinf(x) || x < 0x1p-1022) return log(x);
     if (x == 1.0) return 0.0;
+    double lo;
+    const double hi = log_fast(x, lo);
+    if (rounding_settled(hi, lo, 0x1p-64)) return hi + lo;
     dd l = dd_log(x);
     return l.h;
+}
+
+// x ** y for y the double nearest 1/k, k = 3 or 6 (2/p of the inverse-power inversion for
+// p = 6, 12), as hi + lo with relative error below 2^-92.  t0 = x^(1/k) from cbrt (and sqrt)
+// to about 2^-51; one Newton step t1 = t0 - t0 (t0^k - x) / (k x) with t0^k - x in
+// double-double leaves (k - 1)/2 (t0 error)^2; then x^y = t1 x^delta = t1 (1 + delta log x +
+// O(2^-104)) with delta = y - 1/k = (k y - 1)/k, |delta log x| < 2^-48 for x < 2^1000.
+ECMC_FN double root_pow_fast(double x, double y, int k, double &lo) {
+    const double t0 = k == 3 ? cbrt(x) : cbrt(sqrt(x));
+    const dd t2 = two_prod(t0, t0);
+    const dd t3 = dd_mul_d(t2, t0);
+    const dd tk = k == 3 ? t3 : dd_mul(t3, t3);
+    const dd dlt = dd_sub(tk, dd{x, 0.0});
+    const double corr = t0 * ((dlt.h + dlt.l) / ((double)k * x));
+    const dd t1 = two_sum(t0, -corr);
+    const double delta = fma(y, (double)k, -1.0) / (double)k;
+    const double c = delta * log(x);
+    const dd r = quick_two_sum(t1.h, fma(t1.h, c, t1.l));
+    lo = r.l;
+    return r.h;
 }
 
 // x ** n for an integer 3 <= n <= 32 by binary powering in double-double (each product
@@ -233,6 +304,11 @@ ECMC_FN double cr_pow(double x, double y) {
     if (y == 0.0) return 1.0;
     if (!(x > 0.0) || isinf(x) || !isfinite(y) || x < 0x1p-1022 || x == 1.0) return pow(x, y);
     if (y >= 3.0 && y <= 32.0 && y == rint(y) && x > 0x1p-30 && x < 0x1p30) return dd_pow_int(x, (int)y);
+    if ((y == 1.0 / 3.0 || y == 1.0 / 6.0) && x > 0x1p-900 && x < 0x1p900) {
+        double lo;
+        const double hi = root_pow_fast(x, y, y == 1.0 / 3.0 ? 3 : 6, lo);
+        if (rounding_settled(hi, lo, 0x1p-92)) return hi + lo;
+    }
     dd l = dd_log(x);
     dd t = dd_mul_d(l, y);
     if (fabs(t.h) > 700.0) return pow(x, y);

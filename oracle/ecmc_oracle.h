@@ -66,6 +66,8 @@ double ora_time_sub(double q1, double r1, double q2, double r2);
 /* ---- math switch ---------------------------------------------------------------------- */
 double ora_pow(double x, double y, int math_mode);
 double ora_log(double x, int math_mode);
+void ora_log_n(int64_t n, const double *x, double *out, int math_mode);
+void ora_pow_n(int64_t n, const double *x, const double *y, double *out, int math_mode);
 
 /* ---- potentials (scalar, reference argument conventions) --------------------------- */
 /* HardSpherePotential.displacement(velocity, separation) */

@@ -171,6 +171,14 @@ double ora_log(double x, int math_mode) {
     return log(x);
 }
 
+/* Array forms of ora_log / ora_pow, for checks over millions of arguments. */
+void ora_log_n(int64_t n, const double *x, double *out, int math_mode) {
+    for (int64_t i = 0; i < n; i++) out[i] = ora_log(x[i], math_mode);
+}
+void ora_pow_n(int64_t n, const double *x, const double *y, double *out, int math_mode) {
+    for (int64_t i = 0; i < n; i++) out[i] = ora_pow(x[i], y[i], math_mode);
+}
+
 /* ===== vectors (base/vectors.py) ======================================================== */
 static double v_norm_sq(int dim, const double *s) {
     double acc = 0.0;             /* sum() starts from int 0 */

@@ -1,0 +1,79 @@
+"""The fast paths of the correctly rounded log and pow (jellyfysh_amd/csrc/ecmc_math.h), built
+for the host from the same header the kernels include, against the oracle's quad-precision
+rounding (oracle/ecmc_oracle_math.c ora_log / ora_pow in correctly rounded mode).
+
+Pins: every result equals the correctly rounded one; the fast log's own error (hi + lo
+against quad precision) stays below the 2^-64 its rounding test assumes, and the k-th root
+path's below 2^-92; the rounding test sends only a small share of calls to the slow path."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROGRAM = r"""
+#define ECMC_HOST_BUILD 1
+#include <stdio.h>
+#include <stdlib.h>
+#include <quadmath.h>
+#include "ecmc_math.h"
+extern "C" double ora_pow(double x, double y, int math_mode);
+extern "C" double ora_log(double x, int math_mode);
+using namespace ecmc;
+static unsigned long long st = 88172645463325252ull;
+static double unif() {                       /* xorshift64, 53-bit uniform in [0, 1) */
+    st ^= st << 13; st ^= st >> 7; st ^= st << 17;
+    return (double)(st >> 11) * 0x1p-53;
+}
+int main(int argc, char **argv) {
+    long n = atol(argv[1]), bad_log = 0, bad_pow = 0, slow_log = 0, slow_pow = 0;
+    __float128 worst_log = 0, worst_pow = 0;
+    for (long i = 0; i < n; i++) {
+        double x;
+        switch (i % 5) {
+        case 0: x = 1.0 - unif(); break;
+        case 1: x = exp(unif() * 80.0 - 40.0); break;
+        case 2: x = 1.0 + (unif() - 0.5) * 0x1p-6; break;
+        case 3: x = 1.0 + (unif() - 0.5) * 0x1p-30; break;
+        default: x = ldexp(1.0 + unif(), (int)(unif() * 2000) - 1000); break;
+        }
+        if (!(x > 0.0) || x == 1.0) continue;
+        double lo, hi = log_fast(x, lo);
+        __float128 t = logq((__float128)x), e = fabsq(((__float128)hi + lo - t) / t);
+        if (e > worst_log) worst_log = e;
+        slow_log += !rounding_settled(hi, lo, 0x1p-64);
+        bad_log += cr_log(x) != ora_log(x, 1);
+        double y = (i & 1) ? 2.0 / 12.0 : 2.0 / 6.0, b = exp(unif() * 30.0 - 8.0);
+        double plo, phi = root_pow_fast(b, y, y == 1.0 / 3.0 ? 3 : 6, plo);
+        __float128 tp = powq((__float128)b, (__float128)y), ep = fabsq(((__float128)phi + plo - tp) / tp);
+        if (ep > worst_pow) worst_pow = ep;
+        slow_pow += !rounding_settled(phi, plo, 0x1p-92);
+        bad_pow += cr_pow(b, y) != ora_pow(b, y, 1);
+    }
+    printf("%ld %ld %ld %ld %.3f %.3f\n", bad_log, bad_pow, slow_log, slow_pow, (double)log2q(worst_log),
+           (double)log2q(worst_pow));
+    return 0;
+}
+"""
+
+
+def test_fast_log_and_root_pow_round_correctly(tmp_path):
+    gxx = shutil.which("g++")
+    oracle_lib = os.path.join(ROOT, "oracle", "_build", "libecmc_oracle.so")
+    if gxx is None or not os.path.exists(oracle_lib):
+        pytest.skip("needs g++ and the built oracle")
+    src = tmp_path / "fast_math.cpp"
+    src.write_text(PROGRAM)
+    exe = tmp_path / "fast_math"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "jellyfysh_amd", "csrc"),
+                    str(src), "-o", str(exe), oracle_lib, "-Wl,-rpath," + os.path.dirname(oracle_lib),
+                    "-lquadmath"], check=True, capture_output=True)
+    n = 400_000
+    out = subprocess.run([str(exe), str(n)], capture_output=True, text=True, check=True).stdout.split()
+    bad_log, bad_pow, slow_log, slow_pow = (int(v) for v in out[:4])
+    worst_log, worst_pow = float(out[4]), float(out[5])
+    assert bad_log == 0 and bad_pow == 0
+    assert worst_log < -64.0 and worst_pow < -92.0
+    assert slow_log < 0.005 * n and slow_pow == 0

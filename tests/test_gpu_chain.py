@@ -61,6 +61,43 @@ def test_device_log_pow_correctly_rounded(device, oracle):
     assert np.array_equal(got, want)
 
 
+def _oracle_cr(oracle, fn, *arrays):
+    import ctypes
+    pd = ctypes.POINTER(ctypes.c_double)
+    arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in arrays]
+    out = np.empty_like(arrays[0])
+    getattr(oracle.lib(), fn)(len(out), *[a.ctypes.data_as(pd) for a in arrays], out.ctypes.data_as(pd),
+                              oracle.MATH_CR)
+    return out
+
+
+def test_device_log_pow_correctly_rounded_at_scale(device, oracle):
+    # the fast paths (double-length log with a rounding test; k-th root plus one Newton step
+    # for 2/p = 1/3, 1/6) against quad precision over millions of the arguments the inversion
+    # meets: 1 - u for the event draws, and (k / (U + dU)) ** (2/p)
+    import torch
+    rng = np.random.default_rng(11)
+    x = np.concatenate([1.0 - rng.random(1_000_000), np.exp(rng.uniform(-40, 40, 500_000)),
+                        1.0 + rng.uniform(-2 ** -7, 2 ** -7, 500_000)])
+    xt = torch.tensor(x, device="cuda")
+    out = torch.empty_like(xt)
+    print("device log at scale", len(x), flush=True)
+    device.check(device.lib().ecmc_math_log(len(x), device.dptr(xt), device.dptr(out), None))
+    got = out.cpu().numpy()
+    want = _oracle_cr(oracle, "ora_log_n", x)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, "log differs at x = %r" % x[bad[:5]]
+    base = np.exp(rng.uniform(-8.0, 22.0, 2_000_000))
+    ys = np.where(rng.random(len(base)) < 0.5, 2.0 / 12.0, 2.0 / 6.0)
+    bt, yt = torch.tensor(base, device="cuda"), torch.tensor(ys, device="cuda")
+    out = torch.empty_like(bt)
+    device.check(device.lib().ecmc_math_pow(len(base), device.dptr(bt), device.dptr(yt), device.dptr(out), None))
+    got = out.cpu().numpy()
+    want = _oracle_cr(oracle, "ora_pow_n", base, ys)
+    bad = np.flatnonzero(got != want)
+    assert bad.size == 0, "pow differs at x = %r" % base[bad[:5]]
+
+
 @pytest.mark.parametrize("name", ["hard_disks_n64", "lj_two_factor_n256", "lj_mexican_hat_n108"])
 def test_gpu_chain_matches_oracle_on_reference_fixture(device, oracle, name):
     from jellyfysh_amd.chain import DeviceChain
