@@ -90,19 +90,25 @@ constexpr float ETA = 0x1p-17f, ETA_POW = 0x1p-14f;
 __device__ __forceinline__ float up(float x) { return x >= 0.f ? x * (1.f + ETA) : x * (1.f - ETA); }
 __device__ __forceinline__ float dn(float x) { return x >= 0.f ? x * (1.f - ETA) : x * (1.f + ETA); }
 __device__ __forceinline__ float sq_lo(float a_abs_lo) { return dn(a_abs_lo * a_abs_lo); }
+// Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32: within 1 ulp = 2^-23 relative,
+// far inside the ETA margins); below 2^-100 the root is bracketed by [0, 2^-50] directly.
+__device__ __forceinline__ float rcp1(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float sqrt_up(float x) { return x > 0x1p-100f ? up(__builtin_amdgcn_sqrtf(x)) : 0x1p-50f; }
+__device__ __forceinline__ float sqrt_dn(float x) { return x > 0x1p-100f ? dn(__builtin_amdgcn_sqrtf(x)) : 0.f; }
 
 // V(r2) = K / r2^(p/2) with p in {6, 12} (integer powers by multiplication), bounded up
-// (hi) or down for an r2 that is itself a lower/upper bound.
+// (hi) or down for an r2 that is itself a lower/upper bound.  Three products (2^-24 each) and
+// K * rcp(rp) (3 x 2^-24) stay inside the 8 x 2^-24 allowance.
 __device__ __forceinline__ float vpow_bound(float K, float r2, int half_p, bool hi) {
     float r3 = r2 * r2 * r2;
     float rp = half_p == 6 ? r3 * r3 : r3;
-    float v = K / rp;
+    float v = K * rcp1(rp);
     return hi ? v * (1.f + 8.f * 0x1p-24f) * (1.f + ETA) : v * (1.f - 8.f * 0x1p-24f) * (1.f - ETA);
 }
 
 // (K / V)^(2/p) bounded up (hi) or down.
 __device__ __forceinline__ float rstar2_bound(float K, float V, float two_over_p, bool hi) {
-    float q = K / V;
+    float q = K * rcp1(V);
     float r = exp2f(two_over_p * log2f(q));
     return hi ? r * (1.f + ETA_POW) : r * (1.f - ETA_POW);
 }
@@ -116,7 +122,7 @@ struct Geo {
 __device__ float factor_lower_bound(const Geo &g, float k, int p, float rc2_hi, float vrc_lo, float dU_lo) {
     const float INF = __int_as_float(0x7f800000);
     const int half_p = p / 2;
-    const float two_over_p = 2.f / (float)p;
+    const float two_over_p = half_p == 6 ? 1.f / 6.f : 1.f / 3.f;
     if (g.perp2_lo >= rc2_hi) return INF;
     if (k > 0.f) {
         if (g.sd_hi <= 0.f) return INF;
@@ -126,7 +132,7 @@ __device__ float factor_lower_bound(const Geo &g, float k, int p, float rc2_hi, 
         if (dU_lo >= room_hi) return INF;
         float ut_lo = dn(ueff_lo + dU_lo);
         float rs2_hi = rstar2_bound(k, ut_lo, two_over_p, true);
-        float x_hi = up(sqrtf(fmaxf(up(rs2_hi - g.perp2_lo), 0.f)));
+        float x_hi = sqrt_up(fmaxf(up(rs2_hi - g.perp2_lo), 0.f));
         float d = dn(g.sd_lo) - x_hi;
         d = d - fabsf(d) * 0x1p-22f;
         return fmaxf(d, 0.f);
@@ -141,7 +147,7 @@ __device__ float factor_lower_bound(const Geo &g, float k, int p, float rc2_hi, 
     float vt_hi = up(vcur_hi - dU_lo);
     if (!(vt_hi > 0.f)) return 0.f;
     float rs2_lo = rstar2_bound(K, vt_hi, two_over_p, false);
-    float y_lo = dn(sqrtf(fmaxf(dn(rs2_lo - g.perp2_hi), 0.f)));
+    float y_lo = sqrt_dn(fmaxf(dn(rs2_lo - g.perp2_hi), 0.f));
     float d = g.sd_lo + y_lo;
     d = d - fabsf(d) * 0x1p-22f - 0x1p-40f;
     return fmaxf(d, 0.f);
@@ -153,7 +159,7 @@ __device__ float factor_lower_bound(const Geo &g, float k, int p, float rc2_hi, 
 __device__ __forceinline__ float factor_geometric_floor(const Geo &g, float k, float rc2_hi) {
     if (k > 0.f) {
         if (g.r2_lo < rc2_hi) return 0.f;
-        float entry_hi = up(sqrtf(fmaxf(up(rc2_hi - g.perp2_lo), 0.f)));
+        float entry_hi = sqrt_up(fmaxf(up(rc2_hi - g.perp2_lo), 0.f));
         float d = dn(g.sd_lo) - entry_hi;
         d = d - fabsf(d) * 0x1p-22f;
         return fmaxf(d, 0.f);
@@ -177,16 +183,20 @@ __device__ __forceinline__ bool bkey_eq(const BKey &a, const BKey &b) {
 
 template <int WPR>
 __device__ void block_min(BKey &b, BKey *sk) {
+    // (rank, j, f) packed into one word orders like the triple (rank < 4, j < 2^27, f < 2)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double d = b.d;
+    int k = (b.rank << 28) | (b.j << 1) | b.f;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
-        BKey o;
-        o.d = __shfl_xor(b.d, off, 64);
-        o.rank = __shfl_xor(b.rank, off, 64);
-        o.j = __shfl_xor(b.j, off, 64);
-        o.f = __shfl_xor(b.f, off, 64);
-        if (bkey_lt(o, b)) b = o;
+        const double od = __shfl_xor(d, off, 64);
+        const int ok = __shfl_xor(k, off, 64);
+        if (od < d || (od == d && ok < k)) { d = od; k = ok; }
     }
+    b.d = d;
+    b.rank = k >> 28;
+    b.j = (k >> 1) & 0x7ffffff;
+    b.f = k & 1;
     if (WPR > 1) {
         if (lane == 0) sk[wave] = b;
         __syncthreads();
@@ -322,7 +332,7 @@ __device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int
 __host__ __device__ inline size_t batched_lds_bytes(int nmax, int cellmax, int wpr) {
     size_t b = 12ull * nmax + 4ull * cellmax + 2ull * cellmax * B_CELL_CAP;
     b = (b + 7) & ~7ull;
-    b += (8ull + 4ull) * B_CONTENDERS;             // random words, identifiers
+    b += (24ull + 8ull + 4ull) * B_CONTENDERS;     // positions, random words, identifiers
     b += 4ull * B_QUEUE * wpr;
     return (b + 15) & ~15ull;
 }
@@ -350,7 +360,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     int32_t *cell_n = (int32_t *)(pz + A.nmax);
     int16_t *cell_l = (int16_t *)(cell_n + A.cellmax);
     size_t off = (12ull * A.nmax + 4ull * A.cellmax + 2ull * A.cellmax * B_CELL_CAP + 7) & ~7ull;
-    uint32_t *cont_w = (uint32_t *)(smem + off);                     // B_CONTENDERS x 2
+    double *cont_pos = (double *)(smem + off);                       // B_CONTENDERS x 3
+    uint32_t *cont_w = (uint32_t *)(cont_pos + 3 * B_CONTENDERS);    // B_CONTENDERS x 2
     int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
     int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
@@ -492,20 +503,25 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             const double h = s_h;
             const float apf[3] = {(float)ap[0], (float)ap[1], (float)ap[2]};
             const float hf = (float)h * (1.f + 0x1p-22f) + 0x1p-60f;
-            BKey best = {INFINITY, 3, 0x7fffffff, 0};
+            BKey best = {INFINITY, 3, 0x7ffffff, 0};   // j < 2^27: the reduction packs (rank, j, f)
             double bestpos[3] = {0.0, 0.0, 0.0};
-            unsigned long long geo_pass = 0;
-            // contenders found by this thread whose FP64 position is held in registers
+            unsigned geo_pass = 0;   // geometric survivors of this wavefront (uniform)
             if (A.mode != 1) {
                 // ---- pass 1b: bounds of the queued candidates of this wavefront
                 auto bound_queue = [&](int cnt) {
                     if (lane >= cnt) return;
                     const int j = queue[lane];
+                    // the FP64 position, needed if a factor turns out a contender: its memory
+                    // latency overlaps the bound computation below
+                    const double qx = P[j], qy = P[(size_t)n + j], qz = P[2 * (size_t)n + j];
+                    int slot0 = -1, slot1 = -1;
                     Geo g;
                     const bool amb = cand_geometry(px[j] - apf[0], py[j] - apf[1], pz[j] - apf[2], d, Lf, E, g);
                     uint32_t w[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, w);
-                    for (int f = 0; f < nf; f++) {
+#pragma unroll
+                    for (int f = 0; f < 2; f++) {
+                        if (f >= nf) break;
                         float t = 0.f;
                         if (!amb) {
                             if (g.perp2_lo >= rc2_hi) continue;
@@ -522,8 +538,12 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                             cont_id[slot] = (j << 1) | f;
                             cont_w[2 * slot] = pick(f, w[0], w[2]);
                             cont_w[2 * slot + 1] = pick(f, w[1], w[3]);
+                            if (f == 0) slot0 = slot;
+                            else slot1 = slot;
                         }
                     }
+                    if (slot0 >= 0) { cont_pos[3 * slot0] = qx; cont_pos[3 * slot0 + 1] = qy; cont_pos[3 * slot0 + 2] = qz; }
+                    if (slot1 >= 0) { cont_pos[3 * slot1] = qx; cont_pos[3 * slot1 + 1] = qy; cont_pos[3 * slot1 + 2] = qz; }
                 };
                 // ---- pass 1a: geometric test of every candidate, survivors queued
                 int qn = 0;
@@ -538,8 +558,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                         if (amb) {
                             surv = true;
                         } else if (g.perp2_lo < rc2_hi) {
-                            for (int f = 0; f < nf; f++)
-                                if (factor_geometric_floor(g, pick(f, kf0, kf1), rc2_hi) <= hf) surv = true;
+                            surv = factor_geometric_floor(g, kf0, rc2_hi) <= hf ||
+                                   (nf > 1 && factor_geometric_floor(g, kf1, rc2_hi) <= hf);
                         }
                     }
                     const unsigned long long m = __ballot(surv);
@@ -553,7 +573,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                         if (lane < rem) queue[lane] = v;
                         qn = rem;
                     }
-                    geo_pass += surv ? 1ull : 0ull;
+                    geo_pass += (unsigned)__popcll(m);
                 }
                 if (qn > 0) bound_queue(qn);
                 bsync();
@@ -576,8 +596,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 } else {
                     j = cont_id[w] >> 1;
                     f = cont_id[w] & 1;
-                    // FP64 position straight from memory: its latency overlaps the logarithm below
-                    pj[0] = P[j]; pj[1] = P[(size_t)n + j]; pj[2] = P[2 * (size_t)n + j];
+                    pj[0] = cont_pos[3 * w]; pj[1] = cont_pos[3 * w + 1]; pj[2] = cont_pos[3 * w + 2];
                     wa = cont_w[2 * w];
                     wb = cont_w[2 * w + 1];
                 }
@@ -627,7 +646,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 s_stats[2] += (unsigned long long)min(s_nc, B_CONTENDERS);
                 s_stats[3] += (unsigned long long)nwork;
             }
-            if (geo_pass) atomicAdd(&s_stats[1], geo_pass);
+            if (lane == 0 && geo_pass) atomicAdd(&s_stats[1], (unsigned long long)geo_pass);
             bsync();
             // ---- commit (wavefront 0; the winner's position is in s_newpos): the values below
             // are uniform across the wavefront, a ballot finds the new active unit's slot in its
