@@ -238,49 +238,35 @@ __device__ __forceinline__ float dU_lower(uint32_t a, float inv_beta_lo) {
 }
 
 // Exact FP64 event displacement of one truncated-and-shifted inverse-power factor
-// (k / r^p - k / rc^p inside rc) for separation s (target minus active, minimum image)
-// along +d with unit speed: the same operations, in the same order, as ipt_displacement
-// (and so the oracle), arranged so that lanes of the repulsive and of the attractive
-// factor share the calls of the correctly rounded pow.  urc = k / rc^p (FP64, as computed
-// by ipt_displacement).
-#ifndef ECMC_EXACT_INLINE
-#define ECMC_EXACT_INLINE __forceinline__
-#endif
+// (k / r^p - k / rc^p inside rc), p = 12 (six: r^p = (r^2)^6) or 6, for separation s (target
+// minus active, minimum image) along +d with unit speed: the values of ipt_displacement (and
+// so of the oracle), each pow correctly rounded.  The repulsive (k > 0) and attractive forms
+// run as one instruction stream with selects, so lanes of a wavefront evaluating different
+// factors do not diverge; sum(v ** 2 for v in others) is also the maximum potential's
+// squared distance (the d component set to zero adds an exact 0).  urc = k / rc^p.
 #ifndef ECMC_BATCHED_WAVES_PER_SIMD
-#define ECMC_BATCHED_WAVES_PER_SIMD 1
+#define ECMC_BATCHED_WAVES_PER_SIMD 1   // minimum waves per SIMD the register allocation targets
 #endif
-__device__ ECMC_EXACT_INLINE double exact_ipt(double s0, double s1, double s2, int d, double power, double prefactor,
+__device__ __forceinline__ double exact_ipt(double s0, double s1, double s2, int d, bool six, double prefactor,
                                             double rc2, double urc, double dU) {
-    double s[3] = {s0, s1, s2};
-    const double perp2 = others_sq<3>(s, d);
-    if (perp2 >= rc2) return INFINITY;
+    const double perp2 = d == 0 ? s1 * s1 + s2 * s2 : (d == 1 ? s0 * s0 + s2 * s2 : s0 * s0 + s1 * s1);
     const bool rep = prefactor > 0.0;
-    double disp = 0.0;
-    bool live = true;
-    if (rep) {
-        if (s[d] <= 0.0) live = false;
-    } else if (s[d] > 0.0) {
-        disp += s[d];
-        s[d] = 0.0;
-    }
-    const double sn2 = norm_sq<3>(s);
-    if (!rep && sn2 >= rc2) live = false;
-    if (!live) return INFINITY;
-    double cur = urc;
-    if (!rep || sn2 < rc2) cur = prefactor / cr_pow(sn2, power / 2.0);  // ip_potential with cp = 1
-    if (rep) {
-        double smax[3] = {s[0], s[1], s[2]};
-        smax[d] = 0.0;
-        const double maxp = prefactor / cr_pow(norm_sq<3>(smax), power / 2.0);
-        live = dU < maxp - cur;
-    } else {
-        live = !(cur + dU >= urc);
-    }
-    if (!live) return INFINITY;
-    const double nsq = cr_pow(prefactor / (cur + dU), 2.0 / power);
+    const double sd = d == 0 ? s0 : (d == 1 ? s1 : s2);
+    // attractive: a unit ahead is first passed (its closest approach is at s[d] = 0)
+    const bool shift = !rep && sd > 0.0;
+    const double disp = shift ? sd : 0.0;
+    const double sdd = shift ? 0.0 : sd;
+    const double c0 = d == 0 ? sdd : s0, c1 = d == 1 ? sdd : s1, c2 = d == 2 ? sdd : s2;
+    const double sn2 = (0.0 + c0 * c0 + c1 * c1) + c2 * c2;
+    bool live = perp2 < rc2 && (rep ? sd > 0.0 : sn2 < rc2);
+    const double cur = (rep && !(sn2 < rc2)) ? urc : prefactor / pow36(live ? sn2 : 1.0, six);
+    const double maxp = prefactor / pow36(rep && live ? perp2 : 1.0, six);
+    live = live && (rep ? dU < maxp - cur : !(cur + dU >= urc));
+    const double nsq = root36(live ? prefactor / (cur + dU) : 1.0, six);
     const double a = nsq - perp2;
-    if (a < 0.0) return NAN;
-    return rep ? s[d] - sqrt(a) : disp + (s[d] + sqrt(a));
+    const double sq = sqrt(fmax(a, 0.0));
+    const double res = a < 0.0 ? NAN : (rep ? sdd - sq : disp + (sdd + sq));
+    return live ? res : INFINITY;
 }
 
 // FP32 geometry of a candidate: separation bounds with the position slack E, and whether
@@ -603,7 +589,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 const double u = unit53(wa, wb);
                 const double dU = -cr_log(1.0 - u) / A.beta;
                 const double dx = exact_ipt(bmin_image(pj[0] - ap[0], L), bmin_image(pj[1] - ap[1], L),
-                                            bmin_image(pj[2] - ap[2], L), d, pick(f, pw0, pw1), pick(f, pk0, pk1),
+                                            bmin_image(pj[2] - ap[2], L), d, pick(f, pf0, pf1) == 12, pick(f, pk0, pk1),
                                             rc2, pick(f, urc0, urc1), dU);
                 BKey c = {dx, 0, j, f};
                 if (bkey_lt(c, best)) {
@@ -631,7 +617,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
                     const double dU = -cr_log(1.0 - unit53(pick(f, ww[0], ww[2]), pick(f, ww[1], ww[3]))) / A.beta;
                     const double dx = exact_ipt(bmin_image(P[j] - ap[0], L), bmin_image(P[(size_t)n + j] - ap[1], L),
-                                                bmin_image(P[2 * (size_t)n + j] - ap[2], L), d, pick(f, pw0, pw1),
+                                                bmin_image(P[2 * (size_t)n + j] - ap[2], L), d, pick(f, pf0, pf1) == 12,
                                                 pick(f, pk0, pk1), rc2, pick(f, urc0, urc1), dU);
                     if (dx <= h) {
                         bool found = false;

@@ -260,20 +260,26 @@ ECMC_FN double cr_log(double x) {
 }
 
 // x ** y for y the double nearest 1/k, k = 3 or 6 (2/p of the inverse-power inversion for
-// p = 6, 12), as hi + lo with relative error below 2^-92.  t0 = x^(1/k) from cbrt (and sqrt)
-// to about 2^-51; one Newton step t1 = t0 - t0 (t0^k - x) / (k x) with t0^k - x in
+// p = 6, 12), as hi + lo with relative error below 2^-92.  t0 = x^(1/k) from cbrt (of sqrt x
+// for k = 6) to about 2^-51; one Newton step t1 = t0 - t0 (t0^k - x) / (k x) with t0^k - x in
 // double-double leaves (k - 1)/2 (t0 error)^2; then x^y = t1 x^delta = t1 (1 + delta log x +
-// O(2^-104)) with delta = y - 1/k = (k y - 1)/k, |delta log x| < 2^-48 for x < 2^1000.
-ECMC_FN double root_pow_fast(double x, double y, int k, double &lo) {
-    const double t0 = k == 3 ? cbrt(x) : cbrt(sqrt(x));
+// O(2^-104)) with delta = y - 1/k = (k y - 1)/k, |delta log x| < 2^-48 for x < 2^1000, log x
+// from the fast path above.  k = 3 and 6 share one instruction stream (selects, no branches),
+// so lanes of a wavefront evaluating different powers do not diverge.
+ECMC_FN double root_pow_fast(double x, bool six, double &lo) {
+    const double k = six ? 6.0 : 3.0;
+    const double y = six ? 1.0 / 6.0 : 1.0 / 3.0;
+    const double t0 = cbrt(six ? sqrt(x) : x);
     const dd t2 = two_prod(t0, t0);
     const dd t3 = dd_mul_d(t2, t0);
-    const dd tk = k == 3 ? t3 : dd_mul(t3, t3);
+    const dd t6 = dd_mul(t3, t3);
+    const dd tk = six ? t6 : t3;
     const dd dlt = dd_sub(tk, dd{x, 0.0});
-    const double corr = t0 * ((dlt.h + dlt.l) / ((double)k * x));
+    const double corr = t0 * ((dlt.h + dlt.l) / (k * x));
     const dd t1 = two_sum(t0, -corr);
-    const double delta = fma(y, (double)k, -1.0) / (double)k;
-    const double c = delta * log(x);
+    const double delta = fma(y, k, -1.0) / k;
+    double llo;
+    const double c = delta * log_fast(x, llo);
     const dd r = quick_two_sum(t1.h, fma(t1.h, c, t1.l));
     lo = r.l;
     return r.h;
@@ -306,7 +312,7 @@ ECMC_FN double cr_pow(double x, double y) {
     if (y >= 3.0 && y <= 32.0 && y == rint(y) && x > 0x1p-30 && x < 0x1p30) return dd_pow_int(x, (int)y);
     if ((y == 1.0 / 3.0 || y == 1.0 / 6.0) && x > 0x1p-900 && x < 0x1p900) {
         double lo;
-        const double hi = root_pow_fast(x, y, y == 1.0 / 3.0 ? 3 : 6, lo);
+        const double hi = root_pow_fast(x, y == 1.0 / 6.0, lo);
         if (rounding_settled(hi, lo, 0x1p-92)) return hi + lo;
     }
     dd l = dd_log(x);
@@ -314,6 +320,26 @@ ECMC_FN double cr_pow(double x, double y) {
     if (fabs(t.h) > 700.0) return pow(x, y);
     dd e = dd_exp(t);
     return e.h;
+}
+
+// x ** 3 or x ** 6 (six) in double-double, rounded once: x^2 exact, x^3 = x^2 x and
+// x^6 = (x^3)^2 each about 2^-104 relative, for 2^-30 < x < 2^30 (else cr_pow).
+ECMC_FN double pow36(double x, bool six) {
+    if (!(x > 0x1p-30 && x < 0x1p30)) return cr_pow(x, six ? 6.0 : 3.0);
+    const dd x2 = two_prod(x, x);
+    const dd x3 = dd_mul_d(x2, x);
+    const dd x6 = dd_mul(x3, x3);
+    return six ? x6.h : x3.h;
+}
+
+// Correctly rounded x ** (1/3) or x ** (1/6) for the double exponents 1.0/3.0 and 1.0/6.0.
+ECMC_FN double root36(double x, bool six) {
+    if (x > 0x1p-900 && x < 0x1p900) {
+        double lo;
+        const double hi = root_pow_fast(x, six, lo);
+        if (rounding_settled(hi, lo, 0x1p-92)) return hi + lo;
+    }
+    return cr_pow(x, six ? 1.0 / 6.0 : 1.0 / 3.0);
 }
 
 }  // namespace ecmc

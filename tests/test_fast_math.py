@@ -4,7 +4,8 @@ rounding (oracle/ecmc_oracle_math.c ora_log / ora_pow in correctly rounded mode)
 
 Pins: every result equals the correctly rounded one; the fast log's own error (hi + lo
 against quad precision) stays below the 2^-64 its rounding test assumes, and the k-th root
-path's below 2^-92; the rounding test sends only a small share of calls to the slow path."""
+path's below 2^-92; the rounding test sends only a small share of calls to the slow path;
+the double-double x^3 and x^6 of the inversion round correctly too."""
 import os
 import shutil
 import subprocess
@@ -46,11 +47,16 @@ int main(int argc, char **argv) {
         slow_log += !rounding_settled(hi, lo, 0x1p-64);
         bad_log += cr_log(x) != ora_log(x, 1);
         double y = (i & 1) ? 2.0 / 12.0 : 2.0 / 6.0, b = exp(unif() * 30.0 - 8.0);
-        double plo, phi = root_pow_fast(b, y, y == 1.0 / 3.0 ? 3 : 6, plo);
+        double plo, phi = root_pow_fast(b, y == 1.0 / 6.0, plo);
         __float128 tp = powq((__float128)b, (__float128)y), ep = fabsq(((__float128)phi + plo - tp) / tp);
         if (ep > worst_pow) worst_pow = ep;
         slow_pow += !rounding_settled(phi, plo, 0x1p-92);
         bad_pow += cr_pow(b, y) != ora_pow(b, y, 1);
+        bad_pow += root36(b, y == 1.0 / 6.0) != ora_pow(b, y, 1);
+        /* the integer powers of the inversion: (r^2)^6 and (r^2)^3 */
+        double r2 = exp(unif() * 6.0 - 3.0);
+        bad_pow += pow36(r2, true) != ora_pow(r2, 6.0, 1);
+        bad_pow += pow36(r2, false) != ora_pow(r2, 3.0, 1);
     }
     printf("%ld %ld %ld %ld %.3f %.3f\n", bad_log, bad_pow, slow_log, slow_pow, (double)log2q(worst_log),
            (double)log2q(worst_pow));
