@@ -9,8 +9,8 @@
 namespace ecmc {
 
 constexpr int B_MAX_WPR = 8;
-constexpr int B_CELL_CAP = 64;       // occupants per cell
-constexpr int B_CONTENDERS = 128;    // contender list capacity per leg
+constexpr int B_CELL_CAP = 64;       // occupants per cell (lattice starts put up to 64 in one)
+constexpr int B_CONTENDERS = 48;     // contenders per leg (about 7 typical; beyond, all candidates go exact)
 constexpr int B_QUEUE = 128;         // per-wavefront queue of candidates past the geometric test
 constexpr int B_MAX_CPS = 8;         // cells per side handled in LDS (512 cells)
 
@@ -269,17 +269,31 @@ __device__ __forceinline__ double exact_ipt(double s0, double s1, double s2, int
     return live ? res : INFINITY;
 }
 
-// FP32 geometry of a candidate: separation bounds with the position slack E, and whether
-// the minimum image is ambiguous (then the candidate is forced into the contender list).
+// Positions in LDS: 21-bit fixed point per coordinate, q = round(x 2^21 / L) mod 2^21 (x in
+// [0, L)), three to a 64-bit word.  The separation q_j - q_a wrapped into [-2^20, 2^20) is the
+// minimum image exactly in integers; times L 2^-21 it is within L 2^-21 (two roundings of
+// L 2^-22) plus L 2^-24 (the float product) of the true minimum-image separation, unless that
+// is within the same distance of +-L/2.  The slack E = L 2^-20 covers both.
+constexpr uint32_t QMASK = (1u << 21) - 1u;
+__device__ __forceinline__ uint64_t quant3(double x, double y, double z, double inv_step) {
+    const uint64_t qx = (uint64_t)(x * inv_step + 0.5) & QMASK, qy = (uint64_t)(y * inv_step + 0.5) & QMASK,
+                   qz = (uint64_t)(z * inv_step + 0.5) & QMASK;
+    return qx | (qy << 21) | (qz << 42);
+}
+__device__ __forceinline__ float qsep(uint64_t w, int c, int qa, float step) {
+    const int dq = (int)(((uint32_t)(w >> (21 * c)) - (uint32_t)qa + (1u << 20)) & QMASK) - (1 << 20);
+    return (float)dq * step;
+}
+
+// FP32 geometry of a candidate from its minimum-image separation: bounds with the position
+// slack E, and whether the minimum image is ambiguous (then the candidate is forced into the
+// contender list).
 __device__ __forceinline__ bool cand_geometry(float dx, float dy, float dz, int d, float Lf, float E, Geo &g) {
     float sv[3] = {dx, dy, dz};
     bool ambiguous = false;
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-        if (sv[q] >= 0.5f * Lf) sv[q] -= Lf;
-        else if (sv[q] < -0.5f * Lf) sv[q] += Lf;
+    for (int q = 0; q < 3; q++)
         if (fabsf(fabsf(sv[q]) - 0.5f * Lf) <= 2.f * E) ambiguous = true;
-    }
     const float sd = d == 0 ? sv[0] : (d == 1 ? sv[1] : sv[2]);
     g.sd_lo = sd - E;
     g.sd_hi = sd + E;
@@ -313,10 +327,10 @@ __device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int
     return cell_l[(size_t)nearby[lo] * B_CELL_CAP + (idx - pref[lo])];
 }
 
-// LDS bytes of the dynamic part for a class (FP32 positions, cell lists, contenders and
+// LDS bytes of the dynamic part for a class (fixed-point positions, cell lists, contenders and
 // the per-wavefront candidate queues); the host sizes launches with the same function.
 __host__ __device__ inline size_t batched_lds_bytes(int nmax, int cellmax, int wpr) {
-    size_t b = 12ull * nmax + 4ull * cellmax + 2ull * cellmax * B_CELL_CAP;
+    size_t b = 8ull * nmax + 4ull * cellmax + 2ull * cellmax * B_CELL_CAP;
     b = (b + 7) & ~7ull;
     b += (24ull + 8ull + 4ull) * B_CONTENDERS;     // positions, random words, identifiers
     b += 4ull * B_QUEUE * wpr;
@@ -340,12 +354,10 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NTH = 64 * WPR;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    float *px = (float *)smem;
-    float *py = px + A.nmax;
-    float *pz = py + A.nmax;
-    int32_t *cell_n = (int32_t *)(pz + A.nmax);
+    uint64_t *pq = (uint64_t *)smem;
+    int32_t *cell_n = (int32_t *)(pq + A.nmax);
     int16_t *cell_l = (int16_t *)(cell_n + A.cellmax);
-    size_t off = (12ull * A.nmax + 4ull * A.cellmax + 2ull * A.cellmax * B_CELL_CAP + 7) & ~7ull;
+    size_t off = (8ull * A.nmax + 4ull * A.cellmax + 2ull * A.cellmax * B_CELL_CAP + 7) & ~7ull;
     double *cont_pos = (double *)(smem + off);                       // B_CONTENDERS x 3
     uint32_t *cont_w = (uint32_t *)(cont_pos + 3 * B_CONTENDERS);    // B_CONTENDERS x 2
     int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
@@ -405,11 +417,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const int n = R.n, cps = R.cps;
         const double L = R.L;
         const double *P = A.pos64 + R.off;
-        for (int i = tid; i < n; i += NTH) {
-            px[i] = (float)P[i];
-            py[i] = (float)P[(size_t)n + i];
-            pz[i] = (float)P[2 * (size_t)n + i];
-        }
+        const double inv_step = 0x1p21 / L;
+        for (int i = tid; i < n; i += NTH) pq[i] = quant3(P[i], P[(size_t)n + i], P[2 * (size_t)n + i], inv_step);
         const int ncell = cps ? cps * cps * cps : 0;
         for (int c = tid; c < ncell; c += NTH) cell_n[c] = 0;
         if (tid < 3) s_ap[tid] = P[(size_t)tid * n + R.active];
@@ -429,7 +438,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     if (cell_n[c] > B_CELL_CAP) R.err = 1;
         }
         const float Lf = (float)L;
-        const float E = Lf * 0x1p-21f;
+        const float step = (float)(L * 0x1p-21);
+        const float E = Lf * 0x1p-20f;
         const double rc2 = R.cutoff * R.cutoff;
         const float rc2_hi = (float)rc2 * (1.f + 0x1p-22f);
         const int nf = A.fac.n_factors;
@@ -487,7 +497,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             bsync();
             const int M = s_M;
             const double h = s_h;
-            const float apf[3] = {(float)ap[0], (float)ap[1], (float)ap[2]};
+            const uint64_t aw = quant3(ap[0], ap[1], ap[2], inv_step);
+            const int qa0 = (int)(aw & QMASK), qa1 = (int)((aw >> 21) & QMASK), qa2 = (int)((aw >> 42) & QMASK);
             const float hf = (float)h * (1.f + 0x1p-22f) + 0x1p-60f;
             BKey best = {INFINITY, 3, 0x7ffffff, 0};   // j < 2^27: the reduction packs (rank, j, f)
             double bestpos[3] = {0.0, 0.0, 0.0};
@@ -502,7 +513,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     const double qx = P[j], qy = P[(size_t)n + j], qz = P[2 * (size_t)n + j];
                     int slot0 = -1, slot1 = -1;
                     Geo g;
-                    const bool amb = cand_geometry(px[j] - apf[0], py[j] - apf[1], pz[j] - apf[2], d, Lf, E, g);
+                    const bool amb = cand_geometry(qsep(pq[j], 0, qa0, step), qsep(pq[j], 1, qa1, step), qsep(pq[j], 2, qa2, step), d, Lf, E, g);
                     uint32_t w[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, w);
 #pragma unroll
@@ -540,7 +551,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     if (idx < M) {
                         j = candidate_unit(idx, cps, a, s_pref, s_nearby, cell_l);
                         Geo g;
-                        const bool amb = cand_geometry(px[j] - apf[0], py[j] - apf[1], pz[j] - apf[2], d, Lf, E, g);
+                        const bool amb = cand_geometry(qsep(pq[j], 0, qa0, step), qsep(pq[j], 1, qa1, step), qsep(pq[j], 2, qa2, step), d, Lf, E, g);
                         if (amb) {
                             surv = true;
                         } else if (g.perp2_lo < rc2_hi) {
@@ -679,10 +690,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     ev.factor = 0;
                     ev.displacement = best.d;
                     Pw[(size_t)d * n + a] = x;
-                    float xf = (float)x;
-                    if (d == 0) px[a] = xf;
-                    else if (d == 1) py[a] = xf;
-                    else pz[a] = xf;
+                    pq[a] = quant3(pold[0], pold[1], pold[2], inv_step);
                     R.est[1] += best.d;
                     if (best.rank == 0) {
                         double sdj = bmin_image(s_newpos[d] - ap[d], L) - best.d;
