@@ -181,8 +181,12 @@ __device__ __forceinline__ bool bkey_eq(const BKey &a, const BKey &b) {
     return a.d == b.d && a.rank == b.rank && a.j == b.j && a.f == b.f;
 }
 
+// The leg's event over the workgroup: a wavefront shuffle over (displacement, packed (kind,
+// target, factor)), then LDS across wavefronts (one barrier).  The lane holding its
+// wavefront's winning key (keys are unique) leaves that candidate's FP64 position in
+// newpos[3 * wave]; the returned index is the winning wavefront.
 template <int WPR>
-__device__ void block_min(BKey &b, BKey *sk) {
+__device__ int block_min(BKey &b, const BKey &mine, const double *pos, BKey *sk, double *newpos) {
     // (rank, j, f) packed into one word orders like the triple (rank < 4, j < 2^27, f < 2)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double d = b.d;
@@ -197,15 +201,24 @@ __device__ void block_min(BKey &b, BKey *sk) {
     b.rank = k >> 28;
     b.j = (k >> 1) & 0x7ffffff;
     b.f = k & 1;
+    if (b.rank == 0 && bkey_eq(mine, b)) {
+        newpos[3 * wave] = pos[0];
+        newpos[3 * wave + 1] = pos[1];
+        newpos[3 * wave + 2] = pos[2];
+    }
+    int wb = 0;
     if (WPR > 1) {
         if (lane == 0) sk[wave] = b;
         __syncthreads();
         b = sk[0];
 #pragma unroll
         for (int w = 1; w < WPR; w++)
-            if (bkey_lt(sk[w], b)) b = sk[w];
-        __syncthreads();
+            if (bkey_lt(sk[w], b)) {
+                b = sk[w];
+                wb = w;
+            }
     }
+    return wb;
 }
 
 __device__ __forceinline__ void bsync() { __syncthreads(); }
@@ -245,7 +258,7 @@ __device__ __forceinline__ float dU_lower(uint32_t a, float inv_beta_lo) {
 // factors do not diverge; sum(v ** 2 for v in others) is also the maximum potential's
 // squared distance (the d component set to zero adds an exact 0).  urc = k / rc^p.
 #ifndef ECMC_BATCHED_WAVES_PER_SIMD
-#define ECMC_BATCHED_WAVES_PER_SIMD 1   // minimum waves per SIMD the register allocation targets
+#define ECMC_BATCHED_WAVES_PER_SIMD 2   // waves per SIMD the register allocation must allow (VGPRs <= 256)
 #endif
 __device__ __forceinline__ double exact_ipt(double s0, double s1, double s2, int d, bool six, double prefactor,
                                             double rc2, double urc, double dU) {
@@ -364,7 +377,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
     __shared__ int s_item, s_chunk, s_nc, s_overflow, s_nearby[27], s_pref[28], s_M;
-    __shared__ double s_ap[3], s_newpos[3], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
+    __shared__ double s_ap[3], s_newpos[3 * B_MAX_WPR], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
     __shared__ ReplicaDev R;
     __shared__ unsigned long long s_stats[4];
 
@@ -451,6 +464,49 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const float kf0 = (float)pk0, kf1 = (float)pk1;
         const int pf0 = (int)pw0, pf1 = (int)pw1;
         const float inv_beta_lo = (float)(1.0 / A.beta) * (1.f - 0x1p-22f);
+        // The next leg's 27 nearby cells with an inclusive scan of their counts, its horizon,
+        // and cleared contender counters: wavefront 0, before the first leg and right after
+        // each commit (so that no separate barrier precedes pass 1)
+        auto prepare_leg = [&]() {
+            const double pa[3] = {s_ap[0], s_ap[1], s_ap[2]};
+            const int dn_ = R.direction;
+            int pc[3] = {0, 0, 0};
+            if (cps) {
+#pragma unroll
+                for (int q = 0; q < 3; q++) pc[q] = bcell1(pa[q], R.side, cps);
+                int cnt = 0;
+                if (lane < 27) {
+                    int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
+                    int cx = (pc[0] + dx + cps) % cps, cy = (pc[1] + dy + cps) % cps, cz = (pc[2] + dz + cps) % cps;
+                    int c = cx + cps * (cy + cps * cz);
+                    s_nearby[lane] = c;
+                    cnt = min(cell_n[c], B_CELL_CAP);
+                }
+                int incl = cnt;
+#pragma unroll
+                for (int off = 1; off < 32; off <<= 1) {
+                    const int o = __shfl_up(incl, off, 64);
+                    if (lane >= off) incl += o;
+                }
+                if (lane < 27) s_pref[lane + 1] = incl;
+                if (lane == 0) s_pref[0] = 0;
+                if (lane == 26) s_M = incl;
+            }
+            if (lane == 0) {
+                double hh = R.chain_left, cb = INFINITY;
+                if (cps) {
+                    cb = s_bounds[pc[dn_] + 1] - pa[dn_];
+                    if (cb < hh) hh = cb;
+                } else {
+                    s_M = n - 1;
+                }
+                s_h = hh;
+                s_cb = cb;
+                s_nc = 0;
+                s_overflow = 0;
+            }
+        };
+        if (wave == 0) prepare_leg();
         bsync();
         for (int leg_i = 0; leg_i < legs_now && !R.err; leg_i++) {
             const int a = R.active, d = R.direction;
@@ -460,41 +516,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             if (cps) {
 #pragma unroll
                 for (int q = 0; q < 3; q++) ac[q] = bcell1(ap[q], R.side, cps);
-                if (wave == 0) {
-                    // the 27 nearby cells and an inclusive scan of their counts across lanes
-                    int cnt = 0;
-                    if (lane < 27) {
-                        int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
-                        int cx = (ac[0] + dx + cps) % cps, cy = (ac[1] + dy + cps) % cps, cz = (ac[2] + dz + cps) % cps;
-                        int c = cx + cps * (cy + cps * cz);
-                        s_nearby[lane] = c;
-                        cnt = min(cell_n[c], B_CELL_CAP);
-                    }
-                    int incl = cnt;
-#pragma unroll
-                    for (int off = 1; off < 32; off <<= 1) {
-                        const int o = __shfl_up(incl, off, 64);
-                        if (lane >= off) incl += o;
-                    }
-                    if (lane < 27) s_pref[lane + 1] = incl;
-                    if (lane == 0) s_pref[0] = 0;
-                    if (lane == 26) s_M = incl;
-                }
             }
-            if (tid == 0) {
-                double h = R.chain_left, cb = INFINITY;
-                if (cps) {
-                    cb = s_bounds[ac[d] + 1] - ap[d];
-                    if (cb < h) h = cb;
-                } else {
-                    s_M = n - 1;
-                }
-                s_h = h;
-                s_cb = cb;
-                s_nc = 0;
-                s_overflow = 0;
-            }
-            bsync();
             const int M = s_M;
             const double h = s_h;
             const uint64_t aw = quant3(ap[0], ap[1], ap[2], inv_step);
@@ -608,19 +630,9 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     bestpos[0] = pj[0]; bestpos[1] = pj[1]; bestpos[2] = pj[2];
                 }
             }
-            BKey mine = best;
-            if (tid == 0) {
-                BKey c1k = {s_cb, 1, 0, 0};
-                if (bkey_lt(c1k, best)) best = c1k;
-                BKey c2k = {R.chain_left, 2, 0, 0};
-                if (bkey_lt(c2k, best)) best = c2k;
-            }
-            block_min<WPR>(best, sk);
-            if (best.rank == 0 && bkey_eq(mine, best)) {
-                s_newpos[0] = bestpos[0]; s_newpos[1] = bestpos[1]; s_newpos[2] = bestpos[2];
-            }
             if (A.mode == 2) {
-                // audit: a candidate whose exact displacement is <= h must be a contender
+                // audit: a candidate whose exact displacement is <= h must be a contender (before
+                // the selection's barrier: after it, wavefront 0 already prepares the next leg)
                 for (int idx = tid; idx < M * nf; idx += NTH) {
                     const int f = idx % nf;
                     const int j = candidate_unit(idx / nf, cps, a, s_pref, s_nearby, cell_l);
@@ -638,13 +650,20 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     }
                 }
             }
+            BKey mine = best;
+            if (tid == 0) {
+                BKey c1k = {s_cb, 1, 0, 0};
+                if (bkey_lt(c1k, best)) best = c1k;
+                BKey c2k = {R.chain_left, 2, 0, 0};
+                if (bkey_lt(c2k, best)) best = c2k;
+            }
+            const int wbest = block_min<WPR>(best, mine, bestpos, sk, s_newpos);
             if (tid == 0) {
                 s_stats[0] += (unsigned long long)M;
                 s_stats[2] += (unsigned long long)min(s_nc, B_CONTENDERS);
                 s_stats[3] += (unsigned long long)nwork;
             }
             if (lane == 0 && geo_pass) atomicAdd(&s_stats[1], (unsigned long long)geo_pass);
-            bsync();
             // ---- commit (wavefront 0; the winner's position is in s_newpos): the values below
             // are uniform across the wavefront, a ballot finds the new active unit's slot in its
             // cell list, lane 0 writes
@@ -664,7 +683,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 const double pold[3] = {np[0], np[1], np[2]};
                 if (best.rank == 0) {
                     newa = best.j;
-                    np[0] = s_newpos[0]; np[1] = s_newpos[1]; np[2] = s_newpos[2];
+                    np[0] = s_newpos[3 * wbest]; np[1] = s_newpos[3 * wbest + 1]; np[2] = s_newpos[3 * wbest + 2];
                 } else if (best.rank == 2) {
                     uint32_t w[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), 0xFFFFFFFFu, 1u, (uint32_t)r, A.seed_lo, w);
@@ -693,7 +712,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     pq[a] = quant3(pold[0], pold[1], pold[2], inv_step);
                     R.est[1] += best.d;
                     if (best.rank == 0) {
-                        double sdj = bmin_image(s_newpos[d] - ap[d], L) - best.d;
+                        double sdj = bmin_image(np[d] - ap[d], L) - best.d;
                         R.est[0] += sdj;
                         R.est[2] += 1.0;
                         R.est[4] += 1.0;
@@ -729,6 +748,11 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     if (A.log && leg < (uint64_t)A.log_legs) A.log[(size_t)r * A.log_legs + leg] = ev;
                     R.leg = leg + 1;
                 }
+                // lane 0's LDS writes above precede the other lanes' reads below (one wavefront,
+                // LDS operations in order; the fence keeps the compiler from reordering them)
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                prepare_leg();
             }
             bsync();
         }
