@@ -445,7 +445,7 @@ static int pull_replicas(const ecmc_batch *b, std::vector<ReplicaDev> &rep) {
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
     for (int r = 0; r < b->n_rep; r++)
         if (rep[r].err == 3) return ecmc_fail(ECMC_ERR_STATE, "a replica's chunk was held by two workgroups");
-        else if (rep[r].err) return ecmc_fail(ECMC_ERR_STATE, "a cell list overflowed (more than 64 units in one cell)");
+        else if (rep[r].err) return ecmc_fail(ECMC_ERR_STATE, "a unit was missing from its cell's occupant list");
 
     return ECMC_OK;
 }

@@ -9,10 +9,11 @@
 namespace ecmc {
 
 constexpr int B_MAX_WPR = 8;
-constexpr int B_CELL_CAP = 64;       // occupants per cell (lattice starts put up to 64 in one)
 constexpr int B_CONTENDERS = 48;     // contenders per leg (about 7 typical; beyond, all candidates go exact)
 constexpr int B_QUEUE = 128;         // per-wavefront queue of candidates past the geometric test
 constexpr int B_MAX_CPS = 8;         // cells per side handled in LDS (512 cells)
+// the chunk start counts units per cell in the (then idle) contender and queue area
+static_assert(36 * B_CONTENDERS + 4 * B_QUEUE >= 4 * B_MAX_CPS * B_MAX_CPS * B_MAX_CPS, "cell counts do not fit");
 
 struct ReplicaDev {
     int32_t n, cps;
@@ -282,20 +283,14 @@ __device__ __forceinline__ double exact_ipt(double s0, double s1, double s2, int
     return live ? res : INFINITY;
 }
 
-// Positions in LDS: 21-bit fixed point per coordinate, q = round(x 2^21 / L) mod 2^21 (x in
-// [0, L)), three to a 64-bit word.  The separation q_j - q_a wrapped into [-2^20, 2^20) is the
-// minimum image exactly in integers; times L 2^-21 it is within L 2^-21 (two roundings of
-// L 2^-22) plus L 2^-24 (the float product) of the true minimum-image separation, unless that
-// is within the same distance of +-L/2.  The slack E = L 2^-20 covers both.
-constexpr uint32_t QMASK = (1u << 21) - 1u;
-__device__ __forceinline__ uint64_t quant3(double x, double y, double z, double inv_step) {
-    const uint64_t qx = (uint64_t)(x * inv_step + 0.5) & QMASK, qy = (uint64_t)(y * inv_step + 0.5) & QMASK,
-                   qz = (uint64_t)(z * inv_step + 0.5) & QMASK;
-    return qx | (qy << 21) | (qz << 42);
-}
-__device__ __forceinline__ float qsep(uint64_t w, int c, int qa, float step) {
-    const int dq = (int)(((uint32_t)(w >> (21 * c)) - (uint32_t)qa + (1u << 20)) & QMASK) - (1 << 20);
-    return (float)dq * step;
+// Positions in LDS: 16-bit fixed point per coordinate, q = round(x 2^16 / L) mod 2^16 (x in
+// [0, L)), one uint16 array per axis.  The separation q_j - q_a wrapped into [-2^15, 2^15) is
+// the minimum image exactly in integers; times L 2^-16 it is within L 2^-16 (two roundings of
+// L 2^-17) plus L 2^-25 (the float product) of the true minimum-image separation, unless that
+// is within the same distance of +-L/2.  The slack E = L 2^-15 covers both.
+__device__ __forceinline__ uint16_t quant1(double x, double inv_step) { return (uint16_t)(uint32_t)(x * inv_step + 0.5); }
+__device__ __forceinline__ float qsep(uint32_t qj, uint32_t qa, float step) {
+    return (float)((int)((qj - qa + 0x8000u) & 0xFFFFu) - 0x8000) * step;
 }
 
 // FP32 geometry of a candidate from its minimum-image separation: bounds with the position
@@ -327,9 +322,62 @@ __device__ __forceinline__ bool cand_geometry(float dx, float dy, float dz, int 
     return ambiguous;
 }
 
+// Occupant lists sorted by cell: move unit x, at slot p of cell X's segment, to cell Y != X
+// (one wavefront, uniform arguments).  Going up (Y > X), x leaves through X's last slot (X's
+// last unit fills p), every non-empty cell between passes its last unit to the slot before its
+// first, and x becomes Y's first; going down is the mirror image.  Starts of the cells
+// between, and of the far end, shift by one.  Each pass reads all its slots before it writes.
+__device__ void csr_move(int16_t *cu, int16_t *cs, int p, int X, int Y, int x, int lane) {
+    if (Y > X) {
+        const int eX = cs[X + 1] - 1;
+        const int vX = cu[eX];
+        if (lane == 0 && p != eX) cu[p] = (int16_t)vX;
+        for (int c0 = X + 1; c0 < Y; c0 += 64) {
+            const int c = c0 + lane;
+            int v = 0, dst = -1;
+            if (c < Y) {
+                const int b = cs[c], e = cs[c + 1];
+                if (e > b) {
+                    v = cu[e - 1];
+                    dst = b - 1;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (dst >= 0) cu[dst] = (int16_t)v;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) cu[cs[Y] - 1] = (int16_t)x;
+        __builtin_amdgcn_wave_barrier();
+        for (int c = X + 1 + lane; c <= Y; c += 64) cs[c] = (int16_t)(cs[c] - 1);
+    } else {
+        const int fX = cs[X];
+        const int vX = cu[fX];
+        if (lane == 0 && p != fX) cu[p] = (int16_t)vX;
+        // downwards, chunks from the top so that a chunk reads its slots before the chunk below
+        // writes into them
+        for (int c0 = X - 1; c0 > Y; c0 -= 64) {
+            const int c = c0 - lane;
+            int v = 0, dst = -1;
+            if (c > Y) {
+                const int b = cs[c], e = cs[c + 1];
+                if (e > b) {
+                    v = cu[b];
+                    dst = e;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (dst >= 0) cu[dst] = (int16_t)v;
+        }
+        __builtin_amdgcn_wave_barrier();
+        if (lane == 0) cu[cs[Y + 1]] = (int16_t)x;
+        __builtin_amdgcn_wave_barrier();
+        for (int c = Y + 1 + lane; c <= X; c += 64) cs[c] = (int16_t)(cs[c] + 1);
+    }
+}
+
 // candidate idx -> unit (binary search over the 27 prefix sums, or all units but `a`)
-__device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int *pref, const int *nearby,
-                                              const int16_t *cell_l) {
+__device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int *pref, const int *start,
+                                              const int16_t *cu) {
     if (!cps) return idx < a ? idx : idx + 1;
     int lo = 0, hi = 27;  // pref[lo] <= idx < pref[hi]
     while (hi - lo > 1) {
@@ -337,13 +385,13 @@ __device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int
         if (pref[mid] <= idx) lo = mid;
         else hi = mid;
     }
-    return cell_l[(size_t)nearby[lo] * B_CELL_CAP + (idx - pref[lo])];
+    return cu[start[lo] + (idx - pref[lo])];
 }
 
 // LDS bytes of the dynamic part for a class (fixed-point positions, cell lists, contenders and
 // the per-wavefront candidate queues); the host sizes launches with the same function.
 __host__ __device__ inline size_t batched_lds_bytes(int nmax, int cellmax, int wpr) {
-    size_t b = 8ull * nmax + 4ull * cellmax + 2ull * cellmax * B_CELL_CAP;
+    size_t b = 8ull * nmax + 2ull * (cellmax + 1);
     b = (b + 7) & ~7ull;
     b += (24ull + 8ull + 4ull) * B_CONTENDERS;     // positions, random words, identifiers
     b += 4ull * B_QUEUE * wpr;
@@ -367,10 +415,10 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NTH = 64 * WPR;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint64_t *pq = (uint64_t *)smem;
-    int32_t *cell_n = (int32_t *)(pq + A.nmax);
-    int16_t *cell_l = (int16_t *)(cell_n + A.cellmax);
-    size_t off = (8ull * A.nmax + 4ull * A.cellmax + 2ull * A.cellmax * B_CELL_CAP + 7) & ~7ull;
+    uint16_t *qx = (uint16_t *)smem, *qy = qx + A.nmax, *qz = qy + A.nmax;
+    int16_t *cu = (int16_t *)(qz + A.nmax);    // units sorted by cell, the active unit left out
+    int16_t *cs = cu + A.nmax;                 // cell c holds cu[cs[c] .. cs[c + 1])
+    size_t off = (8ull * A.nmax + 2ull * (A.cellmax + 1) + 7) & ~7ull;
     double *cont_pos = (double *)(smem + off);                       // B_CONTENDERS x 3
     uint32_t *cont_w = (uint32_t *)(cont_pos + 3 * B_CONTENDERS);    // B_CONTENDERS x 2
     int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
@@ -430,29 +478,61 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const int n = R.n, cps = R.cps;
         const double L = R.L;
         const double *P = A.pos64 + R.off;
-        const double inv_step = 0x1p21 / L;
-        for (int i = tid; i < n; i += NTH) pq[i] = quant3(P[i], P[(size_t)n + i], P[2 * (size_t)n + i], inv_step);
+        const double inv_step = 0x1p16 / L;
+        for (int i = tid; i < n; i += NTH) {
+            qx[i] = quant1(P[i], inv_step);
+            qy[i] = quant1(P[(size_t)n + i], inv_step);
+            qz[i] = quant1(P[2 * (size_t)n + i], inv_step);
+        }
         const int ncell = cps ? cps * cps * cps : 0;
-        for (int c = tid; c < ncell; c += NTH) cell_n[c] = 0;
+        // counting sort of the units by cell; the counts and cursors live in the contender
+        // area, which is idle here (4 * 512 bytes fit in it)
+        int32_t *ccount = (int32_t *)cont_pos;
+        for (int c = tid; c < ncell; c += NTH) ccount[c] = 0;
         if (tid < 3) s_ap[tid] = P[(size_t)tid * n + R.active];
         if (cps && tid <= cps) s_bounds[tid] = A.bounds[R.bound_off + tid];
         bsync();
         if (cps) {
-            for (int i = tid; i < n; i += NTH) {
-                if (i == R.active) continue;
-                int c = bcell1(P[i], R.side, cps) + cps * (bcell1(P[(size_t)n + i], R.side, cps) +
-                                                           cps * bcell1(P[2 * (size_t)n + i], R.side, cps));
-                int slot = atomicAdd(&cell_n[c], 1);
-                if (slot < B_CELL_CAP) cell_l[(size_t)c * B_CELL_CAP + slot] = (int16_t)i;
+            auto unit_cell = [&](int i) {
+                return bcell1(P[i], R.side, cps) +
+                       cps * (bcell1(P[(size_t)n + i], R.side, cps) + cps * bcell1(P[2 * (size_t)n + i], R.side, cps));
+            };
+            for (int i = tid; i < n; i += NTH)
+                if (i != R.active) atomicAdd(&ccount[unit_cell(i)], 1);
+            bsync();
+            if (wave == 0) {
+                // exclusive scan over the cells: each lane a run of up to 8, then across lanes
+                const int per = (ncell + 63) / 64;
+                int sum = 0;
+                for (int k = 0; k < per; k++) {
+                    const int c = lane * per + k;
+                    if (c < ncell) sum += ccount[c];
+                }
+                int incl = sum;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const int v = __shfl_up(incl, o, 64);
+                    if (lane >= o) incl += v;
+                }
+                int run = incl - sum;
+                for (int k = 0; k < per; k++) {
+                    const int c = lane * per + k;
+                    if (c < ncell) {
+                        const int t = ccount[c];
+                        cs[c] = (int16_t)run;
+                        ccount[c] = run;
+                        run += t;
+                    }
+                }
+                if (lane == 63) cs[ncell] = (int16_t)incl;
             }
             bsync();
-            if (tid == 0)
-                for (int c = 0; c < ncell; c++)
-                    if (cell_n[c] > B_CELL_CAP) R.err = 1;
+            for (int i = tid; i < n; i += NTH)
+                if (i != R.active) cu[atomicAdd(&ccount[unit_cell(i)], 1)] = (int16_t)i;
         }
         const float Lf = (float)L;
-        const float step = (float)(L * 0x1p-21);
-        const float E = Lf * 0x1p-20f;
+        const float step = (float)(L * 0x1p-16);
+        const float E = Lf * 0x1p-15f;
         const double rc2 = R.cutoff * R.cutoff;
         const float rc2_hi = (float)rc2 * (1.f + 0x1p-22f);
         const int nf = A.fac.n_factors;
@@ -479,8 +559,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
                     int cx = (pc[0] + dx + cps) % cps, cy = (pc[1] + dy + cps) % cps, cz = (pc[2] + dz + cps) % cps;
                     int c = cx + cps * (cy + cps * cz);
-                    s_nearby[lane] = c;
-                    cnt = min(cell_n[c], B_CELL_CAP);
+                    s_nearby[lane] = cs[c];
+                    cnt = cs[c + 1] - cs[c];
                 }
                 int incl = cnt;
 #pragma unroll
@@ -519,8 +599,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             }
             const int M = s_M;
             const double h = s_h;
-            const uint64_t aw = quant3(ap[0], ap[1], ap[2], inv_step);
-            const int qa0 = (int)(aw & QMASK), qa1 = (int)((aw >> 21) & QMASK), qa2 = (int)((aw >> 42) & QMASK);
+            const uint32_t qa0 = quant1(ap[0], inv_step), qa1 = quant1(ap[1], inv_step), qa2 = quant1(ap[2], inv_step);
             const float hf = (float)h * (1.f + 0x1p-22f) + 0x1p-60f;
             BKey best = {INFINITY, 3, 0x7ffffff, 0};   // j < 2^27: the reduction packs (rank, j, f)
             double bestpos[3] = {0.0, 0.0, 0.0};
@@ -532,10 +611,10 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     const int j = queue[lane];
                     // the FP64 position, needed if a factor turns out a contender: its memory
                     // latency overlaps the bound computation below
-                    const double qx = P[j], qy = P[(size_t)n + j], qz = P[2 * (size_t)n + j];
+                    const double fx = P[j], fy = P[(size_t)n + j], fz = P[2 * (size_t)n + j];
                     int slot0 = -1, slot1 = -1;
                     Geo g;
-                    const bool amb = cand_geometry(qsep(pq[j], 0, qa0, step), qsep(pq[j], 1, qa1, step), qsep(pq[j], 2, qa2, step), d, Lf, E, g);
+                    const bool amb = cand_geometry(qsep(qx[j], qa0, step), qsep(qy[j], qa1, step), qsep(qz[j], qa2, step), d, Lf, E, g);
                     uint32_t w[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, w);
 #pragma unroll
@@ -561,8 +640,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                             else slot1 = slot;
                         }
                     }
-                    if (slot0 >= 0) { cont_pos[3 * slot0] = qx; cont_pos[3 * slot0 + 1] = qy; cont_pos[3 * slot0 + 2] = qz; }
-                    if (slot1 >= 0) { cont_pos[3 * slot1] = qx; cont_pos[3 * slot1 + 1] = qy; cont_pos[3 * slot1 + 2] = qz; }
+                    if (slot0 >= 0) { cont_pos[3 * slot0] = fx; cont_pos[3 * slot0 + 1] = fy; cont_pos[3 * slot0 + 2] = fz; }
+                    if (slot1 >= 0) { cont_pos[3 * slot1] = fx; cont_pos[3 * slot1 + 1] = fy; cont_pos[3 * slot1 + 2] = fz; }
                 };
                 // ---- pass 1a: geometric test of every candidate, survivors queued
                 int qn = 0;
@@ -571,9 +650,9 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     bool surv = false;
                     int j = 0;
                     if (idx < M) {
-                        j = candidate_unit(idx, cps, a, s_pref, s_nearby, cell_l);
+                        j = candidate_unit(idx, cps, a, s_pref, s_nearby, cu);
                         Geo g;
-                        const bool amb = cand_geometry(qsep(pq[j], 0, qa0, step), qsep(pq[j], 1, qa1, step), qsep(pq[j], 2, qa2, step), d, Lf, E, g);
+                        const bool amb = cand_geometry(qsep(qx[j], qa0, step), qsep(qy[j], qa1, step), qsep(qz[j], qa2, step), d, Lf, E, g);
                         if (amb) {
                             surv = true;
                         } else if (g.perp2_lo < rc2_hi) {
@@ -606,7 +685,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 uint32_t wa, wb;
                 if (all_exact) {
                     f = w % nf;
-                    j = candidate_unit(w / nf, cps, a, s_pref, s_nearby, cell_l);
+                    j = candidate_unit(w / nf, cps, a, s_pref, s_nearby, cu);
                     pj[0] = P[j]; pj[1] = P[(size_t)n + j]; pj[2] = P[2 * (size_t)n + j];
                     uint32_t ww[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
@@ -635,7 +714,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 // the selection's barrier: after it, wavefront 0 already prepares the next leg)
                 for (int idx = tid; idx < M * nf; idx += NTH) {
                     const int f = idx % nf;
-                    const int j = candidate_unit(idx / nf, cps, a, s_pref, s_nearby, cell_l);
+                    const int j = candidate_unit(idx / nf, cps, a, s_pref, s_nearby, cu);
                     uint32_t ww[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
                     const double dU = -cr_log(1.0 - unit53(pick(f, ww[0], ww[2]), pick(f, ww[1], ww[3]))) / A.beta;
@@ -692,14 +771,16 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                         np[0] = Pw[newa]; np[1] = Pw[(size_t)n + newa]; np[2] = Pw[2 * (size_t)n + newa];
                     }
                 }
-                int ca = 0, cn = 0, q = -1, m = 0;
+                int ca = 0, cn = 0, q = -1;
                 if (newa != a && cps) {
                     ca = bcell1(pold[0], R.side, cps) + cps * (bcell1(pold[1], R.side, cps) + cps * bcell1(pold[2], R.side, cps));
                     cn = bcell1(np[0], R.side, cps) + cps * (bcell1(np[1], R.side, cps) + cps * bcell1(np[2], R.side, cps));
-                    m = cell_n[cn];
-                    const bool hit = lane < m && cell_l[(size_t)cn * B_CELL_CAP + lane] == newa;
-                    const unsigned long long mask = __ballot(hit);
-                    q = mask ? __ffsll((long long)mask) - 1 : -1;
+                    // the new active unit's slot in its cell's segment
+                    const int b1 = cs[cn + 1];
+                    for (int base = cs[cn]; base < b1 && q < 0; base += 64) {
+                        const unsigned long long mask = __ballot(base + lane < b1 && cu[base + lane] == newa);
+                        if (mask) q = base + __ffsll((long long)mask) - 1;
+                    }
                 }
                 if (lane == 0) {
                     ecmc_batch_event ev;
@@ -709,7 +790,9 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     ev.factor = 0;
                     ev.displacement = best.d;
                     Pw[(size_t)d * n + a] = x;
-                    pq[a] = quant3(pold[0], pold[1], pold[2], inv_step);
+                    qx[a] = quant1(pold[0], inv_step);
+                    qy[a] = quant1(pold[1], inv_step);
+                    qz[a] = quant1(pold[2], inv_step);
                     R.est[1] += best.d;
                     if (best.rank == 0) {
                         double sdj = bmin_image(np[d] - ap[d], L) - best.d;
@@ -730,18 +813,10 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     }
                     ev.x = x;
                     if (newa != a && cps) {
-                        // the old active unit joins the cell it ended in, the new one leaves its
-                        // cell (its slot was found before the append, so q < m)
-                        int na = cell_n[ca];
-                        if (na >= B_CELL_CAP) R.err = 1;
-                        else { cell_l[(size_t)ca * B_CELL_CAP + na] = (int16_t)a; cell_n[ca] = na + 1; }
-                        if (q < 0) {
-                            R.err = 2;
-                        } else {
-                            const int last = cell_n[cn] - 1;
-                            cell_l[(size_t)cn * B_CELL_CAP + q] = cell_l[(size_t)cn * B_CELL_CAP + last];
-                            cell_n[cn] = last;
-                        }
+                        // the old active unit takes the new one's slot (and is moved to the cell
+                        // it ended in below)
+                        if (q < 0) R.err = 2;
+                        else cu[q] = (int16_t)a;
                     }
                     s_ap[0] = np[0]; s_ap[1] = np[1]; s_ap[2] = np[2];
                     R.active = newa;
@@ -752,6 +827,11 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 // LDS operations in order; the fence keeps the compiler from reordering them)
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
+                if (newa != a && cps && q >= 0 && ca != cn) {
+                    csr_move(cu, cs, q, cn, ca, a, lane);
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                }
                 prepare_leg();
             }
             bsync();
