@@ -257,6 +257,13 @@ int ecmc_energy_backward(const ecmc_energy_term *term, int32_t n, const double *
                          double *d_grad_charge, void *d_work, void *stream);
 
 /* ---- math and random stream (device) ----------------------------------------------------- */
+/* Norms of the shortest separations of all n (n - 1) / 2 pairs i < j of n positions (n x dim
+ * row-major, device memory), ordered (0,1), (0,2), ..., (1,2), ...: what
+ * SeparationOutputHandler.write prints for point masses
+ * (input_output_handler/output_handler/separation_output_handler.py:81-95, with
+ * setting/hypercubic_setting.py:120-172 and base/vectors.py:29-43). n <= 65535. */
+int ecmc_separation_norms(int32_t dim, int64_t n, const double *d_positions, double system_length, double *d_out,
+                          void *stream);
 int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream);
 int ecmc_math_pow(int64_t n, const double *d_x, const double *d_y, double *d_out, void *stream);
 /* Largest error over every float with bit pattern in [bits_lo, bits_hi) of the device's FP32

@@ -94,6 +94,7 @@ def _signatures():
         "ecmc_batch_run_graph": (i32, [vp, i32, i32]),
         "ecmc_batch_last_ms": (i32, [vp, pd]),
         "ecmc_batch_set_max_blocks": (i32, [vp, i32]),
+        "ecmc_separation_norms": (i32, [i32, i64, vp, d, vp, vp]),
         "ecmc_batch_set_class_timing": (i32, [vp, i32]),
         "ecmc_batch_class_ms": (i32, [vp, i32, pd, ctypes.POINTER(i32)]),
         "ecmc_batch_states": (i32, [vp, vp, vp, vp, vp, vp, vp]),

@@ -36,10 +36,7 @@ import re
 import numpy as np
 
 from .chain import DeviceChain, make_desc
-
-
-class ConfigurationError(ValueError):
-    """base.exceptions.ConfigurationError of the reference."""
+from .output import ConfigurationError, make_output_handler
 
 
 def camel(snake):
@@ -126,6 +123,14 @@ class RunConfig:
         if sched not in ("heap_scheduler", "list_scheduler"):
             raise ConfigurationError("unknown scheduler %s" % sched)
         self.scheduler = "heap" if sched == "heap_scheduler" else "list"
+        # output handlers (input_output_handler/input_output_handler.py:43-61): alias -> (class, options)
+        self.output_handlers = {}
+        if "input_output_handler" in m:
+            io = self._section(camel(split_alias(m["input_output_handler"])[0]), {"output_handlers", "input_handler"})
+            for item in split_list(io.get("output_handlers", "")):
+                alias, cls = split_alias(item)
+                opts = dict(self.cp[camel(alias)]) if camel(alias) in self.cp else {}
+                self.output_handlers[alias] = (cls, opts)
         act = split_alias(m.get("activator", "tag_activator"))[1]
         if act != "tag_activator":
             raise ConfigurationError("only the tag activator is supported (got %s)" % act)
@@ -239,6 +244,7 @@ class RunConfig:
             t["kind"] = "sampling"
             t["sampling_interval"] = float(h["sampling_interval"])
             t["first_event_time_zero"] = h.get("first_event_time_zero", "false").lower() in ("1", "yes", "true", "on")
+            t["output_handler"] = h.get("output_handler")
         elif hcls == "single_independent_active_periodic_direction_end_of_chain_event_handler":
             t["kind"] = "end_of_chain"
             t["chain_time"] = float(h["chain_time"])
@@ -283,18 +289,39 @@ class SingleProcessMediator:
         pos = np.asarray(positions, dtype=np.float64)
         self.chain = DeviceChain(config.desc(pos.shape[0], seed), pos, charges)
         self.sampling = [i for i, t in enumerate(config.taggers) if t["kind"] == "sampling"]
+        self.output_handlers = {alias: make_output_handler(cls, opts)
+                                for alias, (cls, opts) in config.output_handlers.items()}
+        # sampling tagger -> the output handler its event handler writes to
+        self.writers = {i: self.output_handlers[config.taggers[i]["output_handler"]] for i in self.sampling
+                        if config.taggers[i].get("output_handler") in self.output_handlers}
 
     def run(self, on_sample=None, chunk=1 << 20):
-        """Run to the end-of-run event; returns the number of committed legs."""
+        """Run to the end-of-run event; returns the number of committed legs.  At each sampling
+        event the positions go to the sampling handler's output handler and to ``on_sample``;
+        at the end every output handler's ``post_run`` runs (as the mediator's post_run does)."""
         mask = 0
-        if on_sample is not None:
-            for i in self.sampling:
+        for i in self.sampling:
+            if on_sample is not None or i in self.writers:
                 mask |= 1 << i
         self.chain.set_stop_mask(mask)
         total = 0
         while True:
-            total += self.chain.run(chunk, log=False)
+            if self.writers:
+                # a short log tells which tagger's event stopped the run
+                ev = self.chain.run(min(chunk, 4096), log=True)
+                total += len(ev)
+                last = int(ev["tagger"][-1]) if len(ev) else -1
+            else:
+                total += self.chain.run(chunk, log=False)
+                last = -1
             if self.chain.active()[4]:      # end of run committed
+                for h in self.output_handlers.values():
+                    h.post_run()
                 return total
-            if on_sample is not None and mask:
-                on_sample(self.chain.positions())
+            if mask:
+                pos = None
+                if last in self.writers:
+                    pos = self.chain.positions()
+                    self.writers[last].write(pos, self.config.system_length)
+                if on_sample is not None and (last < 0 or last in self.sampling):
+                    on_sample(pos if pos is not None else self.chain.positions())

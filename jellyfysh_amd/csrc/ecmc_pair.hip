@@ -223,6 +223,42 @@ extern "C" int ecmc_fp32_math_error(int32_t fn, uint32_t bits_lo, uint32_t bits_
     return hipGetLastError() == hipSuccess ? ECMC_OK : ecmc_fail(ECMC_ERR_HIP, "math check launch failed");
 }
 
+// Norms of the shortest separations of every pair i < j, in the order SeparationOutputHandler
+// writes them for point masses (output_handler/separation_output_handler.py:81-95): separation
+// = target - reference with each entry ((s + L/2) % L) - L/2 (hypercubic_setting.py:120-172),
+// norm = (0 + sum of squares) ** 0.5 (vectors.py:29-43), ** 0.5 being the correctly rounded
+// square root.  Row i is blockIdx.y; pair (i, j) lands at i n - i (i + 1) / 2 + j - i - 1.
+template <int D>
+__global__ void separation_norm_kernel(int64_t n, const double *pos, double L, double *out) {
+    const int64_t i = blockIdx.y;
+    const int64_t j = i + 1 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double half = L / 2.0;
+    double acc = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const double sd = py_mod((pos[D * j + d] - pos[D * i + d]) + half, L) - half;
+        acc = acc + sd * sd;
+    }
+    out[i * n - i * (i + 1) / 2 + (j - i - 1)] = cr_pow(acc, 0.5);
+}
+
+extern "C" int ecmc_separation_norms(int32_t dim, int64_t n, const double *d_positions, double system_length,
+                                     double *d_out, void *stream) {
+    if (!(dim == 2 || dim == 3) || n < 0 || n > 65535 || !(system_length > 0.0))
+        return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (n < 2) return ECMC_OK;
+    if (!d_positions || !d_out) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    const dim3 grid((unsigned)((n - 1 + PT - 1) / PT), (unsigned)(n - 1));
+    if (dim == 2)
+        hipLaunchKernelGGL(separation_norm_kernel<2>, grid, dim3(PT), 0, (hipStream_t)stream, n, d_positions,
+                           system_length, d_out);
+    else
+        hipLaunchKernelGGL(separation_norm_kernel<3>, grid, dim3(PT), 0, (hipStream_t)stream, n, d_positions,
+                           system_length, d_out);
+    return hipGetLastError() == hipSuccess ? ECMC_OK : ecmc_fail(ECMC_ERR_HIP, "separation kernel launch failed");
+}
+
 extern "C" int ecmc_math_log(int64_t n, const double *d_x, double *d_out, void *stream) {
     if (n < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
     if (n == 0) return ECMC_OK;

@@ -67,6 +67,7 @@ double ora_time_sub(double q1, double r1, double q2, double r2);
 double ora_pow(double x, double y, int math_mode);
 double ora_log(double x, int math_mode);
 void ora_log_n(int64_t n, const double *x, double *out, int math_mode);
+void ora_separation_norms(int dim, int64_t n, const double *pos, double L, double *out, int math_mode);
 void ora_pow_n(int64_t n, const double *x, const double *y, double *out, int math_mode);
 
 /* ---- potentials (scalar, reference argument conventions) --------------------------- */

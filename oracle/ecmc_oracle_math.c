@@ -186,6 +186,20 @@ static double v_norm_sq(int dim, const double *s) {
     return acc;
 }
 static double v_norm(int dim, const double *s, int mm) { return ora_pow(v_norm_sq(dim, s), 0.5, mm); }
+
+/* SeparationOutputHandler.write for point masses (separation_output_handler.py:81-95): for
+ * i < j in order, norm(separation_vector(x_i, x_j)) with the entries corrected as in
+ * hypercubic_setting.py:156-172 and the norm of vectors.py:29-43. */
+void ora_separation_norms(int dim, int64_t n, const double *pos, double L, double *out, int mm) {
+    const double half = L / 2.0;
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; i++)
+        for (int64_t j = i + 1; j < n; j++) {
+            double s[3];
+            for (int d = 0; d < dim; d++) s[d] = ora_py_mod((pos[dim * j + d] - pos[dim * i + d]) + half, L) - half;
+            out[k++] = v_norm(dim, s, mm);
+        }
+}
 static double v_others_sq(int dim, const double *s, int dir, int mm) {
     /* sum([value ** 2 for index, value in enumerate(old_vector) if index != direction]) */
     double acc = 0.0;
