@@ -97,7 +97,9 @@ typedef struct {
 } ecmc_tagger_desc;
 
 /* A run configuration: HypercubicSetting + CuboidPeriodicCells + SingleActiveCellOccupancy
- * + scheduler + taggers, and the seed of the random stream (random.seed(seed)). */
+ * + scheduler + taggers, and the seed of the random stream (random.seed(seed)).  rng_skip
+ * words of that stream are drawn before the run starts, as a random input handler draws the
+ * initial positions from it (random_input_handler.py:79-92: two words per coordinate). */
 typedef struct {
     int32_t dim, n;
     double system_length, beta;
@@ -105,7 +107,7 @@ typedef struct {
     int32_t neighbor_layers;
     int32_t max_occupants;   /* <= 0: unbounded */
     int32_t scheduler;
-    int32_t reserved;
+    int32_t rng_skip;
     uint64_t seed;
     int32_t n_taggers;
     ecmc_tagger_desc taggers[ECMC_MAX_TAGGERS];

@@ -42,7 +42,7 @@ class TaggerDesc(ctypes.Structure):
 class SystemDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("n", ctypes.c_int32), ("system_length", ctypes.c_double),
                 ("beta", ctypes.c_double), ("cells_per_side", ctypes.c_int32 * 3), ("neighbor_layers", ctypes.c_int32),
-                ("max_occupants", ctypes.c_int32), ("scheduler", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("max_occupants", ctypes.c_int32), ("scheduler", ctypes.c_int32), ("rng_skip", ctypes.c_int32),
                 ("seed", ctypes.c_uint64), ("n_taggers", ctypes.c_int32), ("taggers", TaggerDesc * MAX_TAGGERS)]
 
 

@@ -33,7 +33,7 @@ def potential_code(name):
 
 
 def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, scheduler="heap", neighbor_layers=1,
-              max_occupants=-1):
+              max_occupants=-1, rng_skip=0):
     """Fill an ecmc_system_desc.
 
     Each tagger is a dict with ``tag``, ``kind`` (pair_cells, pair_all, cell_boundary,
@@ -42,9 +42,11 @@ def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, schedu
     constructors), ``bounding_potential``/``bounding_params`` for the bounding handler,
     ``charge`` (bool), ``number_event_handlers``; ``sampling_interval`` and
     ``first_event_time_zero``; ``chain_time``; ``end_of_run_time``;
-    ``initial_direction_of_motion``, ``speed``, ``initial_active_identifier``.
+    ``initial_direction_of_motion``, ``speed``, ``initial_active_identifier``.  ``rng_skip``
+    words of the random stream are drawn before the run (a random input handler's draws).
     """
     d = L.SystemDesc()
+    d.rng_skip = int(rng_skip)
     d.dim, d.n, d.system_length, d.beta = int(dim), int(n), float(system_length), float(beta)
     cps = list(cells_per_side) if np.ndim(cells_per_side) else [int(cells_per_side)]
     for i in range(3):

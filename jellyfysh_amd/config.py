@@ -36,6 +36,7 @@ import re
 import numpy as np
 
 from .chain import DeviceChain, make_desc
+from .input_handlers import AtomRandomNodeCreator, ChargeValues, PdbInputHandler, RandomInputHandler
 from .output import ConfigurationError, make_output_handler
 
 
@@ -125,12 +126,15 @@ class RunConfig:
         self.scheduler = "heap" if sched == "heap_scheduler" else "list"
         # output handlers (input_output_handler/input_output_handler.py:43-61): alias -> (class, options)
         self.output_handlers = {}
+        self.input_handler = None
         if "input_output_handler" in m:
             io = self._section(camel(split_alias(m["input_output_handler"])[0]), {"output_handlers", "input_handler"})
             for item in split_list(io.get("output_handlers", "")):
                 alias, cls = split_alias(item)
                 opts = dict(self.cp[camel(alias)]) if camel(alias) in self.cp else {}
                 self.output_handlers[alias] = (cls, opts)
+            if io.get("input_handler"):
+                self.input_handler = self._input_handler(io["input_handler"])
         act = split_alias(m.get("activator", "tag_activator"))[1]
         if act != "tag_activator":
             raise ConfigurationError("only the tag activator is supported (got %s)" % act)
@@ -163,6 +167,48 @@ class RunConfig:
                         raise ConfigurationError("tag %s of tagger %s is no tagger" % (tag, t["tag"]))
         if any(t["kind"] == "pair_cells" for t in self.taggers) and self.cells_per_side is None:
             raise ConfigurationError("excluded_cells_tagger needs a cell occupancy")
+
+    def _charge_values(self, value):
+        out = []
+        for item in split_list(value or ""):
+            alias, cls = split_alias(item)
+            if cls != "charge_values":
+                raise ConfigurationError("unsupported charge values %s" % cls)
+            c = self._section(camel(alias), {"charge_name", "charge_values"})
+            out.append(ChargeValues([float(v) for v in split_list(c["charge_values"])], c.get("charge_name", alias)))
+        return out
+
+    def _input_handler(self, value):
+        """input_output_handler/input_handler: random placement of point masses or a .pdb file."""
+        alias, cls = split_alias(value)
+        if cls == "random_input_handler":
+            h = self._section(camel(alias), {"random_node_creator", "number_of_root_nodes"})
+            calias, ccls = split_alias(h["random_node_creator"])
+            if ccls != "atom_random_node_creator":
+                raise ConfigurationError("only point masses (atom_random_node_creator) run on the device (got %s)" % ccls)
+            c = self._section(camel(calias), {"charge_values"})
+            return RandomInputHandler(int(h["number_of_root_nodes"]),
+                                      AtomRandomNodeCreator(self._charge_values(c.get("charge_values"))))
+        if cls == "pdb_input_handler":
+            h = self._section(camel(alias), {"filename", "charge_values"})
+            return PdbInputHandler(h["filename"], self._charge_values(h.get("charge_values")), self.dim,
+                                   self.system_length)
+        raise ConfigurationError("unsupported input handler %s" % cls)
+
+    def initial_state(self, seed=0):
+        """Positions, charges (the electric charge, or the first given; None without charges) and the
+        number of random words the input handler drew: random.seed(seed), then the handler reads,
+        then the run continues the same stream."""
+        import random
+        if self.input_handler is None:
+            raise ConfigurationError("the configuration has no input handler; give the positions")
+        if isinstance(self.input_handler, RandomInputHandler):
+            pos, charges, words = self.input_handler.read(random.Random(seed), self.dim, self.system_length)
+        else:
+            pos, charges = self.input_handler.read_point_masses()
+            words = 0
+        q = charges.get("electric_charge", next(iter(charges.values()), None)) if charges else None
+        return pos, q, words
 
     # ---- helpers ---------------------------------------------------------------------------
     def _section(self, name, allowed):
@@ -272,22 +318,28 @@ class RunConfig:
             cp.read_file(f)
         return cls(cp)
 
-    def desc(self, n, seed=0):
+    def desc(self, n, seed=0, rng_skip=0):
         cps = self.cells_per_side if self.cells_per_side is not None else 1
         return make_desc(self.dim, n, self.system_length, self.beta, cps, self.taggers, seed,
                          scheduler=self.scheduler, neighbor_layers=self.neighbor_layers,
-                         max_occupants=self.max_occupants)
+                         max_occupants=self.max_occupants, rng_skip=rng_skip)
 
 
 class SingleProcessMediator:
     """mediator/single_process_mediator.py:84-156 with the whole leg loop on the device."""
 
-    def __init__(self, config, positions, charges=None, seed=0):
+    def __init__(self, config, positions=None, charges=None, seed=0):
         if isinstance(config, str):
             config = RunConfig.from_file(config)
         self.config = config
+        skip = 0
+        if positions is None:
+            # the configuration's input handler (random placement draws from the run's stream)
+            positions, q, skip = config.initial_state(seed)
+            if charges is None:
+                charges = q
         pos = np.asarray(positions, dtype=np.float64)
-        self.chain = DeviceChain(config.desc(pos.shape[0], seed), pos, charges)
+        self.chain = DeviceChain(config.desc(pos.shape[0], seed, rng_skip=skip), pos, charges)
         self.sampling = [i for i, t in enumerate(config.taggers) if t["kind"] == "sampling"]
         self.output_handlers = {alias: make_output_handler(cls, opts)
                                 for alias, (cls, opts) in config.output_handlers.items()}

@@ -949,6 +949,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     *out = nullptr;
     const int dim = desc->dim, n = desc->n;
     if (dim != 2 && dim != 3) return ecmc_fail(ECMC_ERR_CONFIGURATION, "dimension must be 2 or 3");
+    if (desc->rng_skip < 0) return ecmc_fail(ECMC_ERR_CONFIGURATION, "rng_skip must be >= 0");
     if (n < 2) return ecmc_fail(ECMC_ERR_CONFIGURATION, "at least two units are required");
     if (desc->n_taggers < 1 || desc->n_taggers > ECMC_MAX_TAGGERS)
         return ecmc_fail(ECMC_ERR_CONFIGURATION, "number of taggers out of range");
@@ -1065,6 +1066,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     mt_seed_host(desc->seed, mt);
     ChainState st;
     memset(&st, 0, sizeof(st));
+    st.cursor = (uint64_t)desc->rng_skip;   // words drawn before the run (random input handler)
     st.active = -1;
     st.aspeed = 1.0;
     st.occ_id = -1;

@@ -174,6 +174,7 @@ typedef struct {
     int32_t max_occupants;       /* <= 0: unbounded */
     int32_t scheduler;
     int32_t math_mode;
+    int32_t rng_skip;            /* words drawn before the run (the random input handler's draws) */
     uint64_t seed;
     int32_t n_taggers;
     ora_tagger taggers[ORA_MAX_TAGGERS];

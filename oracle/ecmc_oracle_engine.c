@@ -705,6 +705,7 @@ ora_engine *ora_engine_create(const ora_config *cfg, const double *positions, co
         occ_insert(e, c, i);
     }
     ora_mt_seed(&e->rng, cfg->seed);
+    for (int32_t k = 0; k < cfg->rng_skip; k++) ora_mt_u32(&e->rng);
     int total = 0;
     for (int t = 0; t < cfg->n_taggers; t++) total += cfg->taggers[t].n_handlers > 0 ? cfg->taggers[t].n_handlers : 1;
     e->h = (handler *)calloc(total, sizeof(handler));

@@ -38,6 +38,7 @@ class Config(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("n", ctypes.c_int32), ("system_length", ctypes.c_double),
                 ("beta", ctypes.c_double), ("cells_per_side", ctypes.c_int32 * 3), ("neighbor_layers", ctypes.c_int32),
                 ("max_occupants", ctypes.c_int32), ("scheduler", ctypes.c_int32), ("math_mode", ctypes.c_int32),
+                ("rng_skip", ctypes.c_int32),
                 ("seed", ctypes.c_uint64), ("n_taggers", ctypes.c_int32), ("taggers", Tagger * MAX_TAGGERS)]
 
 

@@ -29,8 +29,7 @@ def sampled_hard_disk_ini(filename, end_time=2.0, interval=0.37):
              "event_handler = fixed_interval_sampling_event_handler\n\n"
              "[FixedIntervalSamplingEventHandler]\nsampling_interval = %r\n"
              "output_handler = separation_output_handler\n\n"
-             "[InputOutputHandler]\noutput_handlers = separation_output_handler\n"
-             "input_handler = random_input_handler\n\n"
+             "[InputOutputHandler]\noutput_handlers = separation_output_handler\n\n"
              "[SeparationOutputHandler]\nfilename = %s\n" % (interval, filename))
     return text
 
