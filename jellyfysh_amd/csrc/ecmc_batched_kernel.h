@@ -322,6 +322,23 @@ __device__ __forceinline__ bool cand_geometry(float dx, float dy, float dz, int 
     return ambiguous;
 }
 
+// Pre-filter of pass 1 on the fixed-point separation (dx, dy, dz): false only for a candidate
+// that no factor can reach within the horizon.  Such a candidate lies outside the cylinder of
+// radius rc about the axis of motion, or more than rc behind the active unit, or more than
+// h + rc ahead of it: a repulsive factor has no event before its unit enters the cutoff
+// sphere ahead, an attractive one none behind beyond the cutoff (its distance only grows)
+// and none ahead before the closest approach.  RB = (rc + 2E)(1 + 2^-20) and HB = (h + rc +
+// 2E)(1 + 2^-20) cover the position slack E of each component (sqrt 2 E across the axis)
+// and the FP32 roundings; a separation within 2E of +-L/2 on any axis (minimum image not
+// certain) always passes.
+__device__ __forceinline__ bool cand_near(float dx, float dy, float dz, int d, float halfL, float E2, float RB2,
+                                          float RB, float HB) {
+    const bool amb = fabsf(fabsf(dx) - halfL) <= E2 || fabsf(fabsf(dy) - halfL) <= E2 || fabsf(fabsf(dz) - halfL) <= E2;
+    const float sd = d == 0 ? dx : (d == 1 ? dy : dz);
+    const float u = d == 0 ? dy : dx, v = d == 2 ? dy : dz;
+    return amb || (u * u + v * v < RB2 && sd > -RB && sd < HB);
+}
+
 // Occupant lists sorted by cell: move unit x, at slot p of cell X's segment, to cell Y != X
 // (one wavefront, uniform arguments).  Going up (Y > X), x leaves through X's last slot (X's
 // last unit fills p), every non-empty cell between passes its last unit to the slot before its
@@ -544,6 +561,13 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const float kf0 = (float)pk0, kf1 = (float)pk1;
         const int pf0 = (int)pw0, pf1 = (int)pw1;
         const float inv_beta_lo = (float)(1.0 / A.beta) * (1.f - 0x1p-22f);
+        // pre-filter constants (cand_near): rc + 2E rounded up, its square, half the box
+        const float RBc = (float)R.cutoff * (1.f + 0x1p-22f) + 2.f * E;
+        const float RB = RBc * (1.f + 0x1p-20f), RB2 = RB * RB * (1.f + 0x1p-20f);
+        const float halfL = 0.5f * Lf, E2 = 2.f * E;
+        // with 3 cells per side the 27 nearby cells are the whole box: candidate idx is unit idx
+        // or idx + 1, past the active unit
+        const bool whole_box = cps == 0 || cps == 3;
         // The next leg's 27 nearby cells with an inclusive scan of their counts, its horizon,
         // and cleared contender counters: wavefront 0, before the first leg and right after
         // each commit (so that no separate barrier precedes pass 1)
@@ -601,6 +625,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             const double h = s_h;
             const uint32_t qa0 = quant1(ap[0], inv_step), qa1 = quant1(ap[1], inv_step), qa2 = quant1(ap[2], inv_step);
             const float hf = (float)h * (1.f + 0x1p-22f) + 0x1p-60f;
+            const float HB = (hf + RBc) * (1.f + 0x1p-20f);
             BKey best = {INFINITY, 3, 0x7ffffff, 0};   // j < 2^27: the reduction packs (rank, j, f)
             double bestpos[3] = {0.0, 0.0, 0.0};
             unsigned geo_pass = 0;   // geometric survivors of this wavefront (uniform)
@@ -650,15 +675,9 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     bool surv = false;
                     int j = 0;
                     if (idx < M) {
-                        j = candidate_unit(idx, cps, a, s_pref, s_nearby, cu);
-                        Geo g;
-                        const bool amb = cand_geometry(qsep(qx[j], qa0, step), qsep(qy[j], qa1, step), qsep(qz[j], qa2, step), d, Lf, E, g);
-                        if (amb) {
-                            surv = true;
-                        } else if (g.perp2_lo < rc2_hi) {
-                            surv = factor_geometric_floor(g, kf0, rc2_hi) <= hf ||
-                                   (nf > 1 && factor_geometric_floor(g, kf1, rc2_hi) <= hf);
-                        }
+                        j = whole_box ? (idx < a ? idx : idx + 1) : candidate_unit(idx, cps, a, s_pref, s_nearby, cu);
+                        surv = cand_near(qsep(qx[j], qa0, step), qsep(qy[j], qa1, step), qsep(qz[j], qa2, step), d, halfL,
+                                         E2, RB2, RB, HB);
                     }
                     const unsigned long long m = __ballot(surv);
                     const int before = __popcll(m & ((1ull << lane) - 1ull));

@@ -26,10 +26,14 @@
 #define ECMC_FN __device__ __forceinline__
 #define ECMC_HD __host__ __device__ __forceinline__
 #define ECMC_TABLE __constant__
+// the rarely taken double-double fallbacks: called, not inlined, so that their registers do
+// not add to those of the fast paths around every call site
+#define ECMC_SLOW static __device__ __noinline__
 #else
 #define ECMC_FN static inline
 #define ECMC_HD static inline
 #define ECMC_TABLE
+#define ECMC_SLOW static
 #endif
 
 #include "ecmc_tables.h"
@@ -247,6 +251,8 @@ ECMC_FN double log_fast(double x, double &lo) {
     return r.h;
 }
 
+ECMC_SLOW double cr_log_slow(double x) { return dd_log(x).h; }
+
 // Correctly rounded natural logarithm: the fast path when its rounding is settled (all but
 // about one call in 500), else the double-double path.
 ECMC_FN double cr_log(double x) {
@@ -255,8 +261,7 @@ ECMC_FN double cr_log(double x) {
     double lo;
     const double hi = log_fast(x, lo);
     if (rounding_settled(hi, lo, 0x1p-64)) return hi + lo;
-    dd l = dd_log(x);
-    return l.h;
+    return cr_log_slow(x);
 }
 
 // x ** y for y the double nearest 1/k, k = 3 or 6 (2/p of the inverse-power inversion for
@@ -302,12 +307,9 @@ ECMC_FN double dd_pow_int(double x, int n) {
     return acc.h;
 }
 
-// Correctly rounded x ** y for the arguments the potentials use (x > 0).
-ECMC_FN double cr_pow(double x, double y) {
-    if (y == 2.0) return x * x;
-    if (y == 0.5 && x >= 0.0) return x == 0.0 ? 0.0 : sqrt(x);
-    if (y == 1.0) return x;
-    if (y == 0.0) return 1.0;
+// Correctly rounded x ** y for the arguments the potentials use (x > 0): the cases that need
+// more than one operation.
+ECMC_SLOW double cr_pow_slow(double x, double y) {
     if (!(x > 0.0) || isinf(x) || !isfinite(y) || x < 0x1p-1022 || x == 1.0) return pow(x, y);
     if (y >= 3.0 && y <= 32.0 && y == rint(y) && x > 0x1p-30 && x < 0x1p30) return dd_pow_int(x, (int)y);
     if ((y == 1.0 / 3.0 || y == 1.0 / 6.0) && x > 0x1p-900 && x < 0x1p900) {
@@ -322,10 +324,18 @@ ECMC_FN double cr_pow(double x, double y) {
     return e.h;
 }
 
+ECMC_FN double cr_pow(double x, double y) {
+    if (y == 2.0) return x * x;
+    if (y == 0.5 && x >= 0.0) return x == 0.0 ? 0.0 : sqrt(x);
+    if (y == 1.0) return x;
+    if (y == 0.0) return 1.0;
+    return cr_pow_slow(x, y);
+}
+
 // x ** 3 or x ** 6 (six) in double-double, rounded once: x^2 exact, x^3 = x^2 x and
 // x^6 = (x^3)^2 each about 2^-104 relative, for 2^-30 < x < 2^30 (else cr_pow).
 ECMC_FN double pow36(double x, bool six) {
-    if (!(x > 0x1p-30 && x < 0x1p30)) return cr_pow(x, six ? 6.0 : 3.0);
+    if (!(x > 0x1p-30 && x < 0x1p30)) return cr_pow_slow(x, six ? 6.0 : 3.0);
     const dd x2 = two_prod(x, x);
     const dd x3 = dd_mul_d(x2, x);
     const dd x6 = dd_mul(x3, x3);
@@ -339,7 +349,7 @@ ECMC_FN double root36(double x, bool six) {
         const double hi = root_pow_fast(x, six, lo);
         if (rounding_settled(hi, lo, 0x1p-92)) return hi + lo;
     }
-    return cr_pow(x, six ? 1.0 / 6.0 : 1.0 / 3.0);
+    return cr_pow_slow(x, six ? 1.0 / 6.0 : 1.0 / 3.0);
 }
 
 }  // namespace ecmc
