@@ -150,7 +150,9 @@ static int launch_class(ecmc_batch *b, int c, int legs, hipStream_t st) {
 // One step: reset the work queues, then every size class on its own stream.
 static int enqueue_step(ecmc_batch *b, int legs, hipStream_t main, bool timed) {
     hipEventRecord(b->fork[0], main);
-    for (int c = 0; c < b->n_classes; c++) {
+    // largest replicas first: their blocks are resident from the start, and the smaller
+    // classes fill the slots left beside them
+    for (int c = b->n_classes - 1; c >= 0; c--) {
         hipStreamWaitEvent(b->streams[c], b->fork[0], 0);
         if (timed) hipEventRecord(b->cls_ev0[c], b->streams[c]);
         int rc = launch_class(b, c, legs, b->streams[c]);

@@ -182,21 +182,54 @@ __device__ __forceinline__ bool bkey_eq(const BKey &a, const BKey &b) {
     return a.d == b.d && a.rank == b.rank && a.j == b.j && a.f == b.f;
 }
 
-// The leg's event over the workgroup: a wavefront shuffle over (displacement, packed (kind,
-// target, factor)), then LDS across wavefronts (one barrier).  The lane holding its
-// wavefront's winning key (keys are unique) leaves that candidate's FP64 position in
-// newpos[3 * wave]; the returned index is the winning wavefront.
+// One step of the (displacement, key) minimum across lanes with a DPP lane permutation of
+// the 16-lane row (CTRL: 0xB1 swaps neighbours, 0x4E pairs of neighbours, 0x141 mirrors each
+// half row, 0x140 the row), in registers.
+template <int CTRL>
+__device__ __forceinline__ void dpp_min_step(double &d, int &k) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(d), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(d), CTRL, 0xF, 0xF, false);
+    const int ok = __builtin_amdgcn_update_dpp(0, k, CTRL, 0xF, 0xF, false);
+    const double od = __hiloint2double(hi, lo);
+    if (od < d || (od == d && ok < k)) { d = od; k = ok; }
+}
+
+// The leg's event over the workgroup: the minimum over lanes of (displacement, packed (kind,
+// target, factor)), then LDS across wavefronts (one barrier).  With one wavefront per replica
+// the first 16 lanes reduce by DPP and the other rows join only when more than 16 lanes hold
+// keys (nlive: lanes beyond hold +inf), the result taken from lane 0; with several, a
+// shuffle butterfly.  The lane holding its wavefront's winning key (keys are unique) leaves
+// that candidate's FP64 position in newpos[3 * wave]; the returned index is the winning
+// wavefront.
 template <int WPR>
-__device__ int block_min(BKey &b, const BKey &mine, const double *pos, BKey *sk, double *newpos) {
+__device__ int block_min(BKey &b, const BKey &mine, const double *pos, BKey *sk, double *newpos, int nlive) {
     // (rank, j, f) packed into one word orders like the triple (rank < 4, j < 2^27, f < 2)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double d = b.d;
     int k = (b.rank << 28) | (b.j << 1) | b.f;
+    if (WPR == 1) {
+        dpp_min_step<0xB1>(d, k);
+        dpp_min_step<0x4E>(d, k);
+        dpp_min_step<0x141>(d, k);
+        dpp_min_step<0x140>(d, k);
+        if (nlive > 16) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double od = __shfl_xor(d, off, 64);
-        const int ok = __shfl_xor(k, off, 64);
-        if (od < d || (od == d && ok < k)) { d = od; k = ok; }
+            for (int off = 16; off <= 32; off <<= 1) {
+                const double od = __shfl_xor(d, off, 64);
+                const int ok = __shfl_xor(k, off, 64);
+                if (od < d || (od == d && ok < k)) { d = od; k = ok; }
+            }
+        }
+        d = __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(d)),
+                             __builtin_amdgcn_readfirstlane(__double2loint(d)));
+        k = __builtin_amdgcn_readfirstlane(k);
+    } else {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            const double od = __shfl_xor(d, off, 64);
+            const int ok = __shfl_xor(k, off, 64);
+            if (od < d || (od == d && ok < k)) { d = od; k = ok; }
+        }
     }
     b.d = d;
     b.rank = k >> 28;
@@ -283,6 +316,18 @@ __device__ __forceinline__ double exact_ipt(double s0, double s1, double s2, int
     return live ? res : INFINITY;
 }
 
+// The exact FP64 event displacement of one contender: the exponential draw dU = -log(1 - u) /
+// beta from its two words (correctly rounded log), then exact_ipt.
+#ifndef ECMC_BATCHED_EXACT_ATTR
+#define ECMC_BATCHED_EXACT_ATTR __forceinline__
+#endif
+__device__ ECMC_BATCHED_EXACT_ATTR double exact_event(uint32_t wa, uint32_t wb, double beta, double s0, double s1,
+                                                      double s2, int d, bool six, double prefactor, double rc2,
+                                                      double urc) {
+    const double dU = -cr_log(1.0 - unit53(wa, wb)) / beta;
+    return exact_ipt(s0, s1, s2, d, six, prefactor, rc2, urc, dU);
+}
+
 // Positions in LDS: 16-bit fixed point per coordinate, q = round(x 2^16 / L) mod 2^16 (x in
 // [0, L)), one uint16 array per axis.  The separation q_j - q_a wrapped into [-2^15, 2^15) is
 // the minimum image exactly in integers; times L 2^-16 it is within L 2^-16 (two roundings of
@@ -323,20 +368,21 @@ __device__ __forceinline__ bool cand_geometry(float dx, float dy, float dz, int 
 }
 
 // Pre-filter of pass 1 on the fixed-point separation (dx, dy, dz): false only for a candidate
-// that no factor can reach within the horizon.  Such a candidate lies outside the cylinder of
-// radius rc about the axis of motion, or more than rc behind the active unit, or more than
-// h + rc ahead of it: a repulsive factor has no event before its unit enters the cutoff
-// sphere ahead, an attractive one none behind beyond the cutoff (its distance only grows)
-// and none ahead before the closest approach.  RB = (rc + 2E)(1 + 2^-20) and HB = (h + rc +
-// 2E)(1 + 2^-20) cover the position slack E of each component (sqrt 2 E across the axis)
-// and the FP32 roundings; a separation within 2E of +-L/2 on any axis (minimum image not
-// certain) always passes.
+// that no factor can reach within the horizon h.  Every candidate with an event at most h
+// away lies within rc of the segment [0, h] along the axis of motion: a repulsive factor has
+// no event before its unit enters the cutoff sphere (ahead), an attractive one none for a
+// unit behind beyond the cutoff (its distance only grows) and none ahead before the closest
+// approach (at s_d <= h).  The fixed-point separation is within E of the true one on each
+// axis, so within sqrt(3) E < 2E of it; hf >= h; RB2 = ((rc + 2E)(1 + 2^-20))^2 (1 + 2^-20)
+// covers these and the FP32 roundings.  A separation within 2E of +-L/2 on any axis (minimum
+// image not certain) always passes.
 __device__ __forceinline__ bool cand_near(float dx, float dy, float dz, int d, float halfL, float E2, float RB2,
-                                          float RB, float HB) {
+                                          float hf) {
     const bool amb = fabsf(fabsf(dx) - halfL) <= E2 || fabsf(fabsf(dy) - halfL) <= E2 || fabsf(fabsf(dz) - halfL) <= E2;
     const float sd = d == 0 ? dx : (d == 1 ? dy : dz);
     const float u = d == 0 ? dy : dx, v = d == 2 ? dy : dz;
-    return amb || (u * u + v * v < RB2 && sd > -RB && sd < HB);
+    const float e = sd - fminf(fmaxf(sd, 0.f), hf);
+    return amb || u * u + v * v + e * e < RB2;
 }
 
 // Occupant lists sorted by cell: move unit x, at slot p of cell X's segment, to cell Y != X
@@ -417,15 +463,15 @@ __host__ __device__ inline size_t batched_lds_bytes(int nmax, int cellmax, int w
 
 // One leg of a replica, for the workgroup that owns it:
 //   1. horizon h = min(distance to the next cell wall, rest of the chain) (FP64);
-//   2. pass 1a over every candidate: FP32 geometry and the geometric floors of its
-//      factors; candidates whose floors are all beyond h are done, the others are appended
-//      to the wavefront's queue (ballot compaction);
-//   3. pass 1b over each full queue: Philox draws, FP32 lower bounds of the event
-//      displacements (FP32 dU bound from the draw's leading bits); a factor whose bound is
-//      <= h is a contender and its FP64 target position is loaded right away (the load's
-//      latency overlaps the rest of the pass);
-//   4. pass 2, one contender per thread: exact FP64 inversion with correctly rounded
-//      pow/log, workgroup min-reduce over (displacement, kind, target, factor);
+//   2. pass 1a over every candidate: the fixed-point separation against the capsule of
+//      radius rc about the segment of motion (cand_near); candidates outside are done, the
+//      others are appended to the wavefront's queue (ballot compaction);
+//   3. pass 1b over each full queue: FP32 geometry with the position slack, Philox draws,
+//      FP32 lower bounds of the event displacements (FP32 dU bound from the draw's leading
+//      bits); a factor whose bound is <= h is a contender;
+//   4. pass 2, one contender per thread: its FP64 position from global memory, exact FP64
+//      inversion with correctly rounded pow/log, workgroup min-reduce over (displacement,
+//      kind, target, factor);
 //   5. thread 0 commits.
 template <int WPR>
 __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched_kernel(BatchArgs A) {
@@ -561,9 +607,9 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const float kf0 = (float)pk0, kf1 = (float)pk1;
         const int pf0 = (int)pw0, pf1 = (int)pw1;
         const float inv_beta_lo = (float)(1.0 / A.beta) * (1.f - 0x1p-22f);
-        // pre-filter constants (cand_near): rc + 2E rounded up, its square, half the box
-        const float RBc = (float)R.cutoff * (1.f + 0x1p-22f) + 2.f * E;
-        const float RB = RBc * (1.f + 0x1p-20f), RB2 = RB * RB * (1.f + 0x1p-20f);
+        // pre-filter constants (cand_near): (rc + 2E)^2 rounded up, half the box
+        const float RB = ((float)R.cutoff * (1.f + 0x1p-22f) + 2.f * E) * (1.f + 0x1p-20f);
+        const float RB2 = RB * RB * (1.f + 0x1p-20f);
         const float halfL = 0.5f * Lf, E2 = 2.f * E;
         // with 3 cells per side the 27 nearby cells are the whole box: candidate idx is unit idx
         // or idx + 1, past the active unit
@@ -625,48 +671,58 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             const double h = s_h;
             const uint32_t qa0 = quant1(ap[0], inv_step), qa1 = quant1(ap[1], inv_step), qa2 = quant1(ap[2], inv_step);
             const float hf = (float)h * (1.f + 0x1p-22f) + 0x1p-60f;
-            const float HB = (hf + RBc) * (1.f + 0x1p-20f);
             BKey best = {INFINITY, 3, 0x7ffffff, 0};   // j < 2^27: the reduction packs (rank, j, f)
             double bestpos[3] = {0.0, 0.0, 0.0};
             unsigned geo_pass = 0;   // geometric survivors of this wavefront (uniform)
             if (A.mode != 1) {
                 // ---- pass 1b: bounds of the queued candidates of this wavefront
+                int nc = 0;   // contenders so far (one wavefront per replica: uniform register)
                 auto bound_queue = [&](int cnt) {
-                    if (lane >= cnt) return;
-                    const int j = queue[lane];
-                    // the FP64 position, needed if a factor turns out a contender: its memory
-                    // latency overlaps the bound computation below
-                    const double fx = P[j], fy = P[(size_t)n + j], fz = P[2 * (size_t)n + j];
-                    int slot0 = -1, slot1 = -1;
-                    Geo g;
-                    const bool amb = cand_geometry(qsep(qx[j], qa0, step), qsep(qy[j], qa1, step), qsep(qz[j], qa2, step), d, Lf, E, g);
-                    uint32_t w[4];
-                    philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, w);
+                    const bool act = lane < cnt;
+                    int j = 0;
+                    uint32_t w[4] = {0u, 0u, 0u, 0u};
+                    bool con[2] = {false, false};
+                    if (act) {
+                        j = queue[lane];
+                        Geo g;
+                        const bool amb = cand_geometry(qsep(qx[j], qa0, step), qsep(qy[j], qa1, step), qsep(qz[j], qa2, step),
+                                                       d, Lf, E, g);
+                        philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, w);
 #pragma unroll
-                    for (int f = 0; f < 2; f++) {
-                        if (f >= nf) break;
-                        float t = 0.f;
-                        if (!amb) {
-                            if (g.perp2_lo >= rc2_hi) continue;
-                            if (factor_geometric_floor(g, pick(f, kf0, kf1), rc2_hi) > hf) continue;
-                            t = factor_lower_bound(g, pick(f, kf0, kf1), pick(f, pf0, pf1), rc2_hi, pick(f, vrc0, vrc1),
-                                                   dU_lower(pick(f, w[0], w[2]), inv_beta_lo));
-                        }
-                        if (t <= hf) {
-                            const int slot = atomicAdd(&s_nc, 1);
-                            if (slot >= B_CONTENDERS) {
-                                s_overflow = 1;
-                                continue;
+                        for (int f = 0; f < 2; f++) {
+                            if (f >= nf) break;
+                            float t = 0.f;
+                            if (!amb) {
+                                if (g.perp2_lo >= rc2_hi) continue;
+                                if (factor_geometric_floor(g, pick(f, kf0, kf1), rc2_hi) > hf) continue;
+                                t = factor_lower_bound(g, pick(f, kf0, kf1), pick(f, pf0, pf1), rc2_hi, pick(f, vrc0, vrc1),
+                                                       dU_lower(pick(f, w[0], w[2]), inv_beta_lo));
                             }
-                            cont_id[slot] = (j << 1) | f;
-                            cont_w[2 * slot] = pick(f, w[0], w[2]);
-                            cont_w[2 * slot + 1] = pick(f, w[1], w[3]);
-                            if (f == 0) slot0 = slot;
-                            else slot1 = slot;
+                            con[f] = t <= hf;
                         }
                     }
-                    if (slot0 >= 0) { cont_pos[3 * slot0] = fx; cont_pos[3 * slot0 + 1] = fy; cont_pos[3 * slot0 + 2] = fz; }
-                    if (slot1 >= 0) { cont_pos[3 * slot1] = fx; cont_pos[3 * slot1 + 1] = fy; cont_pos[3 * slot1 + 2] = fz; }
+                    // contender slots: with one wavefront per replica from ballots over the whole
+                    // (converged) wavefront, else from the workgroup's LDS counter
+#pragma unroll
+                    for (int f = 0; f < 2; f++) {
+                        int slot;
+                        if (WPR == 1) {
+                            const unsigned long long m = __ballot(con[f]);
+                            slot = nc + __popcll(m & ((1ull << lane) - 1ull));
+                            nc += __popcll(m);
+                            if (!con[f]) continue;
+                        } else {
+                            if (!con[f]) continue;
+                            slot = atomicAdd(&s_nc, 1);
+                        }
+                        if (slot >= B_CONTENDERS) {
+                            s_overflow = 1;
+                            continue;
+                        }
+                        cont_id[slot] = (j << 1) | f;
+                        cont_w[2 * slot] = pick(f, w[0], w[2]);
+                        cont_w[2 * slot + 1] = pick(f, w[1], w[3]);
+                    }
                 };
                 // ---- pass 1a: geometric test of every candidate, survivors queued
                 int qn = 0;
@@ -677,7 +733,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     if (idx < M) {
                         j = whole_box ? (idx < a ? idx : idx + 1) : candidate_unit(idx, cps, a, s_pref, s_nearby, cu);
                         surv = cand_near(qsep(qx[j], qa0, step), qsep(qy[j], qa1, step), qsep(qz[j], qa2, step), d, halfL,
-                                         E2, RB2, RB, HB);
+                                         E2, RB2, hf);
                     }
                     const unsigned long long m = __ballot(surv);
                     const int before = __popcll(m & ((1ull << lane) - 1ull));
@@ -693,6 +749,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     geo_pass += (unsigned)__popcll(m);
                 }
                 if (qn > 0) bound_queue(qn);
+                if (WPR == 1 && lane == 0) s_nc = nc;
                 bsync();
             }
             const bool all_exact = (A.mode == 1) || s_overflow;
@@ -713,15 +770,15 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 } else {
                     j = cont_id[w] >> 1;
                     f = cont_id[w] & 1;
-                    pj[0] = cont_pos[3 * w]; pj[1] = cont_pos[3 * w + 1]; pj[2] = cont_pos[3 * w + 2];
+                    // the FP64 position from global memory (its latency overlaps the log of
+                    // the draw inside exact_event)
+                    pj[0] = P[j]; pj[1] = P[(size_t)n + j]; pj[2] = P[2 * (size_t)n + j];
                     wa = cont_w[2 * w];
                     wb = cont_w[2 * w + 1];
                 }
-                const double u = unit53(wa, wb);
-                const double dU = -cr_log(1.0 - u) / A.beta;
-                const double dx = exact_ipt(bmin_image(pj[0] - ap[0], L), bmin_image(pj[1] - ap[1], L),
-                                            bmin_image(pj[2] - ap[2], L), d, pick(f, pf0, pf1) == 12, pick(f, pk0, pk1),
-                                            rc2, pick(f, urc0, urc1), dU);
+                const double dx = exact_event(wa, wb, A.beta, bmin_image(pj[0] - ap[0], L), bmin_image(pj[1] - ap[1], L),
+                                              bmin_image(pj[2] - ap[2], L), d, pick(f, pf0, pf1) == 12, pick(f, pk0, pk1),
+                                              rc2, pick(f, urc0, urc1));
                 BKey c = {dx, 0, j, f};
                 if (bkey_lt(c, best)) {
                     best = c;
@@ -736,10 +793,10 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     const int j = candidate_unit(idx / nf, cps, a, s_pref, s_nearby, cu);
                     uint32_t ww[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
-                    const double dU = -cr_log(1.0 - unit53(pick(f, ww[0], ww[2]), pick(f, ww[1], ww[3]))) / A.beta;
-                    const double dx = exact_ipt(bmin_image(P[j] - ap[0], L), bmin_image(P[(size_t)n + j] - ap[1], L),
-                                                bmin_image(P[2 * (size_t)n + j] - ap[2], L), d, pick(f, pf0, pf1) == 12,
-                                                pick(f, pk0, pk1), rc2, pick(f, urc0, urc1), dU);
+                    const double dx = exact_event(pick(f, ww[0], ww[2]), pick(f, ww[1], ww[3]), A.beta,
+                                                  bmin_image(P[j] - ap[0], L), bmin_image(P[(size_t)n + j] - ap[1], L),
+                                                  bmin_image(P[2 * (size_t)n + j] - ap[2], L), d, pick(f, pf0, pf1) == 12,
+                                                  pick(f, pk0, pk1), rc2, pick(f, urc0, urc1));
                     if (dx <= h) {
                         bool found = false;
                         for (int q = 0; q < min(s_nc, B_CONTENDERS); q++)
@@ -755,7 +812,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 BKey c2k = {R.chain_left, 2, 0, 0};
                 if (bkey_lt(c2k, best)) best = c2k;
             }
-            const int wbest = block_min<WPR>(best, mine, bestpos, sk, s_newpos);
+            const int wbest = block_min<WPR>(best, mine, bestpos, sk, s_newpos, nwork);
             if (tid == 0) {
                 s_stats[0] += (unsigned long long)M;
                 s_stats[2] += (unsigned long long)min(s_nc, B_CONTENDERS);
