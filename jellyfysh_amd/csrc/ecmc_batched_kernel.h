@@ -631,8 +631,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         // with 3 cells per side the 27 nearby cells are the whole box: candidate idx is unit idx
         // or idx + 1, past the active unit
         const bool whole_box = cps == 0 || cps == 3;
-        // The next leg's 27 nearby cells with an inclusive scan of their counts, its horizon,
-        // and cleared contender counters: wavefront 0, before the first leg and right after
+        // The next leg's 27 nearby cells with an inclusive scan of their counts (not needed when
+        // they are the whole box), its horizon, and cleared contender counters: wavefront 0, before the first leg and right after
         // each commit (so that no separate barrier precedes pass 1)
         auto prepare_leg = [&]() {
             const double pa[3] = {s_ap[0], s_ap[1], s_ap[2]};
@@ -641,6 +641,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             if (cps) {
 #pragma unroll
                 for (int q = 0; q < 3; q++) pc[q] = bcell1(pa[q], R.side, cps);
+            }
+            if (cps && !whole_box) {
                 int cnt = 0;
                 if (lane < 27) {
                     int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
@@ -649,12 +651,15 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     s_nearby[lane] = cs[c];
                     cnt = cs[c + 1] - cs[c];
                 }
+                // inclusive scan of the 27 counts: within each 16-lane row by DPP row shifts
+                // (a lane whose source is outside the row adds 0), then lanes 16-26 add row 0's sum
                 int incl = cnt;
-#pragma unroll
-                for (int off = 1; off < 32; off <<= 1) {
-                    const int o = __shfl_up(incl, off, 64);
-                    if (lane >= off) incl += o;
-                }
+                incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xF, 0xF, false);
+                incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xF, 0xF, false);
+                incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xF, 0xF, false);
+                incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xF, 0xF, false);
+                const int row0 = __builtin_amdgcn_readlane(incl, 15);
+                if (lane >= 16) incl += row0;
                 if (lane < 27) s_pref[lane + 1] = incl;
                 if (lane == 0) s_pref[0] = 0;
                 if (lane == 26) s_M = incl;
@@ -664,9 +669,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 if (cps) {
                     cb = s_bounds[pc[dn_] + 1] - pa[dn_];
                     if (cb < hh) hh = cb;
-                } else {
-                    s_M = n - 1;
                 }
+                if (!cps || whole_box) s_M = n - 1;
                 s_h = hh;
                 s_cb = cb;
                 s_nc = 0;
@@ -791,7 +795,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 uint32_t wa, wb;
                 if (all_exact) {
                     f = w % nf;
-                    j = candidate_unit(w / nf, cps, a, s_pref, s_nearby, cu);
+                    const int idx = w / nf;
+                    j = whole_box ? (idx < a ? idx : idx + 1) : candidate_unit(idx, cps, a, s_pref, s_nearby, cu);
                     pj[0] = P[j]; pj[1] = P[(size_t)n + j]; pj[2] = P[2 * (size_t)n + j];
                     uint32_t ww[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
@@ -820,7 +825,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 // the selection's barrier: after it, wavefront 0 already prepares the next leg)
                 for (int idx = tid; idx < M * nf; idx += NTH) {
                     const int f = idx % nf;
-                    const int j = candidate_unit(idx / nf, cps, a, s_pref, s_nearby, cu);
+                    const int ci = idx / nf;
+                    const int j = whole_box ? (ci < a ? ci : ci + 1) : candidate_unit(ci, cps, a, s_pref, s_nearby, cu);
                     uint32_t ww[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
                     const double dx = exact_event(pick(f, ww[0], ww[2]), pick(f, ww[1], ww[3]), A.beta,
