@@ -8,7 +8,9 @@ Workload of the headline number (one "step" = every replica advances --legs legs
   starts.  Inputs are resident on the GPU before the timed region.
 Beside it, the plain form of the same potential at N = 4096, rho = 0.8 (16^3 lattice start) runs
 the reference's leg loop on one workgroup, and the oracle runs a bounded sample of the
-batched workload on the host cores (the CPU baseline).
+batched workload on the host cores (the CPU baseline).  256 charged-soft-sphere replicas
+(N = 512, merged-image Coulomb + inverse power 9) report their rate in one launch of the plain
+chain kernel against one replica alone (--no-charged skips them).
 
     python bench.py [--steps K] [--warmup W] [--legs L]
 """
@@ -45,6 +47,9 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-charged", action="store_true", help="skip the charged-soft-sphere replicas")
+    ap.add_argument("--charged-replicas", type=int, default=256)
+    ap.add_argument("--charged-legs", type=int, default=2000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel 10 times over BYTES (scales hardware byte counters)")
     return ap.parse_args()
@@ -166,6 +171,35 @@ def batched_pressure_at(rho, z, rho0=0.8, width=0.02, n=None, n_min=256, n_max=5
     cov = s2 * np.linalg.inv(A.T @ A)
     return dict(mean=float(coef[0]), err=float(np.sqrt(cov[0, 0])), slope=float(coef[1]), replicas=int(len(x)),
                 window="|rho - %.2f| < %.2f, %d <= N <= %d" % (rho0, width, n_min, n_max))
+
+
+def charged_spheres_rate(replicas, legs):
+    """Charged soft spheres (N = 512, L = 8, merged-image Coulomb + inverse power 9, beta = 1):
+    `replicas` independent systems advanced together by the plain chain kernel (one workgroup
+    each, one launch), against one replica alone."""
+    import torch
+    from jellyfysh_amd.chain import run_together
+    from jellyfysh_amd.workloads import charged_soft_sphere_chains
+    t0 = time.perf_counter()
+    chains = charged_soft_sphere_chains(replicas, seed=1000)
+    create_s = time.perf_counter() - t0
+    run_together(chains, 200)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    k, ms = run_together(chains, legs)
+    wall = time.perf_counter() - t0
+    k1, ms1 = run_together(chains[:1], legs)
+    est = np.array([c.estimators() for c in chains])
+    for c in chains:
+        c.close()
+    return dict(replicas=replicas, n=512, system_length=8.0, legs_per_replica=legs,
+                events_per_sec=float(k.sum()) / wall, kernel_ms=ms,
+                single_replica_events_per_sec=float(k1[0]) / (ms1 * 1e-3),
+                lifts_per_leg=float(est[:, 4].sum() / max(k.sum(), 1)),
+                rejected_thinning_per_leg=float(est[:, 6].sum() / max(k.sum(), 1)), create_s=create_s,
+                note="plain chain kernel, one workgroup per replica in one launch; FP64 event times; Coulomb "
+                     "through cell bounding (FP64 Ewald tables), nearby and surplus bounds with thinning; "
+                     "repulsion inverted exactly over all pairs")
 
 
 def cpu_baseline(n, lengths, pos, seconds, threads):
@@ -296,6 +330,9 @@ def main():
         "batched": {"step_ms_sum": kernel_ms, "events": events, "candidates_per_leg": cands / max(events, 1),
                     "class_kernel_ms": [ms for ms, _ in cls_ms], "pressure_rho_0_80": pz},
     }
+    if not args.no_charged:
+        log("charged soft spheres ...")
+        out["charged_soft_spheres"] = charged_spheres_rate(args.charged_replicas, args.charged_legs)
     if not args.no_plain:
         log("plain chain ...")
         pl = plain_chain_rate(args.plain_legs, args.plain_seconds, args.plain_equilibration)

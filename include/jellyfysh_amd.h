@@ -155,6 +155,13 @@ int ecmc_system_estimators(const ecmc_system *sys, double *est8);
 int ecmc_system_rng_words(const ecmc_system *sys, int64_t *words);
 /* Kernel time of the last ecmc_system_run in milliseconds (HIP events on its stream). */
 int ecmc_system_last_kernel_ms(const ecmc_system *sys, double *ms);
+/* Independent systems advanced together: one launch, one workgroup per system, each system
+ * committing up to max_events legs exactly as ecmc_system_run would (no event logs).  All
+ * systems have one dimension and appear once; n_committed (count entries, may be NULL) gets
+ * each system's committed legs; the launch runs on the first system's stream, whose kernel
+ * time every system then reports.  Replaces, for a set of independent runs, one
+ * SingleProcessMediator.run per system (mediator/single_process_mediator.py:84-156). */
+int ecmc_systems_run(ecmc_system *const *systems, int32_t count, int64_t max_events, int64_t *n_committed);
 
 /* ---- batched independent replicas (no reference counterpart as a whole) ------------------
  * Each replica is a reference chain (straight chains of chain_length cycling +x,+y,+z,

@@ -78,6 +78,7 @@ def _signatures():
         "ecmc_system_estimators": (i32, [vp, pd]),
         "ecmc_system_rng_words": (i32, [vp, pi64]),
         "ecmc_system_last_kernel_ms": (i32, [vp, pd]),
+        "ecmc_systems_run": (i32, [vp, i32, i64, pi64]),
         "ecmc_pair_displacement": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, vp, d, vp, vp]),
         "ecmc_pair_derivative": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, d, vp, vp]),
         "ecmc_math_log": (i32, [i64, vp, vp, vp]),

@@ -163,3 +163,15 @@ class DeviceChain:
         ms = ctypes.c_double()
         L.check(L.lib().ecmc_system_last_kernel_ms(self._h, ctypes.byref(ms)))
         return ms.value
+
+
+def run_together(chains, max_events):
+    """Advance independent chains in one launch (ecmc_systems_run): one workgroup per chain,
+    each committing up to max_events legs exactly as DeviceChain.run would (no event logs).
+    Returns the committed legs per chain and the launch's kernel time in milliseconds."""
+    if not chains:
+        raise ValueError("no chains")
+    hs = (ctypes.c_void_p * len(chains))(*[c._h.value for c in chains])
+    n = np.zeros(len(chains), dtype=np.int64)
+    L.check(L.lib().ecmc_systems_run(hs, len(chains), int(max_events), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+    return n, chains[0].last_kernel_ms()

@@ -29,6 +29,7 @@
 #include <math.h>
 #include <vector>
 #include <string>
+#include <mutex>
 
 #include "../../include/jellyfysh_amd.h"
 #include "ecmc_math.h"
@@ -610,8 +611,9 @@ __device__ bool create_end_of_chain(const ChainArgs &a, ChainState &st, const ec
     return false;
 }
 
+// One workgroup runs one system's legs (the whole leg loop of the reference's mediator).
 template <int D>
-__global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
+__device__ __forceinline__ void chain_body(const ChainArgs &a) {
     __shared__ ChainState st;
     __shared__ ecmc_tagger_desc tg[ECMC_MAX_TAGGERS];
     __shared__ uint32_t mt[624];
@@ -841,6 +843,19 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
     for (int i = tid; i < 624; i += NT) a.mt[i] = mt[i];
 }
 
+template <int D>
+__global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
+    chain_body<D>(a);
+}
+
+// Independent systems in one launch: workgroup b runs system b (its own positions, cell
+// occupancy, random stream and state), exactly as chain_kernel runs it alone.
+template <int D>
+__global__ __launch_bounds__(NT) void chain_many_kernel(const ChainArgs *args) {
+    const ChainArgs a = args[blockIdx.x];
+    chain_body<D>(a);
+}
+
 // InnerPointEstimator.derivative_bound (inner_point_estimator.py:108-171) for the cell
 // separations of cell_bounding_potential.py:100-126: one workgroup per (cell, direction)
 // evaluates the merged-image Coulomb derivative on the (pps + 1)^3 grid spanning the corners
@@ -941,6 +956,36 @@ extern "C" int ecmc_system_set_stop_mask(ecmc_system *sys, int32_t mask) {
     if (!sys) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
     sys->stop_mask = mask;
     return ECMC_OK;
+}
+
+// Cell-bound tables already built in this process, per device, keyed by everything they
+// depend on (cell corners, box, potential and estimator parameters); the last eight are kept.
+struct CellBoundCacheEntry {
+    int device;
+    std::vector<double> key, table;
+};
+static std::mutex g_cb_mutex;
+static std::vector<CellBoundCacheEntry> g_cb_cache;
+
+static bool cell_bound_cache_find(const std::vector<double> &key, std::vector<double> &table) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    std::lock_guard<std::mutex> lock(g_cb_mutex);
+    for (const CellBoundCacheEntry &e : g_cb_cache)
+        if (e.device == dev && e.key.size() == key.size() &&
+            memcmp(e.key.data(), key.data(), sizeof(double) * key.size()) == 0) {
+            table = e.table;
+            return true;
+        }
+    return false;
+}
+
+static void cell_bound_cache_store(const std::vector<double> &key, const std::vector<double> &table) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lock(g_cb_mutex);
+    if (g_cb_cache.size() >= 8) g_cb_cache.erase(g_cb_cache.begin());
+    g_cb_cache.push_back({dev, key, table});
 }
 
 extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *positions, const double *charges,
@@ -1186,18 +1231,38 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
                 corners[(size_t)c * 6 + 3 + d] = is_near ? NAN : grid.cmax[(size_t)c * dim + d] - grid.cmin[d];
             }
         }
-        if (hipMalloc((void **)&s->d_corners, sizeof(double) * corners.size()) != hipSuccess ||
-            hipMalloc((void **)&s->d_cbt, sizeof(double) * grid.count * 6) != hipSuccess ||
-            hipMemcpy(s->d_corners, corners.data(), sizeof(double) * corners.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        if (hipMalloc((void **)&s->d_cbt, sizeof(double) * grid.count * 6) != hipSuccess) {
             ecmc_system_destroy(s);
             return ecmc_fail(ECMC_ERR_HIP, "cell-bound table allocation failed");
         }
-        hipLaunchKernelGGL(cell_bound_kernel, dim3(grid.count * 3), dim3(NT), 0, s->stream, a.ewald, s->d_corners,
-                           desc->system_length, tg.par[3], tg.bpar[2], (int)tg.bpar[1], tg.bpar[0], tg.bpar[3],
-                           s->d_cbt);
-        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess) {
-            ecmc_system_destroy(s);
-            return ecmc_fail(ECMC_ERR_HIP, "cell-bound table kernel failed");
+        // the table depends only on the cell geometry, the box, the potential and the
+        // estimator: replicas of one system take it from the cache instead of rebuilding it
+        std::vector<double> key(corners);
+        key.push_back(desc->system_length);
+        for (int q = 0; q < 4; q++) key.push_back(tg.par[q]);
+        for (int q = 0; q < 4; q++) key.push_back(tg.bpar[q]);
+        std::vector<double> table;
+        if (cell_bound_cache_find(key, table)) {
+            if (hipMemcpy(s->d_cbt, table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                ecmc_system_destroy(s);
+                return ecmc_fail(ECMC_ERR_HIP, "cell-bound table upload failed");
+            }
+        } else {
+            if (hipMalloc((void **)&s->d_corners, sizeof(double) * corners.size()) != hipSuccess ||
+                hipMemcpy(s->d_corners, corners.data(), sizeof(double) * corners.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                ecmc_system_destroy(s);
+                return ecmc_fail(ECMC_ERR_HIP, "cell-bound table allocation failed");
+            }
+            hipLaunchKernelGGL(cell_bound_kernel, dim3(grid.count * 3), dim3(NT), 0, s->stream, a.ewald, s->d_corners,
+                               desc->system_length, tg.par[3], tg.bpar[2], (int)tg.bpar[1], tg.bpar[0], tg.bpar[3],
+                               s->d_cbt);
+            table.resize((size_t)grid.count * 6);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s->stream) != hipSuccess ||
+                hipMemcpy(table.data(), s->d_cbt, sizeof(double) * table.size(), hipMemcpyDeviceToHost) != hipSuccess) {
+                ecmc_system_destroy(s);
+                return ecmc_fail(ECMC_ERR_HIP, "cell-bound table kernel failed");
+            }
+            cell_bound_cache_store(key, table);
         }
         a.cbt = s->d_cbt;
     }
@@ -1262,6 +1327,58 @@ extern "C" int ecmc_system_run(ecmc_system *s, int64_t max_events, ecmc_event *h
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the event log failed");
     if (n_committed) *n_committed = k;
     return status_of_state(st);
+}
+
+extern "C" int ecmc_systems_run(ecmc_system *const *systems, int32_t count, int64_t max_events,
+                                int64_t *n_committed) {
+    if (!systems || count < 1 || max_events < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    const int dim = systems[0] ? systems[0]->dim : 0;
+    std::vector<ChainArgs> args((size_t)count);
+    for (int i = 0; i < count; i++) {
+        ecmc_system *s = systems[i];
+        if (!s) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
+        if (s->dim != dim) return ecmc_fail(ECMC_ERR_ARGUMENT, "all systems of one launch must have one dimension");
+        for (int k = 0; k < i; k++)
+            if (systems[k] == s) return ecmc_fail(ECMC_ERR_ARGUMENT, "a system appears twice in one launch");
+        args[i] = s->args;
+        args[i].log = nullptr;
+        args[i].max_events = max_events;
+        args[i].stop_mask = s->stop_mask;
+        if (n_committed) n_committed[i] = 0;
+    }
+    if (max_events == 0) return ECMC_OK;
+    ecmc_system *s0 = systems[0];
+    ChainArgs *d_args = nullptr;
+    if (hipMalloc((void **)&d_args, sizeof(ChainArgs) * count) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMalloc of the launch arguments failed");
+    if (hipMemcpyAsync(d_args, args.data(), sizeof(ChainArgs) * count, hipMemcpyHostToDevice, s0->stream) != hipSuccess) {
+        hipFree(d_args);
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
+    }
+    hipEventRecord(s0->ev0, s0->stream);
+    if (dim == 2) hipLaunchKernelGGL(chain_many_kernel<2>, dim3(count), dim3(NT), 0, s0->stream, d_args);
+    else hipLaunchKernelGGL(chain_many_kernel<3>, dim3(count), dim3(NT), 0, s0->stream, d_args);
+    hipEventRecord(s0->ev1, s0->stream);
+    const bool launched = hipGetLastError() == hipSuccess;
+    const bool ran = hipStreamSynchronize(s0->stream) == hipSuccess;
+    hipFree(d_args);
+    if (!launched) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
+    if (!ran) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, s0->ev0, s0->ev1);
+    int rc = ECMC_OK;
+    for (int i = 0; i < count; i++) {
+        ecmc_system *s = systems[i];
+        s->last_ms = ms;
+        int64_t k = 0;
+        ChainState st;
+        if (hipMemcpy(&k, s->d_nout, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(&st, s->d_st, sizeof(ChainState), hipMemcpyDeviceToHost) != hipSuccess)
+            return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+        if (n_committed) n_committed[i] = k;
+        if (rc == ECMC_OK) rc = status_of_state(st);
+    }
+    return rc;
 }
 
 extern "C" int ecmc_system_positions(const ecmc_system *s, double *out) {
