@@ -197,10 +197,11 @@ __device__ __forceinline__ void dpp_min_step(double &d, int &k) {
 // The leg's event over the workgroup: the minimum over lanes of (displacement, packed (kind,
 // target, factor)), then LDS across wavefronts (one barrier).  With one wavefront per replica
 // the first 16 lanes reduce by DPP and the other rows join only when more than 16 lanes hold
-// keys (nlive: lanes beyond hold +inf), the result taken from lane 0; with several, a
-// shuffle butterfly.  The lane holding its wavefront's winning key (keys are unique) leaves
-// that candidate's FP64 position in newpos[3 * wave]; the returned index is the winning
-// wavefront.
+// keys (nlive: lanes beyond hold +inf), the result taken from lane 0, and the returned index
+// is the lane holding the winning key (keys are unique; 0 unless a pair wins), from which the
+// caller reads the candidate's position; with several, a shuffle butterfly, the winning lane
+// of each wavefront leaves that candidate's FP64 position in newpos[3 * wave], and the
+// returned index is the winning wavefront.
 template <int WPR>
 __device__ int block_min(BKey &b, const BKey &mine, const double *pos, BKey *sk, double *newpos, int nlive) {
     // (rank, j, f) packed into one word orders like the triple (rank < 4, j < 2^27, f < 2)
@@ -223,6 +224,12 @@ __device__ int block_min(BKey &b, const BKey &mine, const double *pos, BKey *sk,
         d = __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(d)),
                              __builtin_amdgcn_readfirstlane(__double2loint(d)));
         k = __builtin_amdgcn_readfirstlane(k);
+        b.d = d;
+        b.rank = k >> 28;
+        b.j = (k >> 1) & 0x7ffffff;
+        b.f = k & 1;
+        const unsigned long long wm = __ballot(b.rank == 0 && bkey_eq(mine, b));
+        return wm ? __ffsll((long long)wm) - 1 : 0;
     } else {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
@@ -556,7 +563,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const int chunk = s_chunk;
         const int legs_now = min(A.chunk_legs, A.legs - chunk * A.chunk_legs);
         const int n = R.n, cps = R.cps;
-        const double L = R.L;
+        const double L = R.L, side = R.side;
         const double *P = A.pos64 + R.off;
         const double inv_step = 0x1p16 / L;
         for (int i = tid; i < n; i += NTH) {
@@ -631,17 +638,29 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         // with 3 cells per side the 27 nearby cells are the whole box: candidate idx is unit idx
         // or idx + 1, past the active unit
         const bool whole_box = cps == 0 || cps == 3;
+        // Replica state through the chunk, in registers (with one wavefront per replica the LDS
+        // copy R is only read at the start and written at the end; with several, wavefront 0
+        // publishes it to LDS after each commit and every wavefront reloads it)
+        int st_a = R.active, st_d = R.direction, st_err = R.err;
+        uint64_t st_leg = R.leg;
+        double st_cl = R.chain_left;
+        double st_ap[3] = {s_ap[0], s_ap[1], s_ap[2]};
+        double st_h = 0.0, st_cb = 0.0;
+        int st_M = 0, st_pc[3] = {0, 0, 0};
+        // the estimators, summed leg by leg in the oracle's order (wavefront 0)
+        double dest[6] = {R.est[0], R.est[1], R.est[2], R.est[3], R.est[4], R.est[5]};
+        unsigned long long dst[4] = {0ull, 0ull, 0ull, 0ull};  // statistics increments (wavefront 0)
         // The next leg's 27 nearby cells with an inclusive scan of their counts (not needed when
-        // they are the whole box), its horizon, and cleared contender counters: wavefront 0, before the first leg and right after
-        // each commit (so that no separate barrier precedes pass 1)
-        auto prepare_leg = [&]() {
-            const double pa[3] = {s_ap[0], s_ap[1], s_ap[2]};
-            const int dn_ = R.direction;
+        // they are the whole box), its cell, horizon and candidate count, and cleared contender
+        // counters: wavefront 0, before the first leg and right after each commit (so that no
+        // separate barrier precedes pass 1)
+        auto prepare_leg = [&](const double *pa, int dn_, double cl) {
             int pc[3] = {0, 0, 0};
             if (cps) {
 #pragma unroll
-                for (int q = 0; q < 3; q++) pc[q] = bcell1(pa[q], R.side, cps);
+                for (int q = 0; q < 3; q++) pc[q] = bcell1(pa[q], side, cps);
             }
+            int Mv = n - 1;
             if (cps && !whole_box) {
                 int cnt = 0;
                 if (lane < 27) {
@@ -662,34 +681,41 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 if (lane >= 16) incl += row0;
                 if (lane < 27) s_pref[lane + 1] = incl;
                 if (lane == 0) s_pref[0] = 0;
-                if (lane == 26) s_M = incl;
+                Mv = __builtin_amdgcn_readlane(incl, 26);
             }
+            double hh = cl, cb = INFINITY;
+            if (cps) {
+                cb = s_bounds[pc[dn_] + 1] - pa[dn_];
+                if (cb < hh) hh = cb;
+            }
+            st_h = hh;
+            st_cb = cb;
+            st_M = Mv;
+#pragma unroll
+            for (int q = 0; q < 3; q++) st_pc[q] = pc[q];
             if (lane == 0) {
-                double hh = R.chain_left, cb = INFINITY;
-                if (cps) {
-                    cb = s_bounds[pc[dn_] + 1] - pa[dn_];
-                    if (cb < hh) hh = cb;
-                }
-                if (!cps || whole_box) s_M = n - 1;
-                s_h = hh;
-                s_cb = cb;
                 s_nc = 0;
                 s_overflow = 0;
+                if (WPR > 1) {
+                    s_h = hh;
+                    s_cb = cb;
+                    s_M = Mv;
+                }
             }
         };
-        if (wave == 0) prepare_leg();
+        if (wave == 0) prepare_leg(st_ap, st_d, st_cl);
         bsync();
-        for (int leg_i = 0; leg_i < legs_now && !R.err; leg_i++) {
-            const int a = R.active, d = R.direction;
-            const uint64_t leg = R.leg;
-            const double ap[3] = {s_ap[0], s_ap[1], s_ap[2]};
-            int ac[3] = {0, 0, 0};
-            if (cps) {
-#pragma unroll
-                for (int q = 0; q < 3; q++) ac[q] = bcell1(ap[q], R.side, cps);
-            }
-            const int M = s_M;
-            const double h = s_h;
+        if (WPR > 1) {
+            st_h = s_h;
+            st_cb = s_cb;
+            st_M = s_M;
+        }
+        for (int leg_i = 0; leg_i < legs_now && !st_err; leg_i++) {
+            const int a = st_a, d = st_d;
+            const uint64_t leg = st_leg;
+            const double ap[3] = {st_ap[0], st_ap[1], st_ap[2]};
+            const int M = st_M;
+            const double h = st_h;
             const uint32_t qa0 = quant1(ap[0], inv_step), qa1 = quant1(ap[1], inv_step), qa2 = quant1(ap[2], inv_step);
             const float hf = (float)h * (1.f + 0x1p-22f) + 0x1p-60f;
             BKey best = {INFINITY, 3, 0x7ffffff, 0};   // j < 2^27: the reduction packs (rank, j, f)
@@ -843,26 +869,27 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             }
             BKey mine = best;
             if (tid == 0) {
-                BKey c1k = {s_cb, 1, 0, 0};
+                BKey c1k = {st_cb, 1, 0, 0};
                 if (bkey_lt(c1k, best)) best = c1k;
-                BKey c2k = {R.chain_left, 2, 0, 0};
+                BKey c2k = {st_cl, 2, 0, 0};
                 if (bkey_lt(c2k, best)) best = c2k;
             }
             const int wbest = block_min<WPR>(best, mine, bestpos, sk, s_newpos, nwork);
-            if (tid == 0) {
-                s_stats[0] += (unsigned long long)M;
-                s_stats[2] += (unsigned long long)min(s_nc, B_CONTENDERS);
-                s_stats[3] += (unsigned long long)nwork;
+            if (wave == 0) {
+                dst[0] += (unsigned long long)M;
+                dst[2] += (unsigned long long)min(s_nc, B_CONTENDERS);
+                dst[3] += (unsigned long long)nwork;
             }
-            if (lane == 0 && geo_pass) atomicAdd(&s_stats[1], (unsigned long long)geo_pass);
-            // ---- commit (wavefront 0; the winner's position is in s_newpos): the values below
-            // are uniform across the wavefront, a ballot finds the new active unit's slot in its
-            // cell list, lane 0 writes
+            if (WPR == 1) dst[1] += geo_pass;
+            else if (lane == 0 && geo_pass) atomicAdd(&s_stats[1], (unsigned long long)geo_pass);
+            // ---- commit (wavefront 0; the winner's position is in the winning lane's registers
+            // or, with several wavefronts, in s_newpos): the values below are uniform across the
+            // wavefront, a ballot finds the new active unit's slot in its cell list, lane 0 writes
             if (wave == 0) {
                 double *Pw = A.pos64 + R.off;
                 double x;
                 if (best.rank == 1) {
-                    int c = ac[d];
+                    int c = st_pc[d];
                     x = c + 1 < cps ? s_bounds[c + 1] : 0.0;
                 } else {
                     x = ap[d] + best.d;
@@ -874,7 +901,14 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 const double pold[3] = {np[0], np[1], np[2]};
                 if (best.rank == 0) {
                     newa = best.j;
-                    np[0] = s_newpos[3 * wbest]; np[1] = s_newpos[3 * wbest + 1]; np[2] = s_newpos[3 * wbest + 2];
+                    if (WPR == 1) {
+#pragma unroll
+                        for (int k = 0; k < 3; k++)
+                            np[k] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(bestpos[k]), wbest),
+                                                     __builtin_amdgcn_readlane(__double2loint(bestpos[k]), wbest));
+                    } else {
+                        np[0] = s_newpos[3 * wbest]; np[1] = s_newpos[3 * wbest + 1]; np[2] = s_newpos[3 * wbest + 2];
+                    }
                 } else if (best.rank == 2) {
                     uint32_t w[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), 0xFFFFFFFFu, 1u, (uint32_t)r, A.seed_lo, w);
@@ -885,8 +919,8 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 }
                 int ca = 0, cn = 0, q = -1;
                 if (newa != a && cps) {
-                    ca = bcell1(pold[0], R.side, cps) + cps * (bcell1(pold[1], R.side, cps) + cps * bcell1(pold[2], R.side, cps));
-                    cn = bcell1(np[0], R.side, cps) + cps * (bcell1(np[1], R.side, cps) + cps * bcell1(np[2], R.side, cps));
+                    ca = bcell1(pold[0], side, cps) + cps * (bcell1(pold[1], side, cps) + cps * bcell1(pold[2], side, cps));
+                    cn = bcell1(np[0], side, cps) + cps * (bcell1(np[1], side, cps) + cps * bcell1(np[2], side, cps));
                     // the new active unit's slot in its cell's segment
                     const int b1 = cs[cn + 1];
                     for (int base = cs[cn]; base < b1 && q < 0; base += 64) {
@@ -894,46 +928,52 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                         if (mask) q = base + __ffsll((long long)mask) - 1;
                     }
                 }
+                // the replica's state, alike in every lane of wavefront 0
+                dest[1] += best.d;
+                if (best.rank == 0) {
+                    dest[0] += bmin_image(np[d] - ap[d], L) - best.d;
+                    dest[2] += 1.0;
+                    dest[4] += 1.0;
+                    st_cl -= best.d;
+                } else if (best.rank == 1) {
+                    dest[3] += 1.0;
+                    st_cl -= best.d;
+                } else {
+                    dest[5] += 1.0;
+                    st_cl = A.chain_length;
+                    st_d = (d + 1) % 3;
+                }
+                if (newa != a && cps && q < 0) st_err = 2;
+                st_a = newa;
+                st_leg = leg + 1;
+#pragma unroll
+                for (int k = 0; k < 3; k++) st_ap[k] = np[k];
                 if (lane == 0) {
-                    ecmc_batch_event ev;
-                    ev.kind = best.rank;
-                    ev.active = a;
-                    ev.target = -1;
-                    ev.factor = 0;
-                    ev.displacement = best.d;
                     Pw[(size_t)d * n + a] = x;
                     qx[a] = quant1(pold[0], inv_step);
                     qy[a] = quant1(pold[1], inv_step);
                     qz[a] = quant1(pold[2], inv_step);
-                    R.est[1] += best.d;
-                    if (best.rank == 0) {
-                        double sdj = bmin_image(np[d] - ap[d], L) - best.d;
-                        R.est[0] += sdj;
-                        R.est[2] += 1.0;
-                        R.est[4] += 1.0;
-                        R.chain_left -= best.d;
-                        ev.target = best.j;
-                        ev.factor = best.f;
-                    } else if (best.rank == 1) {
-                        R.est[3] += 1.0;
-                        R.chain_left -= best.d;
-                    } else {
-                        R.est[5] += 1.0;
-                        R.chain_left = A.chain_length;
-                        R.direction = (d + 1) % 3;
-                        ev.target = newa;
+                    // the old active unit takes the new one's slot (and is moved to the cell it
+                    // ended in below)
+                    if (newa != a && cps && q >= 0) cu[q] = (int16_t)a;
+                    if (A.log && leg < (uint64_t)A.log_legs) {
+                        ecmc_batch_event ev;
+                        ev.kind = best.rank;
+                        ev.active = a;
+                        ev.target = best.rank == 0 ? best.j : (best.rank == 2 ? newa : -1);
+                        ev.factor = best.rank == 0 ? best.f : 0;
+                        ev.displacement = best.d;
+                        ev.x = x;
+                        A.log[(size_t)r * A.log_legs + leg] = ev;
                     }
-                    ev.x = x;
-                    if (newa != a && cps) {
-                        // the old active unit takes the new one's slot (and is moved to the cell
-                        // it ended in below)
-                        if (q < 0) R.err = 2;
-                        else cu[q] = (int16_t)a;
+                    if (WPR > 1) {
+                        R.active = st_a;
+                        R.direction = st_d;
+                        R.leg = st_leg;
+                        R.chain_left = st_cl;
+                        R.err = st_err;
+                        s_ap[0] = np[0]; s_ap[1] = np[1]; s_ap[2] = np[2];
                     }
-                    s_ap[0] = np[0]; s_ap[1] = np[1]; s_ap[2] = np[2];
-                    R.active = newa;
-                    if (A.log && leg < (uint64_t)A.log_legs) A.log[(size_t)r * A.log_legs + leg] = ev;
-                    R.leg = leg + 1;
                 }
                 // lane 0's LDS writes above precede the other lanes' reads below (one wavefront,
                 // LDS operations in order; the fence keeps the compiler from reordering them)
@@ -944,12 +984,32 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                 }
-                prepare_leg();
+                prepare_leg(st_ap, st_d, st_cl);
             }
             bsync();
+            if (WPR > 1) {
+                st_a = R.active;
+                st_d = R.direction;
+                st_leg = R.leg;
+                st_cl = R.chain_left;
+                st_err = R.err;
+                st_ap[0] = s_ap[0]; st_ap[1] = s_ap[1]; st_ap[2] = s_ap[2];
+                st_h = s_h;
+                st_cb = s_cb;
+                st_M = s_M;
+            }
         }
         if (tid == 0) {
-            for (int q = 0; q < 4; q++) R.stats[q] += (int64_t)s_stats[q];
+            R.active = st_a;
+            R.direction = st_d;
+            R.leg = st_leg;
+            R.chain_left = st_cl;
+            R.err = st_err;
+            for (int q = 0; q < 6; q++) R.est[q] = dest[q];
+            R.stats[0] += (int64_t)dst[0];
+            R.stats[1] += (int64_t)(dst[1] + s_stats[1]);
+            R.stats[2] += (int64_t)dst[2];
+            R.stats[3] += (int64_t)dst[3];
             A.rep[r] = R;
         }
         bsync();   // every store of this chunk (positions, log, state) has completed
