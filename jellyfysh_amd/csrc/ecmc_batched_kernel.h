@@ -88,8 +88,11 @@ __device__ __forceinline__ int bcell1(double x, double side, int cps) {
 // margin (ETA for products/quotients/roots, ETA_POW for the exp2/log2 power), far above
 // FP32 rounding (2^-24 per operation) and the documented accuracy of log2f/exp2f.
 constexpr float ETA = 0x1p-17f, ETA_POW = 0x1p-14f;
-__device__ __forceinline__ float up(float x) { return x >= 0.f ? x * (1.f + ETA) : x * (1.f - ETA); }
-__device__ __forceinline__ float dn(float x) { return x >= 0.f ? x * (1.f - ETA) : x * (1.f + ETA); }
+// x moved outward by ETA |x| with one fused operation: the single rounding leaves a margin of
+// at least ETA - 2^-23 relative.  No argument is -inf (every caller passes a finite value or
+// +inf, which stays +inf).
+__device__ __forceinline__ float up(float x) { return fmaf(fabsf(x), ETA, x); }
+__device__ __forceinline__ float dn(float x) { return fmaf(-fabsf(x), ETA, x); }
 __device__ __forceinline__ float sq_lo(float a_abs_lo) { return dn(a_abs_lo * a_abs_lo); }
 // Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32: within 1 ulp = 2^-23 relative,
 // far inside the ETA margins); below 2^-100 the root is bracketed by [0, 2^-50] directly.
@@ -99,12 +102,14 @@ __device__ __forceinline__ float sqrt_dn(float x) { return x > 0x1p-100f ? dn(__
 
 // V(r2) = K / r2^(p/2) with p in {6, 12} (integer powers by multiplication), bounded up
 // (hi) or down for an r2 that is itself a lower/upper bound.  Three products (2^-24 each) and
-// K * rcp(rp) (3 x 2^-24) stay inside the 8 x 2^-24 allowance.
+// K * rcp(rp) (3 x 2^-24) stay inside an 8 x 2^-24 allowance, to which ETA is added; the two
+// factors are one constant, (1 +- 2^-21)(1 +- 2^-17) without its 2^-38 term (the side that
+// widens the bound), applied with one more rounding.
 __device__ __forceinline__ float vpow_bound(float K, float r2, int half_p, bool hi) {
     float r3 = r2 * r2 * r2;
     float rp = half_p == 6 ? r3 * r3 : r3;
     float v = K * rcp1(rp);
-    return hi ? v * (1.f + 8.f * 0x1p-24f) * (1.f + ETA) : v * (1.f - 8.f * 0x1p-24f) * (1.f - ETA);
+    return v * (hi ? 1.f + 0x1p-17f + 0x1p-21f : 1.f - 0x1p-17f - 0x1p-21f);
 }
 
 // (K / V)^(2/p) bounded up (hi) or down.
