@@ -353,12 +353,15 @@ __device__ __forceinline__ float qsep(uint32_t qj, uint32_t qa, float step) {
 // FP32 geometry of a candidate from its minimum-image separation: bounds with the position
 // slack E, and whether the minimum image is ambiguous (then the candidate is forced into the
 // contender list).
-__device__ __forceinline__ bool cand_geometry(float dx, float dy, float dz, int d, float Lf, float E, Geo &g) {
+__device__ __forceinline__ bool cand_geometry(float dx, float dy, float dz, int d, float Lf, float E, bool check_amb,
+                                              Geo &g) {
     float sv[3] = {dx, dy, dz};
     bool ambiguous = false;
+    if (check_amb) {
 #pragma unroll
-    for (int q = 0; q < 3; q++)
-        if (fabsf(fabsf(sv[q]) - 0.5f * Lf) <= 2.f * E) ambiguous = true;
+        for (int q = 0; q < 3; q++)
+            if (fabsf(fabsf(sv[q]) - 0.5f * Lf) <= 2.f * E) ambiguous = true;
+    }
     const float sd = d == 0 ? sv[0] : (d == 1 ? sv[1] : sv[2]);
     g.sd_lo = sd - E;
     g.sd_hi = sd + E;
@@ -387,10 +390,13 @@ __device__ __forceinline__ bool cand_geometry(float dx, float dy, float dz, int 
 // approach (at s_d <= h).  The fixed-point separation is within E of the true one on each
 // axis, so within sqrt(3) E < 2E of it; hf >= h; RB2 = ((rc + 2E)(1 + 2^-20))^2 (1 + 2^-20)
 // covers these and the FP32 roundings.  A separation within 2E of +-L/2 on any axis (minimum
-// image not certain) always passes.
+// image not certain) always passes; check_amb false (uniform) skips that test where no such
+// separation can lie in the capsule.
 __device__ __forceinline__ bool cand_near(float dx, float dy, float dz, int d, float halfL, float E2, float RB2,
-                                          float hf) {
-    const bool amb = fabsf(fabsf(dx) - halfL) <= E2 || fabsf(fabsf(dy) - halfL) <= E2 || fabsf(fabsf(dz) - halfL) <= E2;
+                                          float hf, bool check_amb) {
+    bool amb = false;
+    if (check_amb)
+        amb = fabsf(fabsf(dx) - halfL) <= E2 || fabsf(fabsf(dy) - halfL) <= E2 || fabsf(fabsf(dz) - halfL) <= E2;
     const float sd = d == 0 ? dx : (d == 1 ? dy : dz);
     const float u = d == 0 ? dy : dx, v = d == 2 ? dy : dz;
     const float e = sd - fminf(fmaxf(sd, 0.f), hf);
@@ -640,6 +646,12 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const float RB = ((float)R.cutoff * (1.f + 0x1p-22f) + 2.f * E) * (1.f + 0x1p-20f);
         const float RB2 = RB * RB * (1.f + 0x1p-20f);
         const float halfL = 0.5f * Lf, E2 = 2.f * E;
+        // A separation within 2E of +-L/2 on some axis has |s_q| >= L/2 - 2E.  When that exceeds
+        // the capsule's reach rc + 2E ahead of the largest horizon (the chain length, rounded
+        // up as hf is), such a candidate fails the capsule test on either image, so neither
+        // pass needs the ambiguity test.
+        const float hf_max = (float)A.chain_length * (1.f + 0x1p-22f) + 0x1p-60f;
+        const bool check_amb = !(halfL - E2 > (hf_max + RB) * (1.f + 0x1p-20f));
         // with 3 cells per side the 27 nearby cells are the whole box: candidate idx is unit idx
         // or idx + 1, past the active unit
         const bool whole_box = cps == 0 || cps == 3;
@@ -738,7 +750,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                         j = queue[lane];
                         Geo g;
                         const bool amb = cand_geometry(qsep(qx[j], qa0, step), qsep(qy[j], qa1, step), qsep(qz[j], qa2, step),
-                                                       d, Lf, E, g);
+                                                       d, Lf, E, check_amb, g);
                         philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, w);
 #pragma unroll
                         for (int f = 0; f < 2; f++) {
@@ -806,10 +818,10 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     bool s0 = false, s1 = false;
                     if (i0 < M)
                         s0 = cand_near(qsep(qx[j0], qa0, step), qsep(qy[j0], qa1, step), qsep(qz[j0], qa2, step), d, halfL,
-                                       E2, RB2, hf);
+                                       E2, RB2, hf, check_amb);
                     if (i1 < M)
                         s1 = cand_near(qsep(qx[j1], qa0, step), qsep(qy[j1], qa1, step), qsep(qz[j1], qa2, step), d, halfL,
-                                       E2, RB2, hf);
+                                       E2, RB2, hf, check_amb);
                     append(s0, j0);
                     append(s1, j1);
                 }
