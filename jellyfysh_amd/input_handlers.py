@@ -6,8 +6,13 @@ input_output_handler/input_handler package builds it.
   atom_random_node_creator.py:26-60, hypercubic_setting.py:226-235: every root node is a point
   mass at ``[random.uniform(0.0, L) for _ in range(dimension)]``, drawn from the run's random
   stream (CPython's ``random``, the reference's generator).  ``read`` also reports how many
-  words of the stream the draws took (two per coordinate), so that the device chain starts its
-  own draws after them (``rng_skip``).
+  words of the stream the draws took (two per ``uniform``), so that the device chain starts
+  its own draws after them (``rng_skip``).
+- ``DipoleRandomNodeCreator`` -- dipole_random_node_creator.py:69-99 with
+  vectors.py:242-262: a random centre, a direction by rejection (components uniform in
+  [-1, 1] until 0 < norm <= 1, then divided by the norm), a half-separation uniform in
+  [min / 2, max / 2], the two atoms at centre +- direction x half-separation, each coordinate
+  taken % L; the first atom gets each charge's first value, the second its second.
 - ``PdbInputHandler`` -- pdb_input_handler.py:33-191.  The reference reads the file through
   MDAnalysis (not installed here); this is a restatement of what that reader yields for the
   fixed-column PDB records the reference writes and tests: the box from CRYST1 (columns 7-33),
@@ -49,21 +54,64 @@ class AtomRandomNodeCreator:
         return position, {c.charge_name: c[0] for c in self.charge_values}
 
 
+class DipoleRandomNodeCreator:
+    number_of_nodes_per_root_node = 2
+    number_of_node_levels = 2
+
+    def __init__(self, min_initial_dipole_separation=0.0, max_initial_dipole_separation=0.05, charge_values=()):
+        self.min_initial_dipole_separation = float(min_initial_dipole_separation)
+        self.max_initial_dipole_separation = float(max_initial_dipole_separation)
+        self.charge_values = list(charge_values)
+
+    @staticmethod
+    def random_vector_on_unit_sphere(rng, dimension):
+        while True:
+            vector = [rng.uniform(-1, 1) for _ in range(dimension)]
+            vector_norm = sum(c * c for c in vector) ** 0.5
+            if 0.0 < vector_norm <= 1.0:
+                break
+        return [v / vector_norm * 1.0 for v in vector]
+
+    def fill_root_node(self, rng, dimension, system_length):
+        """([atom one, atom two] positions, {name: [charge one, charge two]})."""
+        center = [rng.uniform(0.0, system_length) for _ in range(dimension)]
+        direction = self.random_vector_on_unit_sphere(rng, dimension)
+        half = rng.uniform(self.min_initial_dipole_separation / 2.0, self.max_initial_dipole_separation / 2.0)
+        one = [_py_mod(center[d] + direction[d] * half, system_length) for d in range(dimension)]
+        two = [_py_mod(center[d] - direction[d] * half, system_length) for d in range(dimension)]
+        return [one, two], {c.charge_name: [c[0], c[1]] for c in self.charge_values}
+
+
+class _CountingRandom:
+    """The run's random.Random with a count of the 32-bit words its uniform draws take
+    (random() combines two)."""
+
+    def __init__(self, rng):
+        self._rng = rng
+        self.words = 0
+
+    def uniform(self, a, b):
+        self.words += 2
+        return self._rng.uniform(a, b)
+
+
 class RandomInputHandler:
     def __init__(self, number_of_root_nodes, random_node_creator):
         self.number_of_root_nodes = int(number_of_root_nodes)
         self.random_node_creator = random_node_creator
 
     def read(self, rng, dimension, system_length):
-        """Positions (n, dim), charges {name: (n,)} and the number of random words drawn."""
+        """Positions (n, dim) -- (n, children, dim) for composite objects --, charges {name: (n,)
+        or (n, children)} and the number of random words drawn."""
+        counting = _CountingRandom(rng)
         positions, charges = [], {}
         for _ in range(self.number_of_root_nodes):
-            p, c = self.random_node_creator.fill_root_node(rng, dimension, system_length)
+            p, c = self.random_node_creator.fill_root_node(counting, dimension, system_length)
             positions.append(p)
             for k, v in c.items():
                 charges.setdefault(k, []).append(v)
-        words = 2 * self.number_of_root_nodes * dimension     # random.uniform -> random(): 2 words
-        return np.array(positions, dtype=np.float64), {k: np.array(v) for k, v in charges.items()}, words
+        return (np.array(positions, dtype=np.float64), {k: np.array(v) for k, v in charges.items()},
+                counting.words)
 
 
 def _py_mod(x, y):

@@ -137,3 +137,29 @@ def test_gpu_chain_continues_the_stream_after_random_placement(device, oracle):
         assert np.array_equal(got[f], want[f]), f
     t_got, t_want = got["tq"] + got["tr"], want["tq"] + want["tr"]
     assert np.max(np.abs(t_got - t_want) / np.maximum(np.abs(t_want), 1.0)) < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_dipole_creator_invariants_and_stream_position(dim):
+    """DipoleRandomNodeCreator (dipole_random_node_creator.py:69-99): the atoms' minimum-image
+    separation lies in [min, max]; the charges follow the atom order; the run's random stream
+    continues right after the reported words, which counts the rejected direction draws."""
+    from jellyfysh_amd.input_handlers import ChargeValues, DipoleRandomNodeCreator, RandomInputHandler
+    L = 3.0
+    h = RandomInputHandler(300, DipoleRandomNodeCreator(0.1, 0.5, [ChargeValues([0.5, -0.5], "electric_charge")]))
+    rng = random.Random(21)
+    pos, charges, words = h.read(rng, dim, L)
+    assert pos.shape == (300, 2, dim) and pos.min() >= 0.0 and pos.max() < L
+    s = pos[:, 0] - pos[:, 1]
+    s -= L * np.round(s / L)
+    sep = np.sqrt((s * s).sum(1))
+    assert sep.min() >= 0.1 * (1 - 1e-12) and sep.max() <= 0.5 * (1 + 1e-12)
+    assert charges["electric_charge"].shape == (300, 2)
+    assert np.all(charges["electric_charge"] == np.array([0.5, -0.5]))
+    ref = random.Random(21)
+    for _ in range(words):
+        ref.getrandbits(32)
+    assert ref.random() == rng.random()
+    # centre, direction and separation take (2 dim + 1) uniforms at least; the rejection loop
+    # (acceptance pi/4 in 2D, pi/6 in 3D) adds more
+    assert words > 300 * 2 * (2 * dim + 1)
