@@ -126,3 +126,20 @@ def test_gpu_soft_sphere_replicas_at_size(device):
         assert alone.run(legs, log=False) == legs
         assert np.array_equal(alone.positions(), chains[i].positions())
         assert np.array_equal(alone.estimators(), chains[i].estimators())
+
+
+def test_charged_soft_sphere_ini_is_the_workload_wiring():
+    """jellyfysh_amd/configs/charged_soft_spheres.ini, read by the INI front end, describes the
+    same run (byte for byte) as the tagger list of jellyfysh_amd.workloads."""
+    import os
+    from jellyfysh_amd.chain import make_desc
+    from jellyfysh_amd.config import RunConfig
+    from jellyfysh_amd.workloads import charged_soft_spheres
+    ini = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jellyfysh_amd", "configs",
+                       "charged_soft_spheres.ini")
+    cfg = RunConfig.from_file(ini)
+    sysd = charged_soft_spheres(seed=3)
+    a = cfg.desc(512, seed=3)
+    b = make_desc(3, 512, sysd["system_length"], sysd["beta"], sysd["cells_per_side"], sysd["taggers"], 3,
+                  scheduler="heap", max_occupants=sysd["max_occupants"])
+    assert bytes(a) == bytes(b)
