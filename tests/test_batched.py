@@ -117,9 +117,10 @@ def test_gpu_batched_matches_oracle(device, oracle, mode):
 
 
 @pytest.mark.gpu
-def test_gpu_batched_bounds_never_exceed_exact(device):
-    b, ns, lengths, pos = _device_batch(([128, 512, 4096], [1, 2, 4]), 2, 0)
-    print("batched audit", flush=True)
+@pytest.mark.parametrize("classes", [([128, 512, 4096], [1, 2, 4]), ([16384], [1])])
+def test_gpu_batched_bounds_never_exceed_exact(device, classes):
+    b, ns, lengths, pos = _device_batch(classes, 2, 0)
+    print("batched audit", classes, flush=True)
     b.run(800)
     assert b.audit() == 0
     st = b.states()["stats"]
@@ -184,3 +185,33 @@ def test_gpu_batched_chunks_migrate_between_blocks(device):
     assert np.array_equal(s1["est"], s2["est"])
     for p1, p2 in zip(b1.positions(), b2.positions()):
         assert np.array_equal(p1, p2)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("chain_length", [1.0, 4.0])
+def test_gpu_batched_one_wavefront_per_replica_matches_oracle(device, oracle, chain_length):
+    """The benchmark's layout (one size class, every replica on one wavefront) against the
+    oracle, for boxes without cells (N 64, 110, 300), with three cells per side, where the
+    nearby cells are the whole box (N 700), and with five (N 1500).  Chains of 4.0 reach half
+    the box, which turns the minimum-image ambiguity tests on in the boxes with cells."""
+    from jellyfysh_amd.batch import DeviceBatch
+    ns, lengths, pos = small_batch()
+    legs = 1200
+    print("one wavefront per replica, chain length", chain_length, flush=True)
+    b = DeviceBatch(ns, lengths, pos, seed=9, classes=([16384], [1]), chain_length=chain_length)
+    b.set_log(legs)
+    b.run(legs // 3)
+    b.run(legs - legs // 3)
+    got = b.log()
+    st = b.states()
+    allpos = b.positions()
+    for r in range(len(ns)):
+        rep = oracle.BatchedReplica(pos[r], lengths[r], replica=r, seed=9, chain_length=chain_length)
+        want = rep.run(legs)
+        for f in ("kind", "active", "target", "factor", "displacement", "x"):
+            bad = np.flatnonzero(got[r][f] != want[f])
+            assert bad.size == 0, (r, f, int(bad[0]), got[r][bad[0]], want[bad[0]])
+        assert np.array_equal(st["est"][r][:6], np.array(rep.state.est)[:6])
+        assert st["active"][r] == rep.state.active
+        assert np.array_equal(allpos[r], rep.pos)
+    b.close()
