@@ -522,7 +522,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
     int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
-    __shared__ int s_item, s_chunk, s_nc, s_overflow, s_nearby[27], s_pref[28], s_M;
+    __shared__ int s_item, s_chunk, s_nc, s_overflow, s_nearby[27], s_pref[28], s_M, s_Mfull;
     __shared__ double s_ap[3], s_newpos[3 * B_MAX_WPR], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
     __shared__ ReplicaDev R;
     __shared__ unsigned long long s_stats[4];
@@ -664,6 +664,15 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         double st_ap[3] = {s_ap[0], s_ap[1], s_ap[2]};
         double st_h = 0.0, st_cb = 0.0;
         int st_M = 0, st_pc[3] = {0, 0, 0};
+        // Nearby cells that cannot meet the capsule of the next leg are left out of its candidate
+        // list (their count is taken as zero): the box distance from the segment [0, h] (h
+        // widened to cover hf) must be at least rc + 2E rounded up, plus 2E for the position
+        // slack of the fixed-point test and L 2^-18 for the FP32 box bounds (each rounded once
+        // from an FP64 difference below L, then a few FP32 operations).  Not in the audit mode
+        // (which checks every candidate) nor where the ambiguity tests are on.
+        // (the radius is recomputed in each call from the cutoff in LDS, so that it does not
+        // hold a register through the leg)
+        const bool cull = A.mode != 2 && !check_amb;
         // the estimators, summed leg by leg in the oracle's order (wavefront 0)
         double dest[6] = {R.est[0], R.est[1], R.est[2], R.est[3], R.est[4], R.est[5]};
         unsigned long long dst[4] = {0ull, 0ull, 0ull, 0ull};  // statistics increments (wavefront 0)
@@ -677,33 +686,65 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
 #pragma unroll
                 for (int q = 0; q < 3; q++) pc[q] = bcell1(pa[q], side, cps);
             }
-            int Mv = n - 1;
-            if (cps && !whole_box) {
-                int cnt = 0;
-                if (lane < 27) {
-                    int dx = lane % 3 - 1, dy = (lane / 3) % 3 - 1, dz = lane / 9 - 1;
-                    int cx = (pc[0] + dx + cps) % cps, cy = (pc[1] + dy + cps) % cps, cz = (pc[2] + dz + cps) % cps;
-                    int c = cx + cps * (cy + cps * cz);
-                    s_nearby[lane] = cs[c];
-                    cnt = cs[c + 1] - cs[c];
-                }
-                // inclusive scan of the 27 counts: within each 16-lane row by DPP row shifts
-                // (a lane whose source is outside the row adds 0), then lanes 16-26 add row 0's sum
-                int incl = cnt;
-                incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xF, 0xF, false);
-                incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xF, 0xF, false);
-                incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xF, 0xF, false);
-                incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xF, 0xF, false);
-                const int row0 = __builtin_amdgcn_readlane(incl, 15);
-                if (lane >= 16) incl += row0;
-                if (lane < 27) s_pref[lane + 1] = incl;
-                if (lane == 0) s_pref[0] = 0;
-                Mv = __builtin_amdgcn_readlane(incl, 26);
-            }
             double hh = cl, cb = INFINITY;
             if (cps) {
                 cb = s_bounds[pc[dn_] + 1] - pa[dn_];
                 if (cb < hh) hh = cb;
+            }
+            int Mv = n - 1, Mf = n - 1;
+            if (cps && !whole_box) {
+                int cnt = 0, cntf = 0;
+                if (lane < 27) {
+                    const int off[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
+                    int cc[3];
+#pragma unroll
+                    for (int q = 0; q < 3; q++) cc[q] = (pc[q] + off[q] + cps) % cps;
+                    const int c = cc[0] + cps * (cc[1] + cps * cc[2]);
+                    s_nearby[lane] = cs[c];
+                    cntf = cs[c + 1] - cs[c];
+                    bool live = true;
+                    if (cull) {
+                        const float rb = ((float)R.cutoff * (1.f + 0x1p-22f) + 2.f * E) * (1.f + 0x1p-20f);
+                        const float cull_r = (rb * (1.f + 0x1p-20f) + 2.f * E + Lf * 0x1p-18f) * (1.f + 0x1p-20f);
+                        const float cull_r2 = cull_r * cull_r * (1.f + 0x1p-20f);
+                        // the cell's box relative to the active unit (shifted by L where the
+                        // neighbour wraps), against the point 0 across the motion and the
+                        // segment [0, h] along it
+                        const float top = (float)hh * (1.f + 0x1p-20f) + 0x1p-50f;
+                        float g2 = 0.f;
+#pragma unroll
+                        for (int q = 0; q < 3; q++) {
+                            const double shift = (off[q] < 0 && pc[q] == 0) ? -L : ((off[q] > 0 && pc[q] == cps - 1) ? L : 0.0);
+                            const float lo = (float)(s_bounds[cc[q]] - pa[q] + shift);
+                            const float hi = (float)(s_bounds[cc[q] + 1] - pa[q] + shift);
+                            const float gap = fmaxf(fmaxf(lo - (q == dn_ ? top : 0.f), -hi), 0.f);
+                            g2 += gap * gap;
+                        }
+                        live = g2 < cull_r2;
+                    }
+                    cnt = live ? cntf : 0;
+                }
+                // inclusive scans of the 27 counts (kept and all): within each 16-lane row by DPP
+                // row shifts (a lane whose source is outside the row adds 0), then lanes 16-26
+                // add row 0's sum
+                int incl = cnt, inclf = cntf;
+                incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xF, 0xF, false);
+                inclf += __builtin_amdgcn_update_dpp(0, inclf, 0x111, 0xF, 0xF, false);
+                incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xF, 0xF, false);
+                inclf += __builtin_amdgcn_update_dpp(0, inclf, 0x112, 0xF, 0xF, false);
+                incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xF, 0xF, false);
+                inclf += __builtin_amdgcn_update_dpp(0, inclf, 0x114, 0xF, 0xF, false);
+                incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xF, 0xF, false);
+                inclf += __builtin_amdgcn_update_dpp(0, inclf, 0x118, 0xF, 0xF, false);
+                const int row0 = __builtin_amdgcn_readlane(incl, 15), row0f = __builtin_amdgcn_readlane(inclf, 15);
+                if (lane >= 16) {
+                    incl += row0;
+                    inclf += row0f;
+                }
+                if (lane < 27) s_pref[lane + 1] = incl;
+                if (lane == 0) s_pref[0] = 0;
+                Mv = __builtin_amdgcn_readlane(incl, 26);
+                Mf = __builtin_amdgcn_readlane(inclf, 26);
             }
             st_h = hh;
             st_cb = cb;
@@ -713,6 +754,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             if (lane == 0) {
                 s_nc = 0;
                 s_overflow = 0;
+                s_Mfull = Mf;   // the statistics count every candidate of the 27 cells
                 if (WPR > 1) {
                     s_h = hh;
                     s_cb = cb;
@@ -893,7 +935,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             }
             const int wbest = block_min<WPR>(best, mine, bestpos, sk, s_newpos, nwork);
             if (wave == 0) {
-                dst[0] += (unsigned long long)M;
+                dst[0] += (unsigned long long)s_Mfull;   // every candidate of the 27 cells
                 dst[2] += (unsigned long long)min(s_nc, B_CONTENDERS);
                 dst[3] += (unsigned long long)nwork;
             }
