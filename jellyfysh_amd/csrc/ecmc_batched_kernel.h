@@ -517,15 +517,15 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     int16_t *cu = (int16_t *)(qz + A.nmax);    // units sorted by cell, the active unit left out
     int16_t *cs = cu + A.nmax;                 // cell c holds cu[cs[c] .. cs[c + 1])
     size_t off = (8ull * A.nmax + 2ull * (A.cellmax + 1) + 7) & ~7ull;
-    double *cont_pos = (double *)(smem + off);                       // B_CONTENDERS x 3
-    uint32_t *cont_w = (uint32_t *)(cont_pos + 3 * B_CONTENDERS);    // B_CONTENDERS x 2
+    int32_t *ccount = (int32_t *)(smem + off);                        // 24 B x B_CONTENDERS: chunk-start cell counts
+    uint32_t *cont_w = (uint32_t *)(smem + off + 24 * B_CONTENDERS);  // B_CONTENDERS x 2
     int32_t *cont_id = (int32_t *)(cont_w + 2 * B_CONTENDERS);       // (j << 1) | f
     int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
     __shared__ int s_item, s_chunk, s_nc, s_overflow, s_nearby[27], s_pref[28], s_M, s_Mfull;
     __shared__ double s_ap[3], s_newpos[3 * B_MAX_WPR], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
     __shared__ ReplicaDev R;
-    __shared__ unsigned long long s_stats[4];
+    __shared__ unsigned long long s_geo;   // geometric survivors of the other wavefronts (several per replica)
 
     for (;;) {
         // Claim a work item: a class member that no block holds and that has chunks left.  Wave
@@ -568,7 +568,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const int r = A.order[member];
         if (tid == 0) {
             R = A.rep[r];
-            for (int q = 0; q < 4; q++) s_stats[q] = 0;
+            s_geo = 0;
         }
         bsync();
         const int chunk = s_chunk;
@@ -585,7 +585,6 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         const int ncell = cps ? cps * cps * cps : 0;
         // counting sort of the units by cell; the counts and cursors live in the contender
         // area, which is idle here (4 * 512 bytes fit in it)
-        int32_t *ccount = (int32_t *)cont_pos;
         for (int c = tid; c < ncell; c += NTH) ccount[c] = 0;
         if (tid < 3) s_ap[tid] = P[(size_t)tid * n + R.active];
         if (cps && tid <= cps) s_bounds[tid] = A.bounds[R.bound_off + tid];
@@ -940,7 +939,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 dst[3] += (unsigned long long)nwork;
             }
             if (WPR == 1) dst[1] += geo_pass;
-            else if (lane == 0 && geo_pass) atomicAdd(&s_stats[1], (unsigned long long)geo_pass);
+            else if (lane == 0 && geo_pass) atomicAdd(&s_geo, (unsigned long long)geo_pass);
             // ---- commit (wavefront 0; the winner's position is in the winning lane's registers
             // or, with several wavefronts, in s_newpos): the values below are uniform across the
             // wavefront, a ballot finds the new active unit's slot in its cell list, lane 0 writes
@@ -1066,7 +1065,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
             R.err = st_err;
             for (int q = 0; q < 6; q++) R.est[q] = dest[q];
             R.stats[0] += (int64_t)dst[0];
-            R.stats[1] += (int64_t)(dst[1] + s_stats[1]);
+            R.stats[1] += (int64_t)(dst[1] + s_geo);
             R.stats[2] += (int64_t)dst[2];
             R.stats[3] += (int64_t)dst[3];
             A.rep[r] = R;
