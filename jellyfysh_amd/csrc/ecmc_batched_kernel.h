@@ -329,9 +329,11 @@ __device__ __forceinline__ double exact_ipt(double s0, double s1, double s2, int
 }
 
 // The exact FP64 event displacement of one contender: the exponential draw dU = -log(1 - u) /
-// beta from its two words (correctly rounded log), then exact_ipt.
+// beta from its two words (correctly rounded log), then exact_ipt.  A call, not inlined: the
+// leg loop keeps its state in registers, and inlining this chain pushes the kernel past 256
+// VGPRs into spills (measured slower than the call).
 #ifndef ECMC_BATCHED_EXACT_ATTR
-#define ECMC_BATCHED_EXACT_ATTR __forceinline__
+#define ECMC_BATCHED_EXACT_ATTR __noinline__
 #endif
 __device__ ECMC_BATCHED_EXACT_ATTR double exact_event(uint32_t wa, uint32_t wb, double beta, double s0, double s1,
                                                       double s2, int d, bool six, double prefactor, double rc2,
