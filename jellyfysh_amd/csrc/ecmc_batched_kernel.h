@@ -471,21 +471,23 @@ __device__ __forceinline__ int candidate_unit(int idx, int cps, int a, const int
     return cu[start[lo] + (idx - pref[lo])];
 }
 
-// Two candidates at once (indices i0, i1 < M with cells): a fixed five-step search over the
-// 28 prefix sums (largest lo with pref[lo] <= idx, pref[27] = M), branch-free, so that the two
-// chains of LDS reads interleave.
-__device__ __forceinline__ void candidate_units2(int i0, int i1, const int *pref, const int *start, const int16_t *cu,
-                                                 int &j0, int &j1) {
-    int lo0 = 0, lo1 = 0;
+// Three candidates at once (indices i0, i1, i2 < M with cells): a fixed five-step search over
+// the 28 prefix sums (largest lo with pref[lo] <= idx, pref[27] = M), branch-free, so that the
+// three chains of LDS reads interleave.
+__device__ __forceinline__ void candidate_units3(int i0, int i1, int i2, const int *pref, const int *start,
+                                                 const int16_t *cu, int &j0, int &j1, int &j2) {
+    int lo0 = 0, lo1 = 0, lo2 = 0;
 #pragma unroll
     for (int st = 16; st > 0; st >>= 1) {
-        const int t0 = min(lo0 + st, 27), t1 = min(lo1 + st, 27);
-        const int p0 = pref[t0], p1 = pref[t1];
+        const int t0 = min(lo0 + st, 27), t1 = min(lo1 + st, 27), t2 = min(lo2 + st, 27);
+        const int p0 = pref[t0], p1 = pref[t1], p2 = pref[t2];
         lo0 = p0 <= i0 ? t0 : lo0;
         lo1 = p1 <= i1 ? t1 : lo1;
+        lo2 = p2 <= i2 ? t2 : lo2;
     }
     j0 = cu[start[lo0] + (i0 - pref[lo0])];
     j1 = cu[start[lo1] + (i1 - pref[lo1])];
+    j2 = cu[start[lo2] + (i2 - pref[lo2])];
 }
 
 // LDS bytes of the dynamic part for a class (fixed-point positions, cell lists, contenders and
@@ -831,9 +833,9 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                         cont_w[2 * slot + 1] = pick(f, w[1], w[3]);
                     }
                 };
-                // ---- pass 1a: every candidate against the capsule, survivors queued; two
-                // chunks of 64 per wavefront and iteration, so that the LDS reads of both
-                // (cell search, unit, position) overlap
+                // ---- pass 1a: every candidate against the capsule, survivors queued; three
+                // chunks of 64 per wavefront and iteration, so that their LDS reads (cell search,
+                // unit, position) overlap
                 int qn = 0;
                 auto append = [&](bool surv, int j) {
                     const unsigned long long m = __ballot(surv);
@@ -849,24 +851,29 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                     }
                     geo_pass += (unsigned)__popcll(m);
                 };
-                for (int base = wave * 64; base < M; base += 2 * NTH) {
-                    const int i0 = base + lane, i1 = base + NTH + lane;
-                    int j0, j1;
+                for (int base = wave * 64; base < M; base += 3 * NTH) {
+                    const int i0 = base + lane, i1 = base + NTH + lane, i2 = base + 2 * NTH + lane;
+                    int j0, j1, j2;
                     if (whole_box) {
                         j0 = i0 < a ? i0 : i0 + 1;
                         j1 = i1 < a ? i1 : i1 + 1;
+                        j2 = i2 < a ? i2 : i2 + 1;
                     } else {
-                        candidate_units2(min(i0, M - 1), min(i1, M - 1), s_pref, s_nearby, cu, j0, j1);
+                        candidate_units3(min(i0, M - 1), min(i1, M - 1), min(i2, M - 1), s_pref, s_nearby, cu, j0, j1, j2);
                     }
-                    bool s0 = false, s1 = false;
+                    bool s0 = false, s1 = false, s2 = false;
                     if (i0 < M)
                         s0 = cand_near(qsep(qx[j0], qa0, step), qsep(qy[j0], qa1, step), qsep(qz[j0], qa2, step), d, halfL,
                                        E2, RB2, hf, check_amb);
                     if (i1 < M)
                         s1 = cand_near(qsep(qx[j1], qa0, step), qsep(qy[j1], qa1, step), qsep(qz[j1], qa2, step), d, halfL,
                                        E2, RB2, hf, check_amb);
+                    if (i2 < M)
+                        s2 = cand_near(qsep(qx[j2], qa0, step), qsep(qy[j2], qa1, step), qsep(qz[j2], qa2, step), d, halfL,
+                                       E2, RB2, hf, check_amb);
                     append(s0, j0);
                     append(s1, j1);
+                    append(s2, j2);
                 }
                 if (qn > 0) bound_queue(qn);
                 if (WPR == 1 && lane == 0) s_nc = nc;
