@@ -215,3 +215,29 @@ def test_gpu_batched_one_wavefront_per_replica_matches_oracle(device, oracle, ch
         assert st["active"][r] == rep.state.active
         assert np.array_equal(allpos[r], rep.pos)
     b.close()
+
+
+@pytest.mark.gpu
+def test_gpu_batched_largest_replicas_match_oracle(device, oracle):
+    """The workload's largest replicas (N = 2048 at rho 0.7 and 0.9: 5^3 cells, the most LDS per
+    replica) in the benchmark's layout, bit-exact against the oracle."""
+    from jellyfysh_amd.batch import DeviceBatch, lattice_positions
+    rng = np.random.default_rng(12)
+    ns = [2048, 2048]
+    lengths = [(2048 / 0.7) ** (1 / 3), (2048 / 0.9) ** (1 / 3)]
+    pos = [lattice_positions(n, l, 0.01, rng) for n, l in zip(ns, lengths)]
+    legs = 400
+    print("two replicas of N = 2048, %d legs" % legs, flush=True)
+    b = DeviceBatch(ns, lengths, pos, seed=4, classes=([16384], [1]))
+    b.set_log(legs)
+    b.run(legs)
+    got = b.log()
+    allpos = b.positions()
+    for r in range(2):
+        rep = oracle.BatchedReplica(pos[r], lengths[r], replica=r, seed=4)
+        want = rep.run(legs)
+        for f in ("kind", "active", "target", "factor", "displacement", "x"):
+            bad = np.flatnonzero(got[r][f] != want[f])
+            assert bad.size == 0, (r, f, int(bad[0]))
+        assert np.array_equal(allpos[r], rep.pos)
+    b.close()
