@@ -216,12 +216,15 @@ def cpu_baseline(n, lengths, pos, seconds, threads):
     pool = np.flatnonzero((n >= 256) & (n <= 512))
     order = pool[np.argsort(np.abs(rho[pool] - 0.8), kind="stable")[: 4 * threads]]
     order = order[np.argsort(-n[order], kind="stable")]          # largest first
-    # calibrate legs so that the sample takes about `seconds`
+    # one replica on one thread: the single-core rate, and the legs per replica that make the
+    # sample take about `seconds`
     probe = O.BatchedReplica(pos[order[0]], float(lengths[order[0]]), replica=int(order[0]), seed=1,
                              math_mode=O.MATH_LIBM)
+    probe.run(200, log=False)
+    probe_legs = 3000
     t = time.perf_counter()
-    probe.run(400, log=False)
-    per_leg = (time.perf_counter() - t) / 400
+    probe.run(probe_legs, log=False)
+    per_leg = (time.perf_counter() - t) / probe_legs
     half = max(200, int(seconds * threads / (2 * len(order) * per_leg * 1.1)))
 
     def work(r):
@@ -239,7 +242,9 @@ def cpu_baseline(n, lengths, pos, seconds, threads):
     est = np.stack([e for _, e in res])
     z = 1.0 + est[:, 0] / est[:, 1]
     pz = batched_pressure_at(rho[order], z, width=1.0, n=n[order])
-    return dict(value=done / wall, unit="events/s", cores=threads, kind="port",
+    single = dict(value=1.0 / per_leg, unit="events/s", cores=1,
+                  sample="one replica (N %d) x %d legs on one host thread, after 200" % (int(n[order[0]]), probe_legs))
+    return dict(value=done / wall, unit="events/s", cores=threads, kind="port", single_core=single,
                 sample="%d replicas of the batched workload (256 <= N <= 512, density closest to 0.8, N %d-%d) "
                        "x %d legs each, oracle restatement in FP64 with glibc math, %d host threads, %.1f s"
                        % (len(order), int(n[order].min()), int(n[order].max()), 2 * half, threads, wall),
