@@ -173,15 +173,17 @@ def batched_pressure_at(rho, z, rho0=0.8, width=0.02, n=None, n_min=256, n_max=5
                 window="|rho - %.2f| < %.2f, %d <= N <= %d" % (rho0, width, n_min, n_max))
 
 
-def charged_spheres_rate(replicas, legs):
+def charged_spheres_rate(replicas, legs, far="cell_bounding"):
     """Charged soft spheres (N = 512, L = 8, merged-image Coulomb + inverse power 9, beta = 1):
     `replicas` independent systems advanced together by the plain chain kernel (one workgroup
-    each, one launch), against one replica alone."""
+    each, one launch), against one replica alone.  `far`: the far cells through the
+    cell-bounding potential (one candidate per occupied far cell) or the cell veto (one
+    sampled cell per leg)."""
     import torch
     from jellyfysh_amd.chain import run_together
     from jellyfysh_amd.workloads import charged_soft_sphere_chains
     t0 = time.perf_counter()
-    chains = charged_soft_sphere_chains(replicas, seed=1000)
+    chains = charged_soft_sphere_chains(replicas, seed=1000, far=far)
     create_s = time.perf_counter() - t0
     run_together(chains, 200)
     torch.cuda.synchronize()
@@ -192,14 +194,14 @@ def charged_spheres_rate(replicas, legs):
     est = np.array([c.estimators() for c in chains])
     for c in chains:
         c.close()
-    return dict(replicas=replicas, n=512, system_length=8.0, legs_per_replica=legs,
+    return dict(replicas=replicas, n=512, system_length=8.0, far_cells=far, legs_per_replica=legs,
                 events_per_sec=float(k.sum()) / wall, kernel_ms=ms,
                 single_replica_events_per_sec=float(k1[0]) / (ms1 * 1e-3),
                 lifts_per_leg=float(est[:, 4].sum() / max(k.sum(), 1)),
                 rejected_thinning_per_leg=float(est[:, 6].sum() / max(k.sum(), 1)), create_s=create_s,
                 note="plain chain kernel, one workgroup per replica in one launch; FP64 event times; Coulomb "
-                     "through cell bounding (FP64 Ewald tables), nearby and surplus bounds with thinning; "
-                     "repulsion inverted exactly over all pairs")
+                     "far cells through %s (FP64 Ewald tables), nearby and surplus bounds with thinning; "
+                     "repulsion inverted exactly over all pairs" % far.replace("_", " "))
 
 
 def cpu_baseline(n, lengths, pos, seconds, threads):
@@ -338,6 +340,9 @@ def main():
     if not args.no_charged:
         log("charged soft spheres ...")
         out["charged_soft_spheres"] = charged_spheres_rate(args.charged_replicas, args.charged_legs)
+        log("charged soft spheres, cell veto ...")
+        out["charged_soft_spheres_cell_veto"] = charged_spheres_rate(args.charged_replicas, args.charged_legs,
+                                                                     far="cell_veto")
     if not args.no_plain:
         log("plain chain ...")
         pl = plain_chain_rate(args.plain_legs, args.plain_seconds, args.plain_equilibration)

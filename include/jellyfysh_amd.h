@@ -58,13 +58,18 @@ enum {
     ECMC_TAG_END_OF_RUN = 5,     /* NoInStateTagger + FinalTimeEndOfRunEventHandler */
     ECMC_TAG_START_OF_RUN = 6,   /* NoInStateTagger + InitialChainStartOfRunEventHandler */
     ECMC_TAG_SURPLUS = 7,        /* SurplusCellsTagger: the surplus units of a bounded occupancy */
-    ECMC_TAG_CELL_BOUNDING = 8   /* CellBoundingPotentialTagger: occupants of the non-nearby cells */
+    ECMC_TAG_CELL_BOUNDING = 8,  /* CellBoundingPotentialTagger: occupants of the non-nearby cells */
+    ECMC_TAG_CELL_VETO = 9       /* CellVetoTagger (cell_veto_tagger.py:31-155): the active unit once */
 };
 /* Event handlers of pair taggers */
 /* ECMC_EH_CELL_BOUNDING: TwoLeafUnitCellBoundingPotentialEventHandler
  * (two_leaf_unit_cell_bounding_potential_event_handler.py:113-211) with the merged-image
  * Coulomb potential (par) and ECMC_POT_CELL_BOUNDING (bpar). */
-enum { ECMC_EH_TWO_LEAF = 0, ECMC_EH_TWO_LEAF_BOUNDING = 1, ECMC_EH_CELL_BOUNDING = 2 };
+/* ECMC_EH_CELL_VETO: LeafUnitCellVetoEventHandler (leaf_unit_cell_veto_event_handler.py:31-149,
+ * cell_veto_event_handler.py:93-238): Walker tables over the same InnerPointEstimator bounds
+ * (bpar as for ECMC_EH_CELL_BOUNDING), one sampled target cell per leg, thinning against the
+ * merged-image Coulomb potential (par). */
+enum { ECMC_EH_TWO_LEAF = 0, ECMC_EH_TWO_LEAF_BOUNDING = 1, ECMC_EH_CELL_BOUNDING = 2, ECMC_EH_CELL_VETO = 3 };
 /* Potentials (the modules of potential/) */
 enum {
     ECMC_POT_HARD_SPHERE = 0,              /* par: radius */

@@ -13,8 +13,8 @@ LIB_PATH = os.path.join(_HERE, "libjellyfysh_amd.so")
 
 MAX_TAGGERS = 16
 (TAG_PAIR_CELLS, TAG_PAIR_ALL, TAG_CELL_BOUNDARY, TAG_SAMPLING, TAG_END_OF_CHAIN, TAG_END_OF_RUN,
- TAG_START_OF_RUN, TAG_SURPLUS, TAG_CELL_BOUNDING) = range(9)
-EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING, EH_CELL_BOUNDING = 0, 1, 2
+ TAG_START_OF_RUN, TAG_SURPLUS, TAG_CELL_BOUNDING, TAG_CELL_VETO) = range(10)
+EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING, EH_CELL_BOUNDING, EH_CELL_VETO = 0, 1, 2, 3
 (POT_HARD_SPHERE, POT_INVERSE_POWER, POT_LENNARD_JONES, POT_INVERSE_POWER_TRUNC, POT_MERGED_IMAGE_COULOMB,
  POT_INVERSE_POWER_COULOMB_BOUND, POT_DISPLACED_EVEN_POWER, POT_CELL_BOUNDING) = range(8)
 SCHED_LIST, SCHED_HEAP = 0, 1

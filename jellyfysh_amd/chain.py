@@ -13,8 +13,9 @@ from . import _lib as L
 
 _KINDS = {"pair_cells": L.TAG_PAIR_CELLS, "pair_all": L.TAG_PAIR_ALL, "cell_boundary": L.TAG_CELL_BOUNDARY,
           "sampling": L.TAG_SAMPLING, "end_of_chain": L.TAG_END_OF_CHAIN, "end_of_run": L.TAG_END_OF_RUN,
-          "start_of_run": L.TAG_START_OF_RUN, "surplus": L.TAG_SURPLUS, "cell_bounding": L.TAG_CELL_BOUNDING}
-_PAIR_KINDS = ("pair_cells", "pair_all", "surplus", "cell_bounding")
+          "start_of_run": L.TAG_START_OF_RUN, "surplus": L.TAG_SURPLUS, "cell_bounding": L.TAG_CELL_BOUNDING,
+          "cell_veto": L.TAG_CELL_VETO}
+_PAIR_KINDS = ("pair_cells", "pair_all", "surplus", "cell_bounding", "cell_veto")
 # InnerPointEstimator defaults (inner_point_estimator.py:45-46): prefactor, points_per_side,
 # target_charge, empirical_bound
 CELL_BOUNDING_DEFAULTS = (1.5, 10.0, 1.0, float("inf"))
@@ -72,12 +73,14 @@ def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, schedu
         if t["kind"] in _PAIR_KINDS:
             if t["kind"] == "cell_bounding":
                 g.event_handler = L.EH_CELL_BOUNDING
+            elif t["kind"] == "cell_veto":
+                g.event_handler = L.EH_CELL_VETO
             else:
                 g.event_handler = L.EH_TWO_LEAF_BOUNDING if t.get("bounding_potential") else L.EH_TWO_LEAF
             g.potential = potential_code(t["potential"])
             for j, v in enumerate(t.get("params", ())):
                 g.par[j] = float(v)
-            if t["kind"] == "cell_bounding":
+            if t["kind"] in ("cell_bounding", "cell_veto"):
                 g.bounding_potential = L.POT_CELL_BOUNDING
                 bp = list(t.get("bounding_params", ()))
                 bp = bp + list(CELL_BOUNDING_DEFAULTS[len(bp):])

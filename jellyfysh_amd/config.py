@@ -9,11 +9,12 @@ SingleActiveCellOccupancy over CuboidPeriodicCells, the heap or list scheduler a
 taggers / event handlers / potentials:
 
   taggers        excluded_cells_tagger, factor_type_map_in_state_tagger (every pair),
-                 surplus_cells_tagger, cell_bounding_potential_tagger,
+                 surplus_cells_tagger, cell_bounding_potential_tagger, cell_veto_tagger,
                  cell_boundary_tagger, no_in_state_tagger, active_global_state_in_state_tagger
   handlers       two_leaf_unit_event_handler, two_leaf_unit_bounding_potential_event_handler,
                  two_leaf_unit_cell_bounding_potential_event_handler (cell_bounding_potential
-                 with an inner_point_estimator), cell_boundary_event_handler,
+                 with an inner_point_estimator), leaf_unit_cell_veto_event_handler (with an
+                 inner_point_estimator), cell_boundary_event_handler,
                  fixed_interval_sampling_event_handler,
                  single_independent_active_periodic_direction_end_of_chain_event_handler,
                  final_time_end_of_run_event_handler, initial_chain_start_of_run_event_handler
@@ -90,6 +91,7 @@ _HANDLER_OPTIONS = {
     "two_leaf_unit_event_handler": {"potential"},
     "two_leaf_unit_bounding_potential_event_handler": {"potential", "bounding_potential", "charge"},
     "two_leaf_unit_cell_bounding_potential_event_handler": {"potential", "bounding_potential", "charge"},
+    "leaf_unit_cell_veto_event_handler": {"estimator", "potential", "charge"},
     "cell_boundary_event_handler": set(),
     "fixed_interval_sampling_event_handler": {"sampling_interval", "output_handler", "first_event_time_zero"},
     "single_independent_active_periodic_direction_end_of_chain_event_handler": {"chain_time"},
@@ -244,15 +246,20 @@ class RunConfig:
         if cls != "cell_bounding_potential":
             raise ConfigurationError("the cell-bounding handler needs cell_bounding_potential (got %s)" % cls)
         sec = self._section(camel(alias), {"estimator"})
-        ealias, ecls = split_alias(sec.get("estimator", "inner_point_estimator"))
+        return self._estimator(sec.get("estimator", "inner_point_estimator"))[1]
+
+    def _estimator(self, value):
+        """inner_point_estimator -> (its potential option or None, [prefactor, points_per_side,
+        target_charge, empirical_bound]) with the defaults of inner_point_estimator.py:45-46."""
+        ealias, ecls = split_alias(value)
         if ecls != "inner_point_estimator":
             raise ConfigurationError("only the inner_point_estimator is supported (got %s)" % ecls)
         est = self._section(camel(ealias), {"prefactor", "potential", "target_charge", "points_per_side",
                                             "empirical_bound", "periodic_boundaries"})
         if est.get("periodic_boundaries", "true").lower() not in ("1", "yes", "true", "on"):
             raise ConfigurationError("the estimator corrects separations for periodic boundaries here")
-        return [float(est.get("prefactor", 1.5)), float(est.get("points_per_side", 10)),
-                float(est.get("target_charge", 1.0)), float(est.get("empirical_bound", "inf"))]
+        return est.get("potential"), [float(est.get("prefactor", 1.5)), float(est.get("points_per_side", 10)),
+                                      float(est.get("target_charge", 1.0)), float(est.get("empirical_bound", "inf"))]
 
     def _tagger(self, label, cls):
         sec = self._section(camel(label), _TAGGER_OPTIONS)
@@ -269,6 +276,19 @@ class RunConfig:
             t["kind"] = "cell_bounding"
             t["potential"], t["params"] = self._potential(h["potential"])
             t["bounding_params"] = self._cell_bounding(h["bounding_potential"])
+            t["charge"] = "charge" in h
+        elif hcls == "leaf_unit_cell_veto_event_handler":
+            if cls != "cell_veto_tagger":
+                raise ConfigurationError("the cell-veto handler runs under cell_veto_tagger")
+            t["kind"] = "cell_veto"
+            epot, t["bounding_params"] = self._estimator(h["estimator"])
+            # leaf_unit_cell_veto_event_handler.py:81: the estimator's potential unless one is given
+            pot = h.get("potential", epot)
+            if pot is None:
+                raise ConfigurationError("the cell-veto handler needs a potential (its own or the estimator's)")
+            t["potential"], t["params"] = self._potential(pot)
+            if t["potential"] != "merged_image_coulomb":
+                raise ConfigurationError("the cell veto runs with merged_image_coulomb_potential here")
             t["charge"] = "charge" in h
         elif hcls in ("two_leaf_unit_event_handler", "two_leaf_unit_bounding_potential_event_handler"):
             if cls == "excluded_cells_tagger":

@@ -30,6 +30,8 @@
 #include <vector>
 #include <string>
 #include <mutex>
+#include <utility>
+#include <cmath>
 
 #include "../../include/jellyfysh_amd.h"
 #include "ecmc_math.h"
@@ -51,7 +53,15 @@ struct Slot {
     double cb_boundary;
     int32_t cb_dir, adir;
     double aspeed;
-    int32_t cb_cell, pad_cb;   // active cell of a cell-bounding candidate
+    int32_t cb_cell, pad_cb;   // active cell of a cell-bounding candidate; target cell of a cell veto
+    double cv_rate;            // cell veto: bounding event rate of the sampled cell
+    int32_t cv_index, cv_phase;  // cell veto: drawn table entry; 1 = entry pending, 2 = entry drawn
+};
+
+// One entry of Walker's table (walker.py:73-101): item0 with rate0, else the alias item1
+struct CvEntry {
+    double rate0;
+    int32_t item0, item1;
 };
 
 struct ChainState {
@@ -97,6 +107,10 @@ struct ChainArgs {
     int32_t *sur_cell, *sur_id;   // surplus units in the order the reference's dict yields them
     int32_t *cand;                // candidate list of the cell-bounding tagger
     const double *cbt;            // cell-bound table: (cell offset, direction) -> upper, lower
+    // cell veto: Walker tables [direction * 2 + (0 upper, 1 lower)] of cv_m entries each
+    const CvEntry *cvw;
+    double cv_total[6], cv_mean[6];
+    int32_t cv_m, pad3;
 };
 
 __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
@@ -611,6 +625,70 @@ __device__ bool create_end_of_chain(const ChainArgs &a, ChainState &st, const ec
     return false;
 }
 
+// cell veto send_event_time (cell_veto_event_handler.py:190-238): the Walker table by the sign
+// of the charge correction factor; random.choice over its entries (rejection on bit_length
+// words), uniform(0, mean) for item or alias, then expovariate / (total rate * speed).
+// Thread 0 only.  Returns false when the ring ran out of words; the drawn entry is kept in the
+// slot, so the call resumes after a refill.
+template <int D>
+__device__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
+    Slot &sl = st.slot[tix];
+    const int act = st.active, dir = st.adir;
+    double cf = (tg.use_charge ? a.charge[act] : 1.0) * 1.0 / tg.bpar[2];
+    int b = 0;
+    if (!(cf > 0.0)) { cf *= -1.0; b = 1; }
+    const int wt = dir * 2 + b;
+    const uint32_t m = (uint32_t)a.cv_m;
+    if (sl.cv_phase != 2) {
+        sl.cv_phase = 1;
+        int k = 0;
+        while ((m >> k) != 0u) k++;
+        while (st.cursor < st.produced) {
+            const uint32_t w = ring_word(a, st.cursor);
+            st.cursor++;
+            const uint32_t v = k >= 32 ? w : (w >> (32 - k));
+            if (v < m) { sl.cv_index = (int32_t)v; sl.cv_phase = 2; break; }
+        }
+        if (sl.cv_phase != 2) return false;
+    }
+    if (st.produced - st.cursor < 4) return false;
+    const CvEntry e = a.cvw[(size_t)wt * m + sl.cv_index];
+    const double u = 0.0 + (a.cv_mean[wt] - 0.0) * words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
+    const int rel = u <= e.rate0 ? e.item0 : e.item1;
+    const double u2 = words_to_random(ring_word(a, st.cursor + 2), ring_word(a, st.cursor + 3));
+    const double dU = -cr_log(1.0 - u2) / a.beta;
+    st.cursor += 4;
+    sl.cv_phase = 0;
+    const double total = a.cv_total[wt] * cf;
+    const double row = a.cbt[(size_t)(rel * 3 + dir) * 2 + b];
+    sl.cv_rate = (b ? -row : row) * cf;
+    if (!(sl.cv_rate > 0.0) || rel < 0) { st.err = ECMC_ERR_NUMERIC; return true; }
+    double ap[D];
+    load_pos<D>(a, act, ap);
+    const int acell = pos_to_cell<D>(a, ap);
+    // cuboid_periodic_cells.py:182-207: the cell centre shifted by the relative cell's lower
+    // corner lies half a cell from every wall, so the per-axis identifier sum is that cell
+    int tcell = 0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        const int ai = (acell / a.cum[d]) % a.cps[d], ri = (rel / a.cum[d]) % a.cps[d];
+        tcell += ((ai + ri) % a.cps[d]) * a.cum[d];
+    }
+    const double dt = dU / (total * st.aspeed);
+    sl.t = time_add(st.at, dt);
+    sl.valid = 1;
+    sl.seq = st.seq++;
+    sl.target = -1;
+    sl.cb_cell = tcell;
+    sl.adir = dir;
+    sl.aspeed = st.aspeed;
+    if (!isinf(sl.t.q)) slice_pos<D>(a, ap, dir, st.aspeed, st.at, sl.t);
+#pragma unroll
+    for (int d = 0; d < D; d++) sl.apos[d] = ap[d];
+    sl.astamp = sl.t;
+    return true;
+}
+
 // One workgroup runs one system's legs (the whole leg loop of the reference's mediator).
 template <int D>
 __device__ __forceinline__ void chain_body(const ChainArgs &a) {
@@ -660,6 +738,20 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
                 if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL || c.kind == ECMC_TAG_SURPLUS ||
                     c.kind == ECMC_TAG_CELL_BOUNDING) {
                     create_pair<D>(a, st, mt, c, t, nb_cell, nb_pref, sb, sh, scan);
+                } else if (c.kind == ECMC_TAG_CELL_VETO) {
+                    if (tid == 0) { st.slot[t].valid = 0; st.slot[t].cv_phase = 0; }
+                    __syncthreads();
+                    // the same bounded refill loop as the end-of-chain randint below
+                    for (int attempt = 0;; attempt++) {
+                        ring_ensure(st, mt, a, 64);
+                        if (tid == 0) {
+                            flag = (st.active < 0 || create_cell_veto<D>(a, st, c, t)) ? 1 : 0;
+                            if (!flag && attempt >= 64) { st.err = ECMC_ERR_STATE; flag = 1; }
+                        }
+                        __syncthreads();
+                        if (flag) break;
+                        ring_ensure(st, mt, a, 624);
+                    }
                 } else if (c.kind == ECMC_TAG_END_OF_CHAIN) {
                     if (tid == 0) st.slot[t].valid = 0;
                     __syncthreads();
@@ -703,7 +795,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
         const bool thinning = (tg[w].kind == ECMC_TAG_PAIR_CELLS || tg[w].kind == ECMC_TAG_PAIR_ALL ||
                                tg[w].kind == ECMC_TAG_SURPLUS || tg[w].kind == ECMC_TAG_CELL_BOUNDING) &&
                               (tg[w].event_handler == ECMC_EH_TWO_LEAF_BOUNDING ||
-                               tg[w].event_handler == ECMC_EH_CELL_BOUNDING);
+                               tg[w].event_handler == ECMC_EH_CELL_BOUNDING) ||
+                              tg[w].kind == ECMC_TAG_CELL_VETO;
         if (thinning) ring_ensure(st, mt, a, 2);
         if (tid == 0) {
             Slot &sl = st.slot[w];
@@ -770,6 +863,48 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
                     st.est[4] += 1.0;
                     st.active = sl.target;
                     ev.flags |= 1;
+                }
+                st.est[2] += 1.0;
+                break;
+            }
+            case ECMC_TAG_CELL_VETO: {
+                // mediator.py:265-292: the occupant of the sampled cell, or none;
+                // leaf_unit_cell_veto_event_handler.py:127-149 with
+                // event_handler_with_bounding_potential.py:75-101: thinning of the stored rate
+                const int tc = sl.cb_cell;
+                const int no = a.occ_n[tc];
+                if (no > 1) { st.err = ECMC_ERR_STATE; break; }
+                bool lift = false;
+                double s[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) s[d] = 0.0;
+                if (no == 1) {
+                    const int tid2 = a.occ[(size_t)tc * a.occ_cap];
+                    ev.target = tid2;
+                    double tp[D];
+                    load_pos<D>(a, tid2, tp);
+                    separation_vector<D>(a, sl.apos, tp, s);
+                    double q3[3] = {0, 0, 0};
+                    for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
+                    const double c1 = T.use_charge ? a.charge[act] : 1.0;
+                    const double c2 = T.use_charge ? a.charge[tid2] : 1.0;
+                    const double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+                    if (real > 0) {
+                        double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
+                        st.cursor += 2;
+                        u = 0 + (sl.cv_rate - 0) * u;
+                        if (u < real) lift = true;
+                    }
+                }
+                store_pos<D>(a, act, sl.apos);
+                st.at = sl.astamp;
+                if (lift) {
+                    st.est[0] += s[sl.adir];
+                    st.est[4] += 1.0;
+                    st.active = ev.target;
+                    ev.flags |= 1;
+                } else {
+                    st.est[6] += 1.0;
                 }
                 st.est[2] += 1.0;
                 break;
@@ -931,6 +1066,46 @@ static void mt_seed_host(uint64_t seed, uint32_t *mt) {
 
 using namespace ecmc;
 
+// Walker's table (walker.py:73-101): small and large stacks split at the mean (sequential sum
+// of the rates), one (small, alias large) entry per step with the large item's residual pushed
+// back by comparison with the mean; the leftovers must lie within 1e-6 of the mean.
+static bool walker_table(const std::vector<int32_t> &items, const std::vector<double> &rates, CvEntry *out,
+                         double &total_out, double &mean_out) {
+    const int m = (int)items.size();
+    double total = 0.0;
+    for (int i = 0; i < m; i++) total += rates[i];
+    const double mean = total / m;
+    std::vector<std::pair<int32_t, double>> small, large;
+    for (int i = 0; i < m; i++) {
+        if (!(rates[i] >= 0.0)) return false;
+        if (rates[i] > mean) large.emplace_back(items[i], rates[i]);
+        else small.emplace_back(items[i], rates[i]);
+    }
+    int k = 0;
+    while (!small.empty() && !large.empty()) {
+        const std::pair<int32_t, double> sm = small.back();
+        small.pop_back();
+        std::pair<int32_t, double> lg = large.back();
+        large.pop_back();
+        out[k++] = CvEntry{sm.second, sm.first, lg.first};
+        lg.second -= mean - sm.second;
+        if (lg.second < mean) small.push_back(lg);
+        else large.push_back(lg);
+    }
+    for (std::vector<std::pair<int32_t, double>> *rest : {&small, &large}) {
+        while (!rest->empty()) {
+            const std::pair<int32_t, double> it = rest->back();
+            rest->pop_back();
+            const double q = it.second / mean;
+            if (!(1 - 1e-6 < q && q < 1 + 1e-6)) return false;
+            out[k++] = CvEntry{mean, it.first, -1};
+        }
+    }
+    total_out = total;
+    mean_out = mean;
+    return k == m;
+}
+
 struct ecmc_system {
     ecmc_system_desc desc;
     int dim, n, ncells, nb_stride, occ_cap, ring_words;
@@ -939,6 +1114,7 @@ struct ecmc_system {
     int32_t *d_nearby = nullptr, *d_nearby_n = nullptr, *d_occ = nullptr, *d_occ_n = nullptr;
     int32_t *d_sur_cell = nullptr, *d_sur_id = nullptr, *d_cand = nullptr;
     double *d_cbt = nullptr, *d_corners = nullptr;
+    CvEntry *d_cvw = nullptr;
     uint32_t *d_mt = nullptr, *d_ring = nullptr;
     ChainState *d_st = nullptr;
     ecmc_tagger_desc *d_tg = nullptr;
@@ -1003,12 +1179,14 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     int ewald_tagger = -1, cb_tagger = -1;
     for (int t = 0; t < desc->n_taggers; t++) {
         const ecmc_tagger_desc &tg = desc->taggers[t];
-        if (tg.kind < 0 || tg.kind > ECMC_TAG_CELL_BOUNDING) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
+        if (tg.kind < 0 || tg.kind > ECMC_TAG_CELL_VETO) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
+        if ((tg.kind == ECMC_TAG_CELL_VETO) != (tg.event_handler == ECMC_EH_CELL_VETO))
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "the cell-veto handler runs under the cell-veto tagger only");
         if (tg.kind == ECMC_TAG_SURPLUS && desc->max_occupants <= 0)
             return ecmc_fail(ECMC_ERR_CONFIGURATION, "the surplus tagger needs a bounded cell occupancy");
         if ((tg.kind == ECMC_TAG_CELL_BOUNDING) != (tg.event_handler == ECMC_EH_CELL_BOUNDING))
             return ecmc_fail(ECMC_ERR_CONFIGURATION, "the cell-bounding handler runs under the cell-bounding tagger only");
-        if (tg.kind == ECMC_TAG_CELL_BOUNDING) {
+        if (tg.kind == ECMC_TAG_CELL_BOUNDING || tg.kind == ECMC_TAG_CELL_VETO) {
             if (tg.potential != ECMC_POT_MERGED_IMAGE_COULOMB || tg.bounding_potential != ECMC_POT_CELL_BOUNDING || dim != 3)
                 return ecmc_fail(ECMC_ERR_CONFIGURATION, "cell bounding needs merged-image Coulomb with the cell bounding potential in 3D");
             if (desc->max_occupants != 1)
@@ -1265,6 +1443,41 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
             cell_bound_cache_store(key, table);
         }
         a.cbt = s->d_cbt;
+        bool veto = false;
+        for (int t = 0; t < desc->n_taggers; t++) veto = veto || desc->taggers[t].kind == ECMC_TAG_CELL_VETO;
+        if (veto) {
+            // cell_veto_event_handler.py:147-159: per direction, the non-nearby cells in
+            // yield_cells() order with rates max(upper, 0) and max(-lower, 0)
+            std::vector<int32_t> items;
+            for (int c = 0; c < grid.count; c++)
+                if (!std::isnan(table[(size_t)c * 6])) items.push_back(c);
+            const int m = (int)items.size();
+            std::vector<CvEntry> all((size_t)6 * m);
+            std::vector<double> rates(m);
+            for (int wt = 0; wt < 6 && m > 0; wt++) {
+                const int d = wt / 2, b = wt % 2;
+                for (int i = 0; i < m; i++) {
+                    const double v = table[(size_t)(items[i] * 3 + d) * 2 + b];
+                    const double r = b ? -v : v;
+                    rates[i] = 0.0 > r ? 0.0 : r;
+                }
+                if (!walker_table(items, rates, &all[(size_t)wt * m], a.cv_total[wt], a.cv_mean[wt])) {
+                    ecmc_system_destroy(s);
+                    return ecmc_fail(ECMC_ERR_CONFIGURATION, "cell veto: Walker table failed (negative rate or leftover check)");
+                }
+            }
+            if (m == 0) {
+                ecmc_system_destroy(s);
+                return ecmc_fail(ECMC_ERR_CONFIGURATION, "cell veto: no cell lies outside the nearby cells");
+            }
+            if (hipMalloc((void **)&s->d_cvw, sizeof(CvEntry) * all.size()) != hipSuccess ||
+                hipMemcpy(s->d_cvw, all.data(), sizeof(CvEntry) * all.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                ecmc_system_destroy(s);
+                return ecmc_fail(ECMC_ERR_HIP, "cell veto table upload failed");
+            }
+            a.cvw = s->d_cvw;
+            a.cv_m = m;
+        }
     }
     *out = s;
     return ECMC_OK;
@@ -1275,6 +1488,7 @@ extern "C" void ecmc_system_destroy(ecmc_system *s) {
     hipFree(s->d_pos); hipFree(s->d_charge); hipFree(s->d_cmin); hipFree(s->d_nearby); hipFree(s->d_nearby_n);
     hipFree(s->d_occ); hipFree(s->d_occ_n); hipFree(s->d_mt); hipFree(s->d_ring); hipFree(s->d_st);
     hipFree(s->d_sur_cell); hipFree(s->d_sur_id); hipFree(s->d_cand); hipFree(s->d_cbt); hipFree(s->d_corners);
+    hipFree(s->d_cvw);
     hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);

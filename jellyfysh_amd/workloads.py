@@ -9,6 +9,9 @@ inverse-power-9 repulsion of prefactor 1 between every pair, at beta = 1.
   cell-bounding potential with FP64 Ewald tables (one occupant per cell,
   single_active_cell_occupancy.py:62), nearby cells and the surplus through the
   inverse-power Coulomb bound with thinning.
+- Far cells, alternatively: the cell veto of the reference's coulomb_atoms/cell_veto.ini
+  (cell_veto_event_handler.py:93-238) over the same estimator bounds (``far="cell_veto"``),
+  one sampled target cell per leg in place of one candidate per occupied far cell.
 - Repulsion: one more pair tagger over all other units with the exact inversion of
   InversePowerPotential(power=9, prefactor=1) (inverse_power_potential.py:32-179), which has
   no cutoff in the reference.
@@ -26,6 +29,7 @@ import numpy as np
 from .chain import DeviceChain, make_desc
 
 COULOMB_PAIRS = ["coulomb_cell_bounding", "coulomb_nearby", "cell_boundary", "coulomb_surplus", "soft_sphere"]
+FAR_TAGS = {"cell_bounding": "coulomb_cell_bounding", "cell_veto": "coulomb_cell_veto"}
 
 
 def random_placement_without_overlap(n, system_length, min_separation, rng, dim=3, max_attempts=10_000_000):
@@ -51,15 +55,21 @@ def random_placement_without_overlap(n, system_length, min_separation, rng, dim=
 
 
 def soft_sphere_taggers(alpha=3.45, fourier_cutoff=6, position_cutoff=2, power=9.0, prefactor=1.0, chain_time=1.0,
-                        end_of_run_time=1.0e9):
-    """The tagger list (INI sections) of the charged soft spheres."""
+                        end_of_run_time=1.0e9, far="cell_bounding"):
+    """The tagger list (INI sections) of the charged soft spheres; `far` is "cell_bounding" or
+    "cell_veto"."""
+    if far not in FAR_TAGS:
+        raise ValueError("far cells through cell_bounding or cell_veto (got %r)" % (far,))
     mic = [alpha, fourier_cutoff, position_cutoff, 1.0]
-    pairs = list(COULOMB_PAIRS)
+    pairs = [FAR_TAGS[far]] + COULOMB_PAIRS[1:]
     eoc = ["end_of_chain"] + pairs
     ipcb = dict(bounding_potential="inverse_power_coulomb_bounding", bounding_params=[1.5837])
+    first = dict(tag=FAR_TAGS[far], kind=far, potential="merged_image_coulomb", params=mic,
+                 bounding_params=[1.5, 10, 1.0, float("inf")], charge=True, create=pairs, trash=pairs)
+    if far == "cell_veto":
+        first["number_event_handlers"] = 1
     return [
-        dict(tag="coulomb_cell_bounding", kind="cell_bounding", potential="merged_image_coulomb", params=mic,
-             bounding_params=[1.5, 10, 1.0, float("inf")], charge=True, create=pairs, trash=pairs),
+        first,
         dict(tag="coulomb_nearby", kind="pair_cells", potential="merged_image_coulomb", params=mic, charge=True,
              create=pairs, trash=pairs, **ipcb),
         dict(tag="cell_boundary", kind="cell_boundary", create=pairs, trash=pairs),
@@ -76,7 +86,7 @@ def soft_sphere_taggers(alpha=3.45, fourier_cutoff=6, position_cutoff=2, power=9
 
 
 def charged_soft_spheres(n=512, system_length=8.0, seed=0, min_separation=0.8, cells_per_side=8, beta=1.0,
-                         chain_time=1.0):
+                         chain_time=1.0, far="cell_bounding"):
     """One replica: dict with dim, n, system_length, beta, cells_per_side, seed, taggers,
     positions (n x 3) and charges (+1 for even identifiers, -1 for odd)."""
     if n % 2:
@@ -85,7 +95,7 @@ def charged_soft_spheres(n=512, system_length=8.0, seed=0, min_separation=0.8, c
     pos = random_placement_without_overlap(n, system_length, min_separation, rng)
     charges = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     return dict(dim=3, n=n, system_length=float(system_length), beta=float(beta), cells_per_side=int(cells_per_side),
-                seed=int(seed), taggers=soft_sphere_taggers(chain_time=chain_time), positions=pos, charges=charges,
+                seed=int(seed), taggers=soft_sphere_taggers(chain_time=chain_time, far=far), positions=pos, charges=charges,
                 max_occupants=1)
 
 

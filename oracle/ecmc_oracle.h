@@ -125,6 +125,22 @@ void ora_cell_bound_table(const ora_cells *cells, const ora_ewald *ewald, double
                           double target_charge, int points_per_side, double estimator_prefactor,
                           double empirical_bound, double system_length, double *out);
 
+/* ---- Walker's alias table (walker.py:53-130) ------------------------------------------
+ * Built from m rates in the given order.  Entry i holds (item0[i], rate0[i]) and, when
+ * item1[i] >= 0, the alias item1[i] (its rate is mean - rate0[i]); entries without an alias
+ * carry rate0 = mean.  ora_walker_build returns the number of entries (m) or -1 when a rate is
+ * negative or the leftovers fail the reference's 1e-6 check. */
+typedef struct {
+    int m;
+    double total, mean;
+    int32_t *item0, *item1;
+    double *rate0;
+} ora_walker;
+int ora_walker_build(ora_walker *w, int m, const int32_t *items, const double *rates);
+void ora_walker_free(ora_walker *w);
+/* sample_cell: random.choice over the entries, then uniform(0, mean) <= rate0 keeps item0 */
+int32_t ora_walker_sample(const ora_walker *w, ora_mt *r);
+
 /* ---- the chain engine ---------------------------------------------------------------- */
 enum {
     ORA_TAG_PAIR_CELLS = 0,   /* ExcludedCellsTagger -> pair event handler */
@@ -135,12 +151,16 @@ enum {
     ORA_TAG_END_OF_RUN = 5,
     ORA_TAG_START_OF_RUN = 6,
     ORA_TAG_SURPLUS = 7,      /* SurplusCellsTagger: the surplus units in dict order -> pair handler */
-    ORA_TAG_CELL_BOUNDING = 8 /* CellBoundingPotentialTagger: occupants of the non-nearby cells */
+    ORA_TAG_CELL_BOUNDING = 8, /* CellBoundingPotentialTagger: occupants of the non-nearby cells */
+    ORA_TAG_CELL_VETO = 9      /* CellVetoTagger: one handler per leg for the active unit */
 };
 /* ORA_EH_CELL_BOUNDING: TwoLeafUnitCellBoundingPotentialEventHandler with a merged-image
  * Coulomb potential (par) and a CellBoundingPotential from an InnerPointEstimator
  * (bpar: estimator prefactor, points_per_side, target_charge, empirical_bound). */
-enum { ORA_EH_TWO_LEAF = 0, ORA_EH_TWO_LEAF_BOUNDING = 1, ORA_EH_CELL_BOUNDING = 2 };
+/* ORA_EH_CELL_VETO: LeafUnitCellVetoEventHandler (leaf_unit_cell_veto_event_handler.py:31-149,
+ * cell_veto_event_handler.py:93-238) with the merged-image Coulomb potential (par) and an
+ * InnerPointEstimator (bpar as for ORA_EH_CELL_BOUNDING). */
+enum { ORA_EH_TWO_LEAF = 0, ORA_EH_TWO_LEAF_BOUNDING = 1, ORA_EH_CELL_BOUNDING = 2, ORA_EH_CELL_VETO = 3 };
 enum {
     ORA_POT_HARD_SPHERE = 0,      /* par: radius */
     ORA_POT_INVERSE_POWER = 1,    /* par: power, prefactor */

@@ -165,6 +165,75 @@ void ora_cell_bound_table(const ora_cells *cells, const ora_ewald *ewald, double
     }
 }
 
+/* Walker's table (walker.py:73-101).  small_list / large_list are stacks: items at or below
+ * the mean go to the small one, in input order; each step pops one of each, stores
+ * (small, alias large with rate mean - small.rate), lowers the large item's rate by that
+ * amount and pushes it back by comparison with the mean.  The total is the sequential sum
+ * of the rates (CPython before 3.12 and PyPy; 3.12 compensates the summation of floats). */
+int ora_walker_build(ora_walker *w, int m, const int32_t *items, const double *rates) {
+    memset(w, 0, sizeof(*w));
+    if (m <= 0) return -1;
+    double total = 0.0;
+    for (int i = 0; i < m; i++) total += rates[i];
+    const double mean = total / m;
+    for (int i = 0; i < m; i++) if (!(rates[i] >= 0.0)) return -1;
+    int32_t *sid = (int32_t *)malloc(sizeof(int32_t) * m), *lid = (int32_t *)malloc(sizeof(int32_t) * m);
+    double *sr = (double *)malloc(sizeof(double) * m), *lr = (double *)malloc(sizeof(double) * m);
+    w->item0 = (int32_t *)malloc(sizeof(int32_t) * m);
+    w->item1 = (int32_t *)malloc(sizeof(int32_t) * m);
+    w->rate0 = (double *)malloc(sizeof(double) * m);
+    int ns = 0, nl = 0, k = 0, ok = 1;
+    for (int i = 0; i < m; i++) {
+        if (rates[i] > mean) { lid[nl] = items[i]; lr[nl++] = rates[i]; }
+        else { sid[ns] = items[i]; sr[ns++] = rates[i]; }
+    }
+    while (ns && nl) {
+        const int32_t si = sid[--ns], li = lid[--nl];
+        const double srate = sr[ns];
+        double lrate = lr[nl];
+        w->item0[k] = si; w->rate0[k] = srate; w->item1[k] = li; k++;
+        lrate -= mean - srate;
+        if (lrate < mean) { sid[ns] = li; sr[ns++] = lrate; }
+        else { lid[nl] = li; lr[nl++] = lrate; }
+    }
+    while (ns) {
+        --ns;
+        const double q = sr[ns] / mean;
+        if (!(1 - 1e-6 < q && q < 1 + 1e-6)) ok = 0;
+        w->item0[k] = sid[ns]; w->rate0[k] = mean; w->item1[k] = -1; k++;
+    }
+    while (nl) {
+        --nl;
+        const double q = lr[nl] / mean;
+        if (!(1 - 1e-6 < q && q < 1 + 1e-6)) ok = 0;
+        w->item0[k] = lid[nl]; w->rate0[k] = mean; w->item1[k] = -1; k++;
+    }
+    free(sid); free(lid); free(sr); free(lr);
+    w->m = k; w->total = total; w->mean = mean;
+    if (!ok || k != m) { ora_walker_free(w); return -1; }
+    return k;
+}
+void ora_walker_free(ora_walker *w) {
+    free(w->item0); free(w->item1); free(w->rate0);
+    memset(w, 0, sizeof(*w));
+}
+/* walker.py:103-117: random.choice(table) = table[_randbelow(len)], then uniform(0, mean) */
+int32_t ora_walker_sample(const ora_walker *w, ora_mt *r) {
+    const int64_t i = ora_randint(r, 0, w->m - 1);
+    const double u = ora_uniform(r, 0.0, w->mean);
+    return u <= w->rate0[i] ? w->item0[i] : w->item1[i];
+}
+
+/* cuboid_periodic_cells.py:182-207: the cell holding the centre of `cell` shifted by the
+ * lower corner of `rel` (positions corrected into [0, L)) */
+static int cells_translate(const ora_cells *c, int cell, int rel, double L) {
+    double cmin[3], cmax[3], rmin[3], rmax[3], p[3];
+    ora_cells_bounds(c, cell, cmin, cmax);
+    ora_cells_bounds(c, rel, rmin, rmax);
+    for (int d = 0; d < c->dim; d++) p[d] = ora_py_mod((cmax[d] + cmin[d]) / 2.0 + rmin[d], L);
+    return ora_cells_position_to_cell(c, p);
+}
+
 /* ===== small growable int list ========================================================= */
 typedef struct { int *v; int n, cap; } ilist;
 static void il_push(ilist *l, int x) {
@@ -180,8 +249,8 @@ static int il_remove(ilist *l, int x) { /* list.remove: first occurrence */
 /* ===== handlers ======================================================================== */
 typedef struct {
     int tagger;
-    double cb_rate;         /* bounding event rate of a cell-bounding candidate */
-    int cb_cell;            /* active cell when that candidate was created */
+    double cb_rate;         /* bounding event rate of a cell-bounding candidate or cell-veto event */
+    int cb_cell;            /* active cell of a cell-bounding candidate; target cell of a cell veto */
     int pending;           /* event currently stored in the scheduler */
     double tq, tr;
     int target;            /* pair target or new active (end of chain) */
@@ -208,7 +277,8 @@ struct ora_engine {
     ilist *occ;             /* per cell */
     int *sur_cell, *sur_id; /* surplus units in the order the reference's dict yields them */
     int nsur, capsur;
-    double *cbt[ORA_MAX_TAGGERS]; /* cell-bound tables of cell-bounding taggers */
+    double *cbt[ORA_MAX_TAGGERS]; /* cell-bound tables of cell-bounding and cell-veto taggers */
+    ora_walker *cv[ORA_MAX_TAGGERS]; /* cell veto: Walker tables [direction * 2 + (0 upper, 1 lower)] */
     int occ_active_id, occ_active_cell;
     ora_mt rng;
     handler *h; int nh;
@@ -505,6 +575,38 @@ static void create_tagger(ora_engine *e, int t) {
         free(targets);
         break;
     }
+    case ORA_TAG_CELL_VETO: {
+        /* cell_veto_tagger.py:150-155 yields the active unit once; send_event_time
+         * (cell_veto_event_handler.py:190-238): Walker table by the sign of the charge
+         * correction factor, a sampled relative cell, then expovariate / (total rate * speed) */
+        if (a < 0) return;
+        int hidx = pool_pop(e, t);
+        if (hidx < 0) return;
+        handler *hd = &e->h[hidx];
+        hd->aid = a; hd->adir = e->adir; hd->aspeed = e->aspeed; hd->aq = e->aq; hd->ar = e->ar;
+        memcpy(hd->apos, e->pos + (size_t)a * e->dim, sizeof(double) * e->dim);
+        hd->target = -1;
+        const int acell = ora_cells_position_to_cell(e->cells, hd->apos);
+        if (acell < 0) { eng_fail(e, "position outside of the cell system"); return; }
+        double cf = (tg->use_charge ? e->charge[a] : 1.0) * 1.0 / tg->bpar[2];
+        int b = 0;
+        if (!(cf > 0.0)) { cf *= -1.0; b = 1; }
+        const ora_walker *wk = &e->cv[t][e->adir * 2 + b];
+        const double total = wk->total * cf;
+        const int rel = ora_walker_sample(wk, &e->rng);
+        const double row = e->cbt[t][(size_t)(rel * 3 + e->adir) * 2 + b];
+        hd->cb_rate = (b ? -row : row) * cf;
+        if (!(hd->cb_rate > 0.0)) { eng_fail(e, "cell veto: bounding event rate not positive"); return; }
+        hd->cb_cell = cells_translate(e->cells, acell, rel, e->L);
+        const double dU = ora_expovariate(&e->rng, e->cfg.beta, e->cfg.math_mode);
+        const double dt = dU / (total * e->aspeed);
+        double q, r;
+        ora_time_add(e->aq, e->ar, dt, &q, &r);
+        if (!isinf(q)) slice_pos(e, hd->apos, e->adir, e->aspeed, e->aq, e->ar, q, r);
+        hd->aq = q; hd->ar = r;
+        sched_push(e, hidx, q, r);
+        break;
+    }
     case ORA_TAG_CELL_BOUNDARY: {
         if (a < 0) return;
         int hidx = pool_pop(e, t);
@@ -638,6 +740,41 @@ static int out_state(ora_engine *e, int hidx, ora_event *ev) {
         e->est[2] += 1.0;
         break;
     }
+    case ORA_TAG_CELL_VETO: {
+        /* mediator.py:265-292 hands over the occupant of the sampled cell (None when empty);
+         * leaf_unit_cell_veto_event_handler.py:127-149 with
+         * event_handler_with_bounding_potential.py:75-101: thinning of the stored rate */
+        const ilist *o = &e->occ[hd->cb_cell];
+        if (o->n > 1) { eng_fail(e, "cell veto: the sampled cell holds more than one unit"); return -1; }
+        int lift = 0;
+        double sep[3] = {0, 0, 0};
+        if (o->n == 1) {
+            const int tid = o->v[0];
+            ev->target = tid;
+            double p[3];
+            separation_vector(e, hd->apos, e->pos + (size_t)tid * e->dim, sep);
+            for (int d = 0; d < 3; d++) p[d] = sep[(hd->adir + d) % 3];
+            double c1 = tg->use_charge ? e->charge[hd->aid] : 1.0;
+            double c2 = tg->use_charge ? e->charge[tid] : 1.0;
+            double real = tg->par[3] * c1 * c2 * ora_ewald_derivative(e->ewald[hd->tagger], p[0], p[1], p[2]) * hd->aspeed;
+            if (real > 0) {
+                double u = ora_uniform(&e->rng, 0, hd->cb_rate);
+                if (u < real) lift = 1;
+            }
+        }
+        memcpy(e->pos + (size_t)a * e->dim, hd->apos, sizeof(double) * e->dim);
+        e->aq = hd->aq; e->ar = hd->ar;
+        if (lift) {
+            e->est[0] += sep[hd->adir];
+            e->est[4] += 1.0;
+            e->active = ev->target;
+            ev->flags |= 1;
+        } else {
+            e->est[6] += 1.0;
+        }
+        e->est[2] += 1.0;
+        break;
+    }
     case ORA_TAG_CELL_BOUNDARY: {
         double *p = e->pos + (size_t)a * e->dim;
         memcpy(p, hd->apos, sizeof(double) * e->dim);
@@ -724,15 +861,42 @@ ora_engine *ora_engine_create(const ora_config *cfg, const double *positions, co
         if (cfg->taggers[t].potential == ORA_POT_MERGED_IMAGE_COULOMB)
             e->ewald[t] = ora_ewald_create((int)cfg->taggers[t].par[1], (int)cfg->taggers[t].par[2],
                                            cfg->taggers[t].par[0], e->L);
-        if (cfg->taggers[t].event_handler == ORA_EH_CELL_BOUNDING) {
+        if (cfg->taggers[t].event_handler == ORA_EH_CELL_BOUNDING || cfg->taggers[t].event_handler == ORA_EH_CELL_VETO) {
             const ora_tagger *tg = &cfg->taggers[t];
             if (e->dim != 3 || !e->ewald[t] || cfg->max_occupants != 1) {
                 eng_fail(e, "cell bounding needs 3-D merged-image Coulomb and one occupant per cell");
                 return e;
             }
+            if ((tg->kind == ORA_TAG_CELL_VETO) != (tg->event_handler == ORA_EH_CELL_VETO)) {
+                eng_fail(e, "the cell-veto handler runs under the cell-veto tagger only");
+                return e;
+            }
             e->cbt[t] = (double *)malloc(sizeof(double) * (size_t)nc * 6);
             ora_cell_bound_table(e->cells, e->ewald[t], tg->par[3], tg->bpar[2], (int)tg->bpar[1], tg->bpar[0],
                                  tg->bpar[3], e->L, e->cbt[t]);
+        }
+        if (cfg->taggers[t].event_handler == ORA_EH_CELL_VETO) {
+            /* cell_veto_event_handler.py:147-159: per direction, the non-nearby cells in
+             * yield_cells() order with rates max(upper, 0) and max(-lower, 0) */
+            int32_t *items = (int32_t *)malloc(sizeof(int32_t) * nc);
+            double *ru = (double *)malloc(sizeof(double) * nc), *rl = (double *)malloc(sizeof(double) * nc);
+            e->cv[t] = (ora_walker *)calloc(6, sizeof(ora_walker));
+            for (int d = 0; d < 3 && !e->err; d++) {
+                int m = 0;
+                for (int c = 0; c < nc; c++) {
+                    const double up = e->cbt[t][(size_t)(c * 3 + d) * 2], lo = e->cbt[t][(size_t)(c * 3 + d) * 2 + 1];
+                    if (isnan(up)) continue;                      /* nearby the zero cell */
+                    items[m] = c;
+                    ru[m] = 0.0 > up ? 0.0 : up;
+                    rl[m] = 0.0 > -lo ? 0.0 : -lo;
+                    m++;
+                }
+                if (ora_walker_build(&e->cv[t][d * 2], m, items, ru) < 0 ||
+                    ora_walker_build(&e->cv[t][d * 2 + 1], m, items, rl) < 0)
+                    eng_fail(e, "cell veto: Walker table failed");
+            }
+            free(items); free(ru); free(rl);
+            if (e->err) return e;
         }
     }
     return e;
@@ -744,6 +908,7 @@ void ora_engine_destroy(ora_engine *e) {
     for (int c = 0; c < nc; c++) free(e->occ[c].v);
     for (int t = 0; t < e->cfg.n_taggers; t++) {
         free(e->running[t].v); free(e->idle[t].v); ora_ewald_destroy(e->ewald[t]); free(e->cbt[t]);
+        if (e->cv[t]) { for (int k = 0; k < 6; k++) ora_walker_free(&e->cv[t][k]); free(e->cv[t]); }
     }
     free(e->occ); free(e->sur_cell); free(e->sur_id); free(e->running); free(e->idle);
     ora_cells_destroy(e->cells);

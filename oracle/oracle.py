@@ -18,8 +18,8 @@ _REF_DIR = os.path.join(_HERE, "_ref")
 MATH_LIBM, MATH_CR = 0, 1
 TAG_PAIR_CELLS, TAG_PAIR_ALL, TAG_CELL_BOUNDARY, TAG_SAMPLING, TAG_END_OF_CHAIN, TAG_END_OF_RUN, \
     TAG_START_OF_RUN = range(7)
-EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING, EH_CELL_BOUNDING = 0, 1, 2
-TAG_SURPLUS, TAG_CELL_BOUNDING = 7, 8
+EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING, EH_CELL_BOUNDING, EH_CELL_VETO = 0, 1, 2, 3
+TAG_SURPLUS, TAG_CELL_BOUNDING, TAG_CELL_VETO = 7, 8, 9
 POT_HARD_SPHERE, POT_INVERSE_POWER, POT_LENNARD_JONES, POT_INVERSE_POWER_TRUNC, POT_MERGED_IMAGE_COULOMB, \
     POT_IPCB = range(6)
 SCHED_LIST, SCHED_HEAP = 0, 1
@@ -98,6 +98,8 @@ def lib():
             "ora_config_hard_disks": (None, [vp, i32, d, d, i32, d, d, u64]),
             "ora_config_inverse_power_pairs": (None, [vp, i32, i32, d, d, i32, i32, pd, pd, d, d, d, u64, i32]),
             "ora_run_threads": (i64, [vp, pd, pd, i32, i64, pd]),
+            "ora_walker_build": (i32, [vp, i32, ctypes.POINTER(i32), pd]), "ora_walker_free": (None, [vp]),
+            "ora_walker_sample": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -113,6 +115,38 @@ def _dp(a):
 
 def vec(x):
     return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+class _WalkerC(ctypes.Structure):
+    _fields_ = [("m", ctypes.c_int32), ("total", ctypes.c_double), ("mean", ctypes.c_double),
+                ("item0", ctypes.POINTER(ctypes.c_int32)), ("item1", ctypes.POINTER(ctypes.c_int32)),
+                ("rate0", ctypes.POINTER(ctypes.c_double))]
+
+
+class Walker:
+    """Walker's alias table of the oracle (walker.py:53-130) over integer items."""
+
+    def __init__(self, items, rates):
+        it = np.ascontiguousarray(items, dtype=np.int32)
+        r = vec(rates)
+        self._w = _WalkerC()
+        if lib().ora_walker_build(ctypes.byref(self._w), len(it), it.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                  _dp(r)) < 0:
+            raise ValueError("Walker table failed (negative rate or leftover check)")
+        m = self._w.m
+        self.total, self.mean = self._w.total, self._w.mean
+        self.item0 = np.ctypeslib.as_array(self._w.item0, (m,)).copy()
+        self.item1 = np.ctypeslib.as_array(self._w.item1, (m,)).copy()
+        self.rate0 = np.ctypeslib.as_array(self._w.rate0, (m,)).copy()
+
+    def sample(self, mt):
+        return lib().ora_walker_sample(ctypes.byref(self._w), mt._buf)
+
+    def __del__(self):
+        try:
+            lib().ora_walker_free(ctypes.byref(self._w))
+        except Exception:
+            pass
 
 
 class MT:

@@ -70,7 +70,7 @@ def wiring_from_meta(meta):
 _ORACLE_POT = {"hard_sphere": 0, "inverse_power": 1, "lennard_jones": 2, "inverse_power_truncated": 3,
                "merged_image_coulomb": 4, "inverse_power_coulomb_bounding": 5}
 _ORACLE_KIND = {"pair_cells": 0, "pair_all": 1, "cell_boundary": 2, "sampling": 3, "end_of_chain": 4,
-                "end_of_run": 5, "start_of_run": 6, "surplus": 7, "cell_bounding": 8}
+                "end_of_run": 5, "start_of_run": 6, "surplus": 7, "cell_bounding": 8, "cell_veto": 9}
 
 
 def oracle_config_from_wiring(dim, n, system_length, beta, cells_per_side, taggers, seed, scheduler="heap",
@@ -99,13 +99,13 @@ def oracle_config_from_wiring(dim, n, system_length, beta, cells_per_side, tagge
             g.trash[j] = idx[tag]
         g.n_trash = len(t.get("trash", []))
         g.n_handlers = t.get("number_event_handlers", n)
-        if t["kind"] in ("pair_cells", "pair_all", "surplus", "cell_bounding"):
+        if t["kind"] in ("pair_cells", "pair_all", "surplus", "cell_bounding", "cell_veto"):
             g.event_handler = 1 if t.get("bounding_potential") else 0
             g.potential = _ORACLE_POT[t["potential"]]
             for j, v in enumerate(t.get("params", ())):
                 g.par[j] = v
-            if t["kind"] == "cell_bounding":
-                g.event_handler = 2
+            if t["kind"] in ("cell_bounding", "cell_veto"):
+                g.event_handler = 2 if t["kind"] == "cell_bounding" else 3
                 bp = list(t.get("bounding_params", ()))
                 for j, v in enumerate(bp + list(CELL_BOUNDING_DEFAULTS[len(bp):])):
                     g.bpar[j] = v
@@ -221,4 +221,20 @@ def coulomb_cell_bounded_system(n=8, system_length=1.0, cells_per_side=(3, 5, 7)
     ]
     meta = dict(dim=3, n=n, system_length=system_length, beta=beta, cells_per_side=list(cells_per_side),
                 seed=seed, scheduler="heap", max_occupants=1)
+    return meta, taggers, pos, charges
+
+
+def coulomb_cell_veto_system(n=8, estimator_prefactor=1.0, **kw):
+    """The system of coulomb_cell_bounded_system with the far cells handled by the cell veto
+    (the reference's coulomb_atoms/cell_veto.ini): one cell_veto tagger with an inner point
+    estimator of prefactor 1.0 in place of the cell-bounding tagger."""
+    meta, taggers, pos, charges = coulomb_cell_bounded_system(n=n, **kw)
+    old, new = "coulomb_cell_bounding", "coulomb_cell_veto"
+    for t in taggers:
+        for key in ("create", "trash"):
+            t[key] = [new if x == old else x for x in t[key]]
+    mic = taggers[0]["params"]
+    taggers[0] = dict(tag=new, kind="cell_veto", potential="merged_image_coulomb", params=mic,
+                      bounding_params=[estimator_prefactor, 10, 1.0, float("inf")], charge=True,
+                      create=taggers[1]["create"], trash=taggers[1]["trash"], number_event_handlers=1)
     return meta, taggers, pos, charges
