@@ -10,7 +10,8 @@ Beside it, the plain form of the same potential at N = 4096, rho = 0.8 (16^3 lat
 the reference's leg loop on one workgroup, and the oracle runs a bounded sample of the
 batched workload on the host cores (the CPU baseline).  256 charged-soft-sphere replicas
 (N = 512, merged-image Coulomb + inverse power 9) report their rate in one launch of the plain
-chain kernel against one replica alone (--no-charged skips them).
+chain kernel against one replica alone, once with the far cells through the cell-bounding
+potential and once through the cell veto (--no-charged skips them).
 
     python bench.py [--steps K] [--warmup W] [--legs L]
 """
@@ -199,6 +200,7 @@ def charged_spheres_rate(replicas, legs, far="cell_bounding"):
                 single_replica_events_per_sec=float(k1[0]) / (ms1 * 1e-3),
                 lifts_per_leg=float(est[:, 4].sum() / max(k.sum(), 1)),
                 rejected_thinning_per_leg=float(est[:, 6].sum() / max(k.sum(), 1)), create_s=create_s,
+                chain_displacement_per_sec=float(est[:, 1].sum()) / wall,
                 note="plain chain kernel, one workgroup per replica in one launch; FP64 event times; Coulomb "
                      "far cells through %s (FP64 Ewald tables), nearby and surplus bounds with thinning; "
                      "repulsion inverted exactly over all pairs" % far.replace("_", " "))
