@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--no-charged", action="store_true", help="skip the charged-soft-sphere replicas")
     ap.add_argument("--charged-replicas", type=int, default=256)
     ap.add_argument("--charged-legs", type=int, default=2000)
+    ap.add_argument("--only-charged", action="store_true",
+                    help="only the charged-soft-sphere lines (cell bounding, then cell veto), one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel 10 times over BYTES (scales hardware byte counters)")
     return ap.parse_args()
@@ -259,6 +261,15 @@ def main():
     args = parse()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.only_charged:
+        from jellyfysh_amd import _lib
+        _lib.require_device()
+        out = {}
+        for far in ("cell_bounding", "cell_veto"):
+            log("charged soft spheres, far cells through %s ..." % far)
+            out[far] = charged_spheres_rate(args.charged_replicas, args.charged_legs, far=far)
+        print(json.dumps({"charged_soft_spheres": out}), flush=True)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
