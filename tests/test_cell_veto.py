@@ -189,3 +189,76 @@ def test_cell_veto_ini_is_the_workload_wiring():
     assert bytes(a) == bytes(b)
     assert cfg.taggers[0]["kind"] == "cell_veto" and cfg.taggers[0]["bounding_params"] == [1.5, 10.0, 1.0,
                                                                                            float("inf")]
+
+
+def _coulomb_atoms_ini():
+    import os
+    from jellyfysh_amd.config import RunConfig
+    return RunConfig.from_file(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                            "jellyfysh_amd", "configs", "coulomb_atoms_cell_veto.ini"))
+
+
+@pytest.mark.gpu
+def test_gpu_coulomb_atoms_ini_matches_oracle(device, oracle):
+    """The two-charge cell-veto INI with its random start: the device chain against the oracle,
+    event for event, after the input handler's words (rng_skip)."""
+    from jellyfysh_amd.chain import DeviceChain
+    cfg = _coulomb_atoms_ini()
+    pos, q, skip = cfg.initial_state(11)
+    legs = 4000
+    print("coulomb atoms with the cell veto from the INI, %d legs" % legs, flush=True)
+    got = DeviceChain(cfg.desc(2, 11, rng_skip=skip), pos, q).run(legs)
+    oc = S.oracle_config_from_wiring(3, 2, cfg.system_length, cfg.beta, cfg.cells_per_side, cfg.taggers, 11,
+                                     cfg.scheduler, oracle.MATH_CR, cfg.max_occupants)
+    oc.rng_skip = skip
+    want = oracle.Engine(oc, pos, q).run(legs)
+    assert len(got) == len(want) == legs
+    for f in ("tagger", "flags", "target", "ncand"):
+        bad = np.flatnonzero(got[f] != want[f])
+        assert bad.size == 0, "field %s differs first at leg %d" % (f, bad[0])
+
+
+@pytest.mark.gpu
+def test_gpu_cell_veto_samples_the_reference_distribution(device):
+    """Two +1 charges (coulomb_atoms_cell_veto.ini) in 256 replicas (seeds 0-255) advanced
+    together, each stopping at its own sampling events (every 0.56789): the empirical
+    cumulative distribution of the sampled separations, after 20 samples of burn-in per
+    replica, against the reference's reversible-Monte-Carlo data
+    (tests/golden/coulomb_atoms_reference_cdf.dat), as the reference's plotting script
+    compares its cell-veto run with it.  ~97 000 samples: the Kolmogorov distance of
+    independent samples would be ~0.005 (measured: 0.0032); the bound 0.012 leaves room for
+    their correlation."""
+    import os
+    from jellyfysh_amd.chain import DeviceChain, run_together
+    cfg = _coulomb_atoms_ini()
+    samp = [i for i, t in enumerate(cfg.taggers) if t["kind"] == "sampling"][0]
+    chains = []
+    for seed in range(256):
+        pos, q, skip = cfg.initial_state(seed)
+        c = DeviceChain(cfg.desc(2, seed, rng_skip=skip), pos, q)
+        c.set_stop_mask(1 << samp)
+        chains.append(c)
+    L = cfg.system_length
+    seps = []
+    print("256 replicas of two charges, 400 sampling stops", flush=True)
+    for it in range(400):
+        n, _ = run_together(chains, 200000)
+        assert np.all((n > 0) & (n < 200000))          # every replica stopped at a sampling event
+        if it < 20:
+            continue
+        for c in chains:
+            p = c.positions()
+            s = np.mod((p[1] - p[0]) + L / 2, L) - L / 2
+            seps.append(np.sqrt(np.sum(s * s)))
+        if it % 100 == 0:
+            print("stop %d" % it, flush=True)
+    for c in chains:
+        c.close()
+    ref = np.loadtxt(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                  "coulomb_atoms_reference_cdf.dat"))
+    seps = np.sort(np.array(seps))
+    F = np.searchsorted(seps, ref[:, 0], side="right") / len(seps)
+    ks = np.max(np.abs(F - ref[:, 1]))
+    print("samples %d, Kolmogorov distance to the reference data %.4f, median %.4f" %
+          (len(seps), ks, np.median(seps)), flush=True)
+    assert ks < 0.012
