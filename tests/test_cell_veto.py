@@ -191,11 +191,32 @@ def test_cell_veto_ini_is_the_workload_wiring():
                                                                                            float("inf")]
 
 
-def _coulomb_atoms_ini():
+def _coulomb_atoms_ini(far="cell_veto"):
+    """coulomb_atoms_cell_veto.ini, or with far="cell_bounding" the same run with the far cells
+    through the cell-bounding potential (estimator prefactor 1.5), as the reference's
+    coulomb_atoms/cell_bounded.ini differs from its cell_veto.ini."""
     import os
     from jellyfysh_amd.config import RunConfig
-    return RunConfig.from_file(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
-                                            "jellyfysh_amd", "configs", "coulomb_atoms_cell_veto.ini"))
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jellyfysh_amd", "configs",
+                        "coulomb_atoms_cell_veto.ini")
+    text = open(path).read()
+    if far == "cell_bounding":
+        text = text.replace("coulomb_cell_veto (cell_veto_tagger)",
+                            "coulomb_cell_bounding (cell_bounding_potential_tagger)")
+        text = text.replace("coulomb_cell_veto", "coulomb_cell_bounding").replace("[CoulombCellVeto]",
+                                                                               "[CoulombCellBounding]")
+        text = text.replace("event_handler = leaf_unit_cell_veto_event_handler",
+                            "event_handler = two_leaf_unit_cell_bounding_potential_event_handler\n"
+                            "number_event_handlers = 1")
+        text = text.replace("[LeafUnitCellVetoEventHandler]\nestimator = inner_point_estimator\n"
+                            "charge = electric_charge",
+                            "[TwoLeafUnitCellBoundingPotentialEventHandler]\n"
+                            "potential = merged_image_coulomb_potential\n"
+                            "bounding_potential = cell_bounding_potential\ncharge = electric_charge\n\n"
+                            "[CellBoundingPotential]\nestimator = inner_point_estimator")
+        text = text.replace("prefactor = 1.0\ntarget_charge = 1.0", "prefactor = 1.5\ntarget_charge = 1.0")
+        assert "veto" not in text.split("[Run]")[1]
+    return RunConfig.from_string(text)
 
 
 @pytest.mark.gpu
@@ -219,7 +240,8 @@ def test_gpu_coulomb_atoms_ini_matches_oracle(device, oracle):
 
 
 @pytest.mark.gpu
-def test_gpu_cell_veto_samples_the_reference_distribution(device):
+@pytest.mark.parametrize("far", ["cell_veto", "cell_bounding"])
+def test_gpu_coulomb_atoms_sample_the_reference_distribution(device, far):
     """Two +1 charges (coulomb_atoms_cell_veto.ini) in 256 replicas (seeds 0-255) advanced
     together, each stopping at its own sampling events (every 0.56789): the empirical
     cumulative distribution of the sampled separations, after 20 samples of burn-in per
@@ -230,7 +252,8 @@ def test_gpu_cell_veto_samples_the_reference_distribution(device):
     their correlation."""
     import os
     from jellyfysh_amd.chain import DeviceChain, run_together
-    cfg = _coulomb_atoms_ini()
+    cfg = _coulomb_atoms_ini(far)
+    assert cfg.taggers[0]["kind"] == far
     samp = [i for i, t in enumerate(cfg.taggers) if t["kind"] == "sampling"][0]
     chains = []
     for seed in range(256):
@@ -240,7 +263,7 @@ def test_gpu_cell_veto_samples_the_reference_distribution(device):
         chains.append(c)
     L = cfg.system_length
     seps = []
-    print("256 replicas of two charges, 400 sampling stops", flush=True)
+    print("256 replicas of two charges (far cells: %s), 400 sampling stops" % far, flush=True)
     for it in range(400):
         n, _ = run_together(chains, 200000)
         assert np.all((n > 0) & (n < 200000))          # every replica stopped at a sampling event
