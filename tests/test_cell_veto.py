@@ -194,12 +194,14 @@ def test_cell_veto_ini_is_the_workload_wiring():
 def _coulomb_atoms_ini(far="cell_veto"):
     """coulomb_atoms_cell_veto.ini, or with far="cell_bounding" the same run with the far cells
     through the cell-bounding potential (estimator prefactor 1.5), as the reference's
-    coulomb_atoms/cell_bounded.ini differs from its cell_veto.ini."""
+    coulomb_atoms/cell_bounded.ini differs from its cell_veto.ini; far="power_bounded" reads
+    coulomb_atoms_power_bounded.ini (the pair through the inverse-power Coulomb bound only)."""
     import os
     from jellyfysh_amd.config import RunConfig
-    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jellyfysh_amd", "configs",
-                        "coulomb_atoms_cell_veto.ini")
-    text = open(path).read()
+    configs = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jellyfysh_amd", "configs")
+    if far == "power_bounded":
+        return RunConfig.from_file(os.path.join(configs, "coulomb_atoms_power_bounded.ini"))
+    text = open(os.path.join(configs, "coulomb_atoms_cell_veto.ini")).read()
     if far == "cell_bounding":
         text = text.replace("coulomb_cell_veto (cell_veto_tagger)",
                             "coulomb_cell_bounding (cell_bounding_potential_tagger)")
@@ -240,7 +242,7 @@ def test_gpu_coulomb_atoms_ini_matches_oracle(device, oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("far", ["cell_veto", "cell_bounding"])
+@pytest.mark.parametrize("far", ["cell_veto", "cell_bounding", "power_bounded"])
 def test_gpu_coulomb_atoms_sample_the_reference_distribution(device, far):
     """Two +1 charges (coulomb_atoms_cell_veto.ini) in 256 replicas (seeds 0-255) advanced
     together, each stopping at its own sampling events (every 0.56789): the empirical
@@ -253,7 +255,7 @@ def test_gpu_coulomb_atoms_sample_the_reference_distribution(device, far):
     import os
     from jellyfysh_amd.chain import DeviceChain, run_together
     cfg = _coulomb_atoms_ini(far)
-    assert cfg.taggers[0]["kind"] == far
+    assert cfg.taggers[0]["kind"] == {"power_bounded": "pair_all"}.get(far, far)
     samp = [i for i, t in enumerate(cfg.taggers) if t["kind"] == "sampling"][0]
     chains = []
     for seed in range(256):
