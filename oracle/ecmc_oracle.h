@@ -112,6 +112,9 @@ void ora_cells_bounds(const ora_cells *c, int cell, double *cell_min, double *ce
 int ora_cells_position_to_cell(const ora_cells *c, const double *position);
 int ora_cells_nearby(const ora_cells *c, int cell, int *out); /* canonical order, returns count */
 int ora_cells_neighbor(const ora_cells *c, int cell, int direction, int positive);
+/* cuboid_periodic_cells.py:182-207: the cell holding the centre of `cell` shifted by the
+ * lower corner of `rel` */
+int ora_cells_translate(const ora_cells *c, int cell, int rel, double system_length);
 
 /* ---- cell bounding potential (cell_bounding_potential.py:70-126 with
  * inner_point_estimator.py:108-171) ---------------------------------------------------------

@@ -226,7 +226,7 @@ int32_t ora_walker_sample(const ora_walker *w, ora_mt *r) {
 
 /* cuboid_periodic_cells.py:182-207: the cell holding the centre of `cell` shifted by the
  * lower corner of `rel` (positions corrected into [0, L)) */
-static int cells_translate(const ora_cells *c, int cell, int rel, double L) {
+int ora_cells_translate(const ora_cells *c, int cell, int rel, double L) {
     double cmin[3], cmax[3], rmin[3], rmax[3], p[3];
     ora_cells_bounds(c, cell, cmin, cmax);
     ora_cells_bounds(c, rel, rmin, rmax);
@@ -597,7 +597,7 @@ static void create_tagger(ora_engine *e, int t) {
         const double row = e->cbt[t][(size_t)(rel * 3 + e->adir) * 2 + b];
         hd->cb_rate = (b ? -row : row) * cf;
         if (!(hd->cb_rate > 0.0)) { eng_fail(e, "cell veto: bounding event rate not positive"); return; }
-        hd->cb_cell = cells_translate(e->cells, acell, rel, e->L);
+        hd->cb_cell = ora_cells_translate(e->cells, acell, rel, e->L);
         const double dU = ora_expovariate(&e->rng, e->cfg.beta, e->cfg.math_mode);
         const double dt = dU / (total * e->aspeed);
         double q, r;

@@ -86,7 +86,7 @@ def lib():
             "ora_cells_create": (vp, [i32, d, ctypes.POINTER(i32), i32]), "ora_cells_destroy": (None, [vp]),
             "ora_cells_count": (i32, [vp]), "ora_cells_bounds": (None, [vp, i32, pd, pd]),
             "ora_cells_position_to_cell": (i32, [vp, pd]), "ora_cells_nearby": (i32, [vp, i32, ctypes.POINTER(i32)]),
-            "ora_cells_neighbor": (i32, [vp, i32, i32, i32]),
+            "ora_cells_neighbor": (i32, [vp, i32, i32, i32]), "ora_cells_translate": (i32, [vp, i32, i32, d]),
             "ora_cell_bound_table": (None, [vp, vp, d, d, i32, d, d, d, pd]),
             "ora_engine_create": (vp, [vp, pd, pd]), "ora_engine_destroy": (None, [vp]),
             "ora_engine_run": (i64, [vp, i64, vp]), "ora_engine_done": (i32, [vp]),

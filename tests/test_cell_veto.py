@@ -287,3 +287,25 @@ def test_gpu_coulomb_atoms_sample_the_reference_distribution(device, far):
     print("samples %d, Kolmogorov distance to the reference data %.4f, median %.4f" %
           (len(seps), ks, np.median(seps)), flush=True)
     assert ks < 0.012
+
+
+@pytest.mark.parametrize("cps, L", [((3, 5, 7), 1.0), ((8, 8, 8), 8.0), ((6, 6, 6), 17.2355)])
+def test_translate_is_the_identifier_sum(oracle, cps, L):
+    """The device places the sampled cell by adding per-axis cell identifiers modulo the cells
+    per side; the oracle restates the reference's translate (the cell centre shifted by the
+    relative cell's lower corner, corrected into the box, then position_to_cell).  Both agree
+    for every (cell, relative cell) pair."""
+    import ctypes
+    lib = oracle.lib()
+    c = lib.ora_cells_create(3, L, (ctypes.c_int32 * 3)(*cps), 1)
+    try:
+        n = lib.ora_cells_count(c)
+        cum = (1, cps[0], cps[0] * cps[1])
+        for cell in range(n):
+            a = [(cell // cum[d]) % cps[d] for d in range(3)]
+            for rel in range(n):
+                r = [(rel // cum[d]) % cps[d] for d in range(3)]
+                want = sum(((a[d] + r[d]) % cps[d]) * cum[d] for d in range(3))
+                assert lib.ora_cells_translate(c, cell, rel, L) == want, (cell, rel)
+    finally:
+        lib.ora_cells_destroy(c)
