@@ -29,7 +29,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "ECMC events/sec on 1 MI355X, batched vs plain, at matched pressure (2σ)"
 FLOPS_PER_INVERSION = 60.0     # FP64-equivalent operations of one inverse-power inversion
-PEAK_TFLOPS = 157.3            # MI355X dense FP32 peak
+PEAK_TFLOPS = 157.3            # MI355X FP32 vector peak
+PEAK_FP64_TFLOPS = 78.6       # MI355X FP64 vector peak
+PEAK_HBM_GBS = 8000.0         # MI355X HBM3E bandwidth
 
 
 def parse():
@@ -53,6 +55,7 @@ def parse():
     ap.add_argument("--charged-legs", type=int, default=2000)
     ap.add_argument("--only-charged", action="store_true",
                     help="only the charged-soft-sphere lines (cell bounding, then cell veto), one JSON line")
+    ap.add_argument("--only-plain", action="store_true", help="only the plain N = 4096 chain, one JSON line")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel 10 times over BYTES (scales hardware byte counters)")
     return ap.parse_args()
@@ -130,7 +133,8 @@ def plain_chain_rate(legs, pressure_seconds, equilibration_legs):
                     trash=["start_of_run"])]
     desc = make_desc(3, n, L, 1 / 1.5, int(L / 2.5), taggers, seed=5, scheduler="heap")
     chain = DeviceChain(desc, pos)
-    chain.run(2000, log=False)                      # warm-up
+    warm = chain.run(2000)                          # warm-up, logged: candidates per leg
+    ncand = float(warm["ncand"].mean())
     # the timed rate: one chunk of `legs` legs
     t0 = time.perf_counter()
     done = chain.run(legs, log=False)
@@ -139,6 +143,8 @@ def plain_chain_rate(legs, pressure_seconds, equilibration_legs):
     out = dict(n=n, rho=0.8, start="16^3 grid (spacing 1.07722, jitter 0.01) equilibrated by %d legs of the batched "
                                    "kernel (%.1f s)" % (done_eq, t_eq),
                events=int(done), kernel_ms=ms, wall_s=wall, events_per_sec=done / (ms * 1e-3))
+    out["roofline"] = plain_roofline("ecmc::chain_kernel<3>", ncand, 3, 27, done, ms, charged=False)
+    out["cpu_single_core"] = plain_cpu_single_core(desc, chain.positions(), seconds=4.0)
     # pressure: chunks of `legs` legs for `pressure_seconds`; the standard error comes from
     # 8 blocks of consecutive chunks (chunks of one chain are correlated)
     chunk_z, t_end = [], time.perf_counter() + pressure_seconds
@@ -156,6 +162,68 @@ def plain_chain_rate(legs, pressure_seconds, equilibration_legs):
                            err=float(blocks.std(ddof=1) / np.sqrt(nb)) if nb > 1 else None,
                            chunks=int(len(z)), legs_per_chunk=int(legs), blocks=int(nb))
     return out
+
+
+def plain_roofline(kernel, ncand, dim, cells, events, ms, charged):
+    """Algorithmic HBM bytes of the plain per-chain form: every candidate's position (dim x 8 B),
+    identifier (4 B) and, with charges, charge (8 B), the nearby-cell table (8 B per cell), the
+    active unit's write-back (dim x 8 B) and the 32 B event record, per committed leg; achieved =
+    that x legs / kernel time of the launch, against the HBM peak.  The kernel is bound by the
+    latency of one workgroup per chain, not by bandwidth, so the fraction is small."""
+    per_leg = ncand * (dim * 8 + 4 + (8 if charged else 0)) + cells * 8 + dim * 8 + 32
+    achieved = per_leg * events / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+            "kernel": kernel, "bytes_per_leg": per_leg, "candidates_per_leg": ncand, "legs_per_launch": int(events),
+            "launch_ms": ms, "traffic": None,
+            "note": "one workgroup per chain; latency-bound (per-leg dependent steps between barriers)"}
+
+
+def plain_cpu_single_core(desc, positions, seconds, charges=None):
+    """The oracle's plain engine (the reference's leg loop in FP64 with glibc math, heap
+    scheduler) on one host thread from the same configuration and start, for about `seconds`."""
+    from oracle import oracle as O
+    O.lib()
+    cfg = _oracle_config_of(desc)
+    e = O.Engine(cfg, positions, charges)
+    e.run(200, log=False)
+    legs, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        legs += e.run(500, log=False)
+    wall = time.perf_counter() - t0
+    return dict(value=legs / wall, unit="events/s", cores=1, kind="port",
+                sample="%d legs of the same chain (same start, same wiring) on one host thread, oracle in FP64 with "
+                       "glibc math" % legs)
+
+
+def _oracle_config_of(desc):
+    """The oracle's configuration of a device run description (same fields, same order)."""
+    from oracle import oracle as O
+    c = O.Config()
+    c.dim, c.n, c.system_length, c.beta = desc.dim, desc.n, desc.system_length, desc.beta
+    for d in range(3):
+        c.cells_per_side[d] = desc.cells_per_side[d]
+    c.neighbor_layers = desc.neighbor_layers
+    c.max_occupants = desc.max_occupants
+    c.scheduler = desc.scheduler
+    c.math_mode = O.MATH_LIBM
+    c.seed = desc.seed
+    c.rng_skip = desc.rng_skip
+    c.leaves_per_root = desc.leaves_per_root
+    c.n_taggers = desc.n_taggers
+    for i in range(desc.n_taggers):
+        g, h = c.taggers[i], desc.taggers[i]
+        for f in ("kind", "event_handler", "potential", "bounding_potential", "use_charge", "n_create", "n_trash",
+                  "n_handlers", "value", "ivalue", "ivalue2", "fmap_local"):
+            setattr(g, f, getattr(h, f))
+        for j in range(4):
+            g.par[j], g.bpar[j] = h.par[j], h.bpar[j]
+        for j in range(len(h.create)):
+            g.create[j], g.trash[j] = h.create[j], h.trash[j]
+        for k in range(len(h.fmap_n)):
+            g.fmap_n[k] = h.fmap_n[k]
+            for j in range(len(h.fmap_target[k])):
+                g.fmap_target[k][j] = h.fmap_target[k][j]
+    return c
 
 
 def batched_pressure_at(rho, z, rho0=0.8, width=0.02, n=None, n_min=256, n_max=512):
@@ -197,7 +265,19 @@ def charged_spheres_rate(replicas, legs, far="cell_bounding"):
     est = np.array([c.estimators() for c in chains])
     for c in chains:
         c.close()
+    # candidates per leg from a logged run of one more replica of the same system
+    from jellyfysh_amd.workloads import charged_soft_spheres, make_chain
+    sysd = charged_soft_spheres(seed=999, far=far)
+    probe = make_chain(sysd)
+    ncand = float(probe.run(500)["ncand"].mean())
+    probe.close()
+    roof = plain_roofline("ecmc::chain_many_kernel<3>", ncand, 3, 27, int(k.sum()), ms, charged=True)
+    from jellyfysh_amd.chain import make_desc
+    desc = make_desc(3, 512, sysd["system_length"], sysd["beta"], sysd["cells_per_side"], sysd["taggers"], 999,
+                     scheduler="heap", max_occupants=1)
+    cpu1 = plain_cpu_single_core(desc, sysd["positions"], seconds=4.0, charges=sysd["charges"])
     return dict(replicas=replicas, n=512, system_length=8.0, far_cells=far, legs_per_replica=legs,
+                roofline=roof, cpu_single_core=cpu1,
                 events_per_sec=float(k.sum()) / wall, kernel_ms=ms,
                 single_replica_events_per_sec=float(k1[0]) / (ms1 * 1e-3),
                 lifts_per_leg=float(est[:, 4].sum() / max(k.sum(), 1)),
@@ -271,6 +351,12 @@ def main():
             out[far] = charged_spheres_rate(args.charged_replicas, args.charged_legs, far=far)
         print(json.dumps({"charged_soft_spheres": out}), flush=True)
         return
+    if args.only_plain:
+        from jellyfysh_amd import _lib
+        _lib.require_device()
+        print(json.dumps({"plain": plain_chain_rate(args.plain_legs, args.plain_seconds, args.plain_equilibration)}),
+              flush=True)
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py measures one GPU: each replica runs independently, and a multi-GPU run would "
@@ -338,11 +424,14 @@ def main():
                    "replicas": int(len(n)), "units": int(np.sum(n)), "legs_per_step": args.legs,
                    "classes": {"max_n": [int(v) for v in batch.classes[0]],
                                "wavefronts_per_replica": [int(v) for v in batch.classes[1]]}},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS, "traffic": traffic,
+                     "peak_fp64": PEAK_FP64_TFLOPS, "frac_fp64": achieved / PEAK_FP64_TFLOPS,
                      "kernel": dom_name, "launch_ms": launch_ms, "launches": dom_launches,
                      "flop_per_launch": flops_per_launch,
-                     "note": "VALU kernel priced against the FP32 vector peak; algorithmic work = %g "
+                     "note": "VALU kernel (no MFMA: nothing here is a dense contraction) priced against the FP32 "
+                             "vector peak, the ceiling of its FP32 filter, and against the FP64 vector peak (frac_fp64), "
+                             "the ceiling of its exact inversions; algorithmic work = %g "
                              "FLOP per factor-candidate inversion (%d factors), exact FP64 inversions per "
                              "candidate = %.4f; traffic = FETCH_SIZE x 2 + WRITE_SIZE per launch from the "
                              "committed counter profile of this workload" % (FLOPS_PER_INVERSION, n_factors,

@@ -59,7 +59,11 @@ enum {
     ECMC_TAG_START_OF_RUN = 6,   /* NoInStateTagger + InitialChainStartOfRunEventHandler */
     ECMC_TAG_SURPLUS = 7,        /* SurplusCellsTagger: the surplus units of a bounded occupancy */
     ECMC_TAG_CELL_BOUNDING = 8,  /* CellBoundingPotentialTagger: occupants of the non-nearby cells */
-    ECMC_TAG_CELL_VETO = 9       /* CellVetoTagger (cell_veto_tagger.py:31-155): the active unit once */
+    ECMC_TAG_CELL_VETO = 9,      /* CellVetoTagger (cell_veto_tagger.py:31-155): the active unit once */
+    ECMC_TAG_FACTOR_MAP = 10     /* FactorTypeMapInStateTagger (factor_type_map_in_state_tagger.py:83-107) over
+                                    two-leaf factors: the leaf units that the active leaf unit meets through the
+                                    tagger's factor type map (activator/tagger/factor_type_maps.py:325-356), other
+                                    composite objects in identifier order, map entries in file order */
 };
 /* Event handlers of pair taggers */
 /* ECMC_EH_CELL_BOUNDING: TwoLeafUnitCellBoundingPotentialEventHandler
@@ -85,6 +89,8 @@ enum {
 enum { ECMC_SCHED_LIST = 0, ECMC_SCHED_HEAP = 1 };
 
 #define ECMC_MAX_TAGGERS 16
+#define ECMC_MAX_LEAVES 4        /* leaf units per composite point object */
+#define ECMC_MAX_FACTOR_MAP 8    /* factor type map entries per leaf number */
 
 /* One tagger of the TagActivator with its event handler (an INI section pair). */
 typedef struct {
@@ -98,11 +104,23 @@ typedef struct {
     int32_t n_handlers;                         /* number_event_handlers */
     double value;   /* sampling_interval | chain_time | end_of_run_time | speed */
     int32_t ivalue; /* first_event_time_zero | initial_direction_of_motion */
-    int32_t ivalue2;/* initial_active_identifier */
+    int32_t ivalue2;/* initial_active_identifier (leaf units: root * leaves_per_root + leaf) */
+    /* ECMC_TAG_FACTOR_MAP: the factor type map of the tagger's factor for two composite objects
+     * of leaves_per_root leaves each (factor_type_maps.py:37-88): for the active unit's leaf
+     * number k, fmap_n[k] two-leaf factors whose other leaf has the number fmap_target[k][e] in
+     * the two-object numbering (0 .. leaves_per_root - 1 this object, the rest the other one);
+     * fmap_local: the factor lies within one composite object */
+    int32_t fmap_local;
+    int32_t fmap_n[ECMC_MAX_LEAVES];
+    int32_t fmap_target[ECMC_MAX_LEAVES][ECMC_MAX_FACTOR_MAP];
 } ecmc_tagger_desc;
 
 /* A run configuration: HypercubicSetting + CuboidPeriodicCells + SingleActiveCellOccupancy
- * + scheduler + taggers, and the seed of the random stream (random.seed(seed)).  rng_skip
+ * + scheduler + taggers, and the seed of the random stream (random.seed(seed)).  With composite
+ * point objects (leaves_per_root > 1, TreeStateHandler with two node levels) the units are the
+ * leaf units, which alone carry velocities here; the new active unit of an end of chain is then
+ * randint(root) followed by randint(leaf)
+ * (single_independent_active_periodic_direction_end_of_chain_event_handler.py:176-237).  rng_skip
  * words of that stream are drawn before the run starts, as a random input handler draws the
  * initial positions from it (random_input_handler.py:79-92: two words per coordinate). */
 typedef struct {
@@ -113,6 +131,8 @@ typedef struct {
     int32_t max_occupants;   /* <= 0: unbounded */
     int32_t scheduler;
     int32_t rng_skip;
+    int32_t leaves_per_root; /* leaf units per composite point object (0 or 1: point masses); unit
+                                root * leaves_per_root + leaf, n a multiple of it */
     uint64_t seed;
     int32_t n_taggers;
     ecmc_tagger_desc taggers[ECMC_MAX_TAGGERS];
@@ -167,6 +187,16 @@ int ecmc_system_last_kernel_ms(const ecmc_system *sys, double *ms);
  * time every system then reports.  Replaces, for a set of independent runs, one
  * SingleProcessMediator.run per system (mediator/single_process_mediator.py:84-156). */
 int ecmc_systems_run(ecmc_system *const *systems, int32_t count, int64_t max_events, int64_t *n_committed);
+/* ecmc_systems_run without any allocation (no hipMalloc, hipFree or other allocating call): the
+ * launch arguments go into a fixed device array of the library, sized for
+ * ECMC_SYSTEMS_NOALLOC_MAX systems (512 B each; every GPU the library runs on holds its own
+ * copy until the process ends), staged through a fixed host array.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  count above the maximum is rejected with ECMC_ERR_ARGUMENT; the
+ * systems must live on the current device.  Calls that overlap on one GPU share the array and
+ * would race: the library serialises the calls of one process with a lock. */
+#define ECMC_SYSTEMS_NOALLOC_MAX 1024
+int ecmc_systems_run_noalloc(ecmc_system *const *systems, int32_t count, int64_t max_events, int64_t *n_committed);
 
 /* ---- batched independent replicas (no reference counterpart as a whole) ------------------
  * Each replica is a reference chain (straight chains of chain_length cycling +x,+y,+z,

@@ -12,8 +12,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjellyfysh_amd.so")
 
 MAX_TAGGERS = 16
+MAX_LEAVES = 4
+MAX_FACTOR_MAP = 8
+SYSTEMS_NOALLOC_MAX = 1024   # ECMC_SYSTEMS_NOALLOC_MAX
 (TAG_PAIR_CELLS, TAG_PAIR_ALL, TAG_CELL_BOUNDARY, TAG_SAMPLING, TAG_END_OF_CHAIN, TAG_END_OF_RUN,
- TAG_START_OF_RUN, TAG_SURPLUS, TAG_CELL_BOUNDING, TAG_CELL_VETO) = range(10)
+ TAG_START_OF_RUN, TAG_SURPLUS, TAG_CELL_BOUNDING, TAG_CELL_VETO, TAG_FACTOR_MAP) = range(11)
 EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING, EH_CELL_BOUNDING, EH_CELL_VETO = 0, 1, 2, 3
 (POT_HARD_SPHERE, POT_INVERSE_POWER, POT_LENNARD_JONES, POT_INVERSE_POWER_TRUNC, POT_MERGED_IMAGE_COULOMB,
  POT_INVERSE_POWER_COULOMB_BOUND, POT_DISPLACED_EVEN_POWER, POT_CELL_BOUNDING) = range(8)
@@ -36,14 +39,16 @@ class TaggerDesc(ctypes.Structure):
                 ("bounding_potential", ctypes.c_int32), ("par", ctypes.c_double * 4), ("bpar", ctypes.c_double * 4),
                 ("use_charge", ctypes.c_int32), ("n_create", ctypes.c_int32), ("create", ctypes.c_int32 * MAX_TAGGERS),
                 ("n_trash", ctypes.c_int32), ("trash", ctypes.c_int32 * MAX_TAGGERS), ("n_handlers", ctypes.c_int32),
-                ("value", ctypes.c_double), ("ivalue", ctypes.c_int32), ("ivalue2", ctypes.c_int32)]
+                ("value", ctypes.c_double), ("ivalue", ctypes.c_int32), ("ivalue2", ctypes.c_int32),
+                ("fmap_local", ctypes.c_int32), ("fmap_n", ctypes.c_int32 * MAX_LEAVES),
+                ("fmap_target", (ctypes.c_int32 * MAX_FACTOR_MAP) * MAX_LEAVES)]
 
 
 class SystemDesc(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("n", ctypes.c_int32), ("system_length", ctypes.c_double),
                 ("beta", ctypes.c_double), ("cells_per_side", ctypes.c_int32 * 3), ("neighbor_layers", ctypes.c_int32),
                 ("max_occupants", ctypes.c_int32), ("scheduler", ctypes.c_int32), ("rng_skip", ctypes.c_int32),
-                ("seed", ctypes.c_uint64), ("n_taggers", ctypes.c_int32), ("taggers", TaggerDesc * MAX_TAGGERS)]
+                ("leaves_per_root", ctypes.c_int32), ("seed", ctypes.c_uint64), ("n_taggers", ctypes.c_int32), ("taggers", TaggerDesc * MAX_TAGGERS)]
 
 
 SCOPE_ALL, SCOPE_INTER, SCOPE_INTRA = 0, 1, 2
@@ -79,6 +84,7 @@ def _signatures():
         "ecmc_system_rng_words": (i32, [vp, pi64]),
         "ecmc_system_last_kernel_ms": (i32, [vp, pd]),
         "ecmc_systems_run": (i32, [vp, i32, i64, pi64]),
+        "ecmc_systems_run_noalloc": (i32, [vp, i32, i64, pi64]),
         "ecmc_pair_displacement": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, vp, d, vp, vp]),
         "ecmc_pair_derivative": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, d, vp, vp]),
         "ecmc_math_log": (i32, [i64, vp, vp, vp]),

@@ -14,8 +14,8 @@ from . import _lib as L
 _KINDS = {"pair_cells": L.TAG_PAIR_CELLS, "pair_all": L.TAG_PAIR_ALL, "cell_boundary": L.TAG_CELL_BOUNDARY,
           "sampling": L.TAG_SAMPLING, "end_of_chain": L.TAG_END_OF_CHAIN, "end_of_run": L.TAG_END_OF_RUN,
           "start_of_run": L.TAG_START_OF_RUN, "surplus": L.TAG_SURPLUS, "cell_bounding": L.TAG_CELL_BOUNDING,
-          "cell_veto": L.TAG_CELL_VETO}
-_PAIR_KINDS = ("pair_cells", "pair_all", "surplus", "cell_bounding", "cell_veto")
+          "cell_veto": L.TAG_CELL_VETO, "factor_map": L.TAG_FACTOR_MAP}
+_PAIR_KINDS = ("pair_cells", "pair_all", "surplus", "cell_bounding", "cell_veto", "factor_map")
 # InnerPointEstimator defaults (inner_point_estimator.py:45-46): prefactor, points_per_side,
 # target_charge, empirical_bound
 CELL_BOUNDING_DEFAULTS = (1.5, 10.0, 1.0, float("inf"))
@@ -33,8 +33,23 @@ def potential_code(name):
         raise ValueError("unknown potential %r" % (name,))
 
 
+def fill_factor_map(g, factor_map, leaves_per_root):
+    """The factor type map of one tagger (``{"local": bool, "targets": [[leaf numbers] per leaf
+    number of the active unit]}``, two-object numbering) into an ecmc_tagger_desc / ora_tagger."""
+    targets = factor_map["targets"]
+    if len(targets) > L.MAX_LEAVES or len(targets) != leaves_per_root:
+        raise ValueError("a factor type map lists the entries of each of the %d leaf numbers" % leaves_per_root)
+    g.fmap_local = 1 if factor_map["local"] else 0
+    for k, lst in enumerate(targets):
+        if len(lst) > L.MAX_FACTOR_MAP:
+            raise ValueError("at most %d factor type map entries per leaf number" % L.MAX_FACTOR_MAP)
+        g.fmap_n[k] = len(lst)
+        for e, tl in enumerate(lst):
+            g.fmap_target[k][e] = int(tl)
+
+
 def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, scheduler="heap", neighbor_layers=1,
-              max_occupants=-1, rng_skip=0):
+              max_occupants=-1, rng_skip=0, leaves_per_root=1):
     """Fill an ecmc_system_desc.
 
     Each tagger is a dict with ``tag``, ``kind`` (pair_cells, pair_all, cell_boundary,
@@ -45,9 +60,12 @@ def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, schedu
     ``first_event_time_zero``; ``chain_time``; ``end_of_run_time``;
     ``initial_direction_of_motion``, ``speed``, ``initial_active_identifier``.  ``rng_skip``
     words of the random stream are drawn before the run (a random input handler's draws).
+    Composite point objects: ``leaves_per_root`` leaf units per object (unit root * leaves + leaf),
+    pair taggers of kind ``factor_map`` with a ``factor_map`` (see fill_factor_map).
     """
     d = L.SystemDesc()
     d.rng_skip = int(rng_skip)
+    d.leaves_per_root = int(leaves_per_root)
     d.dim, d.n, d.system_length, d.beta = int(dim), int(n), float(system_length), float(beta)
     cps = list(cells_per_side) if np.ndim(cells_per_side) else [int(cells_per_side)]
     for i in range(3):
@@ -91,6 +109,8 @@ def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, schedu
                 for j, v in enumerate(t.get("bounding_params", ())):
                     g.bpar[j] = float(v)
             g.use_charge = 1 if t.get("charge") else 0
+            if t["kind"] == "factor_map":
+                fill_factor_map(g, t["factor_map"], int(leaves_per_root))
         elif t["kind"] == "sampling":
             g.value = float(t["sampling_interval"])
             g.ivalue = 1 if t.get("first_event_time_zero") else 0
@@ -109,12 +129,16 @@ class DeviceChain:
     """Device-resident state of one chain plus its event loop (ecmc_system_*)."""
 
     def __init__(self, desc, positions, charges=None):
-        L.require_device()
         self.desc = desc
         pos = np.ascontiguousarray(np.asarray(positions, dtype=np.float64))
         if pos.shape != (desc.n, desc.dim):
             raise ValueError("positions must have shape (n, dim) = (%d, %d)" % (desc.n, desc.dim))
         self._charges = None if charges is None else np.ascontiguousarray(np.asarray(charges, dtype=np.float64))
+        if self._charges is not None and self._charges.shape != (desc.n,):
+            raise ValueError("charges must have shape (n,) = (%d,)" % desc.n)
+        if self._charges is None and any(desc.taggers[t].use_charge for t in range(desc.n_taggers)):
+            raise ValueError("a tagger multiplies by the charges (charge = ...), but no charges were given")
+        L.require_device()
         h = ctypes.c_void_p()
         pd = ctypes.POINTER(ctypes.c_double)
         L.check(L.lib().ecmc_system_create(ctypes.byref(desc), pos.ctypes.data_as(pd),
@@ -168,13 +192,16 @@ class DeviceChain:
         return ms.value
 
 
-def run_together(chains, max_events):
+def run_together(chains, max_events, noalloc=False):
     """Advance independent chains in one launch (ecmc_systems_run): one workgroup per chain,
     each committing up to max_events legs exactly as DeviceChain.run would (no event logs).
-    Returns the committed legs per chain and the launch's kernel time in milliseconds."""
+    With noalloc, ecmc_systems_run_noalloc launches them without allocating device memory (at
+    most SYSTEMS_NOALLOC_MAX chains).  Returns the committed legs per chain and the launch's
+    kernel time in milliseconds."""
     if not chains:
         raise ValueError("no chains")
     hs = (ctypes.c_void_p * len(chains))(*[c._h.value for c in chains])
     n = np.zeros(len(chains), dtype=np.int64)
-    L.check(L.lib().ecmc_systems_run(hs, len(chains), int(max_events), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+    fn = L.lib().ecmc_systems_run_noalloc if noalloc else L.lib().ecmc_systems_run
+    L.check(fn(hs, len(chains), int(max_events), n.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
     return n, chains[0].last_kernel_ms()

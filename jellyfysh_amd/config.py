@@ -8,7 +8,8 @@ comma-separated values.  It accepts the single-process run
 SingleActiveCellOccupancy over CuboidPeriodicCells, the heap or list scheduler and these
 taggers / event handlers / potentials:
 
-  taggers        excluded_cells_tagger, factor_type_map_in_state_tagger (every pair),
+  taggers        excluded_cells_tagger, factor_type_map_in_state_tagger (point masses: every
+                 pair; composite objects: the pairs of its factor type map file),
                  surplus_cells_tagger, cell_bounding_potential_tagger, cell_veto_tagger,
                  cell_boundary_tagger, no_in_state_tagger, active_global_state_in_state_tagger
   handlers       two_leaf_unit_event_handler, two_leaf_unit_bounding_potential_event_handler,
@@ -24,20 +25,25 @@ taggers / event handlers / potentials:
                  (no reference counterpart: an inverse power truncated and shifted at a cutoff)
 
 Anything else raises ConfigurationError, as the reference's factory does for options a
-class does not take.  The initial configuration is passed in by the caller (the reference's
-input handlers are not part of this path); ``seed`` is the seed of the random stream.
+class does not take.  The initial configuration comes from the file's input handler (random
+placement of point masses or dipoles from the run's random stream, or a .pdb file) or is passed
+in by the caller; ``seed`` is the seed of the random stream.  Composite point objects (the
+dipole random node creator) run as leaf units, unit root * leaves + leaf, with their
+factor_type_map_in_state_tagger's factor type map read from its file (two-leaf factors).
 
 ``SingleProcessMediator(config, positions, charges=None, seed=0).run(on_sample=None)``
 then runs the whole leg loop on the device until the end-of-run event; at each sampling
 event (fixed_interval_sampling_event_handler) it hands the positions to ``on_sample``.
 """
 import configparser
+import os
 import re
 
 import numpy as np
 
 from .chain import DeviceChain, make_desc
-from .input_handlers import AtomRandomNodeCreator, ChargeValues, PdbInputHandler, RandomInputHandler
+from .input_handlers import (AtomRandomNodeCreator, ChargeValues, DipoleRandomNodeCreator, PdbInputHandler,
+                             RandomInputHandler)
 from .output import ConfigurationError, make_output_handler
 
 
@@ -101,7 +107,44 @@ _HANDLER_OPTIONS = {
 }
 
 _TAGGER_OPTIONS = {"create", "trash", "event_handler", "number_event_handlers", "internal_state_label",
-                   "factor_type_maps", "tag"}
+                   "factor_type_maps", "factor_type_maps_label", "tag"}
+
+
+def read_factor_type_maps(path, leaves_per_root):
+    """activator/tagger/factor_type_maps.py:160-213: lines '[i, j, ...], FactorName' (single-line
+    comments start with '#'); leaf numbers below leaves_per_root belong to one composite object,
+    the others to a second one.  A factor is local when all its numbers lie in one object.  Returns
+    {FactorName: {"local": bool, "targets": per leaf number k, the other leaf numbers of the
+    two-leaf factors holding k, in file order}}."""
+    P = int(leaves_per_root)
+    line_re = re.compile(r"(\[(?:[0-9]+, )*[0-9]+\]), ((?:[A-Z][a-z]*)+)")
+    out = {}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#"):
+                continue
+            m = line_re.match(line)
+            if m is None:
+                raise ConfigurationError("Line %r does not match pattern '[index, index, ...], Factor'" % line)
+            idx = [int(v) for v in re.findall(r"[0-9]+", m.group(1))]
+            if any(i >= 2 * P for i in idx):
+                raise ConfigurationError("point mass index too large for two composite objects of %d leaves" % P)
+            local = all(i < P for i in idx)
+            entry = out.setdefault(m.group(2), {"local": local, "targets": [[] for _ in range(P)]})
+            if entry["local"] != local:
+                raise ConfigurationError("factor %s mixes local and non-local lines" % m.group(2))
+            if len(idx) != 2 or idx[0] == idx[1]:
+                raise ConfigurationError("only factors of two leaf units run on the device (%r)" % line.strip())
+            for k in idx:
+                if k < P:
+                    entry["targets"][k].append(idx[1] if idx[0] == k else idx[0])
+    return out
+
+
+def all_leaf_factor_type_map(leaves_per_root):
+    """factor_type_maps.py:359-394: every leaf of every other composite object."""
+    P = int(leaves_per_root)
+    return {"local": False, "targets": [list(range(P, 2 * P)) for _ in range(P)]}
 
 
 class RunConfig:
@@ -129,6 +172,8 @@ class RunConfig:
         # output handlers (input_output_handler/input_output_handler.py:43-61): alias -> (class, options)
         self.output_handlers = {}
         self.input_handler = None
+        self.leaves_per_root = 1
+        self._path = getattr(parser, "_ecmc_path", None)
         if "input_output_handler" in m:
             io = self._section(camel(split_alias(m["input_output_handler"])[0]), {"output_handlers", "input_handler"})
             for item in split_list(io.get("output_handlers", "")):
@@ -137,6 +182,9 @@ class RunConfig:
                 self.output_handlers[alias] = (cls, opts)
             if io.get("input_handler"):
                 self.input_handler = self._input_handler(io["input_handler"])
+                creator = getattr(self.input_handler, "random_node_creator", None)
+                if creator is not None:
+                    self.leaves_per_root = creator.number_of_nodes_per_root_node
         act = split_alias(m.get("activator", "tag_activator"))[1]
         if act != "tag_activator":
             raise ConfigurationError("only the tag activator is supported (got %s)" % act)
@@ -186,11 +234,18 @@ class RunConfig:
         if cls == "random_input_handler":
             h = self._section(camel(alias), {"random_node_creator", "number_of_root_nodes"})
             calias, ccls = split_alias(h["random_node_creator"])
-            if ccls != "atom_random_node_creator":
-                raise ConfigurationError("only point masses (atom_random_node_creator) run on the device (got %s)" % ccls)
-            c = self._section(camel(calias), {"charge_values"})
-            return RandomInputHandler(int(h["number_of_root_nodes"]),
-                                      AtomRandomNodeCreator(self._charge_values(c.get("charge_values"))))
+            if ccls == "atom_random_node_creator":
+                c = self._section(camel(calias), {"charge_values"})
+                creator = AtomRandomNodeCreator(self._charge_values(c.get("charge_values")))
+            elif ccls == "dipole_random_node_creator":
+                c = self._section(camel(calias), {"charge_values", "min_initial_dipole_separation",
+                                                  "max_initial_dipole_separation"})
+                creator = DipoleRandomNodeCreator(float(c.get("min_initial_dipole_separation", 0.0)),
+                                                  float(c.get("max_initial_dipole_separation", 0.05)),
+                                                  self._charge_values(c.get("charge_values")))
+            else:
+                raise ConfigurationError("random node creator %s has no device counterpart" % ccls)
+            return RandomInputHandler(int(h["number_of_root_nodes"]), creator)
         if cls == "pdb_input_handler":
             h = self._section(camel(alias), {"filename", "charge_values"})
             return PdbInputHandler(h["filename"], self._charge_values(h.get("charge_values")), self.dim,
@@ -210,6 +265,10 @@ class RunConfig:
             pos, charges = self.input_handler.read_point_masses()
             words = 0
         q = charges.get("electric_charge", next(iter(charges.values()), None)) if charges else None
+        # composite objects: leaf units in identifier order, unit root * leaves + leaf
+        pos = np.asarray(pos, dtype=np.float64).reshape(-1, self.dim)
+        if q is not None:
+            q = np.asarray(q, dtype=np.float64).reshape(-1)
         return pos, q, words
 
     # ---- helpers ---------------------------------------------------------------------------
@@ -293,6 +352,9 @@ class RunConfig:
         elif hcls in ("two_leaf_unit_event_handler", "two_leaf_unit_bounding_potential_event_handler"):
             if cls == "excluded_cells_tagger":
                 t["kind"] = "pair_cells"
+            elif cls == "factor_type_map_in_state_tagger" and self.leaves_per_root > 1:
+                t["kind"] = "factor_map"
+                t["factor_map"] = self._factor_type_map(sec, label)
             elif cls == "factor_type_map_in_state_tagger":
                 t["kind"] = "pair_all"
             elif cls == "surplus_cells_tagger":
@@ -321,8 +383,39 @@ class RunConfig:
             t["kind"] = "start_of_run"
             t["initial_direction_of_motion"] = int(h.get("initial_direction_of_motion", 0))
             t["speed"] = float(h.get("speed", 1.0))
-            t["initial_active_identifier"] = int(h.get("initial_active_identifier", 0))
+            ident = [int(v) for v in split_list(h.get("initial_active_identifier", "0"))]
+            if len(ident) == 2 and self.leaves_per_root > 1:
+                ident = [ident[0] * self.leaves_per_root + ident[1]]
+            if len(ident) != 1:
+                raise ConfigurationError("initial_active_identifier %r does not name a leaf unit"
+                                         % h.get("initial_active_identifier"))
+            t["initial_active_identifier"] = ident[0]
         return t
+
+    def _factor_type_map(self, sec, label):
+        """The factor type map of a factor_type_map_in_state_tagger: its factor_type_maps section's
+        file, the entry named by factor_type_maps_label (or the tag) in CamelCase, or every leaf of
+        every other composite object when the file has no such factor."""
+        import os
+        if "factor_type_maps" not in sec:
+            raise ConfigurationError("factor_type_map_in_state_tagger needs factor_type_maps")
+        alias, cls = split_alias(sec["factor_type_maps"])
+        if cls != "factor_type_maps":
+            raise ConfigurationError("unsupported factor type maps %s" % cls)
+        fname = self._section(camel(alias), {"filename"}).get("filename")
+        if not fname:
+            raise ConfigurationError("section [%s] needs a filename" % camel(alias))
+        here = os.path.dirname(os.path.abspath(__file__))
+        candidates = [fname]
+        if self._path:
+            candidates.append(os.path.join(os.path.dirname(self._path), fname))
+        candidates.append(os.path.join(here, "configs", "factor_set_files", os.path.basename(fname)))
+        path = next((c for c in candidates if os.path.exists(c)), None)
+        if path is None:
+            raise ConfigurationError("factor type map file %s not found" % fname)
+        maps = read_factor_type_maps(path, self.leaves_per_root)
+        name = camel(sec.get("factor_type_maps_label", label))
+        return maps.get(name, all_leaf_factor_type_map(self.leaves_per_root))
 
     # ---- construction ----------------------------------------------------------------------
     @classmethod
@@ -336,13 +429,14 @@ class RunConfig:
         cp = configparser.ConfigParser()
         with open(path) as f:
             cp.read_file(f)
+        cp._ecmc_path = os.path.abspath(path)
         return cls(cp)
 
     def desc(self, n, seed=0, rng_skip=0):
         cps = self.cells_per_side if self.cells_per_side is not None else 1
         return make_desc(self.dim, n, self.system_length, self.beta, cps, self.taggers, seed,
                          scheduler=self.scheduler, neighbor_layers=self.neighbor_layers,
-                         max_occupants=self.max_occupants, rng_skip=rng_skip)
+                         max_occupants=self.max_occupants, rng_skip=rng_skip, leaves_per_root=self.leaves_per_root)
 
 
 class SingleProcessMediator:
@@ -361,7 +455,7 @@ class SingleProcessMediator:
         pos = np.asarray(positions, dtype=np.float64)
         self.chain = DeviceChain(config.desc(pos.shape[0], seed, rng_skip=skip), pos, charges)
         self.sampling = [i for i, t in enumerate(config.taggers) if t["kind"] == "sampling"]
-        self.output_handlers = {alias: make_output_handler(cls, opts)
+        self.output_handlers = {alias: make_output_handler(cls, opts, config.leaves_per_root)
                                 for alias, (cls, opts) in config.output_handlers.items()}
         # sampling tagger -> the output handler its event handler writes to
         self.writers = {i: self.output_handlers[config.taggers[i]["output_handler"]] for i in self.sampling

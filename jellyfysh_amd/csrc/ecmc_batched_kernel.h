@@ -527,6 +527,7 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
     int32_t *queue = cont_id + B_CONTENDERS + wave * B_QUEUE;
     __shared__ BKey sk[B_MAX_WPR];
     __shared__ int s_item, s_chunk, s_nc, s_overflow, s_nearby[27], s_pref[28], s_M, s_Mfull;
+    __shared__ int s_preff[28];   // audit mode: prefix sums over every unit of the 27 cells (none culled)
     __shared__ double s_ap[3], s_newpos[3 * B_MAX_WPR], s_h, s_cb, s_bounds[B_MAX_CPS + 1];
     __shared__ ReplicaDev R;
     __shared__ unsigned long long s_geo;   // geometric survivors of the other wavefronts (several per replica)
@@ -671,11 +672,12 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
         // list (their count is taken as zero): the box distance from the segment [0, h] (h
         // widened to cover hf) must be at least rc + 2E rounded up, plus 2E for the position
         // slack of the fixed-point test and L 2^-18 for the FP32 box bounds (each rounded once
-        // from an FP64 difference below L, then a few FP32 operations).  Not in the audit mode
-        // (which checks every candidate) nor where the ambiguity tests are on.
-        // (the radius is recomputed in each call from the cutoff in LDS, so that it does not
-        // hold a register through the leg)
-        const bool cull = A.mode != 2 && !check_amb;
+        // from an FP64 difference below L, then a few FP32 operations).  Not in the all-FP64
+        // mode (which inverts every candidate of the 27 cells) nor where the ambiguity tests
+        // are on; the audit mode keeps it and checks every candidate of the 27 cells, culled
+        // or not.  (The radius is recomputed in each call from the cutoff in LDS, so that it
+        // does not hold a register through the leg.)
+        const bool cull = A.mode != 1 && !check_amb;
         // the estimators, summed leg by leg in the oracle's order (wavefront 0)
         double dest[6] = {R.est[0], R.est[1], R.est[2], R.est[3], R.est[4], R.est[5]};
         unsigned long long dst[4] = {0ull, 0ull, 0ull, 0ull};  // statistics increments (wavefront 0)
@@ -746,6 +748,10 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 }
                 if (lane < 27) s_pref[lane + 1] = incl;
                 if (lane == 0) s_pref[0] = 0;
+                if (A.mode == 2) {
+                    if (lane < 27) s_preff[lane + 1] = inclf;
+                    if (lane == 0) s_preff[0] = 0;
+                }
                 Mv = __builtin_amdgcn_readlane(incl, 26);
                 Mf = __builtin_amdgcn_readlane(inclf, 26);
             }
@@ -914,12 +920,15 @@ __global__ __launch_bounds__(64 * WPR, ECMC_BATCHED_WAVES_PER_SIMD) void batched
                 }
             }
             if (A.mode == 2) {
-                // audit: a candidate whose exact displacement is <= h must be a contender (before
-                // the selection's barrier: after it, wavefront 0 already prepares the next leg)
-                for (int idx = tid; idx < M * nf; idx += NTH) {
+                // audit: every candidate of the 27 cells (or of the whole box), including those in
+                // cells culled from the leg's list, whose exact displacement is <= h must be a
+                // contender (before the selection's barrier: after it, wavefront 0 already
+                // prepares the next leg)
+                const int Ma = whole_box ? M : s_Mfull;
+                for (int idx = tid; idx < Ma * nf; idx += NTH) {
                     const int f = idx % nf;
                     const int ci = idx / nf;
-                    const int j = whole_box ? (ci < a ? ci : ci + 1) : candidate_unit(ci, cps, a, s_pref, s_nearby, cu);
+                    const int j = whole_box ? (ci < a ? ci : ci + 1) : candidate_unit(ci, cps, a, s_preff, s_nearby, cu);
                     uint32_t ww[4];
                     philox4x32_10((uint32_t)leg, (uint32_t)(leg >> 32), (uint32_t)j, 0u, (uint32_t)r, A.seed_lo, ww);
                     const double dx = exact_event(pick(f, ww[0], ww[2]), pick(f, ww[1], ww[3]), A.beta,

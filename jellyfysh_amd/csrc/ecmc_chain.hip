@@ -83,7 +83,8 @@ struct ChainState {
 
 struct ChainArgs {
     int32_t n, ncells, nb_stride, occ_cap, ring_words, n_taggers, stop_mask, ewald_fc;
-    int32_t pad0, pad1;
+    int32_t leaves;            // leaf units per composite object (1: point masses)
+    int32_t occupancy;         // some tagger reads the cell occupancy (else it is not kept up)
     double L, half, beta;
     int32_t cps[3], cum[3];
     double side[3];
@@ -284,6 +285,23 @@ __device__ double pair_dt(const ChainArgs &a, const ecmc_tagger_desc &tg, int di
     return ipcb_displacement(tg.bpar[0] * c1 * c2, p[0], p[1], p[2], dU, a.L) / speed;
 }
 
+// Candidate j of a factor-map tagger (factor_type_maps.py:325-356): a local factor's entries for
+// the active leaf number k; a non-local factor's entries for every other composite object in
+// identifier order, the entry's other leaf in that object (or, for numbers below P, in the
+// active unit's own object).
+__device__ __forceinline__ int fmap_target(const ecmc_tagger_desc &tg, int P, int act, int j) {
+    const int m = act / P, k = act % P, fn = tg.fmap_n[k];
+    if (tg.fmap_local) return m * P + tg.fmap_target[k][j];
+    const int o = j / fn, e = j - o * fn;
+    const int other = o < m ? o : o + 1;
+    const int tl = tg.fmap_target[k][e];
+    return tl < P ? m * P + tl : other * P + (tl - P);
+}
+__device__ __forceinline__ int fmap_count(const ecmc_tagger_desc &tg, int P, int n, int act) {
+    const int fn = tg.fmap_n[act % P];
+    return tg.fmap_local ? fn : (n / P - 1) * fn;
+}
+
 struct Best {
     double q, r;
     int32_t j;
@@ -351,6 +369,7 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
     const bool all = tg.kind == ECMC_TAG_PAIR_ALL;
     const bool sur = tg.kind == ECMC_TAG_SURPLUS;
     const bool far = tg.kind == ECMC_TAG_CELL_BOUNDING;
+    const bool fmap = tg.kind == ECMC_TAG_FACTOR_MAP;
     if (far) {
         // cell_bounding_potential_tagger.py:150-155: the occupied cells that are not nearby the
         // active cell, in cell order; each threads scans a contiguous run of cells, an
@@ -383,7 +402,8 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
         if (multi) st.err = ECMC_ERR_STATE;   // the handler takes exactly one target unit
         __syncthreads();
     }
-    const int M = all ? a.n - 1 : (sur ? st.nsur : (far ? scan[NT] : nb_pref[nc]));
+    const int M = fmap ? (st.active < 0 ? 0 : fmap_count(tg, a.leaves, a.n, st.active))
+                       : (all ? a.n - 1 : (sur ? st.nsur : (far ? scan[NT] : nb_pref[nc])));
     const bool draws = pot_draws(tg);
     ring_ensure(st, mt, a, draws ? 2ull * (uint64_t)M : 0ull);
     const uint64_t base = st.cursor;
@@ -398,7 +418,9 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
     uint32_t hsum = 0;
     for (int j = tid; j < M; j += NT) {
         int target;
-        if (all) {
+        if (fmap) {
+            target = fmap_target(tg, a.leaves, act, j);
+        } else if (all) {
             target = j < act ? j : j + 1;
         } else if (sur) {
             target = a.sur_id[j];
@@ -444,7 +466,9 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
             st.err = ECMC_ERR_NUMERIC;
         } else {
             int j = best.j, target;
-            if (all) {
+            if (fmap) {
+                target = fmap_target(tg, a.leaves, act, j);
+            } else if (all) {
                 target = j < act ? j : j + 1;
             } else if (sur) {
                 target = a.sur_id[j];
@@ -597,32 +621,45 @@ __device__ void create_single(const ChainArgs &a, ChainState &st, const ecmc_tag
 }
 
 // end of chain: chain time and the randint of the next active unit
-// (single_independent_active_periodic_direction_end_of_chain_event_handler.py:176-237).
-// Returns false when the ring ran out of words before the draw completed.
+// (single_independent_active_periodic_direction_end_of_chain_event_handler.py:176-237): with
+// composite objects the root's randint, then the leaf's.  Returns false when the ring ran out of
+// words before the draws completed; the root already drawn is kept in the slot (cv_index).
 __device__ bool create_end_of_chain(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
     Slot &sl = st.slot[tix];
     if (!sl.valid || sl.target >= 0x40000000) {
-        double since = time_sub(st.at, st.eoc_last);
-        if (!(0.0 <= since && since <= tg.value)) { st.err = ECMC_ERR_CONFIGURATION; return true; }
-        double chain = time_sub(st.eoc_last, st.at) + tg.value;
-        sl.t = time_add(st.at, chain);
-        sl.valid = 1;
-        sl.target = 0x40000000;  // draw pending
-    }
-    uint32_t nrange = (uint32_t)a.n;
-    int k = 0;
-    while ((nrange >> k) != 0u) k++;
-    while (st.cursor < st.produced) {
-        uint32_t w = ring_word(a, st.cursor);
-        st.cursor++;
-        uint32_t v = k >= 32 ? w : (w >> (32 - k));
-        if (v < nrange) {
-            sl.target = (int32_t)v;
-            sl.seq = st.seq++;
-            return true;
+        if (!sl.valid || sl.target != 0x40000000) {
+            double since = time_sub(st.at, st.eoc_last);
+            if (!(0.0 <= since && since <= tg.value)) { st.err = ECMC_ERR_CONFIGURATION; return true; }
+            double chain = time_sub(st.eoc_last, st.at) + tg.value;
+            sl.t = time_add(st.at, chain);
+            sl.valid = 1;
+            sl.target = 0x40000000;  // draws pending
+            sl.cv_index = -1;        // no root drawn yet
         }
     }
-    return false;
+    const int P = a.leaves;
+    while (true) {
+        const bool root_phase = sl.cv_index < 0;
+        const uint32_t nrange = root_phase ? (uint32_t)(a.n / P) : (uint32_t)P;
+        int k = 0;
+        while ((nrange >> k) != 0u) k++;
+        bool got = false;
+        uint32_t v = 0;
+        while (st.cursor < st.produced) {
+            uint32_t w = ring_word(a, st.cursor);
+            st.cursor++;
+            v = k >= 32 ? w : (w >> (32 - k));
+            if (v < nrange) { got = true; break; }
+        }
+        if (!got) return false;
+        if (root_phase && P > 1) {
+            sl.cv_index = (int32_t)v;
+            continue;
+        }
+        sl.target = root_phase ? (int32_t)v : sl.cv_index * P + (int32_t)v;
+        sl.seq = st.seq++;
+        return true;
+    }
 }
 
 // cell veto send_event_time (cell_veto_event_handler.py:190-238): the Walker table by the sign
@@ -726,7 +763,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
             __syncthreads();
         } else if (pending) {
             if (tid == 0) {
-                occupancy_update<D>(a, st);
+                if (a.occupancy) occupancy_update<D>(a, st);
                 st.ncand = 0;
                 st.chash = 0;
             }
@@ -736,7 +773,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
                 const int t = w.create[ci];
                 const ecmc_tagger_desc &c = tg[t];
                 if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL || c.kind == ECMC_TAG_SURPLUS ||
-                    c.kind == ECMC_TAG_CELL_BOUNDING) {
+                    c.kind == ECMC_TAG_CELL_BOUNDING || c.kind == ECMC_TAG_FACTOR_MAP) {
                     create_pair<D>(a, st, mt, c, t, nb_cell, nb_pref, sb, sh, scan);
                 } else if (c.kind == ECMC_TAG_CELL_VETO) {
                     if (tid == 0) { st.slot[t].valid = 0; st.slot[t].cv_phase = 0; }
@@ -793,7 +830,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
         if (st.err) break;
         const int w = flag;
         const bool thinning = (tg[w].kind == ECMC_TAG_PAIR_CELLS || tg[w].kind == ECMC_TAG_PAIR_ALL ||
-                               tg[w].kind == ECMC_TAG_SURPLUS || tg[w].kind == ECMC_TAG_CELL_BOUNDING) &&
+                               tg[w].kind == ECMC_TAG_SURPLUS || tg[w].kind == ECMC_TAG_CELL_BOUNDING ||
+                               tg[w].kind == ECMC_TAG_FACTOR_MAP) &&
                               (tg[w].event_handler == ECMC_EH_TWO_LEAF_BOUNDING ||
                                tg[w].event_handler == ECMC_EH_CELL_BOUNDING) ||
                               tg[w].kind == ECMC_TAG_CELL_VETO;
@@ -816,7 +854,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
             case ECMC_TAG_PAIR_CELLS:
             case ECMC_TAG_PAIR_ALL:
             case ECMC_TAG_SURPLUS:
-            case ECMC_TAG_CELL_BOUNDING: {
+            case ECMC_TAG_CELL_BOUNDING:
+            case ECMC_TAG_FACTOR_MAP: {
                 ev.target = sl.target;
                 bool lift = true;
                 double s[D];
@@ -1125,6 +1164,7 @@ struct ecmc_system {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
     int32_t stop_mask = 0;
+    int device = 0;          // the device current at creation, which holds its memory
     ChainArgs args;
 };
 
@@ -1176,10 +1216,35 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
         return ecmc_fail(ECMC_ERR_CONFIGURATION, "number of taggers out of range");
     if (desc->neighbor_layers < 0 || desc->neighbor_layers > 1)
         return ecmc_fail(ECMC_ERR_CONFIGURATION, "neighbor_layers must be 0 or 1");
+    const int P = desc->leaves_per_root > 1 ? desc->leaves_per_root : 1;
+    if (P > ECMC_MAX_LEAVES) return ecmc_fail(ECMC_ERR_CONFIGURATION, "more leaf units per composite object than ECMC_MAX_LEAVES");
+    if (n % P != 0) return ecmc_fail(ECMC_ERR_CONFIGURATION, "the number of units is not a multiple of leaves_per_root");
     int ewald_tagger = -1, cb_tagger = -1;
+    int64_t max_cand = n;   // the most pair candidates one tagger creates in a leg
     for (int t = 0; t < desc->n_taggers; t++) {
         const ecmc_tagger_desc &tg = desc->taggers[t];
-        if (tg.kind < 0 || tg.kind > ECMC_TAG_CELL_VETO) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
+        if (tg.kind < 0 || tg.kind > ECMC_TAG_FACTOR_MAP) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
+        if (P > 1 && (tg.kind == ECMC_TAG_PAIR_CELLS || tg.kind == ECMC_TAG_PAIR_ALL || tg.kind == ECMC_TAG_SURPLUS ||
+                      tg.kind == ECMC_TAG_CELL_BOUNDING || tg.kind == ECMC_TAG_CELL_VETO || tg.kind == ECMC_TAG_CELL_BOUNDARY))
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "with composite objects the pair taggers are factor-type-map taggers and no cells are used");
+        if (tg.kind == ECMC_TAG_FACTOR_MAP) {
+            int fmax = 0;
+            for (int k = 0; k < P; k++) {
+                if (tg.fmap_n[k] < 0 || tg.fmap_n[k] > ECMC_MAX_FACTOR_MAP)
+                    return ecmc_fail(ECMC_ERR_CONFIGURATION, "factor type map entries out of range");
+                fmax = tg.fmap_n[k] > fmax ? tg.fmap_n[k] : fmax;
+                for (int e = 0; e < tg.fmap_n[k]; e++) {
+                    const int tl = tg.fmap_target[k][e];
+                    const bool ok = tg.fmap_local ? (tl >= 0 && tl < P && tl != k) : (tl >= P && tl < 2 * P);
+                    if (!ok) return ecmc_fail(ECMC_ERR_CONFIGURATION, "factor type map leaf number out of range");
+                }
+            }
+            const int64_t m = tg.fmap_local ? fmax : (int64_t)(n / P - 1) * fmax;
+            max_cand = m > max_cand ? m : max_cand;
+            if (!tg.fmap_local && n / P < 2) return ecmc_fail(ECMC_ERR_CONFIGURATION, "a non-local factor needs two composite objects");
+            if (tg.n_handlers > 0 && tg.n_handlers < m)
+                return ecmc_fail(ECMC_ERR_TAG_ACTIVATOR, "not-running event handler list would run empty");
+        }
         if ((tg.kind == ECMC_TAG_CELL_VETO) != (tg.event_handler == ECMC_EH_CELL_VETO))
             return ecmc_fail(ECMC_ERR_CONFIGURATION, "the cell-veto handler runs under the cell-veto tagger only");
         if (tg.kind == ECMC_TAG_SURPLUS && desc->max_occupants <= 0)
@@ -1220,7 +1285,8 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
             if (tg.create[i] < 0 || tg.create[i] >= desc->n_taggers) return ecmc_fail(ECMC_ERR_CONFIGURATION, "create tag out of range");
         for (int i = 0; i < tg.n_trash; i++)
             if (tg.trash[i] < 0 || tg.trash[i] >= desc->n_taggers) return ecmc_fail(ECMC_ERR_CONFIGURATION, "trash tag out of range");
-        if (tg.kind == ECMC_TAG_PAIR_CELLS || tg.kind == ECMC_TAG_PAIR_ALL || tg.kind == ECMC_TAG_SURPLUS) {
+        if (tg.kind == ECMC_TAG_PAIR_CELLS || tg.kind == ECMC_TAG_PAIR_ALL || tg.kind == ECMC_TAG_SURPLUS ||
+            tg.kind == ECMC_TAG_FACTOR_MAP) {
             if (tg.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
                 if (tg.potential != ECMC_POT_MERGED_IMAGE_COULOMB || tg.bounding_potential != ECMC_POT_INVERSE_POWER_COULOMB_BOUND || dim != 3)
                     return ecmc_fail(ECMC_ERR_CONFIGURATION, "bounding handler needs merged-image Coulomb with the inverse-power Coulomb bound in 3D");
@@ -1239,6 +1305,10 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     }
     ecmc_system *s = new ecmc_system();
     s->desc = *desc;
+    if (hipGetDevice(&s->device) != hipSuccess) {
+        delete s;
+        return ecmc_fail(ECMC_ERR_HIP, "no current HIP device");
+    }
     s->dim = dim;
     s->n = n;
     CellGrid grid;
@@ -1280,11 +1350,8 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
         for (int d = 0; d < dim; d++) pos_soa[(size_t)d * n + i] = positions[(size_t)i * dim + d];
     std::vector<double> ch(n, 1.0);
     if (charges) for (int i = 0; i < n; i++) ch[i] = charges[i];
-    int max_pairs = 0;
-    for (int t = 0; t < desc->n_taggers; t++)
-        if (desc->taggers[t].kind == ECMC_TAG_PAIR_CELLS || desc->taggers[t].kind == ECMC_TAG_PAIR_ALL) max_pairs++;
-    (void)max_pairs;
-    s->ring_words = 624 * ((2 * n + 256) / 624 + 3);
+    // the ring holds one tagger's draws (two words per candidate) plus room for the twist
+    s->ring_words = (int)(624 * ((2 * max_cand + 256) / 624 + 3));
     uint32_t mt[624];
     mt_seed_host(desc->seed, mt);
     ChainState st;
@@ -1364,6 +1431,12 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
         return ecmc_fail(ECMC_ERR_HIP, "stream/event creation failed");
     }
     a.n = n;
+    a.leaves = P;
+    for (int t = 0; t < desc->n_taggers; t++) {
+        const int k = desc->taggers[t].kind;
+        if (k == ECMC_TAG_PAIR_CELLS || k == ECMC_TAG_SURPLUS || k == ECMC_TAG_CELL_BOUNDING || k == ECMC_TAG_CELL_VETO)
+            a.occupancy = 1;
+    }
     a.ncells = grid.count;
     a.nb_stride = grid.nb_stride;
     a.occ_cap = cap;
@@ -1543,39 +1616,30 @@ extern "C" int ecmc_system_run(ecmc_system *s, int64_t max_events, ecmc_event *h
     return status_of_state(st);
 }
 
-extern "C" int ecmc_systems_run(ecmc_system *const *systems, int32_t count, int64_t max_events,
-                                int64_t *n_committed) {
-    if (!systems || count < 1 || max_events < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
-    const int dim = systems[0] ? systems[0]->dim : 0;
-    std::vector<ChainArgs> args((size_t)count);
+// The launch of independent systems with their arguments staged in `host_args` (count
+// entries, filled here) and copied to the device array `d_args`: one workgroup per system on the
+// first system's stream, then every system's committed legs and state are read back.
+static int systems_run_staged(ecmc_system *const *systems, int32_t count, int64_t max_events, int64_t *n_committed,
+                              ChainArgs *host_args, ChainArgs *d_args) {
+    const int dim = systems[0]->dim;
     for (int i = 0; i < count; i++) {
         ecmc_system *s = systems[i];
-        if (!s) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
-        if (s->dim != dim) return ecmc_fail(ECMC_ERR_ARGUMENT, "all systems of one launch must have one dimension");
-        for (int k = 0; k < i; k++)
-            if (systems[k] == s) return ecmc_fail(ECMC_ERR_ARGUMENT, "a system appears twice in one launch");
-        args[i] = s->args;
-        args[i].log = nullptr;
-        args[i].max_events = max_events;
-        args[i].stop_mask = s->stop_mask;
+        host_args[i] = s->args;
+        host_args[i].log = nullptr;
+        host_args[i].max_events = max_events;
+        host_args[i].stop_mask = s->stop_mask;
         if (n_committed) n_committed[i] = 0;
     }
     if (max_events == 0) return ECMC_OK;
     ecmc_system *s0 = systems[0];
-    ChainArgs *d_args = nullptr;
-    if (hipMalloc((void **)&d_args, sizeof(ChainArgs) * count) != hipSuccess)
-        return ecmc_fail(ECMC_ERR_HIP, "hipMalloc of the launch arguments failed");
-    if (hipMemcpyAsync(d_args, args.data(), sizeof(ChainArgs) * count, hipMemcpyHostToDevice, s0->stream) != hipSuccess) {
-        hipFree(d_args);
+    if (hipMemcpyAsync(d_args, host_args, sizeof(ChainArgs) * count, hipMemcpyHostToDevice, s0->stream) != hipSuccess)
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
-    }
     hipEventRecord(s0->ev0, s0->stream);
     if (dim == 2) hipLaunchKernelGGL(chain_many_kernel<2>, dim3(count), dim3(NT), 0, s0->stream, d_args);
     else hipLaunchKernelGGL(chain_many_kernel<3>, dim3(count), dim3(NT), 0, s0->stream, d_args);
     hipEventRecord(s0->ev1, s0->stream);
     const bool launched = hipGetLastError() == hipSuccess;
     const bool ran = hipStreamSynchronize(s0->stream) == hipSuccess;
-    hipFree(d_args);
     if (!launched) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
     if (!ran) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
     float ms = 0.f;
@@ -1593,6 +1657,71 @@ extern "C" int ecmc_systems_run(ecmc_system *const *systems, int32_t count, int6
         if (rc == ECMC_OK) rc = status_of_state(st);
     }
     return rc;
+}
+
+static int check_systems(ecmc_system *const *systems, int32_t count, int64_t max_events) {
+    if (!systems || count < 1 || max_events < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (!systems[0]) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
+    const int dim = systems[0]->dim;
+    for (int i = 0; i < count; i++) {
+        ecmc_system *s = systems[i];
+        if (!s) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
+        if (s->dim != dim) return ecmc_fail(ECMC_ERR_ARGUMENT, "all systems of one launch must have one dimension");
+        if (s->device != systems[0]->device)
+            return ecmc_fail(ECMC_ERR_ARGUMENT, "all systems of one launch must live on one device");
+        for (int k = 0; k < i; k++)
+            if (systems[k] == s) return ecmc_fail(ECMC_ERR_ARGUMENT, "a system appears twice in one launch");
+    }
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_systems_run(ecmc_system *const *systems, int32_t count, int64_t max_events,
+                                int64_t *n_committed) {
+    int rc = check_systems(systems, count, max_events);
+    if (rc) return rc;
+    std::vector<ChainArgs> args((size_t)count);
+    ChainArgs *d_args = nullptr;
+    if (hipMalloc((void **)&d_args, sizeof(ChainArgs) * count) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMalloc of the launch arguments failed");
+    rc = systems_run_staged(systems, count, max_events, n_committed, args.data(), d_args);
+    hipFree(d_args);
+    return rc;
+}
+
+// ---- independent systems without allocation -------------------------------------------------
+// The launch arguments of ecmc_systems_run_noalloc live in a fixed device array of the library
+// (each GPU the library runs on holds its own copy until the process ends) and are staged in a
+// fixed host array; one process-wide lock serialises the calls, so calls from one process never
+// share the arrays at the same time.  The array's device address comes from
+// hipGetSymbolAddress once per device (the name itself is not a device address in host code).
+static_assert(sizeof(ChainArgs) <= 512, "the no-allocation argument array is sized for 512 B per system");
+static __device__ __align__(256) ChainArgs scratch[ECMC_SYSTEMS_NOALLOC_MAX];
+static ChainArgs g_noalloc_host[ECMC_SYSTEMS_NOALLOC_MAX];
+constexpr int NOALLOC_DEVICES = 64;
+static void *g_scratch_ptr[NOALLOC_DEVICES];
+static std::mutex g_noalloc_mutex;
+
+extern "C" int ecmc_systems_run_noalloc(ecmc_system *const *systems, int32_t count, int64_t max_events,
+                                        int64_t *n_committed) {
+    if (count > ECMC_SYSTEMS_NOALLOC_MAX)
+        return ecmc_fail(ECMC_ERR_ARGUMENT, "more systems than ECMC_SYSTEMS_NOALLOC_MAX in one no-allocation launch");
+    int rc = check_systems(systems, count, max_events);
+    if (rc) return rc;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "no current HIP device");
+    if (dev != systems[0]->device)
+        return ecmc_fail(ECMC_ERR_ARGUMENT, "the systems' device must be the current device");
+    if (dev < 0 || dev >= NOALLOC_DEVICES) return ecmc_fail(ECMC_ERR_HIP, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_scratch_ptr[dev]) {
+        // set-up on the first call on this device, with the device current
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+            return ecmc_fail(ECMC_ERR_HIP, "hipGetSymbolAddress of the argument array failed");
+        g_scratch_ptr[dev] = p;
+    }
+    return systems_run_staged(systems, count, max_events, n_committed, g_noalloc_host,
+                              (ChainArgs *)g_scratch_ptr[dev]);
 }
 
 extern "C" int ecmc_system_positions(const ecmc_system *s, double *out) {

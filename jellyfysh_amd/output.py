@@ -84,17 +84,26 @@ class DummyOutputHandler(OutputHandler):
 
 
 class SeparationOutputHandler(OutputHandler):
-    """Norms of all pair separations of point masses at every sample (one output file: the
-    reference opens one per leaf-identifier distance, a single one without composite objects).
+    """Norms of the pair separations at every sample
+    (input_output_handler/output_handler/separation_output_handler.py:46-106): for point masses
+    every pair i < j into one file; with composite objects of P leaf units, every pair of leaf
+    units of different objects (objects i < j, then leaves in order), into P files by the distance
+    of their leaf numbers, named <stem>_1<d + P + 1>.<suffix> (for dipoles _13: equal leaf
+    numbers, _14: different ones).
 
-    ``write(positions, system_length)`` takes the (n, dim) positions of the extracted global state
-    in identifier order."""
+    ``write(positions, system_length)`` takes the (n, dim) positions of the leaf units in
+    identifier order (unit root * P + leaf)."""
 
-    def __init__(self, filename):
+    def __init__(self, filename, leaves_per_root=1):
         super().__init__(filename)
-        if len(filename.split(".")) != 2:
+        parts = filename.split(".")
+        if len(parts) != 2:
             raise AssertionError("the filename must contain one file format suffix")
-        self._files = [HardBufferedTextWriter(filename)]
+        self.leaves_per_root = P = int(leaves_per_root)
+        if P > 1:
+            self._files = [HardBufferedTextWriter("%s_1%d.%s" % (parts[0], d + P + 1, parts[1])) for d in range(P)]
+        else:
+            self._files = [HardBufferedTextWriter(filename)]
 
     @staticmethod
     def separation_norms(positions, system_length):
@@ -112,11 +121,33 @@ class SeparationOutputHandler(OutputHandler):
         torch.cuda.current_stream().synchronize()
         return out.cpu().numpy()
 
+    @staticmethod
+    def composite_pairs(n, leaves_per_root):
+        """(index into the i < j pair list, leaf-number distance) of every pair of leaf units of
+        different composite objects, in the reference's loop order."""
+        P = int(leaves_per_root)
+        R = n // P
+        out = []
+        for i in range(R):
+            for j in range(i + 1, R):
+                for a in range(P):
+                    for b in range(P):
+                        u, v = i * P + a, j * P + b
+                        out.append((u * n - u * (u + 1) // 2 + (v - u - 1), abs(a - b)))
+        return out
+
     def write(self, positions, system_length):
         super().write(positions, system_length)
         norms = self.separation_norms(positions, system_length)
         # print(float) writes repr(float): the shortest string that reads back as the same double
-        self._files[0].write("".join(repr(float(v)) + "\n" for v in norms))
+        if self.leaves_per_root == 1:
+            self._files[0].write("".join(repr(float(v)) + "\n" for v in norms))
+            return
+        lines = [[] for _ in self._files]
+        for k, d in self.composite_pairs(len(positions), self.leaves_per_root):
+            lines[d].append(repr(float(norms[k])) + "\n")
+        for f, ls in zip(self._files, lines):
+            f.write("".join(ls))
 
     def post_run(self):
         for f in self._files:
@@ -126,8 +157,10 @@ class SeparationOutputHandler(OutputHandler):
 _HANDLERS = {"separation_output_handler": SeparationOutputHandler, "dummy_output_handler": DummyOutputHandler}
 
 
-def make_output_handler(cls, options):
+def make_output_handler(cls, options, leaves_per_root=1):
     """An output handler from its INI class name (snake case) and section options."""
     if cls not in _HANDLERS:
         raise ConfigurationError("output handler %s is not supported" % cls)
+    if cls == "separation_output_handler":
+        return SeparationOutputHandler(leaves_per_root=leaves_per_root, **options)
     return _HANDLERS[cls](**options)

@@ -155,7 +155,8 @@ enum {
     ORA_TAG_START_OF_RUN = 6,
     ORA_TAG_SURPLUS = 7,      /* SurplusCellsTagger: the surplus units in dict order -> pair handler */
     ORA_TAG_CELL_BOUNDING = 8, /* CellBoundingPotentialTagger: occupants of the non-nearby cells */
-    ORA_TAG_CELL_VETO = 9      /* CellVetoTagger: one handler per leg for the active unit */
+    ORA_TAG_CELL_VETO = 9,     /* CellVetoTagger: one handler per leg for the active unit */
+    ORA_TAG_FACTOR_MAP = 10    /* FactorTypeMapInStateTagger over two-leaf factors (see ora_tagger.fmap_*) */
 };
 /* ORA_EH_CELL_BOUNDING: TwoLeafUnitCellBoundingPotentialEventHandler with a merged-image
  * Coulomb potential (par) and a CellBoundingPotential from an InnerPointEstimator
@@ -170,11 +171,18 @@ enum {
     ORA_POT_LENNARD_JONES = 2,    /* par: prefactor, sigma */
     ORA_POT_INVERSE_POWER_TRUNC = 3, /* par: power, prefactor, cutoff */
     ORA_POT_MERGED_IMAGE_COULOMB = 4, /* par: alpha, fourier_cutoff, position_cutoff, prefactor */
-    ORA_POT_IPCB = 5              /* par: prefactor */
+    ORA_POT_IPCB = 5,             /* par: prefactor */
+    ORA_POT_DISPLACED_EVEN_POWER = 6 /* par: equilibrium_separation, power, prefactor */
 };
 enum { ORA_SCHED_LIST = 0, ORA_SCHED_HEAP = 1 };
+/* Events of equal time: ORA_TIES_SCHEDULER ranks them as the configured scheduler does
+ * (heap.c's structural order, or the list scheduler's first push); ORA_TIES_FIRST_PUSH ranks
+ * them by push order under either scheduler (the device chain's rule). */
+enum { ORA_TIES_SCHEDULER = 0, ORA_TIES_FIRST_PUSH = 1 };
 
 #define ORA_MAX_TAGGERS 16
+#define ORA_MAX_LEAVES 4
+#define ORA_MAX_FACTOR_MAP 8
 typedef struct {
     int32_t kind;
     int32_t event_handler;
@@ -186,7 +194,16 @@ typedef struct {
     int32_t n_handlers;          /* pool size (number_event_handlers) */
     double value;                /* sampling_interval | chain_time | end_of_run_time | speed */
     int32_t ivalue;              /* first_event_time_zero | initial_direction_of_motion */
-    int32_t ivalue2;             /* initial_active_identifier */
+    int32_t ivalue2;             /* initial_active_identifier (leaf units: root * leaves_per_root + leaf) */
+    /* ORA_TAG_FACTOR_MAP (factor_type_maps.py:263-356, factor_type_map_in_state_tagger.py:83-107):
+     * for leaf number k of the active unit, fmap_n[k] two-leaf factors whose other leaf has
+     * number fmap_target[k][e] in the numbering of two composite objects; local factors lie
+     * within the active unit's own object, non-local ones are yielded for every other object in
+     * identifier order.  The tagger yields them through a set: the order here is the insertion
+     * order (the order of an insertion-ordered set, as PyPy's). */
+    int32_t fmap_local;
+    int32_t fmap_n[ORA_MAX_LEAVES];
+    int32_t fmap_target[ORA_MAX_LEAVES][ORA_MAX_FACTOR_MAP];
 } ora_tagger;
 
 typedef struct {
@@ -198,6 +215,8 @@ typedef struct {
     int32_t scheduler;
     int32_t math_mode;
     int32_t rng_skip;            /* words drawn before the run (the random input handler's draws) */
+    int32_t ties;                /* ORA_TIES_* */
+    int32_t leaves_per_root;     /* leaf units per composite point object (0 or 1: point masses) */
     uint64_t seed;
     int32_t n_taggers;
     ora_tagger taggers[ORA_MAX_TAGGERS];
@@ -233,6 +252,10 @@ void ora_engine_estimators(const ora_engine *e, double *out8);
 int32_t ora_engine_last_candidates(const ora_engine *e, int32_t *out, int32_t cap);
 /* Number of random words consumed so far. */
 int64_t ora_engine_rng_words(const ora_engine *e);
+
+/* heap.c on its own: a sequence of pushes, trashes and root requests (see the engine source) */
+int ora_heap_replay(int n_ops, const int32_t *op, const int32_t *h, const double *q, const double *r, int n_handlers,
+                    int first_push_ties, int32_t *out);
 
 /* ---- batched replicas (the product's batched form; see ecmc_oracle_batched.c) ---------- */
 void ora_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -264,7 +264,63 @@ typedef struct {
     uint32_t heap_counter;  /* heap scheduler: _minimal_valid_counter */
 } handler;
 
-typedef struct { double q, r; int h; uint32_t counter; } heap_entry;
+/* heap.c:81-326 restated on a plain array: entry 0 is a -inf sentinel, insert bubbles the new
+ * entry up from the end, root pops stale entries (lazy deletion through the per-handler
+ * counters, heap_scheduler.py:242-261) and bubbles the last entry down from the root.  `seq`
+ * numbers the pushes; with first-push ties the root's time is shared by the valid entry
+ * pushed first (the rule of the list scheduler, list_scheduler.py:129-156). */
+typedef struct { double q, r; int h; uint32_t counter; uint64_t seq; } heap_entry;
+typedef struct { heap_entry *a; int len, cap; uint64_t pushes; } ora_heap;
+
+static void oheap_insert(ora_heap *H, double q, double r, int h, uint32_t counter) {
+    if (H->len + 2 > H->cap) {
+        int old = H->cap;
+        H->cap = H->cap ? H->cap * 2 : 64;
+        H->a = (heap_entry *)realloc(H->a, sizeof(heap_entry) * H->cap);
+        if (!old) { H->a[0].q = -INFINITY; H->a[0].r = -INFINITY; H->a[0].h = -1; H->a[0].seq = 0; H->len = 1; }
+    }
+    int pos = H->len++;
+    int par = pos >> 1;
+    while (q < H->a[par].q || (q == H->a[par].q && r < H->a[par].r)) {
+        H->a[pos] = H->a[par];
+        pos = par; par = pos >> 1;
+    }
+    H->a[pos].q = q; H->a[pos].r = r; H->a[pos].h = h; H->a[pos].counter = counter; H->a[pos].seq = H->pushes++;
+}
+/* heap.c:209-247: the entry at a[len] moves down from pos */
+static void oheap_bubble_down(ora_heap *H, int pos) {
+    while (pos < H->len) {
+        int cmp = H->len, ch = pos << 1;
+        if (ch < H->len && (H->a[ch].q < H->a[cmp].q || (H->a[ch].q == H->a[cmp].q && H->a[ch].r < H->a[cmp].r))) cmp = ch;
+        if (ch + 1 < H->len && (H->a[ch + 1].q < H->a[cmp].q || (H->a[ch + 1].q == H->a[cmp].q && H->a[ch + 1].r < H->a[cmp].r))) cmp = ch + 1;
+        H->a[pos] = H->a[cmp];
+        pos = cmp;
+    }
+}
+typedef int (*oheap_stale_fn)(void *ctx, int h, uint32_t counter);
+/* heap.c:263-276; returns the handler of the root entry after lazy deletion, or -1 */
+static int oheap_root(ora_heap *H, oheap_stale_fn stale, void *ctx, int first_push_ties) {
+    while (H->len > 1 && stale(ctx, H->a[1].h, H->a[1].counter)) {
+        H->a[1] = H->a[--H->len];
+        oheap_bubble_down(H, 1);
+    }
+    if (H->len <= 1) return -1;
+    if (!first_push_ties) return H->a[1].h;
+    /* every entry of the root's time lies in a subtree holding the root (children never come
+     * before their parent): search it for the valid entry pushed first */
+    int best = 1;
+    int stack[64], top = 0;
+    stack[top++] = 1;
+    while (top > 0) {
+        const int i = stack[--top];
+        for (int c = 2 * i; c <= 2 * i + 1 && c < H->len; c++) {
+            if (H->a[c].q != H->a[1].q || H->a[c].r != H->a[1].r) continue;
+            if (!stale(ctx, H->a[c].h, H->a[c].counter) && H->a[c].seq < H->a[best].seq) best = c;
+            if (top < 62) stack[top++] = c;
+        }
+    }
+    return H->a[best].h;
+}
 
 struct ora_engine {
     ora_config cfg;
@@ -288,7 +344,7 @@ struct ora_engine {
     /* list scheduler */
     int *lst; int nlst, caplst;
     /* heap scheduler */
-    heap_entry *heap; int hlen, hcap;
+    ora_heap hp;
     double last_q, last_r;
     ora_ewald *ewald[ORA_MAX_TAGGERS];
     double est[8];
@@ -321,28 +377,11 @@ static void sched_push(ora_engine *e, int hidx, double q, double r) {
     }
     /* heap_scheduler.py:139 drops infinite times; heap.c:156-189 insert */
     if (!time_lt_pub(q, r, INFINITY, INFINITY)) return;
-    if (e->hlen + 2 > e->hcap) {
-        int old = e->hcap;
-        e->hcap = e->hcap ? e->hcap * 2 : 64;
-        e->heap = (heap_entry *)realloc(e->heap, sizeof(heap_entry) * e->hcap);
-        if (!old) { e->heap[0].q = -INFINITY; e->heap[0].r = -INFINITY; e->heap[0].h = -1; e->hlen = 1; }
-    }
-    int pos = e->hlen++;
-    int par = pos >> 1;
-    while (q < e->heap[par].q || (q == e->heap[par].q && r < e->heap[par].r)) {
-        e->heap[pos] = e->heap[par];
-        pos = par; par = pos >> 1;
-    }
-    e->heap[pos].q = q; e->heap[pos].r = r; e->heap[pos].h = hidx; e->heap[pos].counter = hd->heap_counter;
+    oheap_insert(&e->hp, q, r, hidx, hd->heap_counter);
 }
-static void heap_bubble_down(ora_engine *e, int pos) {
-    while (pos < e->hlen) {
-        int cmp = e->hlen, ch = pos << 1;
-        if (ch < e->hlen && (e->heap[ch].q < e->heap[cmp].q || (e->heap[ch].q == e->heap[cmp].q && e->heap[ch].r < e->heap[cmp].r))) cmp = ch;
-        if (ch + 1 < e->hlen && (e->heap[ch + 1].q < e->heap[cmp].q || (e->heap[ch + 1].q == e->heap[cmp].q && e->heap[ch + 1].r < e->heap[cmp].r))) cmp = ch + 1;
-        e->heap[pos] = e->heap[cmp];
-        pos = cmp;
-    }
+static int engine_stale(void *ctx, int h, uint32_t counter) {
+    const ora_engine *e = (const ora_engine *)ctx;
+    return e->h[h].heap_counter > counter;
 }
 static int sched_pop_min(ora_engine *e) {
     int best = -1;
@@ -354,11 +393,7 @@ static int sched_pop_min(ora_engine *e) {
         }
     } else {
         /* heap.c:263-276 with heap_scheduler.py:340-362 (stale when counter moved on) */
-        while (e->hlen > 1 && e->h[e->heap[1].h].heap_counter > e->heap[1].counter) {
-            e->heap[1] = e->heap[--e->hlen];
-            heap_bubble_down(e, 1);
-        }
-        if (e->hlen > 1) best = e->heap[1].h;
+        best = oheap_root(&e->hp, engine_stale, e, e->cfg.ties == ORA_TIES_FIRST_PUSH);
     }
     if (best < 0) { eng_fail(e, "SchedulerError: scheduler holds no events"); return -1; }
     if (time_lt_pub(e->h[best].tq, e->h[best].tr, e->last_q, e->last_r)) {
@@ -465,6 +500,10 @@ static double pair_displacement(ora_engine *e, const ora_tagger *tg, int tidx, c
             double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
             return ora_lj_displacement(e->dim, dir, speed, s, tg->par[0], tg->par[1], dU, mm);
         }
+        case ORA_POT_DISPLACED_EVEN_POWER: {
+            double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
+            return ora_dep_displacement(e->dim, dir, speed, s, tg->par[0], (int)tg->par[1], tg->par[2], dU, mm);
+        }
         default: eng_fail(e, "unsupported potential for TwoLeafUnitEventHandler"); return INFINITY;
         }
     }
@@ -516,14 +555,16 @@ static void create_tagger(ora_engine *e, int t) {
     case ORA_TAG_PAIR_CELLS:
     case ORA_TAG_PAIR_ALL:
     case ORA_TAG_SURPLUS:
-    case ORA_TAG_CELL_BOUNDING: {
+    case ORA_TAG_CELL_BOUNDING:
+    case ORA_TAG_FACTOR_MAP: {
         if (a < 0) return;
         /* the candidate targets in the order the tagger yields them:
          * excluded_cells_tagger.py:129-132 (nearby cells, occupants in list order),
          * surplus_cells_tagger.py:128-131 (the surplus dict's order),
          * cell_bounding_potential_tagger.py:150-155 (cells in yield_cells() order that are
          * occupied and not nearby), or every other unit in identifier order */
-        int *targets = (int *)malloc(sizeof(int) * (size_t)(e->n + 1));
+        const int P = e->cfg.leaves_per_root > 1 ? e->cfg.leaves_per_root : 1;
+        int *targets = (int *)malloc(sizeof(int) * ((size_t)(e->n + 1) * ORA_MAX_FACTOR_MAP));
         int nt = 0;
         int cells[27 * 27];
         int ncells = 0;
@@ -531,7 +572,22 @@ static void create_tagger(ora_engine *e, int t) {
             if (e->occ_active_cell < 0) { free(targets); return; }
             ncells = ora_cells_nearby(e->cells, e->occ_active_cell, cells);
         }
-        if (tg->kind == ORA_TAG_PAIR_CELLS) {
+        if (tg->kind == ORA_TAG_FACTOR_MAP) {
+            /* factor_type_maps.py:325-356: local factors of the active leaf's own object, or for
+             * every other root in identifier order the map's entries of the active leaf number */
+            const int m = a / P, k = a % P;
+            if (tg->fmap_local) {
+                for (int x = 0; x < tg->fmap_n[k]; x++) targets[nt++] = m * P + tg->fmap_target[k][x];
+            } else {
+                for (int other = 0; other < e->n / P; other++) {
+                    if (other == m) continue;
+                    for (int x = 0; x < tg->fmap_n[k]; x++) {
+                        const int tl = tg->fmap_target[k][x];
+                        targets[nt++] = tl < P ? m * P + tl : other * P + (tl - P);
+                    }
+                }
+            }
+        } else if (tg->kind == ORA_TAG_PAIR_CELLS) {
             for (int c = 0; c < ncells; c++)
                 for (int o = 0; o < e->occ[cells[c]].n; o++) targets[nt++] = e->occ[cells[c]].v[o];
         } else if (tg->kind == ORA_TAG_PAIR_ALL) {
@@ -650,7 +706,13 @@ static void create_tagger(ora_engine *e, int t) {
         double chain = ora_time_sub(hd->last_q, hd->last_r, e->aq, e->ar) + tg->value;
         double q, r;
         ora_time_add(e->aq, e->ar, chain, &q, &r);
-        hd->target = (int)ora_randint(&e->rng, 0, e->n - 1);
+        /* ..._end_of_chain_event_handler.py:213-237: a random root, and with composite objects
+         * then a random leaf of it */
+        {
+            const int P = e->cfg.leaves_per_root > 1 ? e->cfg.leaves_per_root : 1;
+            const int root = (int)ora_randint(&e->rng, 0, e->n / P - 1);
+            hd->target = P > 1 ? root * P + (int)ora_randint(&e->rng, 0, P - 1) : root;
+        }
         sched_push(e, hidx, q, r);
         break;
     }
@@ -688,7 +750,8 @@ static int out_state(ora_engine *e, int hidx, ora_event *ev) {
     case ORA_TAG_PAIR_CELLS:
     case ORA_TAG_PAIR_ALL:
     case ORA_TAG_SURPLUS:
-    case ORA_TAG_CELL_BOUNDING: {
+    case ORA_TAG_CELL_BOUNDING:
+    case ORA_TAG_FACTOR_MAP: {
         ev->target = hd->target;
         int lift = 1;
         if (tg->event_handler == ORA_EH_CELL_BOUNDING) {
@@ -912,7 +975,7 @@ void ora_engine_destroy(ora_engine *e) {
     }
     free(e->occ); free(e->sur_cell); free(e->sur_id); free(e->running); free(e->idle);
     ora_cells_destroy(e->cells);
-    free(e->pos); free(e->charge); free(e->h); free(e->lst); free(e->heap); free(e->cand);
+    free(e->pos); free(e->charge); free(e->h); free(e->lst); free(e->hp.a); free(e->cand);
     free(e);
 }
 
@@ -1082,4 +1145,29 @@ int64_t ora_run_threads(const ora_config *cfg, const double *positions, const do
     if (seconds) *seconds = (t1.tv_sec - t0.tv_sec) + 1e-9 * (t1.tv_nsec - t0.tv_nsec);
     free(th); free(args);
     return total;
+}
+
+/* ---- the heap alone (test entry) ----------------------------------------------------------
+ * op[i] = 0: push (h[i], q[i], r[i]) with the handler's current counter; 1: trash h[i] (its
+ * counter moves on); 2: root -> out[i] = handler of the root entry or -1.  Handlers
+ * 0..n_handlers-1. */
+struct replay_ctx { uint32_t *counter; };
+static int replay_stale(void *ctx, int h, uint32_t counter) {
+    return ((struct replay_ctx *)ctx)->counter[h] > counter;
+}
+int ora_heap_replay(int n_ops, const int32_t *op, const int32_t *h, const double *q, const double *r, int n_handlers,
+                    int first_push_ties, int32_t *out) {
+    ora_heap H = {0};
+    struct replay_ctx ctx;
+    ctx.counter = (uint32_t *)calloc((size_t)n_handlers, sizeof(uint32_t));
+    if (!ctx.counter) return -1;
+    for (int i = 0; i < n_ops; i++) {
+        out[i] = -2;
+        if (op[i] == 0) oheap_insert(&H, q[i], r[i], h[i], ctx.counter[h[i]]);
+        else if (op[i] == 1) ctx.counter[h[i]]++;
+        else out[i] = oheap_root(&H, replay_stale, &ctx, first_push_ties);
+    }
+    free(H.a);
+    free(ctx.counter);
+    return 0;
 }

@@ -23,7 +23,10 @@ TAG_SURPLUS, TAG_CELL_BOUNDING, TAG_CELL_VETO = 7, 8, 9
 POT_HARD_SPHERE, POT_INVERSE_POWER, POT_LENNARD_JONES, POT_INVERSE_POWER_TRUNC, POT_MERGED_IMAGE_COULOMB, \
     POT_IPCB = range(6)
 SCHED_LIST, SCHED_HEAP = 0, 1
+TIES_SCHEDULER, TIES_FIRST_PUSH = 0, 1
 MAX_TAGGERS = 16
+MAX_LEAVES = 4
+MAX_FACTOR_MAP = 8
 
 
 class Tagger(ctypes.Structure):
@@ -31,14 +34,16 @@ class Tagger(ctypes.Structure):
                 ("bounding_potential", ctypes.c_int32), ("par", ctypes.c_double * 4), ("bpar", ctypes.c_double * 4),
                 ("use_charge", ctypes.c_int32), ("n_create", ctypes.c_int32), ("create", ctypes.c_int32 * MAX_TAGGERS),
                 ("n_trash", ctypes.c_int32), ("trash", ctypes.c_int32 * MAX_TAGGERS), ("n_handlers", ctypes.c_int32),
-                ("value", ctypes.c_double), ("ivalue", ctypes.c_int32), ("ivalue2", ctypes.c_int32)]
+                ("value", ctypes.c_double), ("ivalue", ctypes.c_int32), ("ivalue2", ctypes.c_int32),
+                ("fmap_local", ctypes.c_int32), ("fmap_n", ctypes.c_int32 * MAX_LEAVES),
+                ("fmap_target", (ctypes.c_int32 * MAX_FACTOR_MAP) * MAX_LEAVES)]
 
 
 class Config(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("n", ctypes.c_int32), ("system_length", ctypes.c_double),
                 ("beta", ctypes.c_double), ("cells_per_side", ctypes.c_int32 * 3), ("neighbor_layers", ctypes.c_int32),
                 ("max_occupants", ctypes.c_int32), ("scheduler", ctypes.c_int32), ("math_mode", ctypes.c_int32),
-                ("rng_skip", ctypes.c_int32),
+                ("rng_skip", ctypes.c_int32), ("ties", ctypes.c_int32), ("leaves_per_root", ctypes.c_int32),
                 ("seed", ctypes.c_uint64), ("n_taggers", ctypes.c_int32), ("taggers", Tagger * MAX_TAGGERS)]
 
 
@@ -100,6 +105,8 @@ def lib():
             "ora_run_threads": (i64, [vp, pd, pd, i32, i64, pd]),
             "ora_walker_build": (i32, [vp, i32, ctypes.POINTER(i32), pd]), "ora_walker_free": (None, [vp]),
             "ora_walker_sample": (i32, [vp, vp]),
+            "ora_heap_replay": (i32, [i32, ctypes.POINTER(i32), ctypes.POINTER(i32), pd, pd, i32, i32,
+                                      ctypes.POINTER(i32)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

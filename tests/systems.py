@@ -68,15 +68,16 @@ def wiring_from_meta(meta):
 
 
 _ORACLE_POT = {"hard_sphere": 0, "inverse_power": 1, "lennard_jones": 2, "inverse_power_truncated": 3,
-               "merged_image_coulomb": 4, "inverse_power_coulomb_bounding": 5}
+               "merged_image_coulomb": 4, "inverse_power_coulomb_bounding": 5, "displaced_even_power": 6}
 _ORACLE_KIND = {"pair_cells": 0, "pair_all": 1, "cell_boundary": 2, "sampling": 3, "end_of_chain": 4,
-                "end_of_run": 5, "start_of_run": 6, "surplus": 7, "cell_bounding": 8, "cell_veto": 9}
+                "end_of_run": 5, "start_of_run": 6, "surplus": 7, "cell_bounding": 8, "cell_veto": 9,
+                "factor_map": 10}
 
 
 def oracle_config_from_wiring(dim, n, system_length, beta, cells_per_side, taggers, seed, scheduler="heap",
-                              math_mode=0, max_occupants=-1):
+                              math_mode=0, max_occupants=-1, leaves_per_root=1, ties=0):
     from oracle import oracle as O
-    from jellyfysh_amd.chain import CELL_BOUNDING_DEFAULTS
+    from jellyfysh_amd.chain import CELL_BOUNDING_DEFAULTS, fill_factor_map
     c = O.Config()
     c.dim, c.n, c.system_length, c.beta = dim, n, system_length, beta
     cps = list(cells_per_side) if np.ndim(cells_per_side) else [cells_per_side] * 3
@@ -87,6 +88,8 @@ def oracle_config_from_wiring(dim, n, system_length, beta, cells_per_side, tagge
     c.scheduler = O.SCHED_LIST if scheduler == "list" else O.SCHED_HEAP
     c.math_mode = math_mode
     c.seed = seed
+    c.leaves_per_root = leaves_per_root
+    c.ties = ties
     idx = {t["tag"]: i for i, t in enumerate(taggers)}
     c.n_taggers = len(taggers)
     for i, t in enumerate(taggers):
@@ -99,7 +102,7 @@ def oracle_config_from_wiring(dim, n, system_length, beta, cells_per_side, tagge
             g.trash[j] = idx[tag]
         g.n_trash = len(t.get("trash", []))
         g.n_handlers = t.get("number_event_handlers", n)
-        if t["kind"] in ("pair_cells", "pair_all", "surplus", "cell_bounding", "cell_veto"):
+        if t["kind"] in ("pair_cells", "pair_all", "surplus", "cell_bounding", "cell_veto", "factor_map"):
             g.event_handler = 1 if t.get("bounding_potential") else 0
             g.potential = _ORACLE_POT[t["potential"]]
             for j, v in enumerate(t.get("params", ())):
@@ -114,6 +117,8 @@ def oracle_config_from_wiring(dim, n, system_length, beta, cells_per_side, tagge
                 for j, v in enumerate(t.get("bounding_params", ())):
                     g.bpar[j] = v
             g.use_charge = 1 if t.get("charge") else 0
+            if t["kind"] == "factor_map":
+                fill_factor_map(g, t["factor_map"], leaves_per_root)
         elif t["kind"] == "sampling":
             g.value = t["sampling_interval"]
             g.ivalue = 1 if t.get("first_event_time_zero") else 0
@@ -238,3 +243,57 @@ def coulomb_cell_veto_system(n=8, estimator_prefactor=1.0, **kw):
                       bounding_params=[estimator_prefactor, 10, 1.0, float("inf")], charge=True,
                       create=taggers[1]["create"], trash=taggers[1]["trash"], number_event_handlers=1)
     return meta, taggers, pos, charges
+
+
+def dipole_wiring(meta):
+    """The taggers of a composite-object fixture (tests/golden/make_dipole_event_log.py): one
+    factor-type-map tagger per factor, sampling, end of chain, end of run, start of run."""
+    factors = meta["factors"]
+    pair = [f["tag"] for f in factors]
+    eoc = ["end_of_chain"] + pair
+    hmax = 2 * meta["roots"] * meta["leaves"]
+    taggers = []
+    for f in factors:
+        t = dict(tag=f["tag"], kind="factor_map", create=pair, trash=pair, number_event_handlers=hmax,
+                 factor_map=dict(local=f["local"], targets=f["targets"]))
+        if f["kind"] == "displaced_even_power":
+            t.update(potential="displaced_even_power", params=[f["equilibrium_separation"], f["power"], f["prefactor"]])
+        else:
+            t.update(potential="inverse_power", params=[f["power"], f["prefactor"]])
+        taggers.append(t)
+    taggers.append(dict(tag="sampling", kind="sampling", create=["sampling"], trash=["sampling"],
+                        sampling_interval=meta["sampling_interval"]))
+    taggers.append(dict(tag="end_of_chain", kind="end_of_chain", create=eoc, trash=eoc, chain_time=meta["chain_time"]))
+    taggers.append(dict(tag="end_of_run", kind="end_of_run", create=["end_of_run"], trash=eoc + ["sampling", "end_of_run"],
+                        end_of_run_time=meta["end_time"]))
+    taggers.append(dict(tag="start_of_run", kind="start_of_run", create=pair + ["sampling", "end_of_chain", "end_of_run"],
+                        trash=["start_of_run"], initial_direction_of_motion=0, speed=1.0, initial_active_identifier=0))
+    return taggers
+
+
+def dipole_words(meta):
+    """Random words the reference's random input handler drew for the fixture's dipoles (the
+    restatement in jellyfysh_amd.input_handlers, checked against the fixture's positions)."""
+    import random
+    from jellyfysh_amd.input_handlers import ChargeValues, DipoleRandomNodeCreator, RandomInputHandler
+    h = RandomInputHandler(meta["roots"], DipoleRandomNodeCreator(0.0, meta["max_dipole_separation"],
+                                                                  [ChargeValues(meta["charges"], "electric_charge")]))
+    pos, charges, words = h.read(random.Random(meta["seed"]), meta["dim"], meta["system_length"])
+    return pos.reshape(-1, meta["dim"]), charges["electric_charge"].reshape(-1), words
+
+
+def dipole_oracle_config(meta, math_mode=0, ties=0):
+    _, _, words = dipole_words(meta)
+    c = oracle_config_from_wiring(meta["dim"], meta["roots"] * meta["leaves"], meta["system_length"], meta["beta"], 1,
+                                  dipole_wiring(meta), meta["seed"], scheduler=meta["scheduler"], math_mode=math_mode,
+                                  leaves_per_root=meta["leaves"], ties=ties)
+    c.rng_skip = words
+    return c
+
+
+def dipole_device_desc(meta):
+    from jellyfysh_amd.chain import make_desc
+    _, _, words = dipole_words(meta)
+    return make_desc(meta["dim"], meta["roots"] * meta["leaves"], meta["system_length"], meta["beta"], 1,
+                     dipole_wiring(meta), meta["seed"], scheduler=meta["scheduler"], rng_skip=words,
+                     leaves_per_root=meta["leaves"])

@@ -192,33 +192,14 @@ def test_cell_veto_ini_is_the_workload_wiring():
 
 
 def _coulomb_atoms_ini(far="cell_veto"):
-    """coulomb_atoms_cell_veto.ini, or with far="cell_bounding" the same run with the far cells
-    through the cell-bounding potential (estimator prefactor 1.5), as the reference's
-    coulomb_atoms/cell_bounded.ini differs from its cell_veto.ini; far="power_bounded" reads
-    coulomb_atoms_power_bounded.ini (the pair through the inverse-power Coulomb bound only)."""
+    """The reference's three coulomb_atoms runs as committed INI files: the cell veto
+    (coulomb_atoms_cell_veto.ini), the cell-bounding potential (coulomb_atoms_cell_bounded.ini)
+    and the pair through the inverse-power Coulomb bound only (coulomb_atoms_power_bounded.ini)."""
     import os
     from jellyfysh_amd.config import RunConfig
     configs = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "jellyfysh_amd", "configs")
-    if far == "power_bounded":
-        return RunConfig.from_file(os.path.join(configs, "coulomb_atoms_power_bounded.ini"))
-    text = open(os.path.join(configs, "coulomb_atoms_cell_veto.ini")).read()
-    if far == "cell_bounding":
-        text = text.replace("coulomb_cell_veto (cell_veto_tagger)",
-                            "coulomb_cell_bounding (cell_bounding_potential_tagger)")
-        text = text.replace("coulomb_cell_veto", "coulomb_cell_bounding").replace("[CoulombCellVeto]",
-                                                                               "[CoulombCellBounding]")
-        text = text.replace("event_handler = leaf_unit_cell_veto_event_handler",
-                            "event_handler = two_leaf_unit_cell_bounding_potential_event_handler\n"
-                            "number_event_handlers = 1")
-        text = text.replace("[LeafUnitCellVetoEventHandler]\nestimator = inner_point_estimator\n"
-                            "charge = electric_charge",
-                            "[TwoLeafUnitCellBoundingPotentialEventHandler]\n"
-                            "potential = merged_image_coulomb_potential\n"
-                            "bounding_potential = cell_bounding_potential\ncharge = electric_charge\n\n"
-                            "[CellBoundingPotential]\nestimator = inner_point_estimator")
-        text = text.replace("prefactor = 1.0\ntarget_charge = 1.0", "prefactor = 1.5\ntarget_charge = 1.0")
-        assert "veto" not in text.split("[Run]")[1]
-    return RunConfig.from_string(text)
+    name = {"cell_veto": "cell_veto", "cell_bounding": "cell_bounded", "power_bounded": "power_bounded"}[far]
+    return RunConfig.from_file(os.path.join(configs, "coulomb_atoms_%s.ini" % name))
 
 
 @pytest.mark.gpu

@@ -74,3 +74,28 @@ def test_mediator_from_ini_commits_the_oracle_events(device, oracle):
     want = ref.run(legs + 10)
     assert len(want) == legs
     assert np.array_equal(med.chain.positions(), ref.positions())
+
+
+def test_device_chain_checks_charges_before_any_device_call():
+    """The charges reach ecmc_system_create as n doubles: a wrong shape, or no charges for a
+    wiring that multiplies by them, raises on the host before the library reads them."""
+    from jellyfysh_amd.chain import DeviceChain, make_desc
+    meta, taggers, pos, charges = S.coulomb_cell_bounded_system(n=8)
+    desc = make_desc(3, 8, meta["system_length"], meta["beta"], meta["cells_per_side"], taggers, 1,
+                     scheduler="heap", max_occupants=1)
+    with pytest.raises(ValueError, match="shape"):
+        DeviceChain(desc, pos, charges[:5])
+    with pytest.raises(ValueError, match="no charges"):
+        DeviceChain(desc, pos, None)
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/jellyfysh/config_files"), reason="reference tree absent")
+@pytest.mark.parametrize("name", ["cell_veto", "cell_bounded", "power_bounded"])
+def test_coulomb_atoms_inis_keep_the_reference_tag_order(name):
+    """The create and trash lists of the committed two-charge INI files, section by section, in
+    the order of the reference's own files (the order of a create list sets which random words
+    each candidate draws)."""
+    def lists(path):
+        return [ln.strip() for ln in open(path) if ln.startswith(("create", "trash"))]
+    ref = "/root/reference/jellyfysh/config_files/2018_JCP_149_064113/coulomb_atoms/%s.ini" % name
+    assert lists(os.path.join(CONFIGS, "coulomb_atoms_%s.ini" % name)) == lists(ref)
