@@ -364,11 +364,14 @@ def main():
         sys.exit(2)
     import torch
     from jellyfysh_amd import _lib
-    from jellyfysh_amd.batch import DeviceBatch, batched_lj_workload, default_classes, pressure
+    from jellyfysh_amd.batch import DeviceBatch, default_classes, pressure
+    from jellyfysh_amd.config import BatchedRunConfig
     _lib.require_device()
-    n, lengths, pos = batched_lj_workload(args.replicas, seed=2024)
+    cfg = BatchedRunConfig.from_file(os.path.join(ROOT, "jellyfysh_amd", "configs", "batched_lennard_jones.ini"))
+    cfg.number_of_replicas = args.replicas
+    n, lengths, pos = cfg.workload()
     log("workload: %d replicas, %d units" % (len(n), int(np.sum(n))))
-    batch = DeviceBatch(n, lengths, pos, seed=17, classes=default_classes())
+    batch = DeviceBatch(n, lengths, pos, classes=default_classes(), **cfg.batch_arguments())
     for i in range(args.warmup):
         batch.run_graph(args.legs, 1)
         log("warm-up step %d done" % (i + 1))

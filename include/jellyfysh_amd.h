@@ -198,6 +198,29 @@ int ecmc_systems_run(ecmc_system *const *systems, int32_t count, int64_t max_eve
 #define ECMC_SYSTEMS_NOALLOC_MAX 1024
 int ecmc_systems_run_noalloc(ecmc_system *const *systems, int32_t count, int64_t max_events, int64_t *n_committed);
 
+/* ---- the leg step by step: a set-level event handler per tagger -------------------------------
+ * The host keeps the mediator loop, the scheduler and the tag activator's create/trash lists
+ * (mediator/single_process_mediator.py:84-156, mediator/mediator.py:142-211); the state and the
+ * candidate sets stay on the device.  Each tagger acts as one event handler for its whole set
+ * of factors, the in-tree pattern of handlers that return (time, arguments)
+ * (event_handler/event_handler.py:80-125; event_handler/abstracts/cell_veto_event_handler.py:162-238
+ * with mediator/mediator.py:265-292):
+ *   ecmc_system_begin_leg        the internal-state update before a leg's creates (the cell
+ *                                occupancy follows the active unit, tag_activator.py:205-226);
+ *   ecmc_system_send_event_time  send_event_time of tagger `tagger`: every candidate of its set
+ *                                is evaluated on the device (random draws in the reference's
+ *                                order); returns the earliest time as Time(tq, tr) and its target
+ *                                unit (the new active unit for an end of chain, -1 for events
+ *                                without one); valid = 0 (time +INF) when the set is empty;
+ *   ecmc_system_send_out_state   send_out_state of the stored event of `tagger` and its
+ *                                insertion into the state (thinning included); the committed
+ *                                event record goes to *event.
+ * The host's scheduler decides which tagger's event comes next; trashing is the host's alone. */
+int ecmc_system_begin_leg(ecmc_system *sys);
+int ecmc_system_send_event_time(ecmc_system *sys, int32_t tagger, double *tq, double *tr, int32_t *target,
+                                int32_t *valid);
+int ecmc_system_send_out_state(ecmc_system *sys, int32_t tagger, ecmc_event *event);
+
 /* ---- batched independent replicas (no reference counterpart as a whole) ------------------
  * Each replica is a reference chain (straight chains of chain_length cycling +x,+y,+z,
  * exact pair-factor inversion, cell-boundary events of CuboidPeriodicCells, a random new
@@ -257,6 +280,15 @@ int ecmc_batch_states(const ecmc_batch *batch, int32_t *active, int32_t *directi
 int ecmc_batch_positions(const ecmc_batch *batch, double *positions);
 int ecmc_batch_log(const ecmc_batch *batch, ecmc_batch_event *host_log);
 int ecmc_batch_audit(const ecmc_batch *batch, uint64_t *violations);
+/* Snapshot and restore of a batch's state between steps: every replica's state (active unit,
+ * direction, chain left, leg counter, which is the Philox counter, estimators, statistics) and
+ * all positions, into a host buffer of *bytes bytes (buffer NULL: *bytes gets the size).  A
+ * restore into a batch of the same description and replica sizes continues every replica as
+ * the snapshotted batch would have.  Stands for the reference's dumping output handler and
+ * resume.py (input_output_handler/output_handler/dumping_output_handler.py:70-90,
+ * resume.py:63-109), for the batched form. */
+int ecmc_batch_snapshot(const ecmc_batch *batch, void *buffer, int64_t *bytes);
+int ecmc_batch_restore(ecmc_batch *batch, const void *buffer, int64_t bytes);
 
 /* ---- batched per-pair evaluations on device memory ---------------------------------------
  * Row i uses direction dir[i] (velocity speed[i] along +dir), separation sep[3*i..3*i+dim)
@@ -268,6 +300,21 @@ int ecmc_pair_displacement(int32_t potential, const double *par, int32_t dim, in
 int ecmc_pair_derivative(int32_t potential, const double *par, int32_t dim, int64_t n, const int32_t *d_dir,
                          const double *d_speed, const double *d_sep, const double *d_c1, const double *d_c2,
                          double system_length, double *d_out, void *stream);
+
+/* ---- merged-image Coulomb handle ------------------------------------------------------------
+ * The opaque handle of merged_image_coulomb_potential.h:37-44 (construct_, copy_,
+ * destroy_merged_image_coulomb_potential, estimated_size), its Fourier table resident on the
+ * device that is current at creation.  ecmc_ewald_derivative is the C file's derivative
+ * (merged_image_coulomb_potential.c:205-274) for n separations (n x 3, device memory, x along
+ * the direction of motion as vectors.permutation_3d orders them), without prefactor and
+ * charges, into d_out.  NULL / a negative status stands for the reference's MemoryError. */
+typedef struct ecmc_ewald ecmc_ewald;
+int ecmc_ewald_create(int32_t fourier_cutoff, int32_t position_cutoff, double alpha, double system_length,
+                      ecmc_ewald **out);
+int ecmc_ewald_copy(const ecmc_ewald *ewald, ecmc_ewald **out);
+void ecmc_ewald_destroy(ecmc_ewald *ewald);
+int ecmc_ewald_size(const ecmc_ewald *ewald, int64_t *bytes);
+int ecmc_ewald_derivative(const ecmc_ewald *ewald, int64_t n, const double *d_sep, double *d_out, void *stream);
 
 /* Plain device-to-device copy of `bytes` (a multiple of 16).  No reference counterpart: a
  * kernel of known traffic that scales hardware byte counters in profiles. */

@@ -85,6 +85,9 @@ def _signatures():
         "ecmc_system_last_kernel_ms": (i32, [vp, pd]),
         "ecmc_systems_run": (i32, [vp, i32, i64, pi64]),
         "ecmc_systems_run_noalloc": (i32, [vp, i32, i64, pi64]),
+        "ecmc_system_begin_leg": (i32, [vp]),
+        "ecmc_system_send_event_time": (i32, [vp, i32, pd, pd, pi32, pi32]),
+        "ecmc_system_send_out_state": (i32, [vp, i32, vp]),
         "ecmc_pair_displacement": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, vp, d, vp, vp]),
         "ecmc_pair_derivative": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, d, vp, vp]),
         "ecmc_math_log": (i32, [i64, vp, vp, vp]),
@@ -92,6 +95,11 @@ def _signatures():
         "ecmc_mt_words": (i32, [u64, i64, vp, vp]),
         "ecmc_fp32_math_error": (i32, [i32, ctypes.c_uint32, ctypes.c_uint32, vp, vp]),
         "ecmc_copy_bytes": (i32, [vp, vp, i64, vp]),
+        "ecmc_ewald_create": (i32, [i32, i32, d, d, ctypes.POINTER(vp)]),
+        "ecmc_ewald_copy": (i32, [vp, ctypes.POINTER(vp)]),
+        "ecmc_ewald_destroy": (None, [vp]),
+        "ecmc_ewald_size": (i32, [vp, pi64]),
+        "ecmc_ewald_derivative": (i32, [vp, i64, vp, vp, vp]),
         "ecmc_batch_create": (i32, [vp, vp, pd, pd, ctypes.POINTER(vp)]),
         "ecmc_batch_destroy": (None, [vp]),
         "ecmc_batch_set_classes": (i32, [vp, i32, vp, vp]),
@@ -108,6 +116,8 @@ def _signatures():
         "ecmc_batch_positions": (i32, [vp, pd]),
         "ecmc_batch_log": (i32, [vp, vp]),
         "ecmc_batch_audit": (i32, [vp, ctypes.POINTER(u64)]),
+        "ecmc_batch_snapshot": (i32, [vp, vp, pi64]),
+        "ecmc_batch_restore": (i32, [vp, vp, i64]),
         "ecmc_energy_workspace_bytes": (i32, [vp, i32, pi64]),
         "ecmc_energy_forward": (i32, [vp, i32, vp, vp, vp, d, vp, vp, vp]),
         "ecmc_energy_backward": (i32, [vp, i32, vp, vp, vp, d, vp, vp, vp, vp, vp]),

@@ -174,6 +174,19 @@ class DeviceBatch:
         L.check(self._L.ecmc_batch_audit(self._h, ctypes.byref(v)))
         return v.value
 
+    def snapshot(self):
+        """Every replica's state and all positions as bytes (ecmc_batch_snapshot)."""
+        n = ctypes.c_int64()
+        L.check(self._L.ecmc_batch_snapshot(self._h, None, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        L.check(self._L.ecmc_batch_snapshot(self._h, buf, ctypes.byref(n)))
+        return buf.raw
+
+    def restore(self, snapshot):
+        """Continue from a snapshot of a batch of the same description (ecmc_batch_restore)."""
+        buf = ctypes.create_string_buffer(bytes(snapshot), len(snapshot))
+        L.check(self._L.ecmc_batch_restore(self._h, buf, len(snapshot)))
+
 
 def pressure(est):
     """beta P / rho = 1 + sum of lifting separations / total chain displacement, per replica."""

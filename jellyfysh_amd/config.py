@@ -491,3 +491,75 @@ class SingleProcessMediator:
                     self.writers[last].write(pos, self.config.system_length)
                 if on_sample is not None and (last < 0 or last in self.sampling):
                     on_sample(pos if pos is not None else self.chain.positions())
+
+
+class BatchedRunConfig:
+    """An INI file of the batched form (no reference counterpart as a whole; read with the
+    reference's factory conventions, base/factory.py:41-151): [Run] mediator =
+    batched_replicas_mediator, the [HypercubicSetting]'s beta and dimension 3, and a
+    [BatchedReplicasMediator] section with the replica set (number_of_replicas, replica sizes
+    log-uniform in [minimum_number_of_units, maximum_number_of_units], densities uniform in
+    [minimum_density, maximum_density], lattice starts with a jitter, drawn from
+    workload_seed), chain_length, seed (the replicas' Philox key) and one or two potentials,
+    each a truncated_inverse_power_potential of power 6 or 12 with one common cutoff.
+
+    ``workload()`` gives (n, system lengths, positions); ``batch_arguments()`` the keyword
+    arguments of jellyfysh_amd.batch.DeviceBatch; ``batch()`` builds it."""
+
+    def __init__(self, parser):
+        self.cp = parser
+        run = RunConfig._section(self, "Run", {"mediator", "setting"})
+        if split_alias(run.get("mediator", ""))[1] != "batched_replicas_mediator":
+            raise ConfigurationError("not a batched run (mediator = batched_replicas_mediator)")
+        s = RunConfig._section(self, "HypercubicSetting", {"beta", "dimension", "system_length"})
+        if int(s.get("dimension", 3)) != 3:
+            raise ConfigurationError("the batched replicas are three-dimensional")
+        if "system_length" in s:
+            raise ConfigurationError("each replica's system length follows from its size and density")
+        self.beta = float(s["beta"])
+        m = RunConfig._section(self, "BatchedReplicasMediator",
+                               {"number_of_replicas", "minimum_number_of_units", "maximum_number_of_units",
+                                "minimum_density", "maximum_density", "jitter", "workload_seed", "chain_length",
+                                "seed", "potentials"})
+        self.number_of_replicas = int(m["number_of_replicas"])
+        self.n_min, self.n_max = int(m["minimum_number_of_units"]), int(m["maximum_number_of_units"])
+        self.rho_min, self.rho_max = float(m["minimum_density"]), float(m["maximum_density"])
+        self.jitter = float(m.get("jitter", 0.01))
+        self.workload_seed = int(m.get("workload_seed", 0))
+        self.chain_length = float(m.get("chain_length", 1.0))
+        self.seed = int(m.get("seed", 1))
+        self.factors, cutoffs = [], set()
+        for item in split_list(m["potentials"]):
+            alias, cls = split_alias(item)
+            if cls != "truncated_inverse_power_potential":
+                raise ConfigurationError("the batched replicas take truncated_inverse_power_potential (got %s)" % cls)
+            sec = RunConfig._section(self, camel(alias), {"power", "prefactor", "cutoff"})
+            p, k, rc = float(sec["power"]), float(sec["prefactor"]), float(sec["cutoff"])
+            if p not in (6.0, 12.0):
+                raise ConfigurationError("powers 6 and 12 are supported")
+            self.factors.append((p, k))
+            cutoffs.add(rc)
+        if not 1 <= len(self.factors) <= 2 or len(cutoffs) != 1:
+            raise ConfigurationError("one or two factors with one common cutoff")
+        self.cutoff = cutoffs.pop()
+
+    @classmethod
+    def from_file(cls, path):
+        cp = configparser.ConfigParser()
+        with open(path) as f:
+            cp.read_file(f)
+        return cls(cp)
+
+    def workload(self):
+        from .batch import batched_lj_workload
+        return batched_lj_workload(self.number_of_replicas, self.n_min, self.n_max, self.rho_min, self.rho_max,
+                                   seed=self.workload_seed, jitter=self.jitter)
+
+    def batch_arguments(self):
+        return dict(beta=self.beta, cutoff=self.cutoff, chain_length=self.chain_length, factors=tuple(self.factors),
+                    seed=self.seed)
+
+    def batch(self, classes=None):
+        from .batch import DeviceBatch
+        n, lengths, pos = self.workload()
+        return DeviceBatch(n, lengths, pos, classes=classes, **self.batch_arguments())

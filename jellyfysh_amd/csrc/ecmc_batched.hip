@@ -491,6 +491,72 @@ extern "C" int ecmc_batch_log(const ecmc_batch *b, ecmc_batch_event *host_log) {
     return ECMC_OK;
 }
 
+// Snapshot layout: header, the replicas' device records (state, Philox leg counters,
+// estimators, statistics), the SoA positions.
+struct BatchSnapshotHeader {
+    uint64_t magic;
+    int32_t n_rep, version;
+    int64_t total_units;
+    ecmc_batch_desc desc;
+};
+static const uint64_t SNAPSHOT_MAGIC = 0x45434d4342415443ull;   // "ECMCBATC"
+
+static size_t snapshot_bytes(const ecmc_batch *b) {
+    return sizeof(BatchSnapshotHeader) + sizeof(ReplicaDev) * (size_t)b->n_rep + sizeof(double) * 3 * (size_t)b->total_units;
+}
+
+extern "C" int ecmc_batch_snapshot(const ecmc_batch *b, void *buffer, int64_t *bytes) {
+    if (!b || !bytes) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    const size_t need = snapshot_bytes(b);
+    if (!buffer) {
+        *bytes = (int64_t)need;
+        return ECMC_OK;
+    }
+    if (*bytes < (int64_t)need) return ecmc_fail(ECMC_ERR_ARGUMENT, "snapshot buffer too small");
+    if (hipDeviceSynchronize() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "device synchronisation failed");
+    BatchSnapshotHeader h;
+    memset(&h, 0, sizeof(h));
+    h.magic = SNAPSHOT_MAGIC;
+    h.n_rep = b->n_rep;
+    h.version = 1;
+    h.total_units = b->total_units;
+    h.desc = b->desc;
+    char *o = (char *)buffer;
+    memcpy(o, &h, sizeof(h));
+    o += sizeof(h);
+    if (hipMemcpy(o, b->d_rep, sizeof(ReplicaDev) * b->n_rep, hipMemcpyDeviceToHost) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+    o += sizeof(ReplicaDev) * b->n_rep;
+    if (hipMemcpy(o, b->d_pos, sizeof(double) * 3 * b->total_units, hipMemcpyDeviceToHost) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+    *bytes = (int64_t)need;
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_batch_restore(ecmc_batch *b, const void *buffer, int64_t bytes) {
+    if (!b || !buffer) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    if (bytes != (int64_t)snapshot_bytes(b)) return ecmc_fail(ECMC_ERR_ARGUMENT, "snapshot size does not match the batch");
+    BatchSnapshotHeader h;
+    memcpy(&h, buffer, sizeof(h));
+    if (h.magic != SNAPSHOT_MAGIC || h.version != 1) return ecmc_fail(ECMC_ERR_ARGUMENT, "not a batch snapshot");
+    if (h.n_rep != b->n_rep || h.total_units != b->total_units || memcmp(&h.desc, &b->desc, sizeof(h.desc)) != 0)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the snapshot belongs to another batch configuration");
+    const char *in = (const char *)buffer + sizeof(h);
+    std::vector<ReplicaDev> rep((size_t)b->n_rep);
+    memcpy(rep.data(), in, sizeof(ReplicaDev) * b->n_rep);
+    for (int r = 0; r < b->n_rep; r++) {
+        const ReplicaDev &x = rep[r], &y = b->rep_host[r];
+        if (x.n != y.n || x.off != y.off || x.L != y.L || x.cps != y.cps || x.bound_off != y.bound_off)
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "the snapshot's replica sizes or boxes differ from the batch");
+    }
+    in += sizeof(ReplicaDev) * b->n_rep;
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(b->d_rep, rep.data(), sizeof(ReplicaDev) * b->n_rep, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_pos, in, sizeof(double) * 3 * b->total_units, hipMemcpyHostToDevice) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+    return ECMC_OK;
+}
+
 extern "C" int ecmc_batch_audit(const ecmc_batch *b, uint64_t *violations) {
     if (!b || !violations) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
     unsigned long long v = 0;

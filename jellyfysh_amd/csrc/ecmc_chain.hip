@@ -726,18 +726,241 @@ __device__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_
     return true;
 }
 
+// Shared memory of the leg loop's collective steps.
+struct LegShared {
+    int nb_cell[32];
+    int nb_pref[33];
+    Best sb[NT / 64];
+    uint32_t sh[NT / 64];
+    int scan[NT + 1];
+    int flag;
+};
+
+// The event handlers of tagger t send their candidate event times: the pair taggers' candidate
+// sets over the workgroup, the cell veto and the end of chain with their bounded ring refills,
+// the others on thread 0 (collective; tag_activator.py:228-289 for one tag of a create list).
+template <int D>
+__device__ void create_tagger(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &c, int t,
+                              LegShared &ls) {
+    const int tid = threadIdx.x;
+    if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL || c.kind == ECMC_TAG_SURPLUS ||
+        c.kind == ECMC_TAG_CELL_BOUNDING || c.kind == ECMC_TAG_FACTOR_MAP) {
+        create_pair<D>(a, st, mt, c, t, ls.nb_cell, ls.nb_pref, ls.sb, ls.sh, ls.scan);
+    } else if (c.kind == ECMC_TAG_CELL_VETO) {
+        if (tid == 0) { st.slot[t].valid = 0; st.slot[t].cv_phase = 0; }
+        __syncthreads();
+        // the same bounded refill loop as the end-of-chain randint below
+        for (int attempt = 0;; attempt++) {
+            ring_ensure(st, mt, a, 64);
+            if (tid == 0) {
+                ls.flag = (st.active < 0 || create_cell_veto<D>(a, st, c, t)) ? 1 : 0;
+                if (!ls.flag && attempt >= 64) { st.err = ECMC_ERR_STATE; ls.flag = 1; }
+            }
+            __syncthreads();
+            if (ls.flag) break;
+            ring_ensure(st, mt, a, 624);
+        }
+    } else if (c.kind == ECMC_TAG_END_OF_CHAIN) {
+        if (tid == 0) st.slot[t].valid = 0;
+        __syncthreads();
+        // randint's rejection loop: each word is accepted with probability >= 1/2,
+        // so 64 refills (40k words) without success only happen after a defect
+        for (int attempt = 0;; attempt++) {
+            ring_ensure(st, mt, a, 64);
+            if (tid == 0) {
+                ls.flag = create_end_of_chain(a, st, c, t) ? 1 : 0;
+                if (!ls.flag && attempt >= 64) { st.err = ECMC_ERR_STATE; ls.flag = 1; }
+            }
+            __syncthreads();
+            if (ls.flag) break;
+            ring_ensure(st, mt, a, 624);
+        }
+    } else {
+        if (tid == 0) create_single<D>(a, st, c, t);
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ bool commit_thins(const ecmc_tagger_desc &T) {
+    return ((T.kind == ECMC_TAG_PAIR_CELLS || T.kind == ECMC_TAG_PAIR_ALL || T.kind == ECMC_TAG_SURPLUS ||
+             T.kind == ECMC_TAG_CELL_BOUNDING || T.kind == ECMC_TAG_FACTOR_MAP) &&
+            (T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING || T.event_handler == ECMC_EH_CELL_BOUNDING)) ||
+           T.kind == ECMC_TAG_CELL_VETO;
+}
+
+// The out-state of tagger w's stored event, committed to the state (thread 0; the caller made
+// two ring words available when the event thins): send_out_state of its event handler and
+// insert_into_global_state (mediator/single_process_mediator.py:130-150).
+template <int D>
+__device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &T, int w, ecmc_event &ev) {
+    Slot &sl = st.slot[w];
+    ev.tagger = w;
+    ev.flags = 0;
+    ev.active = st.active;
+    ev.target = -1;
+    ev.tq = sl.t.q;
+    ev.tr = sl.t.r;
+    ev.ncand = st.ncand;
+    ev.chash = st.chash;
+    const int act = st.active;
+    double p[D];
+    switch (T.kind) {
+    case ECMC_TAG_PAIR_CELLS:
+    case ECMC_TAG_PAIR_ALL:
+    case ECMC_TAG_SURPLUS:
+    case ECMC_TAG_CELL_BOUNDING:
+    case ECMC_TAG_FACTOR_MAP: {
+        ev.target = sl.target;
+        bool lift = true;
+        double s[D];
+        separation_vector<D>(a, sl.apos, sl.tpos, s);
+        if (T.event_handler == ECMC_EH_CELL_BOUNDING) {
+            // two_leaf_unit_cell_bounding_potential_event_handler.py:181-211: the active
+            // unit is still in its cell; bound = the candidate's rate; thinning as below
+            if (pos_to_cell<D>(a, sl.apos) != sl.cb_cell) { st.err = ECMC_ERR_STATE; break; }
+            double q3[3] = {0, 0, 0};
+            for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
+            const double c1 = T.use_charge ? a.charge[act] : 1.0;
+            const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
+            double bound = cell_rate<D>(a, T, sl.cb_cell, sl.target, sl.adir, c1, c2) * sl.aspeed;
+            double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+            lift = false;
+            if (real > 0) {
+                double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
+                st.cursor += 2;
+                u = 0 + (bound - 0) * u;
+                if (u < real) lift = true;
+            }
+            if (!lift) st.est[6] += 1.0;
+        } else if (T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
+            // event_handler_with_bounding_potential.py:75-101
+            double q3[3] = {0, 0, 0};
+            for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
+            const double c1 = T.use_charge ? a.charge[act] : 1.0;
+            const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
+            double bound = ipcb_derivative(T.bpar[0] * c1 * c2, q3[0], q3[1], q3[2]) * sl.aspeed;
+            double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+            lift = false;
+            if (real > 0) {
+                double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
+                st.cursor += 2;
+                u = 0 + (bound - 0) * u;
+                if (u < real) lift = true;
+            }
+            if (!lift) st.est[6] += 1.0;
+        }
+        store_pos<D>(a, act, sl.apos);
+        st.at = sl.astamp;
+        if (lift) {
+            st.est[0] += s[sl.adir];
+            st.est[4] += 1.0;
+            st.active = sl.target;
+            ev.flags |= 1;
+        }
+        st.est[2] += 1.0;
+        break;
+    }
+    case ECMC_TAG_CELL_VETO: {
+        // mediator.py:265-292: the occupant of the sampled cell, or none;
+        // leaf_unit_cell_veto_event_handler.py:127-149 with
+        // event_handler_with_bounding_potential.py:75-101: thinning of the stored rate
+        const int tc = sl.cb_cell;
+        const int no = a.occ_n[tc];
+        if (no > 1) { st.err = ECMC_ERR_STATE; break; }
+        bool lift = false;
+        double s[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) s[d] = 0.0;
+        if (no == 1) {
+            const int tid2 = a.occ[(size_t)tc * a.occ_cap];
+            ev.target = tid2;
+            double tp[D];
+            load_pos<D>(a, tid2, tp);
+            separation_vector<D>(a, sl.apos, tp, s);
+            double q3[3] = {0, 0, 0};
+            for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
+            const double c1 = T.use_charge ? a.charge[act] : 1.0;
+            const double c2 = T.use_charge ? a.charge[tid2] : 1.0;
+            const double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+            if (real > 0) {
+                double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
+                st.cursor += 2;
+                u = 0 + (sl.cv_rate - 0) * u;
+                if (u < real) lift = true;
+            }
+        }
+        store_pos<D>(a, act, sl.apos);
+        st.at = sl.astamp;
+        if (lift) {
+            st.est[0] += s[sl.adir];
+            st.est[4] += 1.0;
+            st.active = ev.target;
+            ev.flags |= 1;
+        } else {
+            st.est[6] += 1.0;
+        }
+        st.est[2] += 1.0;
+        break;
+    }
+    case ECMC_TAG_CELL_BOUNDARY: {
+#pragma unroll
+        for (int d = 0; d < D; d++) p[d] = sl.apos[d];
+        slice_pos<D>(a, p, sl.adir, sl.aspeed, sl.astamp, sl.t);
+        p[sl.cb_dir] = sl.cb_boundary;
+        store_pos<D>(a, act, p);
+        st.at = sl.t;
+        st.est[3] += 1.0;
+        break;
+    }
+    case ECMC_TAG_SAMPLING:
+    case ECMC_TAG_END_OF_RUN: {
+        load_pos<D>(a, act, p);
+        slice_pos<D>(a, p, st.adir, st.aspeed, st.at, sl.t);
+        store_pos<D>(a, act, p);
+        st.at = sl.t;
+        if (T.kind == ECMC_TAG_END_OF_RUN) st.done = 1;
+        break;
+    }
+    case ECMC_TAG_END_OF_CHAIN: {
+        load_pos<D>(a, act, p);
+        slice_pos<D>(a, p, st.adir, st.aspeed, st.at, sl.t);
+        store_pos<D>(a, act, p);
+        st.at = sl.t;
+        st.eoc_last = sl.t;
+        ev.target = sl.target;
+        st.est[1] += T.value * st.aspeed;
+        st.est[5] += 1.0;
+        st.adir = (st.adir + 1) % D;
+        if (sl.target != act) ev.flags |= 1;
+        st.active = sl.target;
+        break;
+    }
+    case ECMC_TAG_START_OF_RUN:
+        st.active = T.ivalue2;
+        st.adir = T.ivalue;
+        st.aspeed = T.value;
+        st.at.q = 0.0;
+        st.at.r = 0.0;
+        break;
+    default:
+        st.err = ECMC_ERR_CONFIGURATION;
+    }
+    ev.pos[0] = ev.pos[1] = ev.pos[2] = 0.0;
+    if (act >= 0) {
+        double q[D];
+        load_pos<D>(a, act, q);
+        for (int d = 0; d < D; d++) ev.pos[d] = q[d];
+    }
+}
+
 // One workgroup runs one system's legs (the whole leg loop of the reference's mediator).
 template <int D>
 __device__ __forceinline__ void chain_body(const ChainArgs &a) {
     __shared__ ChainState st;
     __shared__ ecmc_tagger_desc tg[ECMC_MAX_TAGGERS];
     __shared__ uint32_t mt[624];
-    __shared__ int nb_cell[32];
-    __shared__ int nb_pref[33];
-    __shared__ Best sb[NT / 64];
-    __shared__ uint32_t sh[NT / 64];
-    __shared__ int scan[NT + 1];
-    __shared__ int flag;
+    __shared__ LegShared ls;
+    int &flag = ls.flag;
     const int tid = threadIdx.x;
     if (tid == 0) st = *a.st;
     for (int i = tid; i < a.n_taggers; i += NT) tg[i] = a.tg[i];
@@ -771,43 +994,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
             const ecmc_tagger_desc &w = tg[st.last_tagger];
             for (int ci = 0; ci < w.n_create && !st.err; ci++) {
                 const int t = w.create[ci];
-                const ecmc_tagger_desc &c = tg[t];
-                if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL || c.kind == ECMC_TAG_SURPLUS ||
-                    c.kind == ECMC_TAG_CELL_BOUNDING || c.kind == ECMC_TAG_FACTOR_MAP) {
-                    create_pair<D>(a, st, mt, c, t, nb_cell, nb_pref, sb, sh, scan);
-                } else if (c.kind == ECMC_TAG_CELL_VETO) {
-                    if (tid == 0) { st.slot[t].valid = 0; st.slot[t].cv_phase = 0; }
-                    __syncthreads();
-                    // the same bounded refill loop as the end-of-chain randint below
-                    for (int attempt = 0;; attempt++) {
-                        ring_ensure(st, mt, a, 64);
-                        if (tid == 0) {
-                            flag = (st.active < 0 || create_cell_veto<D>(a, st, c, t)) ? 1 : 0;
-                            if (!flag && attempt >= 64) { st.err = ECMC_ERR_STATE; flag = 1; }
-                        }
-                        __syncthreads();
-                        if (flag) break;
-                        ring_ensure(st, mt, a, 624);
-                    }
-                } else if (c.kind == ECMC_TAG_END_OF_CHAIN) {
-                    if (tid == 0) st.slot[t].valid = 0;
-                    __syncthreads();
-                    // randint's rejection loop: each word is accepted with probability >= 1/2,
-                    // so 64 refills (40k words) without success only happen after a defect
-                    for (int attempt = 0;; attempt++) {
-                        ring_ensure(st, mt, a, 64);
-                        if (tid == 0) {
-                            flag = create_end_of_chain(a, st, c, t) ? 1 : 0;
-                            if (!flag && attempt >= 64) { st.err = ECMC_ERR_STATE; flag = 1; }
-                        }
-                        __syncthreads();
-                        if (flag) break;
-                        ring_ensure(st, mt, a, 624);
-                    }
-                } else {
-                    if (tid == 0) create_single<D>(a, st, c, t);
-                    __syncthreads();
-                }
+                create_tagger<D>(a, st, mt, tg[t], t, ls);
             }
             if (tid == 0) st.pending_creates = 0;
             __syncthreads();
@@ -829,174 +1016,11 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
         __syncthreads();
         if (st.err) break;
         const int w = flag;
-        const bool thinning = (tg[w].kind == ECMC_TAG_PAIR_CELLS || tg[w].kind == ECMC_TAG_PAIR_ALL ||
-                               tg[w].kind == ECMC_TAG_SURPLUS || tg[w].kind == ECMC_TAG_CELL_BOUNDING ||
-                               tg[w].kind == ECMC_TAG_FACTOR_MAP) &&
-                              (tg[w].event_handler == ECMC_EH_TWO_LEAF_BOUNDING ||
-                               tg[w].event_handler == ECMC_EH_CELL_BOUNDING) ||
-                              tg[w].kind == ECMC_TAG_CELL_VETO;
-        if (thinning) ring_ensure(st, mt, a, 2);
+        if (commit_thins(tg[w])) ring_ensure(st, mt, a, 2);
         if (tid == 0) {
-            Slot &sl = st.slot[w];
-            const ecmc_tagger_desc &T = tg[w];
             ecmc_event ev;
-            ev.tagger = w;
-            ev.flags = 0;
-            ev.active = st.active;
-            ev.target = -1;
-            ev.tq = sl.t.q;
-            ev.tr = sl.t.r;
-            ev.ncand = st.ncand;
-            ev.chash = st.chash;
-            const int act = st.active;
-            double p[D];
-            switch (T.kind) {
-            case ECMC_TAG_PAIR_CELLS:
-            case ECMC_TAG_PAIR_ALL:
-            case ECMC_TAG_SURPLUS:
-            case ECMC_TAG_CELL_BOUNDING:
-            case ECMC_TAG_FACTOR_MAP: {
-                ev.target = sl.target;
-                bool lift = true;
-                double s[D];
-                separation_vector<D>(a, sl.apos, sl.tpos, s);
-                if (T.event_handler == ECMC_EH_CELL_BOUNDING) {
-                    // two_leaf_unit_cell_bounding_potential_event_handler.py:181-211: the active
-                    // unit is still in its cell; bound = the candidate's rate; thinning as below
-                    if (pos_to_cell<D>(a, sl.apos) != sl.cb_cell) { st.err = ECMC_ERR_STATE; break; }
-                    double q3[3] = {0, 0, 0};
-                    for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
-                    const double c1 = T.use_charge ? a.charge[act] : 1.0;
-                    const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
-                    double bound = cell_rate<D>(a, T, sl.cb_cell, sl.target, sl.adir, c1, c2) * sl.aspeed;
-                    double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
-                    lift = false;
-                    if (real > 0) {
-                        double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
-                        st.cursor += 2;
-                        u = 0 + (bound - 0) * u;
-                        if (u < real) lift = true;
-                    }
-                    if (!lift) st.est[6] += 1.0;
-                } else if (T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
-                    // event_handler_with_bounding_potential.py:75-101
-                    double q3[3] = {0, 0, 0};
-                    for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
-                    const double c1 = T.use_charge ? a.charge[act] : 1.0;
-                    const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
-                    double bound = ipcb_derivative(T.bpar[0] * c1 * c2, q3[0], q3[1], q3[2]) * sl.aspeed;
-                    double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
-                    lift = false;
-                    if (real > 0) {
-                        double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
-                        st.cursor += 2;
-                        u = 0 + (bound - 0) * u;
-                        if (u < real) lift = true;
-                    }
-                    if (!lift) st.est[6] += 1.0;
-                }
-                store_pos<D>(a, act, sl.apos);
-                st.at = sl.astamp;
-                if (lift) {
-                    st.est[0] += s[sl.adir];
-                    st.est[4] += 1.0;
-                    st.active = sl.target;
-                    ev.flags |= 1;
-                }
-                st.est[2] += 1.0;
-                break;
-            }
-            case ECMC_TAG_CELL_VETO: {
-                // mediator.py:265-292: the occupant of the sampled cell, or none;
-                // leaf_unit_cell_veto_event_handler.py:127-149 with
-                // event_handler_with_bounding_potential.py:75-101: thinning of the stored rate
-                const int tc = sl.cb_cell;
-                const int no = a.occ_n[tc];
-                if (no > 1) { st.err = ECMC_ERR_STATE; break; }
-                bool lift = false;
-                double s[D];
-#pragma unroll
-                for (int d = 0; d < D; d++) s[d] = 0.0;
-                if (no == 1) {
-                    const int tid2 = a.occ[(size_t)tc * a.occ_cap];
-                    ev.target = tid2;
-                    double tp[D];
-                    load_pos<D>(a, tid2, tp);
-                    separation_vector<D>(a, sl.apos, tp, s);
-                    double q3[3] = {0, 0, 0};
-                    for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
-                    const double c1 = T.use_charge ? a.charge[act] : 1.0;
-                    const double c2 = T.use_charge ? a.charge[tid2] : 1.0;
-                    const double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
-                    if (real > 0) {
-                        double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
-                        st.cursor += 2;
-                        u = 0 + (sl.cv_rate - 0) * u;
-                        if (u < real) lift = true;
-                    }
-                }
-                store_pos<D>(a, act, sl.apos);
-                st.at = sl.astamp;
-                if (lift) {
-                    st.est[0] += s[sl.adir];
-                    st.est[4] += 1.0;
-                    st.active = ev.target;
-                    ev.flags |= 1;
-                } else {
-                    st.est[6] += 1.0;
-                }
-                st.est[2] += 1.0;
-                break;
-            }
-            case ECMC_TAG_CELL_BOUNDARY: {
-#pragma unroll
-                for (int d = 0; d < D; d++) p[d] = sl.apos[d];
-                slice_pos<D>(a, p, sl.adir, sl.aspeed, sl.astamp, sl.t);
-                p[sl.cb_dir] = sl.cb_boundary;
-                store_pos<D>(a, act, p);
-                st.at = sl.t;
-                st.est[3] += 1.0;
-                break;
-            }
-            case ECMC_TAG_SAMPLING:
-            case ECMC_TAG_END_OF_RUN: {
-                load_pos<D>(a, act, p);
-                slice_pos<D>(a, p, st.adir, st.aspeed, st.at, sl.t);
-                store_pos<D>(a, act, p);
-                st.at = sl.t;
-                if (T.kind == ECMC_TAG_END_OF_RUN) st.done = 1;
-                break;
-            }
-            case ECMC_TAG_END_OF_CHAIN: {
-                load_pos<D>(a, act, p);
-                slice_pos<D>(a, p, st.adir, st.aspeed, st.at, sl.t);
-                store_pos<D>(a, act, p);
-                st.at = sl.t;
-                st.eoc_last = sl.t;
-                ev.target = sl.target;
-                st.est[1] += T.value * st.aspeed;
-                st.est[5] += 1.0;
-                st.adir = (st.adir + 1) % D;
-                if (sl.target != act) ev.flags |= 1;
-                st.active = sl.target;
-                break;
-            }
-            case ECMC_TAG_START_OF_RUN:
-                st.active = T.ivalue2;
-                st.adir = T.ivalue;
-                st.aspeed = T.value;
-                st.at.q = 0.0;
-                st.at.r = 0.0;
-                break;
-            default:
-                st.err = ECMC_ERR_CONFIGURATION;
-            }
-            ev.pos[0] = ev.pos[1] = ev.pos[2] = 0.0;
-            if (act >= 0) {
-                double q[D];
-                load_pos<D>(a, act, q);
-                for (int d = 0; d < D; d++) ev.pos[d] = q[d];
-            }
+            commit_event<D>(a, st, tg[w], w, ev);
+            const ecmc_tagger_desc &T = tg[w];
             // trash by tag
             for (int i = 0; i < T.n_trash; i++) st.slot[T.trash[i]].valid = 0;
             if (a.log) a.log[k] = ev;
@@ -1020,6 +1044,51 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
 template <int D>
 __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
     chain_body<D>(a);
+}
+
+// One step of a leg driven from the host, for the set-level event handler (the mediator and
+// its scheduler stay on the host, the state and the candidate sets on the device):
+//   op 0  the start of a leg: the cell occupancy follows the active unit and the candidate
+//         statistics restart (the activator's internal-state update, tag_activator.py:205-226);
+//   op 1  send_event_time of tagger t's handlers: its candidates' times, the earliest kept in
+//         slot t (time, target, in-state copies);
+//   op 2  send_out_state of tagger t's stored event, committed to the state; *out receives it.
+template <int D>
+__global__ __launch_bounds__(NT) void chain_step_kernel(ChainArgs a, int op, int t, ecmc_event *out) {
+    __shared__ ChainState st;
+    __shared__ ecmc_tagger_desc tg[ECMC_MAX_TAGGERS];
+    __shared__ uint32_t mt[624];
+    __shared__ LegShared ls;
+    const int tid = threadIdx.x;
+    if (tid == 0) st = *a.st;
+    for (int i = tid; i < a.n_taggers; i += NT) tg[i] = a.tg[i];
+    for (int i = tid; i < 624; i += NT) mt[i] = a.mt[i];
+    __syncthreads();
+    if (op == 0) {
+        if (tid == 0) {
+            if (a.occupancy && st.active >= 0) occupancy_update<D>(a, st);
+            st.ncand = 0;
+            st.chash = 0;
+        }
+        __syncthreads();
+    } else if (op == 1) {
+        // the occupancy follows the active unit here too (a no-op after op 0 in the same leg), so
+        // that a cell tagger never reads the cell of a unit that stopped being active
+        if (tid == 0 && a.occupancy && st.active >= 0) occupancy_update<D>(a, st);
+        __syncthreads();
+        create_tagger<D>(a, st, mt, tg[t], t, ls);
+    } else {
+        if (commit_thins(tg[t])) ring_ensure(st, mt, a, 2);
+        if (tid == 0) {
+            ecmc_event ev;
+            commit_event<D>(a, st, tg[t], t, ev);
+            st.last = st.slot[t].t;
+            *out = ev;
+        }
+    }
+    __syncthreads();
+    if (tid == 0) *a.st = st;
+    for (int i = tid; i < 624; i += NT) a.mt[i] = mt[i];
 }
 
 // Independent systems in one launch: workgroup b runs system b (its own positions, cell
@@ -1160,6 +1229,7 @@ struct ecmc_system {
     ecmc_event *d_log = nullptr;
     int64_t log_cap = 0;
     int64_t *d_nout = nullptr;
+    ecmc_event *d_step_ev = nullptr;   // the event of the last host-driven commit
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -1404,6 +1474,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     ECMC_ALLOC(s->d_st, sizeof(ChainState));
     ECMC_ALLOC(s->d_tg, sizeof(ecmc_tagger_desc) * ECMC_MAX_TAGGERS);
     ECMC_ALLOC(s->d_nout, sizeof(int64_t));
+    ECMC_ALLOC(s->d_step_ev, sizeof(ecmc_event));
     if (!ewald.empty()) { ECMC_ALLOC(s->d_ewald, sizeof(double) * ewald.size()); }
     ECMC_ALLOC(s->d_sur_cell, sizeof(int32_t) * n);
     ECMC_ALLOC(s->d_sur_id, sizeof(int32_t) * n);
@@ -1562,7 +1633,7 @@ extern "C" void ecmc_system_destroy(ecmc_system *s) {
     hipFree(s->d_occ); hipFree(s->d_occ_n); hipFree(s->d_mt); hipFree(s->d_ring); hipFree(s->d_st);
     hipFree(s->d_sur_cell); hipFree(s->d_sur_id); hipFree(s->d_cand); hipFree(s->d_cbt); hipFree(s->d_corners);
     hipFree(s->d_cvw);
-    hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald);
+    hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald); hipFree(s->d_step_ev);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -1722,6 +1793,54 @@ extern "C" int ecmc_systems_run_noalloc(ecmc_system *const *systems, int32_t cou
     }
     return systems_run_staged(systems, count, max_events, n_committed, g_noalloc_host,
                               (ChainArgs *)g_scratch_ptr[dev]);
+}
+
+// ---- the leg driven step by step from the host (set-level event handler) -------------------
+static int system_step(ecmc_system *s, int op, int tagger, ChainState &st) {
+    if (!s) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
+    if (op != 0 && (tagger < 0 || tagger >= s->desc.n_taggers)) return ecmc_fail(ECMC_ERR_ARGUMENT, "no such tagger");
+    if (s->dim == 2) hipLaunchKernelGGL(chain_step_kernel<2>, dim3(1), dim3(NT), 0, s->stream, s->args, op, tagger, s->d_step_ev);
+    else hipLaunchKernelGGL(chain_step_kernel<3>, dim3(1), dim3(NT), 0, s->stream, s->args, op, tagger, s->d_step_ev);
+    if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
+    if (hipStreamSynchronize(s->stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
+    if (hipMemcpy(&st, s->d_st, sizeof(ChainState), hipMemcpyDeviceToHost) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+    return status_of_state(st);
+}
+
+extern "C" int ecmc_system_begin_leg(ecmc_system *s) {
+    ChainState st;
+    return system_step(s, 0, 0, st);
+}
+
+extern "C" int ecmc_system_send_event_time(ecmc_system *s, int32_t tagger, double *tq, double *tr, int32_t *target,
+                                           int32_t *valid) {
+    ChainState st;
+    const int rc = system_step(s, 1, tagger, st);
+    if (rc) return rc;
+    const Slot &sl = st.slot[tagger];
+    if (tq) *tq = sl.valid ? sl.t.q : INFINITY;
+    if (tr) *tr = sl.valid ? sl.t.r : INFINITY;
+    if (target) *target = sl.valid ? sl.target : -1;
+    if (valid) *valid = sl.valid;
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_system_send_out_state(ecmc_system *s, int32_t tagger, ecmc_event *event) {
+    ChainState st;
+    if (s && tagger >= 0 && tagger < s->desc.n_taggers) {
+        // the handler must hold an event (the host's scheduler never returns one without)
+        Slot sl;
+        if (hipMemcpy(&sl, (const char *)s->d_st + offsetof(ChainState, slot) + sizeof(Slot) * tagger, sizeof(Slot),
+                      hipMemcpyDeviceToHost) != hipSuccess)
+            return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+        if (!sl.valid) return ecmc_fail(ECMC_ERR_SCHEDULER, "the event handler holds no event");
+    }
+    const int rc = system_step(s, 2, tagger, st);
+    if (rc) return rc;
+    if (event && hipMemcpy(event, s->d_step_ev, sizeof(ecmc_event), hipMemcpyDeviceToHost) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed");
+    return ECMC_OK;
 }
 
 extern "C" int ecmc_system_positions(const ecmc_system *s, double *out) {

@@ -143,9 +143,90 @@ static int launch_pair(int32_t potential, const double *par, int32_t dim, int64_
     return ECMC_OK;
 }
 
+__global__ __launch_bounds__(PT) void ewald_kernel(Ewald ew, int64_t n, const double *sep, double *out) {
+    int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
+    if (i < n) out[i] = ewald_derivative(ew, sep[3 * i], sep[3 * i + 1], sep[3 * i + 2]);
+}
+
 }  // namespace ecmc
 
 using namespace ecmc;
+
+// ---- merged-image Coulomb handle (merged_image_coulomb_potential.h:37-44) --------------------
+struct ecmc_ewald {
+    int fc = 0, pc = 0;
+    double alpha = 0.0, L = 0.0;
+    double *d_table = nullptr;
+    size_t table_n = 0;
+    int device = 0;
+};
+
+static Ewald ewald_of(const ecmc_ewald *e) {
+    Ewald w;
+    w.f = e->d_table;
+    w.fc = e->fc; w.pc = e->pc; w.L = e->L;
+    w.aol = e->alpha / e->L; w.aol2 = e->alpha * e->alpha / (e->L * e->L);
+    w.two_aol_rpi = 2.0 * e->alpha / (e->L * sqrt(M_PI));
+    w.tpol = 2.0 * M_PI / e->L;
+    return w;
+}
+
+extern "C" int ecmc_ewald_create(int32_t fourier_cutoff, int32_t position_cutoff, double alpha, double system_length,
+                                 ecmc_ewald **out) {
+    if (!out) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (fourier_cutoff < 0 || position_cutoff < 0 || !(alpha > 0.0) || !(system_length > 0.0))
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "bad Ewald parameters");
+    ecmc_ewald *e = new ecmc_ewald();
+    e->fc = fourier_cutoff; e->pc = position_cutoff; e->alpha = alpha; e->L = system_length;
+    std::vector<double> table;
+    build_ewald_table(fourier_cutoff, alpha, system_length, table);
+    e->table_n = table.size();
+    if (hipGetDevice(&e->device) != hipSuccess ||
+        hipMalloc((void **)&e->d_table, sizeof(double) * table.size()) != hipSuccess ||
+        hipMemcpy(e->d_table, table.data(), sizeof(double) * table.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        ecmc_ewald_destroy(e);
+        return ecmc_fail(ECMC_ERR_HIP, "Ewald table upload failed");
+    }
+    *out = e;
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_ewald_copy(const ecmc_ewald *src, ecmc_ewald **out) {
+    if (!src || !out) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    ecmc_ewald *e = new ecmc_ewald(*src);
+    e->d_table = nullptr;
+    if (hipMalloc((void **)&e->d_table, sizeof(double) * src->table_n) != hipSuccess ||
+        hipMemcpy(e->d_table, src->d_table, sizeof(double) * src->table_n, hipMemcpyDeviceToDevice) != hipSuccess) {
+        ecmc_ewald_destroy(e);
+        return ecmc_fail(ECMC_ERR_HIP, "Ewald table copy failed");
+    }
+    *out = e;
+    return ECMC_OK;
+}
+
+extern "C" void ecmc_ewald_destroy(ecmc_ewald *e) {
+    if (!e) return;
+    hipFree(e->d_table);
+    delete e;
+}
+
+extern "C" int ecmc_ewald_size(const ecmc_ewald *e, int64_t *bytes) {
+    if (!e || !bytes) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
+    *bytes = (int64_t)(sizeof(ecmc_ewald) + sizeof(double) * e->table_n);
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_ewald_derivative(const ecmc_ewald *e, int64_t n, const double *d_sep, double *d_out, void *stream) {
+    if (!e || n < 0) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (n == 0) return ECMC_OK;
+    if (!d_sep || !d_out) return ecmc_fail(ECMC_ERR_ARGUMENT, "null device pointer");
+    hipLaunchKernelGGL(ewald_kernel, dim3((unsigned)((n + PT - 1) / PT)), dim3(PT), 0, (hipStream_t)stream, ewald_of(e),
+                       n, d_sep, d_out);
+    if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "Ewald kernel launch failed");
+    return ECMC_OK;
+}
 
 extern "C" int ecmc_device_count(void) {
     int n = 0;

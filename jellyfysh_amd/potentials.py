@@ -201,16 +201,72 @@ class InversePowerCoulombBoundingPotential(DevicePotential):
 
 class MergedImageCoulombPotential(DevicePotential):
     """merged_image_coulomb_potential.py:38-248 and its C file: the Ewald-summed directional
-    derivative of the Coulomb interaction with all images (alpha in units of 1/L)."""
+    derivative of the Coulomb interaction with all images (alpha in units of 1/L).
+
+    The instance owns an ecmc_ewald handle (its Fourier table on the device), as the reference's
+    owns the C struct: destroyed when the instance is collected (ffi.gc, :124-125), copied by
+    deepcopy through ecmc_ewald_copy (:176-209), dropped by pickling and rebuilt on unpickling
+    (:211-248).  ``estimated_size`` gives the handle's bytes."""
     code = L.POT_MERGED_IMAGE_COULOMB
     charged = True
     takes_potential_change = False
 
     def __init__(self, alpha=3.45, fourier_cutoff=6, position_cutoff=2, prefactor=1.0, system_length=1.0):
         super().__init__([alpha, fourier_cutoff, position_cutoff, prefactor], system_length)
+        self._args = (float(alpha), int(fourier_cutoff), int(position_cutoff), float(prefactor), float(system_length))
+        self._construct()
+
+    def _construct(self, source=None):
+        import weakref
+        h = ctypes.c_void_p()
+        if source is None:
+            alpha, fc, pc, _, length = self._args
+            L.check(L.lib().ecmc_ewald_create(fc, pc, alpha, length, ctypes.byref(h)))
+        else:
+            L.check(L.lib().ecmc_ewald_copy(source, ctypes.byref(h)))
+        if not h.value:
+            raise MemoryError("the merged-image Coulomb handle could not be built")
+        self._handle = h
+        self._finalizer = weakref.finalize(self, L.lib().ecmc_ewald_destroy, h)
+
+    @property
+    def estimated_size(self):
+        b = ctypes.c_int64()
+        L.check(L.lib().ecmc_ewald_size(self._handle, ctypes.byref(b)))
+        return b.value
+
+    def __deepcopy__(self, memo):
+        out = MergedImageCoulombPotential.__new__(MergedImageCoulombPotential)
+        out.__dict__.update({k: v for k, v in self.__dict__.items() if k not in ("_handle", "_finalizer")})
+        out._construct(self._handle)
+        memo[id(self)] = out
+        return out
+
+    def __getstate__(self):
+        return {k: v for k, v in self.__dict__.items() if k not in ("_handle", "_finalizer", "_par")}
+
+    def __setstate__(self, state):
+        self.__dict__.update(state)
+        alpha, fc, pc, prefactor, _ = self._args
+        self._par = (ctypes.c_double * 4)(alpha, float(fc), float(pc), prefactor)
+        self._construct()
+
+    def ewald_derivatives(self, separation_along_motion):
+        """The C file's raw derivative for rows (n x 3 device tensor, x along the motion)."""
+        import torch
+        sep = separation_along_motion.to(torch.float64).contiguous()
+        out = torch.empty(sep.shape[0], dtype=torch.float64, device=sep.device)
+        L.check(L.lib().ecmc_ewald_derivative(self._handle, sep.shape[0], L.dptr(sep), L.dptr(out),
+                                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)))
+        return out
 
     def derivative(self, velocity, separation, charge_one, charge_two):
-        return self._scalar(True, velocity, separation, charge_one, charge_two, None)
+        """prefactor * c1 * c2 * derivative(permuted separation) * speed (:150-160)."""
+        import torch
+        d, speed = _axis(velocity)
+        p = [float(separation[(d + i) % 3]) for i in range(3)]
+        raw = self.ewald_derivatives(torch.tensor([p], dtype=torch.float64, device="cuda"))
+        return self._args[3] * charge_one * charge_two * float(raw.cpu()[0]) * speed
 
     def displacement(self, *args, **kwargs):
         # merged_image_coulomb_potential.py: a StandardVelocityPotential, not invertible

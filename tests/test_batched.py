@@ -241,3 +241,33 @@ def test_gpu_batched_largest_replicas_match_oracle(device, oracle):
             assert bad.size == 0, (r, f, int(bad[0]))
         assert np.array_equal(allpos[r], rep.pos)
     b.close()
+
+
+@pytest.mark.gpu
+def test_gpu_batched_snapshot_restore_continues_the_run(device):
+    """A snapshot after 400 legs restored into a fresh batch of the same description continues
+    every replica exactly as the original batch does; a snapshot of another configuration is
+    refused."""
+    from jellyfysh_amd import _lib
+    from jellyfysh_amd.batch import DeviceBatch
+    ns, lengths, pos = small_batch()
+    print("batched snapshot and restore", flush=True)
+    a = DeviceBatch(ns, lengths, pos, seed=31, classes=([16384], [1]))
+    a.run(400)
+    snap = a.snapshot()
+    a.run(500)
+    b = DeviceBatch(ns, lengths, pos, seed=31, classes=([128, 512, 4096], [1, 2, 4]))
+    b.restore(snap)
+    b.run(500)
+    sa, sb = a.states(), b.states()
+    for k in ("active", "direction", "legs", "chain_left", "est", "stats"):
+        assert np.array_equal(sa[k], sb[k]), k
+    for p1, p2 in zip(a.positions(), b.positions()):
+        assert np.array_equal(p1, p2)
+    other = DeviceBatch(ns, lengths, pos, seed=32, classes=([16384], [1]))
+    with pytest.raises(_lib.EcmcError):
+        other.restore(snap)
+    with pytest.raises(_lib.EcmcError):
+        other.restore(snap[:-8])
+    for x in (a, b, other):
+        x.close()

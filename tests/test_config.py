@@ -99,3 +99,27 @@ def test_coulomb_atoms_inis_keep_the_reference_tag_order(name):
         return [ln.strip() for ln in open(path) if ln.startswith(("create", "trash"))]
     ref = "/root/reference/jellyfysh/config_files/2018_JCP_149_064113/coulomb_atoms/%s.ini" % name
     assert lists(os.path.join(CONFIGS, "coulomb_atoms_%s.ini" % name)) == lists(ref)
+
+
+def test_batched_ini_is_the_benchmark_workload():
+    """jellyfysh_amd/configs/batched_lennard_jones.ini describes the benchmark's batched
+    workload: the same replica sizes, boxes and starts as batched_lj_workload(4096, seed=2024),
+    and DeviceBatch's potential, beta, chain length and key seed 17."""
+    from jellyfysh_amd.batch import batched_lj_workload
+    from jellyfysh_amd.config import BatchedRunConfig, ConfigurationError
+    path = os.path.join(CONFIGS, "batched_lennard_jones.ini")
+    cfg = BatchedRunConfig.from_file(path)
+    assert cfg.number_of_replicas == 4096
+    assert cfg.batch_arguments() == dict(beta=1 / 1.5, cutoff=2.5, chain_length=1.0,
+                                         factors=((12.0, 4.0), (6.0, -4.0)), seed=17)
+    cfg.number_of_replicas = 64
+    n, lengths, pos = cfg.workload()
+    n2, l2, p2 = batched_lj_workload(64, seed=2024)
+    assert np.array_equal(n, n2) and np.array_equal(lengths, l2)
+    assert all(np.array_equal(a, b) for a, b in zip(pos, p2))
+    text = open(path).read()
+    import configparser
+    cp = configparser.ConfigParser()
+    cp.read_string(text.replace("power = 12", "power = 9"))
+    with pytest.raises(ConfigurationError):
+        BatchedRunConfig(cp)

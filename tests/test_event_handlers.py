@@ -1,0 +1,156 @@
+"""The set-level event handler boundary: the reference's mediator loop on the host
+(jellyfysh_amd.event_handlers.StepMediator, a list scheduler and the tag activator's create and
+trash lists) over one device event handler per tagger (ecmc_system_send_event_time /
+ecmc_system_send_out_state) commits exactly the events of the device's own leg loop
+(ecmc_system_run) and of the oracle, for hard disks, Lennard-Jones, the charged Coulomb wiring
+with the cell veto, and dipoles with factor type maps.  Also the plugin potentials
+(jellyfysh_amd.potentials) against the reference's own unit-test vectors."""
+import math
+
+import numpy as np
+import pytest
+
+import refapi
+import systems as S
+
+FIELDS = ["tagger", "flags", "active", "target", "tq", "tr", "pos"]
+
+
+def test_time_orders_like_the_reference():
+    from jellyfysh_amd.event_handlers import Time
+    a, b = Time.from_float(2.5), Time(2.0, 0.5)
+    assert a == b and not a < b and a <= b
+    assert Time(1.0, 0.9) < Time(2.0, 0.0) and Time(2.0, 0.1) < Time(2.0, 0.2)
+    assert float(Time.from_float(7.25)) == 7.25
+    assert Time(math.inf, math.inf) > Time(1e300, 0.0)
+
+
+def test_scheduler_first_push_and_trash():
+    from jellyfysh_amd.event_handlers import ListScheduler, Time
+    s = ListScheduler()
+    a, b, c = object(), object(), object()
+    s.push_event(Time(1.0, 0.5), a)
+    s.push_event(Time(1.0, 0.5), b)
+    s.push_event(Time(0.0, 0.7), c)
+    assert s.get_succeeding_event() is c
+    s.trash_event(c)
+    assert s.get_succeeding_event() is a
+    with pytest.raises(RuntimeError):
+        s.trash_event(c)
+
+
+def _same(a, b):
+    assert len(a) == len(b)
+    for f in FIELDS + ["ncand", "chash"]:
+        eq = a[f] == b[f]
+        if eq.ndim > 1:
+            eq = eq.all(axis=1)
+        bad = np.flatnonzero(~eq)
+        assert bad.size == 0, (f, int(bad[0]))
+
+
+def _cases():
+    out = []
+    meta, g = S.load_fixture("hard_disks_n64")
+    out.append(("hard_disks_n64", S.device_desc(meta, scheduler="list"), S.wiring_from_meta(meta), g["positions"],
+                None, 3000, g))
+    meta, g = S.load_fixture("lj_two_factor_n256")
+    out.append(("lj_two_factor_n256", S.device_desc(meta), S.wiring_from_meta(meta), g["positions"], None, 1500, g))
+    from jellyfysh_amd.chain import make_desc
+    meta, taggers, pos, charges = S.coulomb_cell_veto_system(n=12, seed=5)
+    out.append(("coulomb_cell_veto_n12", make_desc(3, 12, meta["system_length"], meta["beta"], meta["cells_per_side"],
+                                                   taggers, 5, scheduler="heap", max_occupants=1), taggers, pos,
+                charges, 2000, None))
+    meta, g = S.load_fixture("dipoles_factor_maps_r6")
+    out.append(("dipoles_factor_maps_r6", S.dipole_device_desc(meta), S.dipole_wiring(meta), g["positions"],
+                g["charges"], 3000, g))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(4))
+def test_gpu_step_mediator_commits_the_device_loop_events(device, case):
+    from jellyfysh_amd.chain import DeviceChain
+    from jellyfysh_amd.event_handlers import StepMediator
+    name, desc, taggers, pos, charges, legs, g = _cases()[case]
+    print("set-level handlers on the host's mediator loop: %s, %d legs" % (name, legs), flush=True)
+    stepped = DeviceChain(desc, pos, charges)
+    med = StepMediator(stepped, taggers)
+    got = np.concatenate([med.run(legs // 2), med.run(legs - legs // 2)])
+    loop = DeviceChain(desc, pos, charges)
+    want = loop.run(legs)
+    _same(got, want)
+    assert np.array_equal(stepped.positions(), loop.positions())
+    assert stepped.rng_words() == loop.rng_words()
+    assert np.array_equal(stepped.estimators(), loop.estimators())
+    if g is not None:
+        assert np.array_equal(got["target"], g["target"][:legs])
+        assert np.array_equal(got["tagger"], g["tagger"][:legs])
+
+
+@pytest.mark.gpu
+def test_gpu_plugin_potentials_reproduce_reference_vectors(device):
+    """Every class of jellyfysh_amd.potentials through its reference-style scalar call
+    (derivative / displacement with the reference's argument lists) and its array form, against
+    the expected values of the reference's potential unit tests to their own places."""
+    from jellyfysh_amd import potentials as P
+    classes = {"HardSpherePotential": P.HardSpherePotential, "InversePowerPotential": P.InversePowerPotential,
+               "LennardJonesPotential": P.LennardJonesPotential,
+               "DisplacedEvenPowerPotential": P.DisplacedEvenPowerPotential,
+               "InversePowerCoulombBoundingPotential": P.InversePowerCoulombBoundingPotential,
+               "MergedImageCoulombPotential": P.MergedImageCoulombPotential}
+    import json
+    import os
+    allv = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                       "reference_unit_vectors.json")))
+    vecs = [v for v in allv if refapi.axis_velocity(v["args"][0]) is not None]
+    print("plugin potentials: %d reference vectors" % len(vecs), flush=True)
+    checked = set()
+    for v in vecs:
+        cls = v["potential"]["cls"]
+        kw = dict(v["potential"]["kwargs"])
+        if cls in ("InversePowerCoulombBoundingPotential", "MergedImageCoulombPotential"):
+            kw["system_length"] = v["system_length"] or 1.0
+        pot = classes[cls](*v["potential"]["args"], **kw)
+        args = list(v["args"])
+        got = getattr(pot, v["method"])(*args, **v["kwargs"])
+        assert refapi.matches(got, v), (cls, v["method"], v["line"], got, v["expected"])
+        checked.add(cls)
+    assert checked == set(classes)
+
+
+@pytest.mark.gpu
+def test_gpu_ewald_handle_lifecycle(device, oracle):
+    """The merged-image Coulomb handle: its derivative equals the array path and the oracle's
+    restatement of the C file (to the device's erfc/exp/cos/sin, 1e-13), deepcopy gives an
+    independent handle with the same values, pickling rebuilds it, collecting destroys it."""
+    import copy
+    import gc
+    import pickle
+    import torch
+    from jellyfysh_amd.potentials import MergedImageCoulombPotential
+    pot = MergedImageCoulombPotential(alpha=3.45, fourier_cutoff=6, position_cutoff=2, prefactor=1.0,
+                                      system_length=8.0)
+    assert pot.estimated_size > 8 * 7 ** 3
+    rng = np.random.default_rng(4)
+    sep = rng.uniform(-4.0, 4.0, (500, 3))
+    d_sep = torch.tensor(sep, device="cuda")
+    raw = pot.ewald_derivatives(d_sep).cpu().numpy()
+    h = oracle.lib().ora_ewald_create(6, 2, 3.45, 8.0)
+    want = np.array([oracle.lib().ora_ewald_derivative(h, *s) for s in sep])
+    oracle.lib().ora_ewald_destroy(h)
+    assert np.max(np.abs(raw - want) / np.maximum(np.abs(want), 1.0)) < 1e-13
+    arr = pot.derivatives(2, 1.0, d_sep, 1.0, -1.0).cpu().numpy()
+    perm = np.stack([sep[:, 2], sep[:, 0], sep[:, 1]], 1)
+    assert np.array_equal(arr, -pot.ewald_derivatives(torch.tensor(perm, device="cuda")).cpu().numpy())
+    dup = copy.deepcopy(pot)
+    assert dup._handle.value != pot._handle.value
+    assert np.array_equal(dup.ewald_derivatives(d_sep).cpu().numpy(), raw)
+    back = pickle.loads(pickle.dumps(pot))
+    assert np.array_equal(back.ewald_derivatives(d_sep).cpu().numpy(), raw)
+    assert back.derivative([0.0, 1.0, 0.0], list(sep[0]), 1.0, 1.0) == pot.derivative([0.0, 1.0, 0.0], list(sep[0]),
+                                                                                       1.0, 1.0)
+    fin = pot._finalizer
+    del pot
+    gc.collect()
+    assert not fin.alive
