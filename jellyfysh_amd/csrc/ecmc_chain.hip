@@ -345,24 +345,36 @@ template <int D>
 __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &tg, int tix,
                             int *nb_cell, int *nb_pref, Best *sb, uint32_t *sh, int *scan) {
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        int nc = 0;
-        if (tg.kind == ECMC_TAG_PAIR_CELLS) {
-            int cell = st.occ_cell;
-            int cnt = a.nearby_n[cell];
-            nb_pref[0] = 0;
-            for (int c = 0; c < cnt; c++) {
-                int cc = a.nearby[(size_t)cell * a.nb_stride + c];
-                nb_cell[c] = cc;
-                nb_pref[c + 1] = nb_pref[c] + a.occ_n[cc];
+    if (tg.kind == ECMC_TAG_PAIR_CELLS) {
+        // the nearby cells (at most 27) and the prefix sums of their occupant counts: one lane
+        // per cell of the first wavefront, an inclusive scan by shuffles
+        if (tid < 64) {
+            const int cell = st.occ_cell;
+            const int cnt = a.nearby_n[cell];
+            int cc = 0, num = 0;
+            if (tid < cnt) {
+                cc = a.nearby[(size_t)cell * a.nb_stride + tid];
+                num = a.occ_n[cc];
             }
-            nc = cnt;
-        } else {
-            nb_pref[0] = 0;
-            nb_pref[1] = a.n;  // identifiers 0..n-1 minus the active one
-            nc = 1;
+            int incl = num;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int v = __shfl_up(incl, o, 64);
+                if (tid >= o) incl += v;
+            }
+            if (tid < cnt) {
+                nb_cell[tid] = cc;
+                nb_pref[tid + 1] = incl;
+            }
+            if (tid == 0) {
+                nb_pref[0] = 0;
+                nb_cell[31] = cnt;
+            }
         }
-        nb_cell[31] = nc;
+    } else if (tid == 0) {
+        nb_pref[0] = 0;
+        nb_pref[1] = a.n;  // identifiers 0..n-1 minus the active one
+        nb_cell[31] = 1;
     }
     __syncthreads();
     const int nc = nb_cell[31];
@@ -380,19 +392,21 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
         int cnt = 0;
         for (int c = lo; c < hi; c++)
             if (a.occ_n[c] > 0 && !is_nearby(a, D, c, ac)) cnt++;
-        scan[tid] = cnt;
-        __syncthreads();
-        if (tid == 0) {
-            int acc = 0;
-            for (int i = 0; i < NT; i++) {
-                const int v = scan[i];
-                scan[i] = acc;
-                acc += v;
-            }
-            scan[NT] = acc;
+        // exclusive scan of the counts over the workgroup: shuffles within each wavefront, the
+        // wavefront totals in scan[NT + 1 ..]
+        const int lane = tid & 63, wave = tid >> 6;
+        int incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int v = __shfl_up(incl, o, 64);
+            if (lane >= o) incl += v;
         }
+        if (lane == 63) scan[NT + 1 + wave] = incl;
         __syncthreads();
-        int k = scan[tid];
+        int before = 0;
+        for (int w = 0; w < wave; w++) before += scan[NT + 1 + w];
+        int k = before + incl - cnt;
+        if (tid == NT - 1) scan[NT] = before + incl;
         bool multi = false;
         for (int c = lo; c < hi; c++)
             if (a.occ_n[c] > 0 && !is_nearby(a, D, c, ac)) {
@@ -502,70 +516,132 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
     __syncthreads();
 }
 
-// self._surplus.setdefault(cell, []).append(id) on the flat ordered list: join the cell's
-// group or open a new group at the end (dict insertion order; empty groups are deleted).
-__device__ bool surplus_append(const ChainArgs &a, ChainState &st, int cell, int id) {
-    if (st.nsur >= a.sur_cap) return false;
-    int at = st.nsur;
-    for (int i = st.nsur - 1; i >= 0; i--)
-        if (a.sur_cell[i] == cell) { at = i + 1; break; }
-    for (int i = st.nsur; i > at; i--) { a.sur_cell[i] = a.sur_cell[i - 1]; a.sur_id[i] = a.sur_id[i - 1]; }
-    a.sur_cell[at] = cell;
-    a.sur_id[at] = id;
-    st.nsur++;
-    return true;
-}
-__device__ bool surplus_remove(const ChainArgs &a, ChainState &st, int cell, int id) {
-    for (int i = 0; i < st.nsur; i++)
-        if (a.sur_cell[i] == cell && a.sur_id[i] == id) {
-            for (int k = i; k + 1 < st.nsur; k++) { a.sur_cell[k] = a.sur_cell[k + 1]; a.sur_id[k] = a.sur_id[k + 1]; }
-            st.nsur--;
-            return true;
+// Collective: moves the entries [from, to) of the parallel arrays x and y (y may be NULL) by
+// one slot up (by > 0) or down, in chunks of one entry per thread, every chunk read before it
+// is written and the chunks ordered so that no entry is overwritten before it was read.
+__device__ void shift_entries(int32_t *x, int32_t *y, int from, int to, int by) {
+    const int tid = threadIdx.x;
+    if (by > 0) {
+        for (int top = to; top > from; top -= NT) {
+            const int lo = top - NT > from ? top - NT : from, i = lo + tid;
+            const bool act = i < top;
+            int32_t vx = 0, vy = 0;
+            if (act) { vx = x[i]; if (y) vy = y[i]; }
+            __syncthreads();
+            if (act) { x[i + 1] = vx; if (y) y[i + 1] = vy; }
+            __syncthreads();
         }
-    return false;
+    } else {
+        for (int lo = from; lo < to; lo += NT) {
+            const int i = lo + tid;
+            const bool act = i < to;
+            int32_t vx = 0, vy = 0;
+            if (act) { vx = x[i]; if (y) vy = y[i]; }
+            __syncthreads();
+            if (act) { x[i - 1] = vx; if (y) y[i - 1] = vy; }
+            __syncthreads();
+        }
+    }
 }
 
+// Collective: the largest (want_max) or smallest index i < count with x[i] == vx and, when y is
+// given, y[i] == vy; -1 when there is none.  red holds NT / 64 ints of workgroup scratch.
+__device__ int find_entry(const int32_t *x, const int32_t *y, int count, int32_t vx, int32_t vy, bool want_max,
+                          int *red) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int best = want_max ? -1 : 0x7fffffff;
+    for (int i = tid; i < count; i += NT)
+        if (x[i] == vx && (!y || y[i] == vy)) best = want_max ? (i > best ? i : best) : (i < best ? i : best);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int v = __shfl_xor(best, o, 64);
+        best = want_max ? (v > best ? v : best) : (v < best ? v : best);
+    }
+    if (lane == 0) red[wave] = best;
+    __syncthreads();
+    int r = red[0];
+    for (int w = 1; w < NT / 64; w++) r = want_max ? (red[w] > r ? red[w] : r) : (red[w] < r ? red[w] : r);
+    __syncthreads();
+    return r == 0x7fffffff ? -1 : r;
+}
+
+// Collective, single_active_cell_occupancy.py:149-203: the unit that stops being active joins
+// its cell's occupants, or the surplus when the cell is full (self._surplus.setdefault(cell,
+// []).append(id) on the flat list in dict order: after the cell's last surplus entry, or at the
+// end); the new active unit leaves the occupants or the surplus (the surplus never moves into
+// the occupants: that branch of the reference needs an empty surplus list, which it deletes
+// right away).  The list searches and shifts run over the workgroup.
 template <int D>
-__device__ void occupancy_update(const ChainArgs &a, ChainState &st) {
-    // single_active_cell_occupancy.py:149-203: the unit that stops being active joins its
-    // cell's occupants, or the surplus when the cell is full; the new active unit leaves the
-    // occupants or the surplus (the surplus never moves into the occupants: that branch of
-    // the reference needs an empty surplus list, which it deletes right away)
-    int id = st.active;
+__device__ void occupancy_update(const ChainArgs &a, ChainState &st, int *red) {
+    const int tid = threadIdx.x;
+    const int id = st.active, old_id = st.occ_id, old_cell = st.occ_cell;
+    const int n_old = (id != old_id && old_id >= 0) ? a.occ_n[old_cell] : 0;
     double p[D];
     load_pos<D>(a, id, p);
-    if (id != st.occ_id) {
-        if (st.occ_id >= 0) {
-            int c = st.occ_cell;
-            int n = a.occ_n[c];
-            if (a.max_occ <= 0 || n < a.max_occ) {
-                if (n >= a.occ_cap) { st.err = ECMC_ERR_STATE; return; }
-                a.occ[(size_t)c * a.occ_cap + n] = st.occ_id;
-                a.occ_n[c] = n + 1;
-            } else if (!surplus_append(a, st, c, st.occ_id)) {
-                st.err = ECMC_ERR_STATE;
-                return;
+    const int c = pos_to_cell<D>(a, p);
+    __syncthreads();   // every thread has read the state and the counts before thread 0 changes them
+    if (id == old_id) {
+        if (tid == 0) {
+            if (c < 0) st.err = ECMC_ERR_STATE;
+            else st.occ_cell = c;
+        }
+        __syncthreads();
+        return;
+    }
+    if (old_id >= 0) {
+        const int n = n_old;
+        if (a.max_occ <= 0 || n < a.max_occ) {
+            if (tid == 0) {
+                if (n >= a.occ_cap) st.err = ECMC_ERR_STATE;
+                else {
+                    a.occ[(size_t)old_cell * a.occ_cap + n] = old_id;
+                    a.occ_n[old_cell] = n + 1;
+                }
+            }
+        } else {
+            const int nsur = st.nsur;
+            if (nsur >= a.sur_cap) {
+                if (tid == 0) st.err = ECMC_ERR_STATE;
+            } else {
+                const int last = find_entry(a.sur_cell, nullptr, nsur, old_cell, 0, true, red);
+                const int at = last >= 0 ? last + 1 : nsur;
+                shift_entries(a.sur_cell, a.sur_id, at, nsur, +1);
+                if (tid == 0) {
+                    a.sur_cell[at] = old_cell;
+                    a.sur_id[at] = old_id;
+                    st.nsur = nsur + 1;
+                }
             }
         }
-        int c = pos_to_cell<D>(a, p);
-        if (c < 0) { st.err = ECMC_ERR_STATE; return; }
+        __syncthreads();
+        if (st.err) return;
+    }
+    if (c < 0) {
+        if (tid == 0) st.err = ECMC_ERR_STATE;
+        __syncthreads();
+        return;
+    }
+    int32_t *Lc = a.occ + (size_t)c * a.occ_cap;
+    const int n = a.occ_n[c];
+    const int i = find_entry(Lc, nullptr, n, id, 0, false, red);
+    if (i >= 0) {
+        shift_entries(Lc, nullptr, i + 1, n, -1);
+        if (tid == 0) a.occ_n[c] = n - 1;
+    } else {
+        const int nsur = st.nsur;
+        const int k = find_entry(a.sur_cell, a.sur_id, nsur, c, id, false, red);
+        if (k < 0) {
+            if (tid == 0) st.err = ECMC_ERR_STATE;
+        } else {
+            shift_entries(a.sur_cell, a.sur_id, k + 1, nsur, -1);
+            if (tid == 0) st.nsur = nsur - 1;
+        }
+    }
+    if (tid == 0) {
         st.occ_cell = c;
         st.occ_id = id;
-        int n = a.occ_n[c];
-        int32_t *L = a.occ + (size_t)c * a.occ_cap;
-        int i = 0;
-        while (i < n && L[i] != id) i++;
-        if (i == n) {
-            if (!surplus_remove(a, st, c, id)) st.err = ECMC_ERR_STATE;
-            return;
-        }
-        for (; i + 1 < n; i++) L[i] = L[i + 1];
-        a.occ_n[c] = n - 1;
-    } else {
-        int c = pos_to_cell<D>(a, p);
-        if (c < 0) { st.err = ECMC_ERR_STATE; return; }
-        st.occ_cell = c;
     }
+    __syncthreads();
 }
 
 // Creates of one non-pair tagger (thread 0 only, ring words made available by the caller).
@@ -726,13 +802,113 @@ __device__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_
     return true;
 }
 
+// ewald_derivative over one wavefront, all 64 lanes calling with the same separation: the
+// real-space images and Fourier terms are spread over the lanes (term g on lane g % 64), each
+// term computed with the same operations as in the sequential loop (the Fourier factors by
+// the same cos/sin recurrence, which every lane steps through), then added in the loop's order
+// by one lane reading the others: the result is the sequential function's, bit for bit.  More
+// than 256 terms fall back to the sequential function.
+__device__ __forceinline__ double wave_lane_double(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+__device__ double ewald_derivative_wave(const Ewald &e, double sx, double sy, double sz) {
+    const int lane = threadIdx.x & 63;
+    const int pc = e.pc, pc2 = pc * pc, fc = e.fc, fc2 = fc * fc, n = fc + 1;
+    const double L = e.L;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    int g = 0;
+    auto keep = [&](double v) {
+        if ((g & 63) == lane) {
+            const int s = g >> 6;
+            if (s == 0) t0 = v; else if (s == 1) t1 = v; else if (s == 2) t2 = v; else t3 = v;
+        }
+        g++;
+    };
+    for (int k = -pc; k < pc + 1; k++) {
+        const int cy = (int)sqrt((double)(pc2 - k * k));
+        for (int j = -cy; j < cy + 1; j++) {
+            const int cx = (int)sqrt((double)(pc2 - j * j - k * k));
+            for (int i = -cx; i < cx + 1; i++) {
+                if ((g & 63) == lane && g < 256) {
+                    const double vz2 = (sz + k * L) * (sz + k * L);
+                    const double vy2 = (sy + j * L) * (sy + j * L);
+                    const double vx = sx + i * L;
+                    const double v2 = vx * vx + vy2 + vz2;
+                    const double vn = sqrt(v2);
+                    keep(vx * (e.two_aol_rpi * exp(-e.aol2 * v2) + erfc(e.aol * vn) / vn) / v2);
+                } else {
+                    g++;
+                }
+            }
+        }
+    }
+    const int nr = g;
+    double dcx = cos(e.tpol * sx), dsx = sin(e.tpol * sx);
+    double dcy = cos(e.tpol * sy), dsy = sin(e.tpol * sy);
+    double dcz = cos(e.tpol * sz), dsz = sin(e.tpol * sz);
+    double cxv = dcx, sxv = dsx, cyv = 1.0, syv = 0.0, czv = 1.0, szv = 0.0, tmp;
+    for (int i = 1; i < fc + 1; i++) {
+        const int cy = (int)sqrt((double)(fc2 - i * i));
+        for (int j = 0; j < cy + 1; j++) {
+            const int cx = (int)sqrt((double)(fc2 - i * i - j * j));
+            for (int k = 0; k < cx + 1; k++) {
+                if ((g & 63) == lane && g < 256) keep(e.f[(i * n + j) * n + k] * sxv * cyv * czv);
+                else g++;
+                if (k != cx) {
+                    tmp = czv; czv = tmp * dcz - szv * dsz; szv = szv * dcz + tmp * dsz;
+                } else if (j != cy) {
+                    tmp = cyv; cyv = tmp * dcy - syv * dsy; syv = syv * dcy + tmp * dsy;
+                    czv = 1.0; szv = 0.0;
+                } else if (i != fc) {
+                    tmp = cxv; cxv = tmp * dcx - sxv * dsx; sxv = sxv * dcx + tmp * dsx;
+                    cyv = 1.0; syv = 0.0; czv = 1.0; szv = 0.0;
+                }
+            }
+        }
+    }
+    if (g > 256) return ewald_derivative(e, sx, sy, sz);
+    (void)nr;
+    double der = 0.0;
+    for (int q = 0; q < g; q++) {
+        const int s = q >> 6;
+        const double v = s == 0 ? t0 : (s == 1 ? t1 : (s == 2 ? t2 : t3));
+        der += wave_lane_double(v, q & 63);
+    }
+    return der;
+}
+
+// The separation (permuted: x along the motion) at which the commit of tagger w's stored
+// event evaluates the merged-image Coulomb derivative, or false when it evaluates none.
+template <int D>
+__device__ bool thinning_separation(const ChainArgs &a, const ChainState &st, const ecmc_tagger_desc &T, int w,
+                                    double *q3) {
+    const Slot &sl = st.slot[w];
+    double s[D];
+    if ((T.kind == ECMC_TAG_PAIR_CELLS || T.kind == ECMC_TAG_PAIR_ALL || T.kind == ECMC_TAG_SURPLUS ||
+         T.kind == ECMC_TAG_CELL_BOUNDING || T.kind == ECMC_TAG_FACTOR_MAP) &&
+        (T.event_handler == ECMC_EH_CELL_BOUNDING || T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING)) {
+        separation_vector<D>(a, sl.apos, sl.tpos, s);
+    } else if (T.kind == ECMC_TAG_CELL_VETO) {
+        if (a.occ_n[sl.cb_cell] != 1) return false;
+        double tp[D];
+        load_pos<D>(a, a.occ[(size_t)sl.cb_cell * a.occ_cap], tp);
+        separation_vector<D>(a, sl.apos, tp, s);
+    } else {
+        return false;
+    }
+    q3[0] = q3[1] = q3[2] = 0.0;
+    for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
+    return true;
+}
+
 // Shared memory of the leg loop's collective steps.
 struct LegShared {
     int nb_cell[32];
     int nb_pref[33];
     Best sb[NT / 64];
     uint32_t sh[NT / 64];
-    int scan[NT + 1];
+    int scan[NT + 1 + NT / 64];
     int flag;
 };
 
@@ -792,7 +968,8 @@ __device__ __forceinline__ bool commit_thins(const ecmc_tagger_desc &T) {
 // two ring words available when the event thins): send_out_state of its event handler and
 // insert_into_global_state (mediator/single_process_mediator.py:130-150).
 template <int D>
-__device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &T, int w, ecmc_event &ev) {
+__device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &T, int w, ecmc_event &ev,
+                             double ewald_pre) {
     Slot &sl = st.slot[w];
     ev.tagger = w;
     ev.flags = 0;
@@ -823,7 +1000,8 @@ __device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagg
             const double c1 = T.use_charge ? a.charge[act] : 1.0;
             const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
             double bound = cell_rate<D>(a, T, sl.cb_cell, sl.target, sl.adir, c1, c2) * sl.aspeed;
-            double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+            const double der = isnan(ewald_pre) ? ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
+            double real = T.par[3] * c1 * c2 * der * sl.aspeed;
             lift = false;
             if (real > 0) {
                 double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
@@ -839,7 +1017,8 @@ __device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagg
             const double c1 = T.use_charge ? a.charge[act] : 1.0;
             const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
             double bound = ipcb_derivative(T.bpar[0] * c1 * c2, q3[0], q3[1], q3[2]) * sl.aspeed;
-            double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+            const double der = isnan(ewald_pre) ? ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
+            double real = T.par[3] * c1 * c2 * der * sl.aspeed;
             lift = false;
             if (real > 0) {
                 double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
@@ -881,7 +1060,8 @@ __device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagg
             for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
             const double c1 = T.use_charge ? a.charge[act] : 1.0;
             const double c2 = T.use_charge ? a.charge[tid2] : 1.0;
-            const double real = T.par[3] * c1 * c2 * ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) * sl.aspeed;
+            const double der = isnan(ewald_pre) ? ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
+            const double real = T.par[3] * c1 * c2 * der * sl.aspeed;
             if (real > 0) {
                 double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
                 st.cursor += 2;
@@ -985,8 +1165,8 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
             }
             __syncthreads();
         } else if (pending) {
+            if (a.occupancy) occupancy_update<D>(a, st, ls.scan);
             if (tid == 0) {
-                if (a.occupancy) occupancy_update<D>(a, st);
                 st.ncand = 0;
                 st.chash = 0;
             }
@@ -1017,9 +1197,14 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
         if (st.err) break;
         const int w = flag;
         if (commit_thins(tg[w])) ring_ensure(st, mt, a, 2);
+        double ewald_pre = NAN;
+        if (tid < 64) {
+            double q3[3];
+            if (thinning_separation<D>(a, st, tg[w], w, q3)) ewald_pre = ewald_derivative_wave(a.ewald, q3[0], q3[1], q3[2]);
+        }
         if (tid == 0) {
             ecmc_event ev;
-            commit_event<D>(a, st, tg[w], w, ev);
+            commit_event<D>(a, st, tg[w], w, ev, ewald_pre);
             const ecmc_tagger_desc &T = tg[w];
             // trash by tag
             for (int i = 0; i < T.n_trash; i++) st.slot[T.trash[i]].valid = 0;
@@ -1065,8 +1250,8 @@ __global__ __launch_bounds__(NT) void chain_step_kernel(ChainArgs a, int op, int
     for (int i = tid; i < 624; i += NT) mt[i] = a.mt[i];
     __syncthreads();
     if (op == 0) {
+        if (a.occupancy && st.active >= 0) occupancy_update<D>(a, st, ls.scan);
         if (tid == 0) {
-            if (a.occupancy && st.active >= 0) occupancy_update<D>(a, st);
             st.ncand = 0;
             st.chash = 0;
         }
@@ -1074,14 +1259,18 @@ __global__ __launch_bounds__(NT) void chain_step_kernel(ChainArgs a, int op, int
     } else if (op == 1) {
         // the occupancy follows the active unit here too (a no-op after op 0 in the same leg), so
         // that a cell tagger never reads the cell of a unit that stopped being active
-        if (tid == 0 && a.occupancy && st.active >= 0) occupancy_update<D>(a, st);
-        __syncthreads();
+        if (a.occupancy && st.active >= 0) occupancy_update<D>(a, st, ls.scan);
         create_tagger<D>(a, st, mt, tg[t], t, ls);
     } else {
         if (commit_thins(tg[t])) ring_ensure(st, mt, a, 2);
+        double ewald_pre = NAN;
+        if (tid < 64) {
+            double q3[3];
+            if (thinning_separation<D>(a, st, tg[t], t, q3)) ewald_pre = ewald_derivative_wave(a.ewald, q3[0], q3[1], q3[2]);
+        }
         if (tid == 0) {
             ecmc_event ev;
-            commit_event<D>(a, st, tg[t], t, ev);
+            commit_event<D>(a, st, tg[t], t, ev, ewald_pre);
             st.last = st.slot[t].t;
             *out = ev;
         }
