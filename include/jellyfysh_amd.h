@@ -17,8 +17,8 @@
  *                          scheduler/heap_scheduler/heap.c:156-304 + list_scheduler.py:116-175)
  *   ecmc_pair_displacement InvertiblePotential.displacement(velocity, separation, charge_one,
  *                          charge_two, potential_change) (potential/potential.py:216-253) for
- *                          the hard-sphere, inverse-power, Lennard-Jones, displaced-even-power
- *                          and Coulomb-bounding potentials, batched
+ *                          the hard-sphere, hard-dipole, inverse-power, Lennard-Jones,
+ *                          displaced-even-power and Coulomb-bounding potentials, batched
  *   ecmc_pair_derivative   Potential.derivative(velocity, separation, charges)
  *                          (potential/potential.py:153-181), batched; for the merged-image
  *                          Coulomb potential this replaces merged_image_coulomb_potential.c:205-274
@@ -83,8 +83,9 @@ enum {
     ECMC_POT_MERGED_IMAGE_COULOMB = 4,     /* par: alpha, fourier_cutoff, position_cutoff, prefactor */
     ECMC_POT_INVERSE_POWER_COULOMB_BOUND = 5, /* par: prefactor */
     ECMC_POT_DISPLACED_EVEN_POWER = 6,     /* par: equilibrium_separation, power, prefactor */
-    ECMC_POT_CELL_BOUNDING = 7             /* CellBoundingPotential + InnerPointEstimator; par: estimator
+    ECMC_POT_CELL_BOUNDING = 7,            /* CellBoundingPotential + InnerPointEstimator; par: estimator
                                               prefactor, points_per_side, target_charge, empirical_bound */
+    ECMC_POT_HARD_DIPOLE = 8               /* HardDipolePotential; par: minimum_separation, maximum_separation */
 };
 enum { ECMC_SCHED_LIST = 0, ECMC_SCHED_HEAP = 1 };
 

@@ -19,7 +19,7 @@ SYSTEMS_NOALLOC_MAX = 1024   # ECMC_SYSTEMS_NOALLOC_MAX
  TAG_START_OF_RUN, TAG_SURPLUS, TAG_CELL_BOUNDING, TAG_CELL_VETO, TAG_FACTOR_MAP) = range(11)
 EH_TWO_LEAF, EH_TWO_LEAF_BOUNDING, EH_CELL_BOUNDING, EH_CELL_VETO = 0, 1, 2, 3
 (POT_HARD_SPHERE, POT_INVERSE_POWER, POT_LENNARD_JONES, POT_INVERSE_POWER_TRUNC, POT_MERGED_IMAGE_COULOMB,
- POT_INVERSE_POWER_COULOMB_BOUND, POT_DISPLACED_EVEN_POWER, POT_CELL_BOUNDING) = range(8)
+ POT_INVERSE_POWER_COULOMB_BOUND, POT_DISPLACED_EVEN_POWER, POT_CELL_BOUNDING, POT_HARD_DIPOLE) = range(9)
 SCHED_LIST, SCHED_HEAP = 0, 1
 
 STATUS = {0: "ok", -1: "argument", -2: "hip", -3: "configuration", -4: "scheduler", -5: "tag activator",

@@ -23,7 +23,8 @@ _POTENTIALS = {"hard_sphere": L.POT_HARD_SPHERE, "inverse_power": L.POT_INVERSE_
                "lennard_jones": L.POT_LENNARD_JONES, "inverse_power_truncated": L.POT_INVERSE_POWER_TRUNC,
                "merged_image_coulomb": L.POT_MERGED_IMAGE_COULOMB,
                "inverse_power_coulomb_bounding": L.POT_INVERSE_POWER_COULOMB_BOUND,
-               "displaced_even_power": L.POT_DISPLACED_EVEN_POWER, "cell_bounding": L.POT_CELL_BOUNDING}
+               "displaced_even_power": L.POT_DISPLACED_EVEN_POWER, "cell_bounding": L.POT_CELL_BOUNDING,
+               "hard_dipole": L.POT_HARD_DIPOLE}
 
 
 def potential_code(name):

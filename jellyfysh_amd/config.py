@@ -81,6 +81,7 @@ def split_list(value):
 # their defaults)
 _POTENTIALS = {
     "hard_sphere_potential": ("hard_sphere", [("radius", None)]),
+    "hard_dipole_potential": ("hard_dipole", [("minimum_separation", None), ("maximum_separation", None)]),
     "inverse_power_potential": ("inverse_power", [("power", None), ("prefactor", None)]),
     "truncated_inverse_power_potential": ("inverse_power_truncated",
                                           [("power", None), ("prefactor", None), ("cutoff", None)]),
@@ -182,9 +183,8 @@ class RunConfig:
                 self.output_handlers[alias] = (cls, opts)
             if io.get("input_handler"):
                 self.input_handler = self._input_handler(io["input_handler"])
-                creator = getattr(self.input_handler, "random_node_creator", None)
-                if creator is not None:
-                    self.leaves_per_root = creator.number_of_nodes_per_root_node
+                creator = getattr(self.input_handler, "random_node_creator", self.input_handler)
+                self.leaves_per_root = int(getattr(creator, "number_of_nodes_per_root_node", 1))
         act = split_alias(m.get("activator", "tag_activator"))[1]
         if act != "tag_activator":
             raise ConfigurationError("only the tag activator is supported (got %s)" % act)
@@ -248,8 +248,20 @@ class RunConfig:
             return RandomInputHandler(int(h["number_of_root_nodes"]), creator)
         if cls == "pdb_input_handler":
             h = self._section(camel(alias), {"filename", "charge_values"})
-            return PdbInputHandler(h["filename"], self._charge_values(h.get("charge_values")), self.dim,
-                                   self.system_length)
+            fname = h["filename"]
+            if not os.path.exists(fname):
+                # the reference's relative path (config_files/...) resolved against this file's folder
+                # or the package's configs folder
+                for base in ([os.path.dirname(self._path)] if self._path else []) + \
+                        [os.path.join(os.path.dirname(os.path.abspath(__file__)), "configs")]:
+                    for cand in (os.path.join(base, fname), os.path.join(base, os.path.basename(fname)),
+                                 os.path.join(base, os.path.basename(os.path.dirname(fname)), os.path.basename(fname))):
+                        if os.path.exists(cand):
+                            fname = cand
+                            break
+                    if os.path.exists(fname):
+                        break
+            return PdbInputHandler(fname, self._charge_values(h.get("charge_values")), self.dim, self.system_length)
         raise ConfigurationError("unsupported input handler %s" % cls)
 
     def initial_state(self, seed=0):
@@ -262,7 +274,7 @@ class RunConfig:
         if isinstance(self.input_handler, RandomInputHandler):
             pos, charges, words = self.input_handler.read(random.Random(seed), self.dim, self.system_length)
         else:
-            pos, charges = self.input_handler.read_point_masses()
+            pos, charges = self.input_handler.read()
             words = 0
         q = charges.get("electric_charge", next(iter(charges.values()), None)) if charges else None
         # composite objects: leaf units in identifier order, unit root * leaves + leaf
@@ -453,7 +465,11 @@ class SingleProcessMediator:
             if charges is None:
                 charges = q
         pos = np.asarray(positions, dtype=np.float64)
-        self.chain = DeviceChain(config.desc(pos.shape[0], seed, rng_skip=skip), pos, charges)
+        self.charges = None if charges is None else np.asarray(charges, dtype=np.float64)
+        # charges reach the device only where a tagger multiplies by them
+        uses_charge = any(t.get("charge") for t in config.taggers)
+        self.chain = DeviceChain(config.desc(pos.shape[0], seed, rng_skip=skip), pos,
+                                 self.charges if uses_charge else None)
         self.sampling = [i for i, t in enumerate(config.taggers) if t["kind"] == "sampling"]
         self.output_handlers = {alias: make_output_handler(cls, opts, config.leaves_per_root)
                                 for alias, (cls, opts) in config.output_handlers.items()}
@@ -488,7 +504,11 @@ class SingleProcessMediator:
                 pos = None
                 if last in self.writers:
                     pos = self.chain.positions()
-                    self.writers[last].write(pos, self.config.system_length)
+                    w = self.writers[last]
+                    if hasattr(w, "polarization"):
+                        w.write(pos, self.config.system_length, self.charges)
+                    else:
+                        w.write(pos, self.config.system_length)
                 if on_sample is not None and (last < 0 or last in self.sampling):
                     on_sample(pos if pos is not None else self.chain.positions())
 

@@ -221,7 +221,7 @@ __device__ __forceinline__ void slice_pos(const ChainArgs &a, double *p, int dir
 
 __device__ __forceinline__ bool pot_draws(const ecmc_tagger_desc &tg) {
     if (tg.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) return true;
-    return tg.potential != ECMC_POT_HARD_SPHERE;
+    return tg.potential != ECMC_POT_HARD_SPHERE && tg.potential != ECMC_POT_HARD_DIPOLE;
 }
 
 // Bounding event rate of a cell-bounding candidate: the InnerPointEstimator bound of the
@@ -264,6 +264,7 @@ __device__ double pair_dt(const ChainArgs &a, const ecmc_tagger_desc &tg, int di
     if (tg.event_handler == ECMC_EH_TWO_LEAF) {
         switch (tg.potential) {
         case ECMC_POT_HARD_SPHERE: return hs_displacement<D>(dir, speed, s, tg.par[0]);
+        case ECMC_POT_HARD_DIPOLE: return hd_displacement<D>(dir, speed, s, tg.par[0], tg.par[1]);
         case ECMC_POT_INVERSE_POWER: return ip_displacement<D>(dir, speed, s, c1, c2, tg.par[0], tg.par[1], dU);
         case ECMC_POT_INVERSE_POWER_TRUNC:
             return ipt_displacement<D>(dir, speed, s, c1, c2, tg.par[0], tg.par[1], tg.par[2], dU);
@@ -1483,9 +1484,11 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     for (int t = 0; t < desc->n_taggers; t++) {
         const ecmc_tagger_desc &tg = desc->taggers[t];
         if (tg.kind < 0 || tg.kind > ECMC_TAG_FACTOR_MAP) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
-        if (P > 1 && (tg.kind == ECMC_TAG_PAIR_CELLS || tg.kind == ECMC_TAG_PAIR_ALL || tg.kind == ECMC_TAG_SURPLUS ||
-                      tg.kind == ECMC_TAG_CELL_BOUNDING || tg.kind == ECMC_TAG_CELL_VETO || tg.kind == ECMC_TAG_CELL_BOUNDARY))
-            return ecmc_fail(ECMC_ERR_CONFIGURATION, "with composite objects the pair taggers are factor-type-map taggers and no cells are used");
+        // with composite objects the cells hold leaf units (cell_level 2): the excluded-cells and
+        // cell-boundary taggers work on leaf units as on point masses; the others are not built
+        if (P > 1 && (tg.kind == ECMC_TAG_PAIR_ALL || tg.kind == ECMC_TAG_SURPLUS || tg.kind == ECMC_TAG_CELL_BOUNDING ||
+                      tg.kind == ECMC_TAG_CELL_VETO))
+            return ecmc_fail(ECMC_ERR_CONFIGURATION, "with composite objects the pair taggers are factor-type-map and excluded-cells taggers");
         if (tg.kind == ECMC_TAG_FACTOR_MAP) {
             int fmax = 0;
             for (int k = 0; k < P; k++) {

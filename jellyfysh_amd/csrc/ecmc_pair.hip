@@ -45,6 +45,9 @@ __device__ void pair_row(const PairArgs &a, int64_t i, bool derivative) {
     case POT_HARD_SPHERE:
         r = derivative ? NAN : hs_displacement<D>(dir, speed, s, a.par[0]);
         break;
+    case POT_HARD_DIPOLE:
+        r = derivative ? NAN : hd_displacement<D>(dir, speed, s, a.par[0], a.par[1]);
+        break;
     case POT_INVERSE_POWER:
         r = derivative ? ip_derivative<D>(dir, speed, s, c1, c2, a.par[0], a.par[1])
                        : ip_displacement<D>(dir, speed, s, c1, c2, a.par[0], a.par[1], dU);
@@ -100,13 +103,14 @@ static int launch_pair(int32_t potential, const double *par, int32_t dim, int64_
     if (n < 0 || (dim != 2 && dim != 3) || !par) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
     if (n == 0) return ECMC_OK;
     if (!dir || !sep || !out) return ecmc_fail(ECMC_ERR_ARGUMENT, "null device pointer");
-    if (potential < 0 || potential > POT_DISPLACED_EVEN_POWER) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown potential");
+    if (potential < 0 || potential > POT_HARD_DIPOLE || potential == POT_CELL_BOUNDING)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown potential");
     if ((potential == POT_MERGED_IMAGE_COULOMB || potential == POT_INVERSE_POWER_COULOMB_BOUND) && dim != 3)
         return ecmc_fail(ECMC_ERR_CONFIGURATION, "the Coulomb potentials need three dimensions");
     if (potential == POT_MERGED_IMAGE_COULOMB && !derivative)
         return ecmc_fail(ECMC_ERR_CONFIGURATION, "the merged-image Coulomb potential has no displacement");
-    if (potential == POT_HARD_SPHERE && derivative)
-        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the hard-sphere potential has no derivative");
+    if ((potential == POT_HARD_SPHERE || potential == POT_HARD_DIPOLE) && derivative)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the hard-sphere and hard-dipole potentials have no derivative");
     PairArgs a;
     memset(&a, 0, sizeof(a));
     a.potential = potential;

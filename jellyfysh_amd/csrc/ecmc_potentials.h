@@ -16,7 +16,9 @@ enum PotentialKind {
     POT_INVERSE_POWER_TRUNC = 3,  // par: power, prefactor, cutoff
     POT_MERGED_IMAGE_COULOMB = 4, // par: alpha, fourier_cutoff, position_cutoff, prefactor
     POT_INVERSE_POWER_COULOMB_BOUND = 5,  // par: prefactor
-    POT_DISPLACED_EVEN_POWER = 6  // par: equilibrium_separation, power, prefactor
+    POT_DISPLACED_EVEN_POWER = 6, // par: equilibrium_separation, power, prefactor
+    POT_CELL_BOUNDING = 7,        // cell-bounding tables (no pair evaluation)
+    POT_HARD_DIPOLE = 8           // par: minimum_separation, maximum_separation
 };
 
 // base/vectors.py:46-60 -- sum() starts from the integer 0.
@@ -52,6 +54,29 @@ ECMC_FN double hs_displacement(int dir, double speed, const double *s, double ra
     for (int i = 0; i < D; i++) vs = vs + v[i] * s[i];
     double srt = vs * vs - v2 * (s2 - diam2);
     return (srt >= 0.0 && vs >= 0.0) ? (vs - sqrt(srt)) / v2 : INFINITY;
+}
+
+// HardDipolePotential.displacement (hard_dipole_potential.py:66-110): the time until the
+// separation reaches the minimum (approaching, if it can) or the maximum separation; NaN when
+// the separation lies outside [minimum, maximum] by more than 1e-13 (the reference asserts).
+template <int D>
+ECMC_FN double hd_displacement(int dir, double speed, const double *s, double rmin, double rmax) {
+    const double min2 = rmin * rmin, max2 = rmax * rmax;
+    double v[D];
+#pragma unroll
+    for (int i = 0; i < D; i++) v[i] = i == dir ? speed : 0.0;
+    const double v2 = norm_sq<D>(v);
+    const double s2 = norm_sq<D>(s);
+    if (!(s2 - min2 > -1.0e-13) || !(max2 - s2 > -1.0e-13) || !(v2 > 0.0)) return NAN;
+    double vs = 0.0;
+#pragma unroll
+    for (int i = 0; i < D; i++) vs = vs + v[i] * s[i];
+    if (vs >= 0.0) {
+        const double m = vs * vs - v2 * (s2 - min2);
+        if (m >= 0.0) return (vs - sqrt(m)) / v2;
+    }
+    const double M = vs * vs - v2 * (s2 - max2);
+    return M >= 0.0 ? (vs + sqrt(M)) / v2 : NAN;
 }
 
 // InversePowerPotential.potential (inverse_power_potential.py:127-143)

@@ -154,13 +154,55 @@ class SeparationOutputHandler(OutputHandler):
             f.close()
 
 
-_HANDLERS = {"separation_output_handler": SeparationOutputHandler, "dummy_output_handler": DummyOutputHandler}
+class PolarizationOutputHandler(OutputHandler):
+    """input_output_handler/output_handler/polarization_output_handler.py:35-110: at every sample
+    the polarization sum over charge-neutral composite objects of charge x position, each leaf
+    position taken at the periodic image closest to its object (for two leaves of charges q, -q
+    that is q times the shortest separation from the second leaf to the first), one line of
+    tab-separated components under a '# Polarization Vector' header.
+
+    ``write(positions, system_length, charges)`` takes the (n, dim) leaf positions in identifier
+    order and the charges of the handler's charge."""
+
+    def __init__(self, filename, charge, leaves_per_root=2):
+        super().__init__(filename)
+        if int(leaves_per_root) < 2:
+            raise ConfigurationError("The output handler PolarizationOutputHandler can only be used with charge neutral "
+                                     "composite point objects.")
+        self.leaves_per_root = int(leaves_per_root)
+        self.charge = charge
+        self._file = HardBufferedTextWriter(filename)
+        print("# Polarization Vector", file=self._file)
+
+    def polarization(self, positions, system_length, charges):
+        P = self.leaves_per_root
+        pos = np.asarray(positions, dtype=np.float64).reshape(-1, P, np.shape(positions)[-1])
+        q = np.asarray(charges, dtype=np.float64).reshape(-1, P)
+        if np.any(q.sum(axis=1) != 0.0):
+            raise AssertionError("every composite object must be charge neutral")
+        L_ = float(system_length)
+        # images closest to the first leaf (for charge-neutral objects the reference's closest
+        # images to the object's position differ from these by a shift that drops out)
+        s = np.mod((pos - pos[:, :1, :]) + L_ / 2, L_) - L_ / 2
+        return (q[:, :, None] * s).sum(axis=(0, 1))
+
+    def write(self, positions, system_length, charges):
+        super().write(positions, system_length, charges)
+        print("\t".join(map(str, (float(v) for v in self.polarization(positions, system_length, charges)))),
+              file=self._file)
+
+    def post_run(self):
+        self._file.close()
+
+
+_HANDLERS = {"separation_output_handler": SeparationOutputHandler, "dummy_output_handler": DummyOutputHandler,
+             "polarization_output_handler": PolarizationOutputHandler}
 
 
 def make_output_handler(cls, options, leaves_per_root=1):
     """An output handler from its INI class name (snake case) and section options."""
     if cls not in _HANDLERS:
         raise ConfigurationError("output handler %s is not supported" % cls)
-    if cls == "separation_output_handler":
-        return SeparationOutputHandler(leaves_per_root=leaves_per_root, **options)
+    if cls in ("separation_output_handler", "polarization_output_handler"):
+        return _HANDLERS[cls](leaves_per_root=leaves_per_root, **options)
     return _HANDLERS[cls](**options)

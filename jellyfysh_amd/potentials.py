@@ -115,6 +115,25 @@ class HardSpherePotential(DevicePotential):
         raise NotImplementedError
 
 
+class HardDipolePotential(DevicePotential):
+    """potential/hard_dipole_potential.py:32-125: U = 0 for minimum <= |s| <= maximum, else inf."""
+    code = L.POT_HARD_DIPOLE
+    takes_potential_change = False
+
+    def __init__(self, minimum_separation, maximum_separation):
+        if not minimum_separation > 0.0 or not maximum_separation > 0.0 or \
+                not minimum_separation < maximum_separation:
+            raise ValueError("0 < minimum_separation < maximum_separation is required")
+        super().__init__([minimum_separation, maximum_separation])
+
+    def displacement(self, velocity, separation):
+        return self._scalar(False, velocity, separation, None, None, None)
+
+    def derivative(self, velocity, separation):
+        # hard_dipole_potential.py:112-125: not defined for ECMC
+        raise NotImplementedError
+
+
 class InversePowerPotential(DevicePotential):
     """potential/inverse_power_potential.py:32-179: U = c1 c2 k / |s|^p."""
     code = L.POT_INVERSE_POWER
@@ -273,6 +292,6 @@ class MergedImageCoulombPotential(DevicePotential):
         raise NotImplementedError
 
 
-__all__ = ["HardSpherePotential", "InversePowerPotential", "TruncatedInversePowerPotential", "LennardJonesPotential",
+__all__ = ["HardSpherePotential", "HardDipolePotential", "InversePowerPotential", "TruncatedInversePowerPotential", "LennardJonesPotential",
            "DisplacedEvenPowerPotential", "InversePowerCoulombBoundingPotential", "MergedImageCoulombPotential",
            "DevicePotential"]

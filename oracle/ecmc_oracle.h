@@ -73,6 +73,9 @@ void ora_pow_n(int64_t n, const double *x, const double *y, double *out, int mat
 /* ---- potentials (scalar, reference argument conventions) --------------------------- */
 /* HardSpherePotential.displacement(velocity, separation) */
 double ora_hs_displacement(int dim, const double *velocity, const double *separation, double radius);
+/* HardDipolePotential.displacement(velocity, separation) */
+double ora_hd_displacement(int dim, const double *velocity, const double *separation, double minimum,
+                           double maximum);
 /* InversePowerPotential standard_velocity_* with direction of motion `dir` and speed. */
 double ora_ip_displacement(int dim, int dir, double speed, const double *separation, double c1, double c2,
                            double power, double prefactor, double potential_change, int math_mode);
@@ -172,7 +175,8 @@ enum {
     ORA_POT_INVERSE_POWER_TRUNC = 3, /* par: power, prefactor, cutoff */
     ORA_POT_MERGED_IMAGE_COULOMB = 4, /* par: alpha, fourier_cutoff, position_cutoff, prefactor */
     ORA_POT_IPCB = 5,             /* par: prefactor */
-    ORA_POT_DISPLACED_EVEN_POWER = 6 /* par: equilibrium_separation, power, prefactor */
+    ORA_POT_DISPLACED_EVEN_POWER = 6, /* par: equilibrium_separation, power, prefactor */
+    ORA_POT_HARD_DIPOLE = 8       /* par: minimum_separation, maximum_separation */
 };
 enum { ORA_SCHED_LIST = 0, ORA_SCHED_HEAP = 1 };
 /* Events of equal time: ORA_TIES_SCHEDULER ranks them as the configured scheduler does

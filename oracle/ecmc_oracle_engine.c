@@ -488,6 +488,11 @@ static double pair_displacement(ora_engine *e, const ora_tagger *tg, int tidx, c
             v[dir] = speed;
             return ora_hs_displacement(e->dim, v, s, tg->par[0]);
         }
+        case ORA_POT_HARD_DIPOLE: {
+            double v[3] = {0, 0, 0};
+            v[dir] = speed;
+            return ora_hd_displacement(e->dim, v, s, tg->par[0], tg->par[1]);
+        }
         case ORA_POT_INVERSE_POWER: {
             double dU = ora_expovariate(&e->rng, e->cfg.beta, mm);
             return ora_ip_displacement(e->dim, dir, speed, s, c1, c2, tg->par[0], tg->par[1], dU, mm);

@@ -233,6 +233,25 @@ double ora_hs_displacement(int dim, const double *velocity, const double *separa
     return (srt >= 0.0 && vs >= 0.0) ? (vs - sqrt(srt)) / v2 : INFINITY;
 }
 
+/* ===== HardDipolePotential (hard_dipole_potential.py:66-110) ============================
+ * NaN where the reference asserts (separation outside [minimum, maximum] by more than 1e-13,
+ * zero velocity). */
+double ora_hd_displacement(int dim, const double *velocity, const double *separation, double minimum,
+                           double maximum) {
+    const double min2 = minimum * minimum, max2 = maximum * maximum;
+    const double v2 = v_norm_sq(dim, velocity);
+    const double s2 = v_norm_sq(dim, separation);
+    if (!(v2 > 0.0) || !(s2 - min2 > -1.0e-13) || !(max2 - s2 > -1.0e-13)) return NAN;
+    double vs = 0.0;
+    for (int i = 0; i < dim; i++) vs = vs + velocity[i] * separation[i];
+    if (vs >= 0.0) {
+        const double m = vs * vs - v2 * (s2 - min2);
+        if (m >= 0.0) return (vs - sqrt(m)) / v2;
+    }
+    const double M = vs * vs - v2 * (s2 - max2);
+    return M >= 0.0 ? (vs + sqrt(M)) / v2 : NAN;
+}
+
 /* ===== InversePowerPotential (inverse_power_potential.py:71-179) ====================== */
 double ora_ip_potential(int dim, const double *s, double charge_product, double power, double prefactor, int mm) {
     return charge_product * prefactor / ora_pow(v_norm_sq(dim, s), power / 2.0, mm);

@@ -77,6 +77,7 @@ def lib():
             "ora_log_n": (None, [i64, pd, pd, i32]), "ora_pow_n": (None, [i64, pd, pd, pd, i32]),
             "ora_separation_norms": (None, [i32, i64, pd, d, pd, i32]),
             "ora_hs_displacement": (d, [i32, pd, pd, d]),
+            "ora_hd_displacement": (d, [i32, pd, pd, d, d]),
             "ora_ip_displacement": (d, [i32, i32, d, pd, d, d, d, d, d, i32]),
             "ora_ip_derivative": (d, [i32, i32, d, pd, d, d, d, d, i32]),
             "ora_ip_potential": (d, [i32, pd, d, d, d, i32]),

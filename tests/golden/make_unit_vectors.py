@@ -19,6 +19,7 @@ import sys
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_unit_vectors.json")
 FILES = ["test_inverse_power_potential.py", "test_lennard_jones_potential.py", "test_hard_sphere_potential.py",
+         "test_hard_dipole_potential.py",
          "test_displaced_even_power_potential.py", "test_inverse_power_coulomb_bounding_potential.py",
          "test_merged_image_coulomb_potential.py"]
 

@@ -6,7 +6,7 @@ import numpy as np
 
 POT_CODE = {"HardSpherePotential": 0, "InversePowerPotential": 1, "LennardJonesPotential": 2,
             "MergedImageCoulombPotential": 4, "InversePowerCoulombBoundingPotential": 5,
-            "DisplacedEvenPowerPotential": 6}
+            "DisplacedEvenPowerPotential": 6, "HardDipolePotential": 8}
 
 
 def ctor_params(pot):
@@ -21,6 +21,8 @@ def ctor_params(pot):
         return default
     if cls == "HardSpherePotential":
         return [get("radius", 0), 0, 0, 0]
+    if cls == "HardDipolePotential":
+        return [get("minimum_separation", 0), get("maximum_separation", 1), 0, 0]
     if cls == "InversePowerPotential":
         return [get("power", 0), get("prefactor", 1), 0, 0]
     if cls == "LennardJonesPotential":
@@ -86,6 +88,8 @@ def oracle_eval(O, vec, math_mode=0):
     pd = O._dp
     if cls == "HardSpherePotential":
         return L.ora_hs_displacement(dim, pd(vv), pd(sv), par[0])
+    if cls == "HardDipolePotential":
+        return L.ora_hd_displacement(dim, pd(vv), pd(sv), par[0], par[1])
     ax = axis_velocity(v)
     d, speed = ax
     if cls == "InversePowerPotential":

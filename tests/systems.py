@@ -68,7 +68,8 @@ def wiring_from_meta(meta):
 
 
 _ORACLE_POT = {"hard_sphere": 0, "inverse_power": 1, "lennard_jones": 2, "inverse_power_truncated": 3,
-               "merged_image_coulomb": 4, "inverse_power_coulomb_bounding": 5, "displaced_even_power": 6}
+               "merged_image_coulomb": 4, "inverse_power_coulomb_bounding": 5, "displaced_even_power": 6,
+               "hard_dipole": 8}
 _ORACLE_KIND = {"pair_cells": 0, "pair_all": 1, "cell_boundary": 2, "sampling": 3, "end_of_chain": 4,
                 "end_of_run": 5, "start_of_run": 6, "surplus": 7, "cell_bounding": 8, "cell_veto": 9,
                 "factor_map": 10}
@@ -297,3 +298,40 @@ def dipole_device_desc(meta):
     return make_desc(meta["dim"], meta["roots"] * meta["leaves"], meta["system_length"], meta["beta"], 1,
                      dipole_wiring(meta), meta["seed"], scheduler=meta["scheduler"], rng_skip=words,
                      leaves_per_root=meta["leaves"])
+
+
+def hard_disk_dipole_wiring(meta):
+    """The taggers of tests/golden/hard_disk_dipoles_r81 (the reference's
+    hard_disk_dipoles_cells.ini): hard spheres between leaf units of nearby cells, the tether
+    through the factor type map, cell boundaries, sampling from time 0, chain ends, run ends."""
+    pair = ["nearby_sphere", "dipole", "cell_boundary"]
+    eoc = ["end_of_chain"] + pair
+    return [
+        dict(tag="nearby_sphere", kind="pair_cells", potential="hard_sphere", params=[meta["radius"]], create=pair,
+             trash=pair, number_event_handlers=40),
+        dict(tag="dipole", kind="factor_map", potential="hard_dipole",
+             params=[meta["minimum_separation"], meta["maximum_separation"]], create=pair, trash=pair,
+             number_event_handlers=1, factor_map=dict(local=True, targets=[[1], [0]])),
+        dict(tag="cell_boundary", kind="cell_boundary", create=pair, trash=pair),
+        dict(tag="polarization_sampling", kind="sampling", create=["polarization_sampling"],
+             trash=["polarization_sampling"], sampling_interval=meta["sampling_interval"], first_event_time_zero=True),
+        dict(tag="end_of_chain", kind="end_of_chain", create=eoc, trash=eoc, chain_time=meta["chain_time"]),
+        dict(tag="end_of_run", kind="end_of_run", create=["end_of_run"],
+             trash=eoc + ["polarization_sampling", "start_of_run", "end_of_run"], end_of_run_time=meta["end_time"]),
+        dict(tag="start_of_run", kind="start_of_run", create=pair + ["polarization_sampling", "end_of_chain", "end_of_run"],
+             trash=["start_of_run"], initial_direction_of_motion=0, speed=1.0, initial_active_identifier=0),
+    ]
+
+
+def hard_disk_dipole_oracle_config(meta, math_mode=0):
+    return oracle_config_from_wiring(meta["dim"], meta["roots"] * meta["leaves"], meta["system_length"], meta["beta"],
+                                     meta["cells_per_side"], hard_disk_dipole_wiring(meta), meta["seed"],
+                                     scheduler=meta["scheduler"], math_mode=math_mode, max_occupants=-1,
+                                     leaves_per_root=meta["leaves"])
+
+
+def hard_disk_dipole_device_desc(meta):
+    from jellyfysh_amd.chain import make_desc
+    return make_desc(meta["dim"], meta["roots"] * meta["leaves"], meta["system_length"], meta["beta"],
+                     meta["cells_per_side"], hard_disk_dipole_wiring(meta), meta["seed"], scheduler=meta["scheduler"],
+                     max_occupants=-1, leaves_per_root=meta["leaves"])

@@ -61,3 +61,61 @@ def test_gpu_dipoles_match_oracle_and_reference(device, oracle):
     assert np.array_equal(log["target"], g["target"]) and np.array_equal(log["tagger"], g["tagger"])
     t, tg = log["tq"] + log["tr"], g["tq"] + g["tr"]
     assert np.max(np.abs(t - tg) / np.maximum(np.abs(tg), 1.0)) < 1e-12
+
+
+def _hdd():
+    return S.load_fixture("hard_disk_dipoles_r81")
+
+
+def test_pdb_reader_gives_the_fixture_start():
+    """The 81-dipole .pdb start through the INI front end (composite objects from a .pdb file)
+    equals the leaf positions the fixture's reference run started from."""
+    import os
+    from jellyfysh_amd.config import RunConfig
+    meta, g = _hdd()
+    cfg = RunConfig.from_file(os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                           "jellyfysh_amd", "configs", "hard_disk_dipoles_cells.ini"))
+    pos, q, words = cfg.initial_state(0)
+    assert cfg.leaves_per_root == 2 and words == 0
+    assert np.array_equal(pos, g["positions"])
+    assert np.array_equal(q, g["charges"])
+
+
+def test_oracle_reproduces_the_reference_hard_disk_dipole_log(oracle):
+    meta, g = _hdd()
+    n = len(g["tagger"])
+    e = oracle.Engine(S.hard_disk_dipole_oracle_config(meta, math_mode=oracle.MATH_LIBM), g["positions"])
+    log = e.run(n)
+    assert len(log) == n
+    for f in FIELDS:
+        eq = log[f] == g[f]
+        if eq.ndim > 1:
+            eq = eq.all(axis=1)
+        bad = np.flatnonzero(~eq)
+        assert bad.size == 0, (f, int(bad[0]))
+    assert np.array_equal(log["active"], g["active"])
+    assert np.array_equal(e.positions(), g["final_positions"])
+
+
+@pytest.mark.gpu
+def test_gpu_hard_disk_dipoles_match_oracle_and_reference(device, oracle):
+    from jellyfysh_amd.chain import DeviceChain
+    meta, g = _hdd()
+    n = len(g["tagger"])
+    print("81 hard-disk dipoles, %d legs" % n, flush=True)
+    chain = DeviceChain(S.hard_disk_dipole_device_desc(meta), g["positions"])
+    log = chain.run(n)
+    e = oracle.Engine(S.hard_disk_dipole_oracle_config(meta, math_mode=oracle.MATH_CR), g["positions"])
+    ref = e.run(n)
+    for f in ("tagger", "flags", "active", "target", "tq", "tr", "pos", "ncand", "chash"):
+        eq = log[f] == ref[f]
+        if eq.ndim > 1:
+            eq = eq.all(axis=1)
+        bad = np.flatnonzero(~eq)
+        assert bad.size == 0, (f, int(bad[0]))
+    assert np.array_equal(chain.positions(), e.positions())
+    for f in FIELDS:
+        eq = log[f] == g[f]
+        if eq.ndim > 1:
+            eq = eq.all(axis=1)
+        assert eq.all(), f

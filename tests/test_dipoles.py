@@ -208,3 +208,53 @@ def test_gpu_flexible_dipole_parameter_derivatives(device):
         um = float(energies(*[torch.tensor(v, dtype=torch.float64, device="cuda") for v in args_m]))
         fd = (up - um) / (2 * h)
         assert abs(grads[i] - fd) <= 1e-6 * max(1.0, abs(fd)), (i, grads[i], fd, vals)
+
+
+@pytest.mark.gpu
+def test_gpu_hard_disk_dipoles_sample_the_reference_polarization(device):
+    """256 replicas of hard_disk_dipoles_cells.ini (the reference's 81 hard-disk dipoles from its
+    .pdb start, seeds 0-255) advanced together, each stopping at its polarization samples (every
+    10.01): after 300 samples of burn-in per replica, the distributions of the polarization's x and
+    y components against the reference's Newtonian-ECMC curves (output/hard_disk_dipoles/
+    ReferenceDataPx/Py_81Dipoles_NewtonianECMC.dat, copied by
+    tests/golden/copy_hard_disk_dipoles_cdf.py).  The polarization is a slow collective
+    observable: the bound 0.04 on the Kolmogorov distance leaves room for the correlation of
+    successive samples."""
+    from jellyfysh_amd.chain import DeviceChain, run_together
+    from jellyfysh_amd.output import PolarizationOutputHandler
+    cfg = _config("hard_disk_dipoles_cells")
+    samp = [i for i, t in enumerate(cfg.taggers) if t["kind"] == "sampling"][0]
+    pos, q, _ = cfg.initial_state(0)
+    chains = []
+    for seed in range(256):
+        c = DeviceChain(cfg.desc(len(pos), seed), pos)
+        c.set_stop_mask(1 << samp)
+        chains.append(c)
+    pol = PolarizationOutputHandler.polarization     # the output handler's sum, without its file
+    samples = []
+    stops, burn = 1500, 300
+    print("256 replicas of 81 hard-disk dipoles, %d sampling stops" % stops, flush=True)
+    for it in range(stops):
+        n, _ = run_together(chains, 1000000)
+        assert np.all((n > 0) & (n < 1000000))
+        if it >= burn:
+            for c in chains:
+                samples.append(pol(_PolarizationView(), c.positions(), cfg.system_length, q))
+        if it % 300 == 0:
+            print("stop %d" % it, flush=True)
+    for c in chains:
+        c.close()
+    samples = np.array(samples)
+    for k, part in enumerate(("px", "py")):
+        ref = np.loadtxt(os.path.join(GOLDEN, "hard_disk_dipoles_reference_cdf_%s.dat" % part))
+        v = np.sort(samples[:, k])
+        F = np.searchsorted(v, ref[:, 0], side="right") / len(v)
+        ks = np.max(np.abs(F - ref[:, 1]))
+        print("%s: samples %d, Kolmogorov distance to the reference data %.4f, mean %.3f, std %.3f" %
+              (part, len(v), ks, v.mean(), v.std()), flush=True)
+        assert ks < 0.04
+
+
+class _PolarizationView:
+    """Stands for a PolarizationOutputHandler of two leaves per object (no output file)."""
+    leaves_per_root = 2

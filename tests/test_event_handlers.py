@@ -94,7 +94,8 @@ def test_gpu_plugin_potentials_reproduce_reference_vectors(device):
     (derivative / displacement with the reference's argument lists) and its array form, against
     the expected values of the reference's potential unit tests to their own places."""
     from jellyfysh_amd import potentials as P
-    classes = {"HardSpherePotential": P.HardSpherePotential, "InversePowerPotential": P.InversePowerPotential,
+    classes = {"HardSpherePotential": P.HardSpherePotential, "HardDipolePotential": P.HardDipolePotential,
+               "InversePowerPotential": P.InversePowerPotential,
                "LennardJonesPotential": P.LennardJonesPotential,
                "DisplacedEvenPowerPotential": P.DisplacedEvenPowerPotential,
                "InversePowerCoulombBoundingPotential": P.InversePowerCoulombBoundingPotential,
