@@ -301,6 +301,17 @@ int ecmc_pair_displacement(int32_t potential, const double *par, int32_t dim, in
 int ecmc_pair_derivative(int32_t potential, const double *par, int32_t dim, int64_t n, const int32_t *d_dir,
                          const double *d_speed, const double *d_sep, const double *d_c1, const double *d_c2,
                          double system_length, double *d_out, void *stream);
+/* FP32 brackets lb <= t <= ub of the time displacement t that ecmc_pair_displacement returns for
+ * the inverse-power potentials (ECMC_POT_INVERSE_POWER, _TRUNC; integer power 1..32), with the
+ * potential change dU = -log(1 - u) / beta of the random pair whose leading MT word is word[i]
+ * (u = random() of the words (word[i], next word), potential/inverse_power_potential.py:96-179
+ * with the expovariate draw of event_handler/two_leaf_unit_event_handler.py).  lb = +inf only
+ * where t is +inf; ub = +inf where an event is not certain.  The chain kernel inverts exactly
+ * only the candidates whose lb does not exceed the smallest ub of their set.  No reference
+ * counterpart as an entry point: the brackets behind the candidate filter, exposed for tests. */
+int ecmc_pair_time_bounds(int32_t potential, const double *par, int32_t dim, int64_t n, const int32_t *d_dir,
+                          const double *d_speed, const double *d_sep, const double *d_c1, const double *d_c2,
+                          const uint32_t *d_word, double beta, float *d_lb, float *d_ub, void *stream);
 
 /* ---- merged-image Coulomb handle ------------------------------------------------------------
  * The opaque handle of merged_image_coulomb_potential.h:37-44 (construct_, copy_,

@@ -90,6 +90,7 @@ def _signatures():
         "ecmc_system_send_out_state": (i32, [vp, i32, vp]),
         "ecmc_pair_displacement": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, vp, d, vp, vp]),
         "ecmc_pair_derivative": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, d, vp, vp]),
+        "ecmc_pair_time_bounds": (i32, [i32, pd, i32, i64, vp, vp, vp, vp, vp, vp, d, vp, vp, vp]),
         "ecmc_math_log": (i32, [i64, vp, vp, vp]),
         "ecmc_math_pow": (i32, [i64, vp, vp, vp, vp]),
         "ecmc_mt_words": (i32, [u64, i64, vp, vp]),

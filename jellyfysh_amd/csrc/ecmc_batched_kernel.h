@@ -5,6 +5,7 @@
 #include "../../include/jellyfysh_amd.h"
 #include "ecmc_math.h"
 #include "ecmc_potentials.h"
+#include "ecmc_fp32_bounds.h"
 
 namespace ecmc {
 
@@ -84,21 +85,7 @@ __device__ __forceinline__ int bcell1(double x, double side, int cps) {
 }
 
 // ---- FP32 lower bound of the truncated inverse-power event displacement ------------------
-// Every quantity is bracketed [lo, hi] and each FP32 operation is covered by a relative
-// margin (ETA for products/quotients/roots, ETA_POW for the exp2/log2 power), far above
-// FP32 rounding (2^-24 per operation) and the documented accuracy of log2f/exp2f.
-constexpr float ETA = 0x1p-17f, ETA_POW = 0x1p-14f;
-// x moved outward by ETA |x| with one fused operation: the single rounding leaves a margin of
-// at least ETA - 2^-23 relative.  No argument is -inf (every caller passes a finite value or
-// +inf, which stays +inf).
-__device__ __forceinline__ float up(float x) { return fmaf(fabsf(x), ETA, x); }
-__device__ __forceinline__ float dn(float x) { return fmaf(-fabsf(x), ETA, x); }
-__device__ __forceinline__ float sq_lo(float a_abs_lo) { return dn(a_abs_lo * a_abs_lo); }
-// Hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32: within 1 ulp = 2^-23 relative,
-// far inside the ETA margins); below 2^-100 the root is bracketed by [0, 2^-50] directly.
-__device__ __forceinline__ float rcp1(float x) { return __builtin_amdgcn_rcpf(x); }
-__device__ __forceinline__ float sqrt_up(float x) { return x > 0x1p-100f ? up(__builtin_amdgcn_sqrtf(x)) : 0x1p-50f; }
-__device__ __forceinline__ float sqrt_dn(float x) { return x > 0x1p-100f ? dn(__builtin_amdgcn_sqrtf(x)) : 0.f; }
+// (the bracketing helpers up, dn, rcp1, sqrt_up, sqrt_dn and dU_lower are in ecmc_fp32_bounds.h)
 
 // V(r2) = K / r2^(p/2) with p in {6, 12} (integer powers by multiplication), bounded up
 // (hi) or down for an r2 that is itself a lower/upper bound.  Three products (2^-24 each) and
@@ -273,28 +260,6 @@ __device__ __forceinline__ void bsync() { __syncthreads(); }
 // the array in scratch memory).
 template <class T>
 __device__ __forceinline__ T pick(int f, T v0, T v1) { return f ? v1 : v0; }
-
-// The device log2f is within LOG2F_ERR * max(|log2 x|, 1) of log2 x for every positive float:
-// an exhaustive scan (ecmc_fp32_math_error) measures 2^-23 and the tests require <= 2^-21.
-constexpr float LOG2F_ERR = 0x1p-20f;
-
-// FP32 lower bound of dU = -log(1 - u) / beta for u = unit53(a, b), from the first word
-// alone: u >= u24 = (a >> 8) 2^-24 exactly, -log(1 - u) >= u, and for u24 >= 2^-10 the
-// float 1 - u24 is exact and -log2f of it is within LOG2F_ERR max(|.|, 1) of
-// -log2(1 - u24).  inv_beta_lo is a lower bound of 1 / beta.  The result never exceeds the
-// exact dU.
-__device__ __forceinline__ float dU_lower(uint32_t a, float inv_beta_lo) {
-    const float u24 = (float)(a >> 8) * 0x1p-24f;
-    float l;
-    if (u24 < 0x1p-10f) {
-        l = u24;                                           // -ln(1 - u) >= u >= u24
-    } else {
-        const float ml2 = -log2f(1.f - u24);
-        const float ml2_lo = ml2 - LOG2F_ERR * fmaxf(ml2, 1.f);  // <= -log2(1 - u24) <= -log2(1 - u)
-        l = ml2_lo * 0x1.62e42ep-1f;                       // ln 2 rounded down
-    }
-    return fmaxf(l * inv_beta_lo * (1.f - 0x1p-20f), 0.f);
-}
 
 // Exact FP64 event displacement of one truncated-and-shifted inverse-power factor
 // (k / r^p - k / rc^p inside rc), p = 12 (six: r^p = (r^2)^6) or 6, for separation s (target

@@ -36,6 +36,7 @@
 #include "../../include/jellyfysh_amd.h"
 #include "ecmc_math.h"
 #include "ecmc_potentials.h"
+#include "ecmc_fp32_bounds.h"
 #include "ecmc_common.h"
 
 namespace ecmc {
@@ -342,6 +343,27 @@ __device__ void block_reduce(Best &b, uint32_t &h, Best *sb, uint32_t *sh) {
     h = sh[0];
 }
 
+// Collective workgroup minimum of a non-negative float (+inf allowed); sh holds NT / 64 words.
+__device__ float block_min_float(float v, uint32_t *sh) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = __float_as_uint(v);
+    __syncthreads();
+    float r = __uint_as_float(sh[0]);
+    for (int w = 1; w < NT / 64; w++) r = fminf(r, __uint_as_float(sh[w]));
+    __syncthreads();
+    return r;
+}
+
+// The candidate sets whose exact inversions are filtered by FP32 brackets first: two-leaf
+// inverse-power factors (plain and truncated-and-shifted) of an integer power up to 32.
+__device__ __forceinline__ bool fp32_filter_applies(const ecmc_tagger_desc &tg) {
+    if (tg.event_handler != ECMC_EH_TWO_LEAF) return false;
+    if (tg.potential != ECMC_POT_INVERSE_POWER && tg.potential != ECMC_POT_INVERSE_POWER_TRUNC) return false;
+    const double p = tg.par[0];
+    return p >= 1.0 && p <= 32.0 && p == floor(p);
+}
+
 template <int D>
 __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &tg, int tix,
                             int *nb_cell, int *nb_pref, Best *sb, uint32_t *sh, int *scan) {
@@ -431,21 +453,77 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
     const int jbase = st.ncand;
     Best best = {INFINITY, INFINITY, 0x7fffffff};
     uint32_t hsum = 0;
-    for (int j = tid; j < M; j += NT) {
-        int target;
-        if (fmap) {
-            target = fmap_target(tg, a.leaves, act, j);
-        } else if (all) {
-            target = j < act ? j : j + 1;
-        } else if (sur) {
-            target = a.sur_id[j];
-        } else if (far) {
-            target = a.cand[j];
-        } else {
-            int c = 0;
-            while (nb_pref[c + 1] <= j) c++;
-            target = a.occ[(size_t)nb_cell[c] * a.occ_cap + (j - nb_pref[c])];
+    auto cand_target = [&](int j) {
+        if (fmap) return fmap_target(tg, a.leaves, act, j);
+        if (all) return j < act ? j : j + 1;
+        if (sur) return (int)a.sur_id[j];
+        if (far) return (int)a.cand[j];
+        int c = 0;
+        while (nb_pref[c + 1] <= j) c++;
+        return (int)a.occ[(size_t)nb_cell[c] * a.occ_cap + (j - nb_pref[c])];
+    };
+    // FP32 filter: every candidate's time displacement is bracketed [lb, ub] from its separation
+    // and the leading word of its draw; a candidate whose lb, added to the active time, comes
+    // after the active time plus the smallest ub of the set cannot be the earliest (nor tie
+    // with it) and is not inverted exactly; lb = +inf marks a candidate without an event.  The
+    // earliest candidate and every candidate of equal time are inverted exactly as before, so
+    // the result is the unfiltered one, unless a candidate left out would have raised the
+    // reference's ValueError (a negative square root from rounding), which the exact inversion
+    // flags.
+    const bool filt = fp32_filter_applies(tg) && M > 1;
+    float lbk0 = 0.f, lbk1 = 0.f;
+    Time thr = {INFINITY, INFINITY};
+    const int fp = (int)tg.par[0];
+    const float two_over_p = 2.f / (float)fp;
+    const bool ftrunc = tg.potential == ECMC_POT_INVERSE_POWER_TRUNC;
+    const float rc2f = ftrunc ? (float)(tg.par[2] * tg.par[2]) : 0.f;
+    const float ibf = (float)(1.0 / a.beta), isf = (float)(1.0 / speed);
+    auto cand_bounds = [&](int j, int target, float &lb, float &ub) {
+        double tp[D], s[D];
+        load_pos<D>(a, target, tp);
+        separation_vector<D>(a, ap, tp, s);
+        const double ct = tg.use_charge ? a.charge[target] : 1.0;
+        const double pp = tg.par[1] * (ca * ct);
+        const float K = (float)fabs(pp);
+        lb = 0.f;
+        ub = INFINITY;
+        if (K >= 0x1p-100f && K <= 0x1p100f) {
+            const uint32_t w0 = ring_word(a, base + 2ull * j);
+            ip_time_bounds<D>(s, dir, K, pp > 0.0, fp, two_over_p, ftrunc, dn(rc2f), up(rc2f),
+                              dU_lower(w0, ibf * (1.f - 0x1p-22f)), dU_upper(w0, ibf * (1.f + 0x1p-22f)),
+                              isf * (1.f - 0x1p-22f), isf * (1.f + 0x1p-22f), lb, ub);
         }
+    };
+    if (filt) {
+        float ubmin = INFINITY;
+        int k = 0;
+        for (int j = tid; j < M; j += NT, k++) {
+            const int target = cand_target(j);
+            float lb, ub;
+            cand_bounds(j, target, lb, ub);
+            if (k == 0) lbk0 = lb;
+            else if (k == 1) lbk1 = lb;
+            ubmin = fminf(ubmin, ub);
+            hsum += cand_mix((uint32_t)(jbase + j), (uint32_t)target);
+        }
+        ubmin = block_min_float(ubmin, sh);
+        if (ubmin < INFINITY) thr = time_add(at, (double)ubmin);
+    }
+    int kk = 0;
+    for (int j = tid; j < M; j += NT, kk++) {
+        if (filt) {
+            float lb, ub;
+            if (kk == 0) lb = lbk0;
+            else if (kk == 1) lb = lbk1;
+            else cand_bounds(j, cand_target(j), lb, ub);
+            if (lb == INFINITY) {
+                const Best c = {INFINITY, INFINITY, j};
+                if (best_lt(c, best)) best = c;
+                continue;
+            }
+            if (time_lt(thr, time_add(at, (double)lb))) continue;
+        }
+        const int target = cand_target(j);
         double tp[D], s[D];
         load_pos<D>(a, target, tp);
         separation_vector<D>(a, ap, tp, s);
@@ -467,7 +545,7 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
         Best c = {t.q, t.r, j};
         if (isnan(dt)) c.q = -INFINITY;  // flags a ValueError of the reference code
         if (best_lt(c, best)) best = c;
-        hsum += cand_mix((uint32_t)(jbase + j), (uint32_t)target);
+        if (!filt) hsum += cand_mix((uint32_t)(jbase + j), (uint32_t)target);
     }
     block_reduce(best, hsum, sb, sh);
     if (tid == 0) {
@@ -480,20 +558,7 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
         } else if (best.q == -INFINITY) {
             st.err = ECMC_ERR_NUMERIC;
         } else {
-            int j = best.j, target;
-            if (fmap) {
-                target = fmap_target(tg, a.leaves, act, j);
-            } else if (all) {
-                target = j < act ? j : j + 1;
-            } else if (sur) {
-                target = a.sur_id[j];
-            } else if (far) {
-                target = a.cand[j];
-            } else {
-                int c = 0;
-                while (nb_pref[c + 1] <= j) c++;
-                target = a.occ[(size_t)nb_cell[c] * a.occ_cap + (j - nb_pref[c])];
-            }
+            const int j = best.j, target = cand_target(j);
             sl.valid = 1;
             sl.cb_cell = st.occ_cell;
             sl.t.q = best.q;
@@ -803,78 +868,91 @@ __device__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_
     return true;
 }
 
-// ewald_derivative over one wavefront, all 64 lanes calling with the same separation: the
-// real-space images and Fourier terms are spread over the lanes (term g on lane g % 64), each
-// term computed with the same operations as in the sequential loop (the Fourier factors by
-// the same cos/sin recurrence, which every lane steps through), then added in the loop's order
-// by one lane reading the others: the result is the sequential function's, bit for bit.  More
-// than 256 terms fall back to the sequential function.
+// ewald_derivative over one wavefront, all 64 lanes calling with the same separation.  Term g
+// of the sequential loop (the real-space images by k, j, i, then the Fourier terms by i, j, k)
+// belongs to lane g % 64, slot g / 64: each lane finds its terms by walking the rows of the two
+// loops (one row per (k, j) or (i, j), no per-term iteration), evaluates them in parallel with
+// the sequential loop's operations (a Fourier factor by stepping the same cos/sin recurrence
+// from its start as far as that term), and the terms are then added in the loop's order by
+// reading the lanes one by one, so the sum rounds as the sequential one does.  More than 256
+// terms fall back to the sequential function.
 __device__ __forceinline__ double wave_lane_double(double v, int lane) {
     return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
                             __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// floor(sqrt(m)) of a small non-negative integer, as (int)sqrt((double)m)
+__device__ __forceinline__ int isqrt_small(int m) {
+    int c = (int)sqrtf((float)m);
+    while (c * c > m) c--;
+    while ((c + 1) * (c + 1) <= m) c++;
+    return c;
 }
 __device__ double ewald_derivative_wave(const Ewald &e, double sx, double sy, double sz) {
     const int lane = threadIdx.x & 63;
     const int pc = e.pc, pc2 = pc * pc, fc = e.fc, fc2 = fc * fc, n = fc + 1;
     const double L = e.L;
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    // code of slot s: kind << 24 | (i + 128) << 16 | (j + 128) << 8 | (k + 128), -1 for none
+    int code[4] = {-1, -1, -1, -1};
     int g = 0;
-    auto keep = [&](double v) {
-        if ((g & 63) == lane) {
-            const int s = g >> 6;
-            if (s == 0) t0 = v; else if (s == 1) t1 = v; else if (s == 2) t2 = v; else t3 = v;
-        }
-        g++;
-    };
     for (int k = -pc; k < pc + 1; k++) {
-        const int cy = (int)sqrt((double)(pc2 - k * k));
+        const int cy = isqrt_small(pc2 - k * k);
         for (int j = -cy; j < cy + 1; j++) {
-            const int cx = (int)sqrt((double)(pc2 - j * j - k * k));
-            for (int i = -cx; i < cx + 1; i++) {
-                if ((g & 63) == lane && g < 256) {
-                    const double vz2 = (sz + k * L) * (sz + k * L);
-                    const double vy2 = (sy + j * L) * (sy + j * L);
-                    const double vx = sx + i * L;
-                    const double v2 = vx * vx + vy2 + vz2;
-                    const double vn = sqrt(v2);
-                    keep(vx * (e.two_aol_rpi * exp(-e.aol2 * v2) + erfc(e.aol * vn) / vn) / v2);
-                } else {
-                    g++;
-                }
+            const int cx = isqrt_small(pc2 - j * j - k * k);
+            const int len = 2 * cx + 1;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int d = lane + 64 * s - g;
+                if (d >= 0 && d < len) code[s] = ((-cx + d + 128) << 16) | ((j + 128) << 8) | (k + 128);
             }
+            g += len;
         }
     }
-    const int nr = g;
-    double dcx = cos(e.tpol * sx), dsx = sin(e.tpol * sx);
-    double dcy = cos(e.tpol * sy), dsy = sin(e.tpol * sy);
-    double dcz = cos(e.tpol * sz), dsz = sin(e.tpol * sz);
-    double cxv = dcx, sxv = dsx, cyv = 1.0, syv = 0.0, czv = 1.0, szv = 0.0, tmp;
     for (int i = 1; i < fc + 1; i++) {
-        const int cy = (int)sqrt((double)(fc2 - i * i));
+        const int cy = isqrt_small(fc2 - i * i);
         for (int j = 0; j < cy + 1; j++) {
-            const int cx = (int)sqrt((double)(fc2 - i * i - j * j));
-            for (int k = 0; k < cx + 1; k++) {
-                if ((g & 63) == lane && g < 256) keep(e.f[(i * n + j) * n + k] * sxv * cyv * czv);
-                else g++;
-                if (k != cx) {
-                    tmp = czv; czv = tmp * dcz - szv * dsz; szv = szv * dcz + tmp * dsz;
-                } else if (j != cy) {
-                    tmp = cyv; cyv = tmp * dcy - syv * dsy; syv = syv * dcy + tmp * dsy;
-                    czv = 1.0; szv = 0.0;
-                } else if (i != fc) {
-                    tmp = cxv; cxv = tmp * dcx - sxv * dsx; sxv = sxv * dcx + tmp * dsx;
-                    cyv = 1.0; syv = 0.0; czv = 1.0; szv = 0.0;
-                }
+            const int cx = isqrt_small(fc2 - i * i - j * j);
+            const int len = cx + 1;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int d = lane + 64 * s - g;
+                if (d >= 0 && d < len) code[s] = (1 << 24) | ((i + 128) << 16) | ((j + 128) << 8) | (d + 128);
             }
+            g += len;
         }
     }
     if (g > 256) return ewald_derivative(e, sx, sy, sz);
-    (void)nr;
+    const double dcx = cos(e.tpol * sx), dsx = sin(e.tpol * sx);
+    const double dcy = cos(e.tpol * sy), dsy = sin(e.tpol * sy);
+    const double dcz = cos(e.tpol * sz), dsz = sin(e.tpol * sz);
+    double term[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        double v = 0.0;
+        const int c = code[s];
+        if (c >= 0) {
+            const int i = ((c >> 16) & 255) - 128, j = ((c >> 8) & 255) - 128, k = (c & 255) - 128;
+            if (c >> 24) {
+                double tmp, cxv = dcx, sxv = dsx, cyv = 1.0, syv = 0.0, czv = 1.0, szv = 0.0;
+                for (int r = 1; r < i; r++) { tmp = cxv; cxv = tmp * dcx - sxv * dsx; sxv = sxv * dcx + tmp * dsx; }
+                for (int r = 0; r < j; r++) { tmp = cyv; cyv = tmp * dcy - syv * dsy; syv = syv * dcy + tmp * dsy; }
+                for (int r = 0; r < k; r++) { tmp = czv; czv = tmp * dcz - szv * dsz; szv = szv * dcz + tmp * dsz; }
+                v = e.f[(i * n + j) * n + k] * sxv * cyv * czv;
+            } else {
+                const double vz2 = (sz + k * L) * (sz + k * L);
+                const double vy2 = (sy + j * L) * (sy + j * L);
+                const double vx = sx + i * L;
+                const double v2 = vx * vx + vy2 + vz2;
+                const double vn = sqrt(v2);
+                v = vx * (e.two_aol_rpi * exp(-e.aol2 * v2) + erfc(e.aol * vn) / vn) / v2;
+            }
+        }
+        term[s] = v;
+    }
     double der = 0.0;
-    for (int q = 0; q < g; q++) {
-        const int s = q >> 6;
-        const double v = s == 0 ? t0 : (s == 1 ? t1 : (s == 2 ? t2 : t3));
-        der += wave_lane_double(v, q & 63);
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int m = min(64, g - 64 * s);
+        for (int l = 0; l < m; l++) der += wave_lane_double(term[s], l);
     }
     return der;
 }

@@ -12,6 +12,7 @@
 #include "ecmc_common.h"
 #include "ecmc_math.h"
 #include "ecmc_potentials.h"
+#include "ecmc_fp32_bounds.h"
 
 namespace ecmc {
 
@@ -88,6 +89,31 @@ __global__ __launch_bounds__(PT) void pair_kernel(PairArgs a, int derivative) {
     pair_row<D>(a, i, derivative != 0);
 }
 
+// FP32 brackets of the time displacement of inverse-power rows (the chain kernel's candidate
+// filter), the potential change drawn from the leading word of the random pair.
+template <int D>
+__global__ __launch_bounds__(PT) void pair_bounds_kernel(int64_t n, int trunc, double power, double prefactor,
+                                                          double cutoff, const int32_t *dir, const double *speed,
+                                                          const double *sep, const double *c1, const double *c2,
+                                                          const uint32_t *word, double beta, float *lb, float *ub) {
+    const int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
+    if (i >= n) return;
+    double s[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) s[d] = sep[i * D + d];
+    const double pp = prefactor * ((c1 ? c1[i] : 1.0) * (c2 ? c2[i] : 1.0));
+    const float K = (float)fabs(pp);
+    const float ib = (float)(1.0 / beta), is = (float)(1.0 / (speed ? speed[i] : 1.0));
+    const float rc2 = trunc ? (float)(cutoff * cutoff) : 0.f;
+    float l = 0.f, u = __int_as_float(0x7f800000);
+    if (K >= 0x1p-100f && K <= 0x1p100f)
+        ip_time_bounds<D>(s, dir[i], K, pp > 0.0, (int)power, 2.f / (float)(int)power, trunc != 0, dn(rc2), up(rc2),
+                          dU_lower(word[i], ib * (1.f - 0x1p-22f)), dU_upper(word[i], ib * (1.f + 0x1p-22f)),
+                          is * (1.f - 0x1p-22f), is * (1.f + 0x1p-22f), l, u);
+    lb[i] = l;
+    ub[i] = u;
+}
+
 __global__ __launch_bounds__(PT) void log_kernel(int64_t n, const double *x, double *out) {
     int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
     if (i < n) out[i] = cr_log(x[i]);
@@ -144,6 +170,31 @@ static int launch_pair(int32_t potential, const double *par, int32_t dim, int64_
         hipFree(d_table);
     }
     if (e != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "pair kernel launch failed");
+    return ECMC_OK;
+}
+
+extern "C" int ecmc_pair_time_bounds(int32_t potential, const double *par, int32_t dim, int64_t n,
+                                     const int32_t *d_dir, const double *d_speed, const double *d_sep,
+                                     const double *d_c1, const double *d_c2, const uint32_t *d_word, double beta,
+                                     float *d_lb, float *d_ub, void *stream) {
+    if (n < 0 || (dim != 2 && dim != 3) || !par || !(beta > 0.0)) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
+    if (potential != POT_INVERSE_POWER && potential != POT_INVERSE_POWER_TRUNC)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "time brackets exist for the inverse-power potentials only");
+    if (!(par[0] >= 1.0 && par[0] <= 32.0 && par[0] == floor(par[0])))
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "time brackets need an integer power from 1 to 32");
+    if (n == 0) return ECMC_OK;
+    if (!d_dir || !d_sep || !d_word || !d_lb || !d_ub) return ecmc_fail(ECMC_ERR_ARGUMENT, "null device pointer");
+    const int trunc = potential == POT_INVERSE_POWER_TRUNC;
+    const double cutoff = trunc ? par[2] : 0.0;
+    dim3 grid((unsigned)((n + PT - 1) / PT));
+    hipStream_t st = (hipStream_t)stream;
+    if (dim == 2)
+        hipLaunchKernelGGL(pair_bounds_kernel<2>, grid, dim3(PT), 0, st, n, trunc, par[0], par[1], cutoff, d_dir, d_speed,
+                           d_sep, d_c1, d_c2, d_word, beta, d_lb, d_ub);
+    else
+        hipLaunchKernelGGL(pair_bounds_kernel<3>, grid, dim3(PT), 0, st, n, trunc, par[0], par[1], cutoff, d_dir, d_speed,
+                           d_sep, d_c1, d_c2, d_word, beta, d_lb, d_ub);
+    if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "bounds kernel launch failed");
     return ECMC_OK;
 }
 

@@ -85,10 +85,13 @@ class CandidateSetEventHandler:
         return Time(q.value, r.value), [target.value]
 
     def send_out_state(self, target):
+        """Commit the event whose time this handler sent last.  ``target`` is the candidate that
+        send_event_time returned (a unit, or a cell for the cell veto); the device holds the same
+        candidate in the handler's slot and resolves it at the commit (a cell to its occupant,
+        a thinning to a lift or a rejection), so the committed record's target can differ from it
+        (start, sampling and end-of-chain events carry -1)."""
         ev = np.zeros(1, dtype=L.EVENT_DTYPE)
         L.check(L.lib().ecmc_system_send_out_state(self._system._h, self.tagger, ev.ctypes.data))
-        if target >= 0 and ev["target"][0] != target and ev["tagger"][0] == self.tagger:
-            raise RuntimeError("the committed event's target differs from the one sent with its time")
         return ev[0]
 
 

