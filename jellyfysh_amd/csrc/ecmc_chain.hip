@@ -113,6 +113,10 @@ struct ChainArgs {
     const CvEntry *cvw;
     double cv_total[6], cv_mean[6];
     int32_t cv_m, pad3;
+    // residency in the launch's dynamic LDS: positions (dim x n doubles at offset 0) and the MT
+    // ring (ring_words words at lds_ring_off), copied in at the start of a launch and back at
+    // its end; 0 keeps them in global memory
+    int32_t lds_pos, lds_ring, lds_ring_off, pad4;
 };
 
 __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
@@ -126,7 +130,8 @@ __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
 // One twist of the 624-word state in LDS, then the tempered words go to the ring.
 // The three phases follow the data dependences of the sequential loop: words 0..226 read
 // only old words; 227..453 read new words 0..226; 454..623 read new words 227..396.
-__device__ void mt_generate_block(uint32_t *mt, uint32_t *ring, int ring_words, uint64_t produced) {
+// Word j of the block goes to ring[(produced + j) & mask] (mask: ring size - 1, all ones for no wrap).
+__device__ void mt_generate_block(uint32_t *mt, uint32_t *ring, uint64_t mask, uint64_t produced) {
     const int t = threadIdx.x;
     uint32_t v = 0;
     if (t < 227) {
@@ -153,21 +158,21 @@ __device__ void mt_generate_block(uint32_t *mt, uint32_t *ring, int ring_words, 
     __syncthreads();
     if (t < 170) mt[t + 454] = v;
     __syncthreads();
-    for (int j = t; j < 624; j += NT) ring[(produced + (uint64_t)j) % (uint64_t)ring_words] = mt_temper(mt[j]);
+    for (int j = t; j < 624; j += NT) ring[(produced + (uint64_t)j) & mask] = mt_temper(mt[j]);
     __syncthreads();
 }
 
 // Collective: make `need` unread words available in the ring.
 __device__ void ring_ensure(ChainState &st, uint32_t *mt, const ChainArgs &a, uint64_t need) {
     while (st.produced < st.cursor + need) {
-        mt_generate_block(mt, a.ring, a.ring_words, st.produced);
+        mt_generate_block(mt, a.ring, (uint64_t)(a.ring_words - 1), st.produced);
         if (threadIdx.x == 0) st.produced += 624;
         __syncthreads();
     }
 }
 
 __device__ __forceinline__ uint32_t ring_word(const ChainArgs &a, uint64_t idx) {
-    return a.ring[idx % (uint64_t)a.ring_words];
+    return a.ring[idx & (uint64_t)(a.ring_words - 1)];   // ring_words is a power of two
 }
 
 // random() from two words (genrand_res53)
@@ -458,9 +463,15 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
         if (all) return j < act ? j : j + 1;
         if (sur) return (int)a.sur_id[j];
         if (far) return (int)a.cand[j];
-        int c = 0;
-        while (nb_pref[c + 1] <= j) c++;
-        return (int)a.occ[(size_t)nb_cell[c] * a.occ_cap + (j - nb_pref[c])];
+        // the nearby cell holding candidate j: the largest c with nb_pref[c] <= j (binary search
+        // over the nc + 1 prefix sums; empty cells repeat a prefix)
+        int lo = 0, hi = nc;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (nb_pref[mid] <= j) lo = mid;
+            else hi = mid;
+        }
+        return (int)a.occ[(size_t)nb_cell[lo] * a.occ_cap + (j - nb_pref[lo])];
     };
     // FP32 filter: every candidate's time displacement is bracketed [lb, ub] from its separation
     // and the leading word of its draw; a candidate whose lb, added to the active time, comes
@@ -521,7 +532,8 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
                 if (best_lt(c, best)) best = c;
                 continue;
             }
-            if (time_lt(thr, time_add(at, (double)lb))) continue;
+            // lb = 0 never culls: the exact FP64 displacement can round below zero there
+            if (lb > 0.f && time_lt(thr, time_add(at, (double)lb))) continue;
         }
         const int target = cand_target(j);
         double tp[D], s[D];
@@ -1214,13 +1226,26 @@ __device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagg
 
 // One workgroup runs one system's legs (the whole leg loop of the reference's mediator).
 template <int D>
-__device__ __forceinline__ void chain_body(const ChainArgs &a) {
+__device__ __forceinline__ void chain_body(const ChainArgs &ga) {
     __shared__ ChainState st;
     __shared__ ecmc_tagger_desc tg[ECMC_MAX_TAGGERS];
     __shared__ uint32_t mt[624];
     __shared__ LegShared ls;
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
     int &flag = ls.flag;
     const int tid = threadIdx.x;
+    // the positions and the MT ring move into LDS for the launch when the host sized it so
+    ChainArgs a = ga;
+    if (ga.lds_pos) {
+        double *lp = (double *)dyn_lds;
+        for (int i = tid; i < D * ga.n; i += NT) lp[i] = ga.pos[i];
+        a.pos = lp;
+    }
+    if (ga.lds_ring) {
+        uint32_t *lr = (uint32_t *)(dyn_lds + ga.lds_ring_off);
+        for (int i = tid; i < ga.ring_words; i += NT) lr[i] = ga.ring[i];
+        a.ring = lr;
+    }
     if (tid == 0) st = *a.st;
     for (int i = tid; i < a.n_taggers; i += NT) tg[i] = a.tg[i];
     for (int i = tid; i < 624; i += NT) mt[i] = a.mt[i];
@@ -1303,6 +1328,10 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
         *a.n_out = k;
     }
     for (int i = tid; i < 624; i += NT) a.mt[i] = mt[i];
+    if (ga.lds_pos)
+        for (int i = tid; i < D * ga.n; i += NT) ga.pos[i] = a.pos[i];
+    if (ga.lds_ring)
+        for (int i = tid; i < ga.ring_words; i += NT) ga.ring[i] = a.ring[i];
 }
 
 template <int D>
@@ -1503,8 +1532,55 @@ struct ecmc_system {
     double last_ms = 0.0;
     int32_t stop_mask = 0;
     int device = 0;          // the device current at creation, which holds its memory
+    size_t lds_bytes = 0;    // dynamic LDS of a launch (positions and ring resident there)
     ChainArgs args;
 };
+
+// Dynamic LDS of the leg loop for a system of n units in dim dimensions with a ring of
+// ring_words words: the positions first, then the ring, each only while the whole fits next to
+// the kernel's static LDS within the device's per-workgroup limit.
+static void plan_lds(ecmc_system *s) {
+    int dev_max = 0;
+    hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
+    hipFuncAttributes fa;
+    size_t stat = 16384;
+    const hipError_t e = s->dim == 2 ? hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&chain_kernel<2>))
+                                     : hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&chain_kernel<3>));
+    if (e == hipSuccess) stat = fa.sharedSizeBytes;
+    const size_t budget = dev_max > (int)stat ? (size_t)dev_max - stat : 0;
+    const size_t pos_b = ((size_t)s->dim * s->n * sizeof(double) + 15) & ~(size_t)15;
+    const size_t ring_b = (size_t)s->ring_words * sizeof(uint32_t);
+    ChainArgs &a = s->args;
+    a.lds_pos = a.lds_ring = 0;
+    a.lds_ring_off = 0;
+    size_t used = 0;
+    if (pos_b <= budget) {
+        a.lds_pos = 1;
+        used = pos_b;
+    }
+    if (used + ring_b <= budget) {
+        a.lds_ring = 1;
+        a.lds_ring_off = (int32_t)used;
+        used += ring_b;
+    }
+    s->lds_bytes = used;
+    if (used > 65536) {
+        const hipError_t r = s->dim == 2
+            ? hipFuncSetAttribute(reinterpret_cast<const void *>(&chain_kernel<2>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)used)
+            : hipFuncSetAttribute(reinterpret_cast<const void *>(&chain_kernel<3>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)used);
+        const hipError_t r2 = s->dim == 2
+            ? hipFuncSetAttribute(reinterpret_cast<const void *>(&chain_many_kernel<2>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)used)
+            : hipFuncSetAttribute(reinterpret_cast<const void *>(&chain_many_kernel<3>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)used);
+        if (r != hipSuccess || r2 != hipSuccess) {
+            // keep within the default limit: the ring alone, or nothing
+            (void)hipGetLastError();
+            a.lds_pos = 0;
+            a.lds_ring = ring_b <= 65536 && ring_b <= budget ? 1 : 0;
+            a.lds_ring_off = 0;
+            s->lds_bytes = a.lds_ring ? ring_b : 0;
+        }
+    }
+}
 
 extern "C" int ecmc_system_set_stop_mask(ecmc_system *sys, int32_t mask) {
     if (!sys) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
@@ -1690,8 +1766,15 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
         for (int d = 0; d < dim; d++) pos_soa[(size_t)d * n + i] = positions[(size_t)i * dim + d];
     std::vector<double> ch(n, 1.0);
     if (charges) for (int i = 0; i < n; i++) ch[i] = charges[i];
-    // the ring holds one tagger's draws (two words per candidate) plus room for the twist
-    s->ring_words = (int)(624 * ((2 * max_cand + 256) / 624 + 3));
+    // the ring holds every word a create may read plus a block in production: a power of two, so
+    // that word indices wrap with a mask
+    {
+        const int64_t need = 624 * ((2 * (int64_t)max_cand + 256) / 624 + 3);
+        int64_t words = 1024;
+        while (words < need) words <<= 1;
+        if (words > (1ll << 30)) { delete s; return ecmc_fail(ECMC_ERR_CONFIGURATION, "candidate sets too large"); }
+        s->ring_words = (int)words;
+    }
     uint32_t mt[624];
     mt_seed_host(desc->seed, mt);
     ChainState st;
@@ -1893,6 +1976,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
             a.cv_m = m;
         }
     }
+    plan_lds(s);
     *out = s;
     return ECMC_OK;
 }
@@ -1937,8 +2021,8 @@ extern "C" int ecmc_system_run(ecmc_system *s, int64_t max_events, ecmc_event *h
     a.max_events = max_events;
     a.stop_mask = s->stop_mask;
     hipEventRecord(s->ev0, s->stream);
-    if (s->dim == 2) hipLaunchKernelGGL(chain_kernel<2>, dim3(1), dim3(NT), 0, s->stream, a);
-    else hipLaunchKernelGGL(chain_kernel<3>, dim3(1), dim3(NT), 0, s->stream, a);
+    if (s->dim == 2) hipLaunchKernelGGL(chain_kernel<2>, dim3(1), dim3(NT), s->lds_bytes, s->stream, a);
+    else hipLaunchKernelGGL(chain_kernel<3>, dim3(1), dim3(NT), s->lds_bytes, s->stream, a);
     hipEventRecord(s->ev1, s->stream);
     if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
     if (hipStreamSynchronize(s->stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
@@ -1963,8 +2047,10 @@ extern "C" int ecmc_system_run(ecmc_system *s, int64_t max_events, ecmc_event *h
 static int systems_run_staged(ecmc_system *const *systems, int32_t count, int64_t max_events, int64_t *n_committed,
                               ChainArgs *host_args, ChainArgs *d_args) {
     const int dim = systems[0]->dim;
+    size_t lds = 0;
     for (int i = 0; i < count; i++) {
         ecmc_system *s = systems[i];
+        if (s->lds_bytes > lds) lds = s->lds_bytes;
         host_args[i] = s->args;
         host_args[i].log = nullptr;
         host_args[i].max_events = max_events;
@@ -1976,8 +2062,8 @@ static int systems_run_staged(ecmc_system *const *systems, int32_t count, int64_
     if (hipMemcpyAsync(d_args, host_args, sizeof(ChainArgs) * count, hipMemcpyHostToDevice, s0->stream) != hipSuccess)
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
     hipEventRecord(s0->ev0, s0->stream);
-    if (dim == 2) hipLaunchKernelGGL(chain_many_kernel<2>, dim3(count), dim3(NT), 0, s0->stream, d_args);
-    else hipLaunchKernelGGL(chain_many_kernel<3>, dim3(count), dim3(NT), 0, s0->stream, d_args);
+    if (dim == 2) hipLaunchKernelGGL(chain_many_kernel<2>, dim3(count), dim3(NT), lds, s0->stream, d_args);
+    else hipLaunchKernelGGL(chain_many_kernel<3>, dim3(count), dim3(NT), lds, s0->stream, d_args);
     hipEventRecord(s0->ev1, s0->stream);
     const bool launched = hipGetLastError() == hipSuccess;
     const bool ran = hipStreamSynchronize(s0->stream) == hipSuccess;
@@ -2069,8 +2155,10 @@ extern "C" int ecmc_systems_run_noalloc(ecmc_system *const *systems, int32_t cou
 static int system_step(ecmc_system *s, int op, int tagger, ChainState &st) {
     if (!s) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
     if (op != 0 && (tagger < 0 || tagger >= s->desc.n_taggers)) return ecmc_fail(ECMC_ERR_ARGUMENT, "no such tagger");
-    if (s->dim == 2) hipLaunchKernelGGL(chain_step_kernel<2>, dim3(1), dim3(NT), 0, s->stream, s->args, op, tagger, s->d_step_ev);
-    else hipLaunchKernelGGL(chain_step_kernel<3>, dim3(1), dim3(NT), 0, s->stream, s->args, op, tagger, s->d_step_ev);
+    ChainArgs sa = s->args;   // one step works on global memory
+    sa.lds_pos = sa.lds_ring = 0;
+    if (s->dim == 2) hipLaunchKernelGGL(chain_step_kernel<2>, dim3(1), dim3(NT), 0, s->stream, sa, op, tagger, s->d_step_ev);
+    else hipLaunchKernelGGL(chain_step_kernel<3>, dim3(1), dim3(NT), 0, s->stream, sa, op, tagger, s->d_step_ev);
     if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
     if (hipStreamSynchronize(s->stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
     if (hipMemcpy(&st, s->d_st, sizeof(ChainState), hipMemcpyDeviceToHost) != hipSuccess)
@@ -2173,7 +2261,7 @@ __global__ __launch_bounds__(NT) void mt_words_kernel(uint32_t *state, uint32_t 
     for (int i = threadIdx.x; i < 624; i += NT) mt[i] = state[i];
     __syncthreads();
     for (int64_t produced = 0; produced < n; produced += 624) {
-        mt_generate_block(mt, out + produced, 624, 0);
+        mt_generate_block(mt, out + produced, ~0ull, 0);
     }
 }
 

@@ -108,7 +108,9 @@ __device__ __forceinline__ float rstar2_g(float K, float V, float two_over_p, bo
 // trunc with the squared cutoff bracketed by [rc2_lo, rc2_hi], dU bracketed by [dU_lo, dU_hi],
 // 1 / speed by [isp_lo, isp_hi].  lb = +inf only where the exact displacement is +inf (no
 // event); ub = +inf where an event is not certain.  Every case the brackets cannot decide gives
-// lb = 0, ub = +inf.
+// lb = 0, ub = +inf.  A positive lb bounds the FP64 displacement of ip_displacement /
+// ipt_displacement from below; where lb = 0 the FP64 value can be a rounding-level negative
+// number (an event at the start, computed as s_d - sqrt(a) with a = s_d^2 in exact terms).
 template <int D>
 __device__ void ip_time_bounds(const double *s, int dir, float K, bool rep, int p, float two_over_p, bool trunc,
                                float rc2_lo, float rc2_hi, float dU_lo, float dU_hi, float isp_lo, float isp_hi,
