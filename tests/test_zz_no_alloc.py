@@ -32,7 +32,7 @@ def _chains(seeds):
         if s % 2:
             out.append(make_chain(charged_soft_spheres(n=32, system_length=4.0, seed=s, cells_per_side=4)))
         else:
-            meta, pos = S.hard_disk_system(n=64, phi=0.5, seed=s)
+            meta, pos = S.lj_system(n_cells=3, rho=0.8, seed=s)
             out.append(DeviceChain(S.device_desc(meta), pos))
     return out
 
