@@ -79,18 +79,21 @@ def copy_calibration(nbytes):
     print(json.dumps({"copy_kernel_bytes_per_launch": nbytes, "launches": 10}), flush=True)
 
 
-def profiled_traffic(kernel, legs):
-    """HBM bytes per launch of `kernel` at `legs` legs per replica, from the committed counter
-    profile of this workload (FETCH_SIZE scaled by the copy calibration's factor, plus
-    WRITE_SIZE); None when the profile has no line for it."""
-    path = os.path.join(ROOT, "profiles", "batched_kernel_lj_4096_replicas_counters.txt")
+def profiled_traffic(kernel, legs, profile="batched_kernel_lj_4096_replicas_counters.txt", tag=None):
+    """HBM bytes per launch of `kernel` at `legs` legs per replica (and workload `tag`), from the
+    committed counter profile of this workload (FETCH_SIZE scaled by the copy calibration's
+    factor, plus WRITE_SIZE); None when the profile has no line for it."""
+    path = os.path.join(ROOT, "profiles", profile)
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        # traffic <kernel> legs=<L> bytes_per_launch=<B>
-        if len(f) == 4 and f[0] == "traffic" and f[1] == kernel and f[2] == "legs=%d" % legs:
-            return float(f[3].split("=")[1])
+        # traffic <kernel> [tag=<T>] legs=<L> bytes_per_launch=<B>
+        if not f or f[0] != "traffic" or len(f) < 4 or f[1] != kernel:
+            continue
+        fields = dict(x.split("=", 1) for x in f[2:] if "=" in x)
+        if fields.get("legs") == str(legs) and fields.get("tag") == tag:
+            return float(fields["bytes_per_launch"])
     return None
 
 
@@ -144,6 +147,8 @@ def plain_chain_rate(legs, pressure_seconds, equilibration_legs):
                                    "kernel (%.1f s)" % (done_eq, t_eq),
                events=int(done), kernel_ms=ms, wall_s=wall, events_per_sec=done / (ms * 1e-3))
     out["roofline"] = plain_roofline("ecmc::chain_kernel<3>", ncand, 3, 27, done, ms, charged=False)
+    out["roofline"]["traffic"] = profiled_traffic("ecmc::chain_kernel<3>", legs,
+                                                  "chain_kernel_plain_lj_4096_counters.txt", "plain")
     out["cpu_single_core"] = plain_cpu_single_core(desc, chain.positions(), seconds=4.0)
     # pressure: chunks of `legs` legs for `pressure_seconds`; the standard error comes from
     # 8 blocks of consecutive chunks (chunks of one chain are correlated)
@@ -272,6 +277,8 @@ def charged_spheres_rate(replicas, legs, far="cell_bounding"):
     ncand = float(probe.run(500)["ncand"].mean())
     probe.close()
     roof = plain_roofline("ecmc::chain_many_kernel<3>", ncand, 3, 27, int(k.sum()), ms, charged=True)
+    roof["traffic"] = profiled_traffic("ecmc::chain_many_kernel<3>", legs,
+                                       "chain_many_kernel_charged_soft_spheres_counters.txt", far)
     from jellyfysh_amd.chain import make_desc
     desc = make_desc(3, 512, sysd["system_length"], sysd["beta"], sysd["cells_per_side"], sysd["taggers"], 999,
                      scheduler="heap", max_occupants=1)

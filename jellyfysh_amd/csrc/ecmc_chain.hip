@@ -117,6 +117,11 @@ struct ChainArgs {
     // ring (ring_words words at lds_ring_off), copied in at the start of a launch and back at
     // its end; 0 keeps them in global memory
     int32_t lds_pos, lds_ring, lds_ring_off, pad4;
+    // the terms of the merged-image Coulomb derivative in the sequential loop's order (real-space
+    // images by k, j, i, then Fourier terms by i, j, k): kind << 24 | (i + 128) << 16 |
+    // (j + 128) << 8 | (k + 128), built on the host
+    const int32_t *ewald_codes;
+    int32_t ewald_ncodes, pad5;
 };
 
 __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
@@ -474,51 +479,76 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
         return (int)a.occ[(size_t)nb_cell[lo] * a.occ_cap + (j - nb_pref[lo])];
     };
     // FP32 filter: every candidate's time displacement is bracketed [lb, ub] from its separation
-    // and the leading word of its draw; a candidate whose lb, added to the active time, comes
-    // after the active time plus the smallest ub of the set cannot be the earliest (nor tie
-    // with it) and is not inverted exactly; lb = +inf marks a candidate without an event.  The
-    // earliest candidate and every candidate of equal time are inverted exactly as before, so
-    // the result is the unfiltered one, unless a candidate left out would have raised the
-    // reference's ValueError (a negative square root from rounding), which the exact inversion
-    // flags.
+    // and the leading word of its draw; a candidate whose lb exceeds the smallest ub of the set
+    // by more than the rounding of the active time can make up ends strictly later, so it can
+    // be neither the earliest nor tie with it and is not inverted exactly; lb = +inf marks a
+    // candidate without an event.  The earliest candidate and every candidate of equal time are
+    // inverted exactly as before, so the result is the unfiltered one, unless a candidate left
+    // out would have raised the reference's ValueError (a negative square root from rounding),
+    // which the exact inversion flags.
     const bool filt = fp32_filter_applies(tg) && M > 1;
     float lbk0 = 0.f, lbk1 = 0.f;
-    Time thr = {INFINITY, INFINITY};
+    double cull_above = INFINITY;
     const int fp = (int)tg.par[0];
     const float two_over_p = 2.f / (float)fp;
     const bool ftrunc = tg.potential == ECMC_POT_INVERSE_POWER_TRUNC;
     const float rc2f = ftrunc ? (float)(tg.par[2] * tg.par[2]) : 0.f;
     const float ibf = (float)(1.0 / a.beta), isf = (float)(1.0 / speed);
+    // the separation of the brackets: the difference wrapped by one box length, within two ulps
+    // of L of separation_vector's Python-modulo form except near +-L/2, where the candidate keeps
+    // the trivial brackets
+    const double edge = a.half * (1.0 - 0x1p-40);
+    const float sep_slack = (float)(a.L * 0x1p-49);
     auto cand_bounds = [&](int j, int target, float &lb, float &ub) {
         double tp[D], s[D];
         load_pos<D>(a, target, tp);
-        separation_vector<D>(a, ap, tp, s);
+        bool near_edge = false;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double v = tp[d] - ap[d];
+            v = v >= a.half ? v - a.L : (v < -a.half ? v + a.L : v);
+            near_edge = near_edge || !(fabs(v) < edge);
+            s[d] = v;
+        }
         const double ct = tg.use_charge ? a.charge[target] : 1.0;
         const double pp = tg.par[1] * (ca * ct);
         const float K = (float)fabs(pp);
         lb = 0.f;
         ub = INFINITY;
-        if (K >= 0x1p-100f && K <= 0x1p100f) {
+        if (!near_edge && K >= 0x1p-100f && K <= 0x1p100f) {
             const uint32_t w0 = ring_word(a, base + 2ull * j);
             ip_time_bounds<D>(s, dir, K, pp > 0.0, fp, two_over_p, ftrunc, dn(rc2f), up(rc2f),
                               dU_lower(w0, ibf * (1.f - 0x1p-22f)), dU_upper(w0, ibf * (1.f + 0x1p-22f)),
-                              isf * (1.f - 0x1p-22f), isf * (1.f + 0x1p-22f), lb, ub);
+                              isf * (1.f - 0x1p-22f), isf * (1.f + 0x1p-22f), lb, ub, sep_slack);
         }
     };
     if (filt) {
         float ubmin = INFINITY;
-        int k = 0;
-        for (int j = tid; j < M; j += NT, k++) {
-            const int target = cand_target(j);
-            float lb, ub;
-            cand_bounds(j, target, lb, ub);
-            if (k == 0) lbk0 = lb;
-            else if (k == 1) lbk1 = lb;
-            ubmin = fminf(ubmin, ub);
-            hsum += cand_mix((uint32_t)(jbase + j), (uint32_t)target);
+        // two candidates per iteration, evaluated side by side (the second one a repeat of the
+        // first past the end of the set, its result dropped)
+        for (int j0 = tid; j0 < M; j0 += 2 * NT) {
+            const int j1 = j0 + NT;
+            const bool has1 = j1 < M;
+            const int t0 = cand_target(j0), t1 = cand_target(has1 ? j1 : j0);
+            float lb0, ub0, lb1, ub1;
+            cand_bounds(j0, t0, lb0, ub0);
+            cand_bounds(has1 ? j1 : j0, t1, lb1, ub1);
+            if (j0 == tid) {
+                lbk0 = lb0;
+                lbk1 = lb1;
+            }
+            ubmin = fminf(ubmin, ub0);
+            hsum += cand_mix((uint32_t)(jbase + j0), (uint32_t)t0);
+            if (has1) {
+                ubmin = fminf(ubmin, ub1);
+                hsum += cand_mix((uint32_t)(jbase + j1), (uint32_t)t1);
+            }
         }
         ubmin = block_min_float(ubmin, sh);
-        if (ubmin < INFINITY) thr = time_add(at, (double)ubmin);
+        // a displacement beyond ubmin + 2^-48 (1 + ubmin) ends later than ubmin does: the active
+        // time's remainder lies in [0, 1), so the two sums with it round to doubles at least one
+        // ulp apart, and Time's divmod by one keeps their order strict
+        if (ubmin < INFINITY) cull_above = (double)ubmin + 0x1p-48 * (1.0 + (double)ubmin);
     }
     int kk = 0;
     for (int j = tid; j < M; j += NT, kk++) {
@@ -533,7 +563,7 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
                 continue;
             }
             // lb = 0 never culls: the exact FP64 displacement can round below zero there
-            if (lb > 0.f && time_lt(thr, time_add(at, (double)lb))) continue;
+            if (lb > 0.f && (double)lb > cull_above) continue;
         }
         const int target = cand_target(j);
         double tp[D], s[D];
@@ -880,63 +910,27 @@ __device__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_
     return true;
 }
 
-// ewald_derivative over one wavefront, all 64 lanes calling with the same separation.  Term g
-// of the sequential loop (the real-space images by k, j, i, then the Fourier terms by i, j, k)
-// belongs to lane g % 64, slot g / 64: each lane finds its terms by walking the rows of the two
-// loops (one row per (k, j) or (i, j), no per-term iteration), evaluates them in parallel with
-// the sequential loop's operations (a Fourier factor by stepping the same cos/sin recurrence
-// from its start as far as that term), and the terms are then added in the loop's order by
-// reading the lanes one by one, so the sum rounds as the sequential one does.  More than 256
-// terms fall back to the sequential function.
-__device__ __forceinline__ double wave_lane_double(double v, int lane) {
-    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
-                            __builtin_amdgcn_readlane(__double2loint(v), lane));
-}
-// floor(sqrt(m)) of a small non-negative integer, as (int)sqrt((double)m)
-__device__ __forceinline__ int isqrt_small(int m) {
-    int c = (int)sqrtf((float)m);
-    while (c * c > m) c--;
-    while ((c + 1) * (c + 1) <= m) c++;
-    return c;
-}
-__device__ double ewald_derivative_wave(const Ewald &e, double sx, double sy, double sz) {
+// ewald_derivative over one wavefront, all 64 lanes calling with the same separation.  The
+// host lists the terms of the sequential loop (the real-space images by k, j, i, then the
+// Fourier terms by i, j, k); term g belongs to lane g % 64, slot g / 64.  Each lane evaluates
+// its terms with the sequential loop's operations (a Fourier factor by stepping the same
+// cos/sin recurrence from its start as far as that term), adds its slots in order, and a
+// butterfly over the lanes adds the 64 partial sums (commutative pairings, so every lane ends
+// with the same sum).  The order of the additions differs from the sequential loop's, so the
+// result agrees with it to rounding.  More than 256 terms fall back to the sequential function.
+__device__ double ewald_derivative_wave(const Ewald &e, const int32_t *codes, int ncodes, double sx, double sy,
+                                        double sz) {
+    if (ncodes > 256 || !codes) return ewald_derivative(e, sx, sy, sz);
     const int lane = threadIdx.x & 63;
-    const int pc = e.pc, pc2 = pc * pc, fc = e.fc, fc2 = fc * fc, n = fc + 1;
+    const int n = e.fc + 1;
     const double L = e.L;
-    // code of slot s: kind << 24 | (i + 128) << 16 | (j + 128) << 8 | (k + 128), -1 for none
-    int code[4] = {-1, -1, -1, -1};
-    int g = 0;
-    for (int k = -pc; k < pc + 1; k++) {
-        const int cy = isqrt_small(pc2 - k * k);
-        for (int j = -cy; j < cy + 1; j++) {
-            const int cx = isqrt_small(pc2 - j * j - k * k);
-            const int len = 2 * cx + 1;
+    int code[4];
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int d = lane + 64 * s - g;
-                if (d >= 0 && d < len) code[s] = ((-cx + d + 128) << 16) | ((j + 128) << 8) | (k + 128);
-            }
-            g += len;
-        }
-    }
-    for (int i = 1; i < fc + 1; i++) {
-        const int cy = isqrt_small(fc2 - i * i);
-        for (int j = 0; j < cy + 1; j++) {
-            const int cx = isqrt_small(fc2 - i * i - j * j);
-            const int len = cx + 1;
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int d = lane + 64 * s - g;
-                if (d >= 0 && d < len) code[s] = (1 << 24) | ((i + 128) << 16) | ((j + 128) << 8) | (d + 128);
-            }
-            g += len;
-        }
-    }
-    if (g > 256) return ewald_derivative(e, sx, sy, sz);
+    for (int s = 0; s < 4; s++) code[s] = lane + 64 * s < ncodes ? codes[lane + 64 * s] : -1;
     const double dcx = cos(e.tpol * sx), dsx = sin(e.tpol * sx);
     const double dcy = cos(e.tpol * sy), dsy = sin(e.tpol * sy);
     const double dcz = cos(e.tpol * sz), dsz = sin(e.tpol * sz);
-    double term[4];
+    double sum = 0.0;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         double v = 0.0;
@@ -958,15 +952,31 @@ __device__ double ewald_derivative_wave(const Ewald &e, double sx, double sy, do
                 v = vx * (e.two_aol_rpi * exp(-e.aol2 * v2) + erfc(e.aol * vn) / vn) / v2;
             }
         }
-        term[s] = v;
+        sum += v;
     }
-    double der = 0.0;
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        const int m = min(64, g - 64 * s);
-        for (int l = 0; l < m; l++) der += wave_lane_double(term[s], l);
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+    return sum;
+}
+
+// The terms of ewald_derivative in its loop order, as the codes of ewald_derivative_wave.
+static std::vector<int32_t> ewald_term_codes(int fc, int pc) {
+    std::vector<int32_t> out;
+    for (int k = -pc; k < pc + 1; k++) {
+        const int cy = (int)sqrt((double)(pc * pc - k * k));
+        for (int j = -cy; j < cy + 1; j++) {
+            const int cx = (int)sqrt((double)(pc * pc - j * j - k * k));
+            for (int i = -cx; i < cx + 1; i++) out.push_back(((i + 128) << 16) | ((j + 128) << 8) | (k + 128));
+        }
     }
-    return der;
+    for (int i = 1; i < fc + 1; i++) {
+        const int cy = (int)sqrt((double)(fc * fc - i * i));
+        for (int j = 0; j < cy + 1; j++) {
+            const int cx = (int)sqrt((double)(fc * fc - i * i - j * j));
+            for (int k = 0; k < cx + 1; k++) out.push_back((1 << 24) | ((i + 128) << 16) | ((j + 128) << 8) | (k + 128));
+        }
+    }
+    return out;
 }
 
 // The separation (permuted: x along the motion) at which the commit of tagger w's stored
@@ -1304,7 +1314,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs &ga) {
         double ewald_pre = NAN;
         if (tid < 64) {
             double q3[3];
-            if (thinning_separation<D>(a, st, tg[w], w, q3)) ewald_pre = ewald_derivative_wave(a.ewald, q3[0], q3[1], q3[2]);
+            if (thinning_separation<D>(a, st, tg[w], w, q3)) ewald_pre = ewald_derivative_wave(a.ewald, a.ewald_codes, a.ewald_ncodes, q3[0], q3[1], q3[2]);
         }
         if (tid == 0) {
             ecmc_event ev;
@@ -1374,7 +1384,7 @@ __global__ __launch_bounds__(NT) void chain_step_kernel(ChainArgs a, int op, int
         double ewald_pre = NAN;
         if (tid < 64) {
             double q3[3];
-            if (thinning_separation<D>(a, st, tg[t], t, q3)) ewald_pre = ewald_derivative_wave(a.ewald, q3[0], q3[1], q3[2]);
+            if (thinning_separation<D>(a, st, tg[t], t, q3)) ewald_pre = ewald_derivative_wave(a.ewald, a.ewald_codes, a.ewald_ncodes, q3[0], q3[1], q3[2]);
         }
         if (tid == 0) {
             ecmc_event ev;
@@ -1516,6 +1526,7 @@ struct ecmc_system {
     int dim, n, ncells, nb_stride, occ_cap, ring_words;
     std::vector<double> cmin;
     double *d_pos = nullptr, *d_charge = nullptr, *d_cmin = nullptr, *d_ewald = nullptr;
+    int32_t *d_ewald_codes = nullptr;
     int32_t *d_nearby = nullptr, *d_nearby_n = nullptr, *d_occ = nullptr, *d_occ_n = nullptr;
     int32_t *d_sur_cell = nullptr, *d_sur_id = nullptr, *d_cand = nullptr;
     double *d_cbt = nullptr, *d_corners = nullptr;
@@ -1796,6 +1807,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
         }
     }
     std::vector<double> ewald;
+    std::vector<int32_t> ewald_codes;
     ChainArgs &a = s->args;
     memset(&a, 0, sizeof(a));
     if (ewald_tagger >= 0) {
@@ -1803,6 +1815,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
         int fc = (int)tg.par[1], pc = (int)tg.par[2];
         double alpha = tg.par[0], L = desc->system_length;
         build_ewald_table(fc, alpha, L, ewald);
+        ewald_codes = ewald_term_codes(fc, pc);
         a.ewald.fc = fc;
         a.ewald.pc = pc;
         a.ewald.L = L;
@@ -1829,6 +1842,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     ECMC_ALLOC(s->d_nout, sizeof(int64_t));
     ECMC_ALLOC(s->d_step_ev, sizeof(ecmc_event));
     if (!ewald.empty()) { ECMC_ALLOC(s->d_ewald, sizeof(double) * ewald.size()); }
+    if (!ewald_codes.empty()) { ECMC_ALLOC(s->d_ewald_codes, sizeof(int32_t) * ewald_codes.size()); }
     ECMC_ALLOC(s->d_sur_cell, sizeof(int32_t) * n);
     ECMC_ALLOC(s->d_sur_id, sizeof(int32_t) * n);
     ECMC_ALLOC(s->d_cand, sizeof(int32_t) * n);
@@ -1847,6 +1861,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     ECMC_UP(s->d_st, &st, sizeof(ChainState));
     ECMC_UP(s->d_tg, desc->taggers, sizeof(ecmc_tagger_desc) * ECMC_MAX_TAGGERS);
     if (!ewald.empty()) { ECMC_UP(s->d_ewald, ewald.data(), sizeof(double) * ewald.size()); }
+    if (!ewald_codes.empty()) { ECMC_UP(s->d_ewald_codes, ewald_codes.data(), sizeof(int32_t) * ewald_codes.size()); }
 #undef ECMC_ALLOC
 #undef ECMC_UP
     if (hipStreamCreate(&s->stream) != hipSuccess || hipEventCreate(&s->ev0) != hipSuccess ||
@@ -1886,6 +1901,8 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     a.st = s->d_st;
     a.tg = s->d_tg;
     a.ewald.f = s->d_ewald;
+    a.ewald_codes = s->d_ewald_codes;
+    a.ewald_ncodes = (int32_t)ewald_codes.size();
     a.n_out = s->d_nout;
     a.max_occ = desc->max_occupants;
     a.sur_cap = n;
@@ -1987,7 +2004,7 @@ extern "C" void ecmc_system_destroy(ecmc_system *s) {
     hipFree(s->d_occ); hipFree(s->d_occ_n); hipFree(s->d_mt); hipFree(s->d_ring); hipFree(s->d_st);
     hipFree(s->d_sur_cell); hipFree(s->d_sur_id); hipFree(s->d_cand); hipFree(s->d_cbt); hipFree(s->d_corners);
     hipFree(s->d_cvw);
-    hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald); hipFree(s->d_step_ev);
+    hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald); hipFree(s->d_ewald_codes); hipFree(s->d_step_ev);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->stream) hipStreamDestroy(s->stream);

@@ -106,7 +106,8 @@ __device__ __forceinline__ float rstar2_g(float K, float V, float two_over_p, bo
 // shifted form ipt_displacement): separation s (FP64, target minus active, minimum image),
 // motion along +dir, K = |prefactor c1 c2| (> 0), rep = the factor repels, p its integer power,
 // trunc with the squared cutoff bracketed by [rc2_lo, rc2_hi], dU bracketed by [dU_lo, dU_hi],
-// 1 / speed by [isp_lo, isp_hi].  lb = +inf only where the exact displacement is +inf (no
+// 1 / speed by [isp_lo, isp_hi]; s may differ from the exact FP64 separation by abs_slack per
+// component.  lb = +inf only where the exact displacement is +inf (no
 // event); ub = +inf where an event is not certain.  Every case the brackets cannot decide gives
 // lb = 0, ub = +inf.  A positive lb bounds the FP64 displacement of ip_displacement /
 // ipt_displacement from below; where lb = 0 the FP64 value can be a rounding-level negative
@@ -114,9 +115,11 @@ __device__ __forceinline__ float rstar2_g(float K, float V, float two_over_p, bo
 template <int D>
 __device__ void ip_time_bounds(const double *s, int dir, float K, bool rep, int p, float two_over_p, bool trunc,
                                float rc2_lo, float rc2_hi, float dU_lo, float dU_hi, float isp_lo, float isp_hi,
-                               float &lb, float &ub) {
+                               float &lb, float &ub, float abs_slack = 0.f) {
     const float INF = __int_as_float(0x7f800000);
-    const float TINY = 0x1p-110f;   // covers a separation component converted below the normal range
+    // absolute slack per separation component: a component converted below the normal range, and
+    // the caller's own (a separation within abs_slack of the exact FP64 one)
+    const float TINY = fmaxf(abs_slack, 0x1p-110f);
     lb = 0.f;
     ub = INF;
     float lo2 = 0.f, hi2 = 0.f;     // the squared perpendicular components, bounded

@@ -307,8 +307,55 @@ ECMC_FN double dd_pow_int(double x, int n) {
     return acc.h;
 }
 
+// x ** n as a double-double by binary powering (each product about 2^-104 relative), n >= 1.
+ECMC_FN dd dd_pow_int_dd(dd base, int n) {
+    dd acc = {1.0, 0.0};
+    bool first = true;
+    while (n > 0) {
+        if (n & 1) {
+            acc = first ? base : dd_mul(acc, base);
+            first = false;
+        }
+        n >>= 1;
+        if (n) base = dd_mul(base, base);
+    }
+    return acc;
+}
+
+// x ** (n + 1/2) for 1 <= n <= 15 as hi + lo, relative error below 2^-98: x^n by binary powering
+// in double-double, sqrt(x) = s + (x - s^2) / (2 s) with the residual exact by fma.
+ECMC_FN double half_pow_fast(double x, int n, double &lo) {
+    const double s = sqrt(x);
+    const dd sq = quick_two_sum(s, fma(-s, s, x) / (2.0 * s));
+    const dd r = dd_mul(dd_pow_int_dd(dd{x, 0.0}, n), sq);
+    lo = r.l;
+    return r.h;
+}
+
+// x ** y for y the double nearest num / den (num = 1, 2; 2 <= den <= 32) as hi + lo, relative
+// error below 2^-92: t0 = pow(x, y) (within a few ulps), one Newton step on t^den = x^num with
+// both powers in double-double, t1 = t0 - t0 (t0^den - x^num) / (den x^num), which leaves
+// (den - 1)/2 (t0 error)^2; then x^y = t1 (1 + delta log x) with delta = y - num/den =
+// (den y - num)/den, |delta log x| < 2^-45 for |log2 x| < 300.
+ECMC_FN double rational_pow_fast(double x, double y, int num, int den, double &lo) {
+    const double t0 = pow(x, y);
+    const dd tk = dd_pow_int_dd(dd{t0, 0.0}, den);
+    const dd xn = num == 1 ? dd{x, 0.0} : two_prod(x, x);
+    const dd dlt = dd_sub(tk, xn);
+    const double corr = t0 * ((dlt.h + dlt.l) / (den * xn.h));
+    const dd t1 = two_sum(t0, -corr);
+    const double delta = fma(y, (double)den, -(double)num) / den;
+    double llo;
+    const double c = delta * log_fast(x, llo);
+    const dd r = quick_two_sum(t1.h, fma(t1.h, c, t1.l));
+    lo = r.l;
+    return r.h;
+}
+
 // Correctly rounded x ** y for the arguments the potentials use (x > 0): the cases that need
-// more than one operation.
+// more than one operation.  Fast paths with a rounding test first (integer powers, half-integer
+// powers r^(p/2) of odd p, and 2/p for the inversion of an integer power p), then the
+// double-double logarithm and exponential.
 ECMC_SLOW double cr_pow_slow(double x, double y) {
     if (!(x > 0.0) || isinf(x) || !isfinite(y) || x < 0x1p-1022 || x == 1.0) return pow(x, y);
     if (y >= 3.0 && y <= 32.0 && y == rint(y) && x > 0x1p-30 && x < 0x1p30) return dd_pow_int(x, (int)y);
@@ -316,6 +363,24 @@ ECMC_SLOW double cr_pow_slow(double x, double y) {
         double lo;
         const double hi = root_pow_fast(x, y == 1.0 / 6.0, lo);
         if (rounding_settled(hi, lo, 0x1p-92)) return hi + lo;
+    }
+    if (y >= 1.5 && y <= 15.5 && y - 0.5 == rint(y - 0.5) && x > 0x1p-30 && x < 0x1p30) {
+        double lo;
+        const double hi = half_pow_fast(x, (int)(y - 0.5), lo);
+        if (rounding_settled(hi, lo, 0x1p-92)) return hi + lo;
+    }
+    if (y > 0.0 && y < 1.0 && x > 0x1p-300 && x < 0x1p300) {
+        // y the double nearest 2/p (p odd, reduced to 2/p) or 1/q (p = 2q)
+        const double pd = rint(2.0 / y);
+        if (pd >= 2.0 && pd <= 32.0 && 2.0 / pd == y) {
+            const int pi = (int)pd;
+            const int num = (pi & 1) ? 2 : 1, den = (pi & 1) ? pi : pi / 2;
+            if (den >= 2) {
+                double lo;
+                const double hi = rational_pow_fast(x, y, num, den, lo);
+                if (rounding_settled(hi, lo, 0x1p-88)) return hi + lo;
+            }
+        }
     }
     dd l = dd_log(x);
     dd t = dd_mul_d(l, y);

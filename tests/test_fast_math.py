@@ -3,9 +3,10 @@ for the host from the same header the kernels include, against the oracle's quad
 rounding (oracle/ecmc_oracle_math.c ora_log / ora_pow in correctly rounded mode).
 
 Pins: every result equals the correctly rounded one; the fast log's own error (hi + lo
-against quad precision) stays below the 2^-64 its rounding test assumes, and the k-th root
-path's below 2^-92; the rounding test sends only a small share of calls to the slow path;
-the double-double x^3 and x^6 of the inversion round correctly too."""
+against quad precision) stays below the 2^-64 its rounding test assumes, the k-th root path's
+and the half-integer power path's below 2^-92, the Newton path for 2/p below 2^-88; the
+rounding test sends only a small share of calls to the slow path; the double-double x^3 and
+x^6 of the inversion round correctly too."""
 import os
 import shutil
 import subprocess
@@ -29,8 +30,8 @@ static double unif() {                       /* xorshift64, 53-bit uniform in [0
     return (double)(st >> 11) * 0x1p-53;
 }
 int main(int argc, char **argv) {
-    long n = atol(argv[1]), bad_log = 0, bad_pow = 0, slow_log = 0, slow_pow = 0;
-    __float128 worst_log = 0, worst_pow = 0;
+    long n = atol(argv[1]), bad_log = 0, bad_pow = 0, slow_log = 0, slow_pow = 0, slow_half = 0, slow_rat = 0;
+    __float128 worst_log = 0, worst_pow = 0, worst_half = 0, worst_rat = 0;
     for (long i = 0; i < n; i++) {
         double x;
         switch (i % 5) {
@@ -57,9 +58,26 @@ int main(int argc, char **argv) {
         double r2 = exp(unif() * 6.0 - 3.0);
         bad_pow += pow36(r2, true) != ora_pow(r2, 6.0, 1);
         bad_pow += pow36(r2, false) != ora_pow(r2, 3.0, 1);
+        /* half-integer powers (r^2)^(p/2) of odd p, and 2/p of other integer powers */
+        static const double hy[4] = {4.5, 3.5, 1.5, 15.5};
+        double hx = exp(unif() * 40.0 - 20.0), hyv = hy[i % 4];
+        double hlo, hhi = half_pow_fast(hx, (int)(hyv - 0.5), hlo);
+        __float128 th = powq((__float128)hx, (__float128)hyv), eh = fabsq(((__float128)hhi + hlo - th) / th);
+        if (eh > worst_half) worst_half = eh;
+        slow_half += !rounding_settled(hhi, hlo, 0x1p-92);
+        bad_pow += cr_pow(hx, hyv) != ora_pow(hx, hyv, 1);
+        static const int pp[6] = {9, 7, 5, 10, 16, 31};
+        const int pv = pp[i % 6];
+        const double ry = 2.0 / pv, rx = exp(unif() * 60.0 - 20.0);
+        double rlo, rhi = rational_pow_fast(rx, ry, (pv & 1) ? 2 : 1, (pv & 1) ? pv : pv / 2, rlo);
+        __float128 tr = powq((__float128)rx, (__float128)ry), er = fabsq(((__float128)rhi + rlo - tr) / tr);
+        if (er > worst_rat) worst_rat = er;
+        slow_rat += !rounding_settled(rhi, rlo, 0x1p-88);
+        bad_pow += cr_pow(rx, ry) != ora_pow(rx, ry, 1);
     }
-    printf("%ld %ld %ld %ld %.3f %.3f\n", bad_log, bad_pow, slow_log, slow_pow, (double)log2q(worst_log),
-           (double)log2q(worst_pow));
+    printf("%ld %ld %ld %ld %.3f %.3f %ld %ld %.3f %.3f\n", bad_log, bad_pow, slow_log, slow_pow,
+           (double)log2q(worst_log), (double)log2q(worst_pow), slow_half, slow_rat, (double)log2q(worst_half),
+           (double)log2q(worst_rat));
     return 0;
 }
 """
@@ -80,6 +98,9 @@ def test_fast_log_and_root_pow_round_correctly(tmp_path):
     out = subprocess.run([str(exe), str(n)], capture_output=True, text=True, check=True).stdout.split()
     bad_log, bad_pow, slow_log, slow_pow = (int(v) for v in out[:4])
     worst_log, worst_pow = float(out[4]), float(out[5])
+    slow_half, slow_rat = int(out[6]), int(out[7])
+    worst_half, worst_rat = float(out[8]), float(out[9])
     assert bad_log == 0 and bad_pow == 0
     assert worst_log < -64.0 and worst_pow < -92.0
-    assert slow_log < 0.005 * n and slow_pow == 0
+    assert worst_half < -92.0 and worst_rat < -88.0        # the accuracies their rounding tests assume
+    assert slow_log < 0.005 * n and slow_pow == 0 and slow_half == 0 and slow_rat == 0

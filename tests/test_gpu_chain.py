@@ -73,8 +73,9 @@ def _oracle_cr(oracle, fn, *arrays):
 
 def test_device_log_pow_correctly_rounded_at_scale(device, oracle):
     # the fast paths (double-length log with a rounding test; k-th root plus one Newton step
-    # for 2/p = 1/3, 1/6) against quad precision over millions of the arguments the inversion
-    # meets: 1 - u for the event draws, and (k / (U + dU)) ** (2/p)
+    # for 2/p = 1/3, 1/6; the Newton step on t^den = x^num for 2/p = 2/9, 2/7, 1/5; half-integer
+    # powers r2^(p/2) for odd p) against quad precision over millions of the arguments the
+    # inversion meets: 1 - u for the event draws, (k / (U + dU)) ** (2/p) and r2 ** (p/2)
     import torch
     rng = np.random.default_rng(11)
     x = np.concatenate([1.0 - rng.random(1_000_000), np.exp(rng.uniform(-40, 40, 500_000)),
@@ -87,8 +88,10 @@ def test_device_log_pow_correctly_rounded_at_scale(device, oracle):
     want = _oracle_cr(oracle, "ora_log_n", x)
     bad = np.flatnonzero(got != want)
     assert bad.size == 0, "log differs at x = %r" % x[bad[:5]]
-    base = np.exp(rng.uniform(-8.0, 22.0, 2_000_000))
-    ys = np.where(rng.random(len(base)) < 0.5, 2.0 / 12.0, 2.0 / 6.0)
+    base = np.exp(rng.uniform(-8.0, 22.0, 3_000_000))
+    ys = rng.choice([2.0 / 12.0, 2.0 / 6.0, 2.0 / 9.0, 2.0 / 7.0, 2.0 / 10.0, 4.5, 3.5, 0.5 + 7], len(base))
+    half = ys > 1.0
+    base[half] = np.exp(rng.uniform(-10.0, 10.0, half.sum()))     # r2 ** (p/2) within 2^+-30
     bt, yt = torch.tensor(base, device="cuda"), torch.tensor(ys, device="cuda")
     out = torch.empty_like(bt)
     device.check(device.lib().ecmc_math_pow(len(base), device.dptr(bt), device.dptr(yt), device.dptr(out), None))
