@@ -146,6 +146,8 @@ def plain_chain_rate(legs, pressure_seconds, equilibration_legs):
     out = dict(n=n, rho=0.8, start="16^3 grid (spacing 1.07722, jitter 0.01) equilibrated by %d legs of the batched "
                                    "kernel (%.1f s)" % (done_eq, t_eq),
                events=int(done), kernel_ms=ms, wall_s=wall, events_per_sec=done / (ms * 1e-3))
+    lds_b, lds_p, lds_r = chain.launch_lds()
+    out["launch_lds"] = dict(bytes=lds_b, positions=lds_p, ring=lds_r)
     out["roofline"] = plain_roofline("ecmc::chain_kernel<3>", ncand, 3, 27, done, ms, charged=False)
     out["roofline"]["traffic"] = profiled_traffic("ecmc::chain_kernel<3>", legs,
                                                   "chain_kernel_plain_lj_4096_counters.txt", "plain")
@@ -268,6 +270,7 @@ def charged_spheres_rate(replicas, legs, far="cell_bounding"):
     wall = time.perf_counter() - t0
     k1, ms1 = run_together(chains[:1], legs)
     est = np.array([c.estimators() for c in chains])
+    lds_b, lds_p, lds_r = chains[0].launch_lds()
     for c in chains:
         c.close()
     # candidates per leg from a logged run of one more replica of the same system
@@ -284,7 +287,7 @@ def charged_spheres_rate(replicas, legs, far="cell_bounding"):
                      scheduler="heap", max_occupants=1)
     cpu1 = plain_cpu_single_core(desc, sysd["positions"], seconds=4.0, charges=sysd["charges"])
     return dict(replicas=replicas, n=512, system_length=8.0, far_cells=far, legs_per_replica=legs,
-                roofline=roof, cpu_single_core=cpu1,
+                roofline=roof, cpu_single_core=cpu1, launch_lds=dict(bytes=lds_b, positions=lds_p, ring=lds_r),
                 events_per_sec=float(k.sum()) / wall, kernel_ms=ms,
                 single_replica_events_per_sec=float(k1[0]) / (ms1 * 1e-3),
                 lifts_per_leg=float(est[:, 4].sum() / max(k.sum(), 1)),

@@ -181,6 +181,10 @@ int ecmc_system_estimators(const ecmc_system *sys, double *est8);
 int ecmc_system_rng_words(const ecmc_system *sys, int64_t *words);
 /* Kernel time of the last ecmc_system_run in milliseconds (HIP events on its stream). */
 int ecmc_system_last_kernel_ms(const ecmc_system *sys, double *ms);
+/* The dynamic LDS of the system's leg-loop launches: bytes, and whether its positions and its
+ * random-word ring live there for a launch (1) or stay in global memory (0).  No reference
+ * counterpart: the device's memory placement, for diagnostics. */
+int ecmc_system_launch_lds(const ecmc_system *sys, int64_t *bytes, int32_t *positions, int32_t *ring);
 /* Independent systems advanced together: one launch, one workgroup per system, each system
  * committing up to max_events legs exactly as ecmc_system_run would (no event logs).  All
  * systems have one dimension and appear once; n_committed (count entries, may be NULL) gets

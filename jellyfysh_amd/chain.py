@@ -187,6 +187,12 @@ class DeviceChain:
         L.check(L.lib().ecmc_system_rng_words(self._h, ctypes.byref(w)))
         return w.value
 
+    def launch_lds(self):
+        """(dynamic LDS bytes of a launch, positions resident in LDS, ring resident in LDS)."""
+        b, p, r = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        L.check(L.lib().ecmc_system_launch_lds(self._h, ctypes.byref(b), ctypes.byref(p), ctypes.byref(r)))
+        return b.value, bool(p.value), bool(r.value)
+
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         L.check(L.lib().ecmc_system_last_kernel_ms(self._h, ctypes.byref(ms)))

@@ -2266,6 +2266,14 @@ extern "C" int ecmc_system_rng_words(const ecmc_system *s, int64_t *words) {
     return ECMC_OK;
 }
 
+extern "C" int ecmc_system_launch_lds(const ecmc_system *s, int64_t *bytes, int32_t *positions, int32_t *ring) {
+    if (!s) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
+    if (bytes) *bytes = (int64_t)s->lds_bytes;
+    if (positions) *positions = s->args.lds_pos;
+    if (ring) *ring = s->args.lds_ring;
+    return ECMC_OK;
+}
+
 extern "C" int ecmc_system_last_kernel_ms(const ecmc_system *s, double *ms) {
     if (!s || !ms) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
     *ms = s->last_ms;

@@ -42,6 +42,13 @@ namespace ecmc {
 
 // ---- Python float semantics ------------------------------------------------------------
 ECMC_HD double py_mod(double x, double y) {
+    if (y > 0.0 && fabs(x) < 2.0 * y) {
+        // within one period either way fmod's exact result is x, x - y or x + y (Sterbenz), and a
+        // negative remainder takes the divisor's sign by one rounded addition, as below
+        double m = x >= y ? x - y : (x <= -y ? x + y : x);
+        if (m < 0.0) m += y;
+        return m == 0.0 ? 0.0 : m;
+    }
     double m = fmod(x, y);
     if (m != 0.0) {
         if ((y < 0) != (m < 0)) m += y;
@@ -52,6 +59,14 @@ ECMC_HD double py_mod(double x, double y) {
 }
 
 ECMC_HD void py_divmod(double vx, double wx, double &q, double &r) {
+    if (wx == 1.0 && vx >= 0.0 && vx < 0x1p52) {
+        // division by one of a non-negative double: the floor and the exact remainder (+0 for an
+        // integer, a -0 quotient for -0 as copysign gives below)
+        const double f = floor(vx);
+        q = f;
+        r = vx - f;
+        return;
+    }
     double mod = fmod(vx, wx);
     double div = (vx - mod) / wx;
     if (mod != 0.0) {

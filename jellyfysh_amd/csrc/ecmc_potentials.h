@@ -80,9 +80,34 @@ ECMC_FN double hd_displacement(int dir, double speed, const double *s, double rm
 }
 
 // InversePowerPotential.potential (inverse_power_potential.py:127-143)
+// Correctly rounded r2 ** (power / 2) and x ** (2 / power) of the inverse-power inversions:
+// the double-double forms inline for the powers 6 and 12 (pow36, root36), else cr_pow.
+// Odd powers take the half-integer and 2/p fast paths of ecmc_math.h inline, with their
+// rounding tests.
+ECMC_FN double pow_half_power(double r2, double power) {
+    if (power == 12.0 || power == 6.0) return pow36(r2, power == 12.0);
+    const double h = power / 2.0;
+    if (h >= 1.5 && h <= 15.5 && h - 0.5 == rint(h - 0.5) && r2 > 0x1p-30 && r2 < 0x1p30) {
+        double lo;
+        const double hi = half_pow_fast(r2, (int)(h - 0.5), lo);
+        if (rounding_settled(hi, lo, 0x1p-92)) return hi + lo;
+    }
+    return cr_pow(r2, h);
+}
+ECMC_FN double root_two_over_power(double x, double power) {
+    if (power == 12.0 || power == 6.0) return root36(x, power == 12.0);
+    const double y = 2.0 / power;
+    if (power >= 3.0 && power <= 31.0 && power == rint(power) && ((int)power & 1) && x > 0x1p-300 && x < 0x1p300) {
+        double lo;
+        const double hi = rational_pow_fast(x, y, 2, (int)power, lo);
+        if (rounding_settled(hi, lo, 0x1p-88)) return hi + lo;
+    }
+    return cr_pow(x, y);
+}
+
 template <int D>
 ECMC_FN double ip_potential(const double *s, double cp, double power, double prefactor) {
-    return cp * prefactor / cr_pow(norm_sq<D>(s), power / 2.0);
+    return cp * prefactor / pow_half_power(norm_sq<D>(s), power);
 }
 
 // InversePowerPotential.standard_velocity_derivative (:71-94) times the speed
@@ -105,7 +130,6 @@ ECMC_FN double ip_displacement(int dir, double speed, const double *sep, double 
     for (int i = 0; i < D; i++) s[i] = sep[i];
     double cp = c1 * c2;
     double pp = prefactor * cp;
-    double two_over_power = 2.0 / power;
     if (pp > 0) {
         if (s[dir] <= 0.0) return INFINITY;
         double smax[D];
@@ -114,7 +138,7 @@ ECMC_FN double ip_displacement(int dir, double speed, const double *sep, double 
         double maxp = ip_potential<D>(smax, cp, power, prefactor);
         double cur = ip_potential<D>(s, cp, power, prefactor);
         if (dU < maxp - cur) {
-            double nsq = cr_pow(cp * prefactor / (cur + dU), two_over_power);
+            double nsq = root_two_over_power(cp * prefactor / (cur + dU), power);
             double a = nsq - others_sq<D>(s, dir);
             if (a < 0.0) return NAN;
             return (s[dir] - sqrt(a)) / speed;
@@ -128,7 +152,7 @@ ECMC_FN double ip_displacement(int dir, double speed, const double *sep, double 
     }
     double cur = ip_potential<D>(s, cp, power, prefactor);
     if (cur + dU >= 0.0) return INFINITY;
-    double nsq = cr_pow(cp * prefactor / (cur + dU), two_over_power);
+    double nsq = root_two_over_power(cp * prefactor / (cur + dU), power);
     double a = nsq - others_sq<D>(s, dir);
     if (a < 0.0) return NAN;
     disp += s[dir] + sqrt(a);
@@ -158,8 +182,7 @@ ECMC_FN double ipt_displacement(int dir, double speed, const double *sep, double
     double rc2 = cutoff * cutoff;
     double perp2 = others_sq<D>(s, dir);
     if (perp2 >= rc2) return INFINITY;
-    double two_over_power = 2.0 / power;
-    double urc = cp * prefactor / cr_pow(rc2, power / 2.0);
+    double urc = cp * prefactor / pow_half_power(rc2, power);
     if (pp > 0) {
         if (s[dir] <= 0.0) return INFINITY;
         double s2 = norm_sq<D>(s);
@@ -169,7 +192,7 @@ ECMC_FN double ipt_displacement(int dir, double speed, const double *sep, double
         for (int i = 0; i < D; i++) smax[i] = i == dir ? 0.0 : s[i];
         double maxp = ip_potential<D>(smax, cp, power, prefactor);
         if (dU < maxp - cur) {
-            double nsq = cr_pow(cp * prefactor / (cur + dU), two_over_power);
+            double nsq = root_two_over_power(cp * prefactor / (cur + dU), power);
             double a = nsq - perp2;
             if (a < 0.0) return NAN;
             return (s[dir] - sqrt(a)) / speed;
@@ -185,7 +208,7 @@ ECMC_FN double ipt_displacement(int dir, double speed, const double *sep, double
     if (s2 >= rc2) return INFINITY;
     double cur = ip_potential<D>(s, cp, power, prefactor);
     if (cur + dU >= urc) return INFINITY;
-    double nsq = cr_pow(cp * prefactor / (cur + dU), two_over_power);
+    double nsq = root_two_over_power(cp * prefactor / (cur + dU), power);
     double a = nsq - perp2;
     if (a < 0.0) return NAN;
     disp += s[dir] + sqrt(a);

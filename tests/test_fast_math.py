@@ -104,3 +104,57 @@ def test_fast_log_and_root_pow_round_correctly(tmp_path):
     assert worst_log < -64.0 and worst_pow < -92.0
     assert worst_half < -92.0 and worst_rat < -88.0        # the accuracies their rounding tests assume
     assert slow_log < 0.005 * n and slow_pow == 0 and slow_half == 0 and slow_rat == 0
+
+
+MODPROG = r"""
+#define ECMC_HOST_BUILD 1
+#include <stdio.h>
+#include "ecmc_math.h"
+using namespace ecmc;
+int main() {
+    double x, y;
+    while (scanf("%la %la", &x, &y) == 2) {
+        double q, r;
+        py_divmod(x, y, q, r);
+        printf("%a %a %a\n", py_mod(x, y), q, r);
+    }
+    return 0;
+}
+"""
+
+
+def test_python_float_mod_and_divmod_of_the_kernels(tmp_path):
+    """py_mod and py_divmod of ecmc_math.h (the kernels' separation wrap and Time's divmod by one,
+    with their one-period and non-negative fast paths) equal CPython's % and divmod, signed zeros
+    included, around every branch point: 0, +-y, +-2y, and far outside."""
+    import math
+    import random
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("needs g++")
+    src = tmp_path / "mod.cpp"
+    src.write_text(MODPROG)
+    exe = tmp_path / "mod"
+    subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-I", os.path.join(ROOT, "jellyfysh_amd", "csrc"),
+                    str(src), "-o", str(exe)], check=True, capture_output=True)
+    rnd = random.Random(5)
+    pairs = []
+    for y in (1.0, 0.5, 17.2355, 8.0, 2.0 ** -3, 3.0, 1e-300):
+        edges = [0.0, -0.0, y, -y, 2 * y, -2 * y, 0.5 * y, -0.5 * y]
+        for e in edges:
+            for k in range(-3, 4):
+                v = e
+                for _ in range(abs(k)):
+                    v = math.nextafter(v, math.inf if k > 0 else -math.inf)
+                pairs.append((v, y))
+        for _ in range(3000):
+            pairs.append((rnd.uniform(-2.5 * y, 2.5 * y), y))
+        pairs += [(rnd.uniform(-1e6, 1e6) * y, y) for _ in range(200)]
+        pairs += [(1e300, y), (-1e300, y), (5e-324, y), (-5e-324, y)]
+    inp = "".join("%s %s\n" % (x.hex(), y.hex()) for x, y in pairs)
+    out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout.split("\n")
+    for (x, y), line in zip(pairs, out):
+        m, q, r = (float.fromhex(v) for v in line.split())
+        wq, wr = divmod(x, y)
+        for got, want in ((m, x % y), (q, wq), (r, wr)):
+            assert got == want and math.copysign(1, got) == math.copysign(1, want), (x, y, got, want)
