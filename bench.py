@@ -298,6 +298,45 @@ def charged_spheres_rate(replicas, legs, far="cell_bounding"):
                      "repulsion inverted exactly over all pairs" % far.replace("_", " "))
 
 
+def flexible_dipoles_rate(replicas, legs):
+    """Flexible dipoles (jellyfysh_amd/configs/flexible_dipoles.ini: 500 dipoles of charges +-0.5,
+    bond p = 2, r0 = 1, k = 400 within each dipole, merged-image Coulomb between dipoles with
+    thinning, r^-12 repulsion between all charges of different dipoles, L = 21.544):
+    `replicas` independent systems (the INI's random start with seeds 0 ..) advanced together
+    by the plain chain kernel, against one alone."""
+    from jellyfysh_amd.chain import DeviceChain, run_together
+    from jellyfysh_amd.config import RunConfig
+    cfg = RunConfig.from_file(os.path.join(ROOT, "jellyfysh_amd", "configs", "flexible_dipoles.ini"))
+    t0 = time.perf_counter()
+    chains, starts = [], []
+    for seed in range(replicas):
+        p, q, w = cfg.initial_state(seed)
+        chains.append(DeviceChain(cfg.desc(len(p), seed, rng_skip=w), p, q))
+        starts.append((p, q, w))
+    create_s = time.perf_counter() - t0
+    run_together(chains, 100)
+    t0 = time.perf_counter()
+    k, ms = run_together(chains, legs)
+    wall = time.perf_counter() - t0
+    k1, ms1 = run_together(chains[:1], legs)
+    lds_b, lds_p, lds_r = chains[0].launch_lds()
+    probe = chains[1].run(300)
+    ncand = float(probe["ncand"][1:].mean()) if len(probe) > 1 else 0.0
+    desc = chains[0].desc
+    for c in chains:
+        c.close()
+    roof = plain_roofline("ecmc::chain_many_kernel<3>", ncand, 3, 0, int(k.sum()), ms, charged=True)
+    p, q, w = starts[0]
+    cpu1 = plain_cpu_single_core(desc, p, seconds=4.0, charges=q)
+    return dict(replicas=replicas, n=len(starts[0][0]), legs_per_replica=legs, roofline=roof, cpu_single_core=cpu1,
+                launch_lds=dict(bytes=lds_b, positions=lds_p, ring=lds_r), events_per_sec=float(k.sum()) / wall,
+                kernel_ms=ms, single_replica_events_per_sec=float(k1[0]) / (ms1 * 1e-3), create_s=create_s,
+                note="plain chain kernel, one workgroup per replica in one launch; leaf units of composite "
+                     "objects with factor type maps; Coulomb through the inverse-power Coulomb bound with "
+                     "thinning (FP64 Ewald derivative), bond and repulsion inverted exactly (FP32 brackets "
+                     "first for the repulsion)")
+
+
 def cpu_baseline(n, lengths, pos, seconds, threads):
     """The oracle's batched restatement (FP64, glibc math, every candidate inverted) on host
     threads, one replica per thread at a time, for about `seconds` of wall time.  The sample
@@ -359,7 +398,9 @@ def main():
         for far in ("cell_bounding", "cell_veto"):
             log("charged soft spheres, far cells through %s ..." % far)
             out[far] = charged_spheres_rate(args.charged_replicas, args.charged_legs, far=far)
-        print(json.dumps({"charged_soft_spheres": out}), flush=True)
+        log("flexible dipoles ...")
+        fd = flexible_dipoles_rate(64, args.charged_legs)
+        print(json.dumps({"charged_soft_spheres": out, "flexible_dipoles": fd}), flush=True)
         return
     if args.only_plain:
         from jellyfysh_amd import _lib
@@ -458,6 +499,9 @@ def main():
         log("charged soft spheres, cell veto ...")
         out["charged_soft_spheres_cell_veto"] = charged_spheres_rate(args.charged_replicas, args.charged_legs,
                                                                      far="cell_veto")
+    if not args.no_charged:
+        log("flexible dipoles ...")
+        out["flexible_dipoles"] = flexible_dipoles_rate(64, args.charged_legs)
     if not args.no_plain:
         log("plain chain ...")
         pl = plain_chain_rate(args.plain_legs, args.plain_seconds, args.plain_equilibration)
