@@ -383,6 +383,9 @@ def cpu_baseline(n, lengths, pos, seconds, threads):
                 sample="%d replicas of the batched workload (256 <= N <= 512, density closest to 0.8, N %d-%d) "
                        "x %d legs each, oracle restatement in FP64 with glibc math, %d host threads, %.1f s"
                        % (len(order), int(n[order].min()), int(n[order].max()), 2 * half, threads, wall),
+                note="the batched restatement (Philox draws, no scheduler, no Time bookkeeping, every candidate "
+                     "inverted): cheaper per leg than the reference's leg loop, so the GPU/CPU ratio is "
+                     "conservative; the plain lines carry the reference-algorithm oracle timed on one core",
                 pressure_rho_0_80=pz)
 
 

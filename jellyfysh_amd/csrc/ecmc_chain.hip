@@ -42,6 +42,10 @@
 namespace ecmc {
 
 constexpr int NT = 256;
+
+// The dynamic LDS of a leg-loop launch (positions, then the ring of tempered words), when the
+// launch's arguments place them there (ChainArgs::lds_pos, lds_ring).
+extern __shared__ __align__(16) unsigned char chain_dyn_lds[];
 constexpr uint32_t MT_UPPER = 0x80000000u, MT_LOWER = 0x7fffffffu, MT_MATRIX = 0x9908b0dfu;
 
 struct Slot {
@@ -136,7 +140,7 @@ __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
 // The three phases follow the data dependences of the sequential loop: words 0..226 read
 // only old words; 227..453 read new words 0..226; 454..623 read new words 227..396.
 // Word j of the block goes to ring[(produced + j) & mask] (mask: ring size - 1, all ones for no wrap).
-__device__ void mt_generate_block(uint32_t *mt, uint32_t *ring, uint64_t mask, uint64_t produced) {
+__device__ __forceinline__ void mt_generate_block(uint32_t *mt, uint32_t *ring, uint64_t mask, uint64_t produced) {
     const int t = threadIdx.x;
     uint32_t v = 0;
     if (t < 227) {
@@ -168,16 +172,19 @@ __device__ void mt_generate_block(uint32_t *mt, uint32_t *ring, uint64_t mask, u
 }
 
 // Collective: make `need` unread words available in the ring.
-__device__ void ring_ensure(ChainState &st, uint32_t *mt, const ChainArgs &a, uint64_t need) {
+__device__ __forceinline__ void ring_ensure(ChainState &st, uint32_t *mt, const ChainArgs &a, uint64_t need) {
     while (st.produced < st.cursor + need) {
-        mt_generate_block(mt, a.ring, (uint64_t)(a.ring_words - 1), st.produced);
+        mt_generate_block(mt, a.lds_ring ? reinterpret_cast<uint32_t *>(chain_dyn_lds + a.lds_ring_off) : a.ring,
+                          (uint64_t)(a.ring_words - 1), st.produced);
         if (threadIdx.x == 0) st.produced += 624;
         __syncthreads();
     }
 }
 
 __device__ __forceinline__ uint32_t ring_word(const ChainArgs &a, uint64_t idx) {
-    return a.ring[idx & (uint64_t)(a.ring_words - 1)];   // ring_words is a power of two
+    const uint64_t i = idx & (uint64_t)(a.ring_words - 1);   // ring_words is a power of two
+    if (a.lds_ring) return reinterpret_cast<const uint32_t *>(chain_dyn_lds + a.lds_ring_off)[i];
+    return a.ring[i];
 }
 
 // random() from two words (genrand_res53)
@@ -204,13 +211,25 @@ __device__ __forceinline__ int pos_to_cell(const ChainArgs &a, const double *p) 
 
 template <int D>
 __device__ __forceinline__ void load_pos(const ChainArgs &a, int id, double *p) {
+    if (a.lds_pos) {
+        const double *lp = reinterpret_cast<const double *>(chain_dyn_lds);
 #pragma unroll
-    for (int d = 0; d < D; d++) p[d] = a.pos[(size_t)d * a.n + id];
+        for (int d = 0; d < D; d++) p[d] = lp[d * a.n + id];
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) p[d] = a.pos[(size_t)d * a.n + id];
+    }
 }
 template <int D>
 __device__ __forceinline__ void store_pos(const ChainArgs &a, int id, const double *p) {
+    if (a.lds_pos) {
+        double *lp = reinterpret_cast<double *>(chain_dyn_lds);
 #pragma unroll
-    for (int d = 0; d < D; d++) a.pos[(size_t)d * a.n + id] = p[d];
+        for (int d = 0; d < D; d++) lp[d * a.n + id] = p[d];
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; d++) a.pos[(size_t)d * a.n + id] = p[d];
+    }
 }
 
 template <int D>
@@ -239,7 +258,7 @@ __device__ __forceinline__ bool pot_draws(const ecmc_tagger_desc &tg) {
 // target cell's offset from the active cell along dir, upper or lower by the sign of the
 // charge correction factor (cell_bounding_potential.py:161-201).
 template <int D>
-__device__ double cell_rate(const ChainArgs &a, const ecmc_tagger_desc &tg, int acell, int target, int dir, double ca,
+__device__ __forceinline__ double cell_rate(const ChainArgs &a, const ecmc_tagger_desc &tg, int acell, int target, int dir, double ca,
                             double ct) {
     double tp[D];
     load_pos<D>(a, target, tp);
@@ -270,7 +289,7 @@ __device__ __forceinline__ bool is_nearby(const ChainArgs &a, int dim, int c, in
 
 // Candidate time displacement of one pair (send_event_time of the pair event handlers).
 template <int D>
-__device__ double pair_dt(const ChainArgs &a, const ecmc_tagger_desc &tg, int dir, double speed, const double *s,
+__device__ __forceinline__ double pair_dt(const ChainArgs &a, const ecmc_tagger_desc &tg, int dir, double speed, const double *s,
                           double c1, double c2, double dU) {
     if (tg.event_handler == ECMC_EH_TWO_LEAF) {
         switch (tg.potential) {
@@ -328,7 +347,7 @@ __device__ __forceinline__ bool best_lt(const Best &x, const Best &y) {
 }
 
 // Collective workgroup min-reduce of (time, index) and sum of the hash terms.
-__device__ void block_reduce(Best &b, uint32_t &h, Best *sb, uint32_t *sh) {
+__device__ __forceinline__ void block_reduce(Best &b, uint32_t &h, Best *sb, uint32_t *sh) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -354,7 +373,7 @@ __device__ void block_reduce(Best &b, uint32_t &h, Best *sb, uint32_t *sh) {
 }
 
 // Collective workgroup minimum of a non-negative float (+inf allowed); sh holds NT / 64 words.
-__device__ float block_min_float(float v, uint32_t *sh) {
+__device__ __forceinline__ float block_min_float(float v, uint32_t *sh) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = __float_as_uint(v);
@@ -375,7 +394,7 @@ __device__ __forceinline__ bool fp32_filter_applies(const ecmc_tagger_desc &tg) 
 }
 
 template <int D>
-__device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &tg, int tix,
+__device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &tg, int tix,
                             int *nb_cell, int *nb_pref, Best *sb, uint32_t *sh, int *scan) {
     const int tid = threadIdx.x;
     if (tg.kind == ECMC_TAG_PAIR_CELLS) {
@@ -627,7 +646,7 @@ __device__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, co
 // Collective: moves the entries [from, to) of the parallel arrays x and y (y may be NULL) by
 // one slot up (by > 0) or down, in chunks of one entry per thread, every chunk read before it
 // is written and the chunks ordered so that no entry is overwritten before it was read.
-__device__ void shift_entries(int32_t *x, int32_t *y, int from, int to, int by) {
+__device__ __forceinline__ void shift_entries(int32_t *x, int32_t *y, int from, int to, int by) {
     const int tid = threadIdx.x;
     if (by > 0) {
         for (int top = to; top > from; top -= NT) {
@@ -654,7 +673,7 @@ __device__ void shift_entries(int32_t *x, int32_t *y, int from, int to, int by) 
 
 // Collective: the largest (want_max) or smallest index i < count with x[i] == vx and, when y is
 // given, y[i] == vy; -1 when there is none.  red holds NT / 64 ints of workgroup scratch.
-__device__ int find_entry(const int32_t *x, const int32_t *y, int count, int32_t vx, int32_t vy, bool want_max,
+__device__ __forceinline__ int find_entry(const int32_t *x, const int32_t *y, int count, int32_t vx, int32_t vy, bool want_max,
                           int *red) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int best = want_max ? -1 : 0x7fffffff;
@@ -680,7 +699,7 @@ __device__ int find_entry(const int32_t *x, const int32_t *y, int count, int32_t
 // the occupants: that branch of the reference needs an empty surplus list, which it deletes
 // right away).  The list searches and shifts run over the workgroup.
 template <int D>
-__device__ void occupancy_update(const ChainArgs &a, ChainState &st, int *red) {
+__device__ __forceinline__ void occupancy_update(const ChainArgs &a, ChainState &st, int *red) {
     const int tid = threadIdx.x;
     const int id = st.active, old_id = st.occ_id, old_cell = st.occ_cell;
     const int n_old = (id != old_id && old_id >= 0) ? a.occ_n[old_cell] : 0;
@@ -754,7 +773,7 @@ __device__ void occupancy_update(const ChainArgs &a, ChainState &st, int *red) {
 
 // Creates of one non-pair tagger (thread 0 only, ring words made available by the caller).
 template <int D>
-__device__ void create_single(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
+__device__ __forceinline__ void create_single(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
     Slot &sl = st.slot[tix];
     switch (tg.kind) {
     case ECMC_TAG_CELL_BOUNDARY: {
@@ -808,7 +827,7 @@ __device__ void create_single(const ChainArgs &a, ChainState &st, const ecmc_tag
 // (single_independent_active_periodic_direction_end_of_chain_event_handler.py:176-237): with
 // composite objects the root's randint, then the leaf's.  Returns false when the ring ran out of
 // words before the draws completed; the root already drawn is kept in the slot (cv_index).
-__device__ bool create_end_of_chain(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
+__device__ __forceinline__ bool create_end_of_chain(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
     Slot &sl = st.slot[tix];
     if (!sl.valid || sl.target >= 0x40000000) {
         if (!sl.valid || sl.target != 0x40000000) {
@@ -852,7 +871,7 @@ __device__ bool create_end_of_chain(const ChainArgs &a, ChainState &st, const ec
 // Thread 0 only.  Returns false when the ring ran out of words; the drawn entry is kept in the
 // slot, so the call resumes after a refill.
 template <int D>
-__device__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
+__device__ __forceinline__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
     Slot &sl = st.slot[tix];
     const int act = st.active, dir = st.adir;
     double cf = (tg.use_charge ? a.charge[act] : 1.0) * 1.0 / tg.bpar[2];
@@ -918,7 +937,7 @@ __device__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_
 // butterfly over the lanes adds the 64 partial sums (commutative pairings, so every lane ends
 // with the same sum).  The order of the additions differs from the sequential loop's, so the
 // result agrees with it to rounding.  More than 256 terms fall back to the sequential function.
-__device__ double ewald_derivative_wave(const Ewald &e, const int32_t *codes, int ncodes, double sx, double sy,
+__device__ __forceinline__ double ewald_derivative_wave(const Ewald &e, const int32_t *codes, int ncodes, double sx, double sy,
                                         double sz) {
     if (ncodes > 256 || !codes) return ewald_derivative(e, sx, sy, sz);
     const int lane = threadIdx.x & 63;
@@ -982,7 +1001,7 @@ static std::vector<int32_t> ewald_term_codes(int fc, int pc) {
 // The separation (permuted: x along the motion) at which the commit of tagger w's stored
 // event evaluates the merged-image Coulomb derivative, or false when it evaluates none.
 template <int D>
-__device__ bool thinning_separation(const ChainArgs &a, const ChainState &st, const ecmc_tagger_desc &T, int w,
+__device__ __forceinline__ bool thinning_separation(const ChainArgs &a, const ChainState &st, const ecmc_tagger_desc &T, int w,
                                     double *q3) {
     const Slot &sl = st.slot[w];
     double s[D];
@@ -1017,7 +1036,7 @@ struct LegShared {
 // sets over the workgroup, the cell veto and the end of chain with their bounded ring refills,
 // the others on thread 0 (collective; tag_activator.py:228-289 for one tag of a create list).
 template <int D>
-__device__ void create_tagger(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &c, int t,
+__device__ __forceinline__ void create_tagger(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &c, int t,
                               LegShared &ls) {
     const int tid = threadIdx.x;
     if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL || c.kind == ECMC_TAG_SURPLUS ||
@@ -1069,7 +1088,7 @@ __device__ __forceinline__ bool commit_thins(const ecmc_tagger_desc &T) {
 // two ring words available when the event thins): send_out_state of its event handler and
 // insert_into_global_state (mediator/single_process_mediator.py:130-150).
 template <int D>
-__device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &T, int w, ecmc_event &ev,
+__device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &T, int w, ecmc_event &ev,
                              double ewald_pre) {
     Slot &sl = st.slot[w];
     ev.tagger = w;
@@ -1236,25 +1255,22 @@ __device__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagg
 
 // One workgroup runs one system's legs (the whole leg loop of the reference's mediator).
 template <int D>
-__device__ __forceinline__ void chain_body(const ChainArgs &ga) {
+__device__ __forceinline__ void chain_body(const ChainArgs &a) {
     __shared__ ChainState st;
     __shared__ ecmc_tagger_desc tg[ECMC_MAX_TAGGERS];
     __shared__ uint32_t mt[624];
     __shared__ LegShared ls;
-    extern __shared__ __align__(16) unsigned char dyn_lds[];
     int &flag = ls.flag;
     const int tid = threadIdx.x;
-    // the positions and the MT ring move into LDS for the launch when the host sized it so
-    ChainArgs a = ga;
-    if (ga.lds_pos) {
-        double *lp = (double *)dyn_lds;
-        for (int i = tid; i < D * ga.n; i += NT) lp[i] = ga.pos[i];
-        a.pos = lp;
+    // the positions and the MT ring move into LDS for the launch when the host sized it so (the
+    // accessors load_pos, store_pos, ring_word and ring_ensure follow the flags)
+    if (a.lds_pos) {
+        double *lp = reinterpret_cast<double *>(chain_dyn_lds);
+        for (int i = tid; i < D * a.n; i += NT) lp[i] = a.pos[i];
     }
-    if (ga.lds_ring) {
-        uint32_t *lr = (uint32_t *)(dyn_lds + ga.lds_ring_off);
-        for (int i = tid; i < ga.ring_words; i += NT) lr[i] = ga.ring[i];
-        a.ring = lr;
+    if (a.lds_ring) {
+        uint32_t *lr = reinterpret_cast<uint32_t *>(chain_dyn_lds + a.lds_ring_off);
+        for (int i = tid; i < a.ring_words; i += NT) lr[i] = a.ring[i];
     }
     if (tid == 0) st = *a.st;
     for (int i = tid; i < a.n_taggers; i += NT) tg[i] = a.tg[i];
@@ -1338,15 +1354,20 @@ __device__ __forceinline__ void chain_body(const ChainArgs &ga) {
         *a.n_out = k;
     }
     for (int i = tid; i < 624; i += NT) a.mt[i] = mt[i];
-    if (ga.lds_pos)
-        for (int i = tid; i < D * ga.n; i += NT) ga.pos[i] = a.pos[i];
-    if (ga.lds_ring)
-        for (int i = tid; i < ga.ring_words; i += NT) ga.ring[i] = a.ring[i];
+    if (a.lds_pos) {
+        const double *lp = reinterpret_cast<const double *>(chain_dyn_lds);
+        for (int i = tid; i < D * a.n; i += NT) a.pos[i] = lp[i];
+    }
+    if (a.lds_ring) {
+        const uint32_t *lr = reinterpret_cast<const uint32_t *>(chain_dyn_lds + a.lds_ring_off);
+        for (int i = tid; i < a.ring_words; i += NT) a.ring[i] = lr[i];
+    }
 }
 
+// One system; its arguments in device memory (s->d_args), read at a uniform address.
 template <int D>
-__global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
-    chain_body<D>(a);
+__global__ __launch_bounds__(NT) void chain_kernel(const ChainArgs *__restrict__ args) {
+    chain_body<D>(*args);
 }
 
 // One step of a leg driven from the host, for the set-level event handler (the mediator and
@@ -1357,7 +1378,9 @@ __global__ __launch_bounds__(NT) void chain_kernel(ChainArgs a) {
 //         slot t (time, target, in-state copies);
 //   op 2  send_out_state of tagger t's stored event, committed to the state; *out receives it.
 template <int D>
-__global__ __launch_bounds__(NT) void chain_step_kernel(ChainArgs a, int op, int t, ecmc_event *out) {
+__global__ __launch_bounds__(NT) void chain_step_kernel(const ChainArgs *__restrict__ args, int op, int t,
+                                                        ecmc_event *out) {
+    const ChainArgs &a = *args;
     __shared__ ChainState st;
     __shared__ ecmc_tagger_desc tg[ECMC_MAX_TAGGERS];
     __shared__ uint32_t mt[624];
@@ -1401,9 +1424,8 @@ __global__ __launch_bounds__(NT) void chain_step_kernel(ChainArgs a, int op, int
 // Independent systems in one launch: workgroup b runs system b (its own positions, cell
 // occupancy, random stream and state), exactly as chain_kernel runs it alone.
 template <int D>
-__global__ __launch_bounds__(NT) void chain_many_kernel(const ChainArgs *args) {
-    const ChainArgs a = args[blockIdx.x];
-    chain_body<D>(a);
+__global__ __launch_bounds__(NT) void chain_many_kernel(const ChainArgs *__restrict__ args) {
+    chain_body<D>(args[blockIdx.x]);   // the arguments stay in global memory, read at a uniform address
 }
 
 // InnerPointEstimator.derivative_bound (inner_point_estimator.py:108-171) for the cell
@@ -1545,6 +1567,7 @@ struct ecmc_system {
     int device = 0;          // the device current at creation, which holds its memory
     size_t lds_bytes = 0;    // dynamic LDS of a launch (positions and ring resident there)
     ChainArgs args;
+    ChainArgs *d_args = nullptr;   // the arguments of the system's own launches, in device memory
 };
 
 // Dynamic LDS of the leg loop for a system of n units in dim dimensions with a ring of
@@ -1829,6 +1852,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
 #define ECMC_UP(dst, src, bytes) \
     if (hipMemcpy((dst), (src), (bytes), hipMemcpyHostToDevice) != hipSuccess) { ecmc_system_destroy(s); return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy failed"); }
     ECMC_ALLOC(s->d_pos, sizeof(double) * dim * n);
+    ECMC_ALLOC(s->d_args, sizeof(ChainArgs));
     ECMC_ALLOC(s->d_charge, sizeof(double) * n);
     ECMC_ALLOC(s->d_cmin, sizeof(double) * grid.cmin.size());
     ECMC_ALLOC(s->d_nearby, sizeof(int32_t) * grid.nearby.size());
@@ -2004,7 +2028,7 @@ extern "C" void ecmc_system_destroy(ecmc_system *s) {
     hipFree(s->d_occ); hipFree(s->d_occ_n); hipFree(s->d_mt); hipFree(s->d_ring); hipFree(s->d_st);
     hipFree(s->d_sur_cell); hipFree(s->d_sur_id); hipFree(s->d_cand); hipFree(s->d_cbt); hipFree(s->d_corners);
     hipFree(s->d_cvw);
-    hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald); hipFree(s->d_ewald_codes); hipFree(s->d_step_ev);
+    hipFree(s->d_tg); hipFree(s->d_nout); hipFree(s->d_log); hipFree(s->d_ewald); hipFree(s->d_ewald_codes); hipFree(s->d_step_ev); hipFree(s->d_args);
     if (s->ev0) hipEventDestroy(s->ev0);
     if (s->ev1) hipEventDestroy(s->ev1);
     if (s->stream) hipStreamDestroy(s->stream);
@@ -2038,8 +2062,10 @@ extern "C" int ecmc_system_run(ecmc_system *s, int64_t max_events, ecmc_event *h
     a.max_events = max_events;
     a.stop_mask = s->stop_mask;
     hipEventRecord(s->ev0, s->stream);
-    if (s->dim == 2) hipLaunchKernelGGL(chain_kernel<2>, dim3(1), dim3(NT), s->lds_bytes, s->stream, a);
-    else hipLaunchKernelGGL(chain_kernel<3>, dim3(1), dim3(NT), s->lds_bytes, s->stream, a);
+    if (hipMemcpyAsync(s->d_args, &a, sizeof(ChainArgs), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
+    if (s->dim == 2) hipLaunchKernelGGL(chain_kernel<2>, dim3(1), dim3(NT), s->lds_bytes, s->stream, s->d_args);
+    else hipLaunchKernelGGL(chain_kernel<3>, dim3(1), dim3(NT), s->lds_bytes, s->stream, s->d_args);
     hipEventRecord(s->ev1, s->stream);
     if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
     if (hipStreamSynchronize(s->stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
@@ -2174,8 +2200,10 @@ static int system_step(ecmc_system *s, int op, int tagger, ChainState &st) {
     if (op != 0 && (tagger < 0 || tagger >= s->desc.n_taggers)) return ecmc_fail(ECMC_ERR_ARGUMENT, "no such tagger");
     ChainArgs sa = s->args;   // one step works on global memory
     sa.lds_pos = sa.lds_ring = 0;
-    if (s->dim == 2) hipLaunchKernelGGL(chain_step_kernel<2>, dim3(1), dim3(NT), 0, s->stream, sa, op, tagger, s->d_step_ev);
-    else hipLaunchKernelGGL(chain_step_kernel<3>, dim3(1), dim3(NT), 0, s->stream, sa, op, tagger, s->d_step_ev);
+    if (hipMemcpyAsync(s->d_args, &sa, sizeof(ChainArgs), hipMemcpyHostToDevice, s->stream) != hipSuccess)
+        return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
+    if (s->dim == 2) hipLaunchKernelGGL(chain_step_kernel<2>, dim3(1), dim3(NT), 0, s->stream, s->d_args, op, tagger, s->d_step_ev);
+    else hipLaunchKernelGGL(chain_step_kernel<3>, dim3(1), dim3(NT), 0, s->stream, s->d_args, op, tagger, s->d_step_ev);
     if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
     if (hipStreamSynchronize(s->stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
     if (hipMemcpy(&st, s->d_st, sizeof(ChainState), hipMemcpyDeviceToHost) != hipSuccess)

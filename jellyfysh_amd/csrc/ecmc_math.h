@@ -58,6 +58,13 @@ ECMC_HD double py_mod(double x, double y) {
     return m;
 }
 
+// fmod(x, y), with the one-period cases taken without fmod's reduction loop: x for
+// 0 <= x < y, the exact x - y for y <= x < 2y (Sterbenz).
+ECMC_HD double fmod_near(double x, double y) {
+    if (x >= 0.0 && y > 0.0 && x < 2.0 * y) return x < y ? x : x - y;
+    return fmod(x, y);
+}
+
 ECMC_HD void py_divmod(double vx, double wx, double &q, double &r) {
     if (wx == 1.0 && vx >= 0.0 && vx < 0x1p52) {
         // division by one of a non-negative double: the floor and the exact remainder (+0 for an

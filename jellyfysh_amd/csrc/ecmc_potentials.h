@@ -385,7 +385,7 @@ ECMC_FN double ipcb_displacement(double pp, double sx, double sy, double sz, dou
     double ph = ipcb_potential(pp, half, sy, sz);
     double per = fabs(p0 - ph);
     double disp = floor(dU / per) * L;
-    dU = fmod(dU, per);
+    dU = fmod_near(dU, per);
     double nn;
     if (pp > 0.0) {
         if (sx <= 0.0) {

@@ -116,7 +116,7 @@ int main() {
     while (scanf("%la %la", &x, &y) == 2) {
         double q, r;
         py_divmod(x, y, q, r);
-        printf("%a %a %a\n", py_mod(x, y), q, r);
+        printf("%a %a %a %a\n", py_mod(x, y), q, r, fmod_near(fabs(x), y));
     }
     return 0;
 }
@@ -125,8 +125,9 @@ int main() {
 
 def test_python_float_mod_and_divmod_of_the_kernels(tmp_path):
     """py_mod and py_divmod of ecmc_math.h (the kernels' separation wrap and Time's divmod by one,
-    with their one-period and non-negative fast paths) equal CPython's % and divmod, signed zeros
-    included, around every branch point: 0, +-y, +-2y, and far outside."""
+    with their one-period and non-negative fast paths) equal CPython's % and divmod, and
+    fmod_near (the image windings of the Coulomb bound) equals math.fmod, signed zeros included,
+    around every branch point: 0, +-y, +-2y, and far outside."""
     import math
     import random
     gxx = shutil.which("g++")
@@ -154,7 +155,7 @@ def test_python_float_mod_and_divmod_of_the_kernels(tmp_path):
     inp = "".join("%s %s\n" % (x.hex(), y.hex()) for x, y in pairs)
     out = subprocess.run([str(exe)], input=inp, capture_output=True, text=True, check=True).stdout.split("\n")
     for (x, y), line in zip(pairs, out):
-        m, q, r = (float.fromhex(v) for v in line.split())
+        m, q, r, f = (float.fromhex(v) for v in line.split())
         wq, wr = divmod(x, y)
-        for got, want in ((m, x % y), (q, wq), (r, wr)):
+        for got, want in ((m, x % y), (q, wq), (r, wr), (f, math.fmod(abs(x), y))):
             assert got == want and math.copysign(1, got) == math.copysign(1, want), (x, y, got, want)
