@@ -326,6 +326,8 @@ def flexible_dipoles_rate(replicas, legs):
     for c in chains:
         c.close()
     roof = plain_roofline("ecmc::chain_many_kernel<3>", ncand, 3, 0, int(k.sum()), ms, charged=True)
+    roof["traffic"] = profiled_traffic("ecmc::chain_many_kernel<3>", legs,
+                                       "chain_many_kernel_charged_soft_spheres_counters.txt", "flexible_dipoles")
     p, q, w = starts[0]
     cpu1 = plain_cpu_single_core(desc, p, seconds=4.0, charges=q)
     return dict(replicas=replicas, n=len(starts[0][0]), legs_per_replica=legs, roofline=roof, cpu_single_core=cpu1,
