@@ -46,6 +46,10 @@ constexpr int NT = 256;
 // The dynamic LDS of a leg-loop launch (positions, then the ring of tempered words), when the
 // launch's arguments place them there (ChainArgs::lds_pos, lds_ring).
 extern __shared__ __align__(16) unsigned char chain_dyn_lds[];
+
+struct ChainArgs;
+__device__ __forceinline__ int32_t *occ_base(const ChainArgs &a);
+__device__ __forceinline__ int32_t *occn_base(const ChainArgs &a);
 constexpr uint32_t MT_UPPER = 0x80000000u, MT_LOWER = 0x7fffffffu, MT_MATRIX = 0x9908b0dfu;
 
 struct Slot {
@@ -111,7 +115,7 @@ struct ChainArgs {
     // bounded occupancy (single_active_cell_occupancy.py:62) and cell bounding
     int32_t max_occ, sur_cap, layers, pad2;
     int32_t *sur_cell, *sur_id;   // surplus units in the order the reference's dict yields them
-    int32_t *cand;                // candidate list of the cell-bounding tagger
+    int32_t *cand;                // the cell-bounding tagger's candidates: their far cells (one occupant each)
     const double *cbt;            // cell-bound table: (cell offset, direction) -> upper, lower
     // cell veto: Walker tables [direction * 2 + (0 upper, 1 lower)] of cv_m entries each
     const CvEntry *cvw;
@@ -120,13 +124,22 @@ struct ChainArgs {
     // residency in the launch's dynamic LDS: positions (dim x n doubles at offset 0) and the MT
     // ring (ring_words words at lds_ring_off), copied in at the start of a launch and back at
     // its end; 0 keeps them in global memory
-    int32_t lds_pos, lds_ring, lds_ring_off, pad4;
+    int32_t lds_pos, lds_ring, lds_ring_off, lds_cand_off;   // lds_cand_off < 0: the candidate list in global memory
+    int32_t lds_occ_off, lds_occn_off;                         // < 0: the occupant lists / counts in global memory
     // the terms of the merged-image Coulomb derivative in the sequential loop's order (real-space
     // images by k, j, i, then Fourier terms by i, j, k): kind << 24 | (i + 128) << 16 |
     // (j + 128) << 8 | (k + 128), built on the host
     const int32_t *ewald_codes;
     int32_t ewald_ncodes, pad5;
 };
+
+// The cell occupant lists (ncells x occ_cap) and their counts, in LDS for a launch when placed there.
+__device__ __forceinline__ int32_t *occ_base(const ChainArgs &a) {
+    return a.lds_occ_off >= 0 ? reinterpret_cast<int32_t *>(chain_dyn_lds + a.lds_occ_off) : a.occ;
+}
+__device__ __forceinline__ int32_t *occn_base(const ChainArgs &a) {
+    return a.lds_occn_off >= 0 ? reinterpret_cast<int32_t *>(chain_dyn_lds + a.lds_occn_off) : a.occ_n;
+}
 
 __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
     y ^= (y >> 11);
@@ -258,11 +271,8 @@ __device__ __forceinline__ bool pot_draws(const ecmc_tagger_desc &tg) {
 // target cell's offset from the active cell along dir, upper or lower by the sign of the
 // charge correction factor (cell_bounding_potential.py:161-201).
 template <int D>
-__device__ __forceinline__ double cell_rate(const ChainArgs &a, const ecmc_tagger_desc &tg, int acell, int target, int dir, double ca,
-                            double ct) {
-    double tp[D];
-    load_pos<D>(a, target, tp);
-    const int tcell = pos_to_cell<D>(a, tp);
+__device__ __forceinline__ double cell_rate_of_cells(const ChainArgs &a, const ecmc_tagger_desc &tg, int acell, int tcell,
+                                                     int dir, double ca, double ct) {
     int rel = 0;
 #pragma unroll
     for (int d = 0; d < D; d++) {
@@ -275,6 +285,14 @@ __device__ __forceinline__ double cell_rate(const ChainArgs &a, const ecmc_tagge
         return cf > 0.0 ? row[0] * cf : row[1] * cf;
     }
     return row[0];
+}
+// The same for a target unit, its cell from its position.
+template <int D>
+__device__ __forceinline__ double cell_rate(const ChainArgs &a, const ecmc_tagger_desc &tg, int acell, int target, int dir,
+                                            double ca, double ct) {
+    double tp[D];
+    load_pos<D>(a, target, tp);
+    return cell_rate_of_cells<D>(a, tg, acell, pos_to_cell<D>(a, tp), dir, ca, ct);
 }
 
 // Is cell c among the nearby cells of cell ac (within `layers` per axis, periodic)?
@@ -406,7 +424,7 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
             int cc = 0, num = 0;
             if (tid < cnt) {
                 cc = a.nearby[(size_t)cell * a.nb_stride + tid];
-                num = a.occ_n[cc];
+                num = occn_base(a)[cc];
             }
             int incl = num;
 #pragma unroll
@@ -434,6 +452,8 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
     const bool sur = tg.kind == ECMC_TAG_SURPLUS;
     const bool far = tg.kind == ECMC_TAG_CELL_BOUNDING;
     const bool fmap = tg.kind == ECMC_TAG_FACTOR_MAP;
+    // the far cells of this create (built and read within it): in LDS when the launch has room
+    int32_t *const cand = a.lds_cand_off >= 0 ? reinterpret_cast<int32_t *>(chain_dyn_lds + a.lds_cand_off) : a.cand;
     if (far) {
         // cell_bounding_potential_tagger.py:150-155: the occupied cells that are not nearby the
         // active cell, in cell order; each threads scans a contiguous run of cells, an
@@ -441,9 +461,15 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
         const int ac = st.occ_cell;
         const int chunk = (a.ncells + NT - 1) / NT;
         const int lo = min(tid * chunk, a.ncells), hi = min(lo + chunk, a.ncells);
+        // the occupied far cells of this thread's run, as bits when the run has at most 32 cells
+        const bool use_bits = chunk <= 32;
+        uint32_t bits = 0;
         int cnt = 0;
         for (int c = lo; c < hi; c++)
-            if (a.occ_n[c] > 0 && !is_nearby(a, D, c, ac)) cnt++;
+            if (occn_base(a)[c] > 0 && !is_nearby(a, D, c, ac)) {
+                cnt++;
+                if (use_bits) bits |= 1u << (c - lo);
+            }
         // exclusive scan of the counts over the workgroup: shuffles within each wavefront, the
         // wavefront totals in scan[NT + 1 ..]
         const int lane = tid & 63, wave = tid >> 6;
@@ -461,9 +487,9 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
         if (tid == NT - 1) scan[NT] = before + incl;
         bool multi = false;
         for (int c = lo; c < hi; c++)
-            if (a.occ_n[c] > 0 && !is_nearby(a, D, c, ac)) {
-                if (a.occ_n[c] != 1) multi = true;
-                a.cand[k++] = a.occ[(size_t)c * a.occ_cap];
+            if (use_bits ? ((bits >> (c - lo)) & 1u) != 0 : (occn_base(a)[c] > 0 && !is_nearby(a, D, c, ac))) {
+                if (occn_base(a)[c] != 1) multi = true;
+                cand[k++] = c;   // the cell; its one occupant is the candidate
             }
         if (multi) st.err = ECMC_ERR_STATE;   // the handler takes exactly one target unit
         __syncthreads();
@@ -486,7 +512,7 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
         if (fmap) return fmap_target(tg, a.leaves, act, j);
         if (all) return j < act ? j : j + 1;
         if (sur) return (int)a.sur_id[j];
-        if (far) return (int)a.cand[j];
+        if (far) return (int)occ_base(a)[(size_t)cand[j] * a.occ_cap];
         // the nearby cell holding candidate j: the largest c with nb_pref[c] <= j (binary search
         // over the nc + 1 prefix sums; empty cells repeat a prefix)
         int lo = 0, hi = nc;
@@ -495,7 +521,7 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
             if (nb_pref[mid] <= j) lo = mid;
             else hi = mid;
         }
-        return (int)a.occ[(size_t)nb_cell[lo] * a.occ_cap + (j - nb_pref[lo])];
+        return (int)occ_base(a)[(size_t)nb_cell[lo] * a.occ_cap + (j - nb_pref[lo])];
     };
     // FP32 filter: every candidate's time displacement is bracketed [lb, ub] from its separation
     // and the leading word of its draw; a candidate whose lb exceeds the smallest ub of the set
@@ -597,7 +623,8 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
         double dt;
         if (tg.event_handler == ECMC_EH_CELL_BOUNDING) {
             // two_leaf_unit_cell_bounding_potential_event_handler.py:171-177
-            const double rate = cell_rate<D>(a, tg, st.occ_cell, target, dir, ca, ct);
+            const double rate = far ? cell_rate_of_cells<D>(a, tg, st.occ_cell, (int)cand[j], dir, ca, ct)
+                                    : cell_rate<D>(a, tg, st.occ_cell, target, dir, ca, ct);
             dt = (rate > 0 ? dU / rate : INFINITY) / speed;
         } else {
             dt = pair_dt<D>(a, tg, dir, speed, s, ca, ct, dU);
@@ -702,7 +729,7 @@ template <int D>
 __device__ __forceinline__ void occupancy_update(const ChainArgs &a, ChainState &st, int *red) {
     const int tid = threadIdx.x;
     const int id = st.active, old_id = st.occ_id, old_cell = st.occ_cell;
-    const int n_old = (id != old_id && old_id >= 0) ? a.occ_n[old_cell] : 0;
+    const int n_old = (id != old_id && old_id >= 0) ? occn_base(a)[old_cell] : 0;
     double p[D];
     load_pos<D>(a, id, p);
     const int c = pos_to_cell<D>(a, p);
@@ -721,8 +748,8 @@ __device__ __forceinline__ void occupancy_update(const ChainArgs &a, ChainState 
             if (tid == 0) {
                 if (n >= a.occ_cap) st.err = ECMC_ERR_STATE;
                 else {
-                    a.occ[(size_t)old_cell * a.occ_cap + n] = old_id;
-                    a.occ_n[old_cell] = n + 1;
+                    occ_base(a)[(size_t)old_cell * a.occ_cap + n] = old_id;
+                    occn_base(a)[old_cell] = n + 1;
                 }
             }
         } else {
@@ -748,12 +775,12 @@ __device__ __forceinline__ void occupancy_update(const ChainArgs &a, ChainState 
         __syncthreads();
         return;
     }
-    int32_t *Lc = a.occ + (size_t)c * a.occ_cap;
-    const int n = a.occ_n[c];
+    int32_t *Lc = occ_base(a) + (size_t)c * a.occ_cap;
+    const int n = occn_base(a)[c];
     const int i = find_entry(Lc, nullptr, n, id, 0, false, red);
     if (i >= 0) {
         shift_entries(Lc, nullptr, i + 1, n, -1);
-        if (tid == 0) a.occ_n[c] = n - 1;
+        if (tid == 0) occn_base(a)[c] = n - 1;
     } else {
         const int nsur = st.nsur;
         const int k = find_entry(a.sur_cell, a.sur_id, nsur, c, id, false, red);
@@ -1010,9 +1037,9 @@ __device__ __forceinline__ bool thinning_separation(const ChainArgs &a, const Ch
         (T.event_handler == ECMC_EH_CELL_BOUNDING || T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING)) {
         separation_vector<D>(a, sl.apos, sl.tpos, s);
     } else if (T.kind == ECMC_TAG_CELL_VETO) {
-        if (a.occ_n[sl.cb_cell] != 1) return false;
+        if (occn_base(a)[sl.cb_cell] != 1) return false;
         double tp[D];
-        load_pos<D>(a, a.occ[(size_t)sl.cb_cell * a.occ_cap], tp);
+        load_pos<D>(a, occ_base(a)[(size_t)sl.cb_cell * a.occ_cap], tp);
         separation_vector<D>(a, sl.apos, tp, s);
     } else {
         return false;
@@ -1164,14 +1191,14 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
         // leaf_unit_cell_veto_event_handler.py:127-149 with
         // event_handler_with_bounding_potential.py:75-101: thinning of the stored rate
         const int tc = sl.cb_cell;
-        const int no = a.occ_n[tc];
+        const int no = occn_base(a)[tc];
         if (no > 1) { st.err = ECMC_ERR_STATE; break; }
         bool lift = false;
         double s[D];
 #pragma unroll
         for (int d = 0; d < D; d++) s[d] = 0.0;
         if (no == 1) {
-            const int tid2 = a.occ[(size_t)tc * a.occ_cap];
+            const int tid2 = occ_base(a)[(size_t)tc * a.occ_cap];
             ev.target = tid2;
             double tp[D];
             load_pos<D>(a, tid2, tp);
@@ -1272,6 +1299,14 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
         uint32_t *lr = reinterpret_cast<uint32_t *>(chain_dyn_lds + a.lds_ring_off);
         for (int i = tid; i < a.ring_words; i += NT) lr[i] = a.ring[i];
     }
+    if (a.lds_occn_off >= 0) {
+        int32_t *lc = occn_base(a);
+        for (int i = tid; i < a.ncells; i += NT) lc[i] = a.occ_n[i];
+    }
+    if (a.lds_occ_off >= 0) {
+        int32_t *lo = occ_base(a);
+        for (int i = tid; i < a.ncells * a.occ_cap; i += NT) lo[i] = a.occ[i];
+    }
     if (tid == 0) st = *a.st;
     for (int i = tid; i < a.n_taggers; i += NT) tg[i] = a.tg[i];
     for (int i = tid; i < 624; i += NT) mt[i] = a.mt[i];
@@ -1361,6 +1396,14 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
     if (a.lds_ring) {
         const uint32_t *lr = reinterpret_cast<const uint32_t *>(chain_dyn_lds + a.lds_ring_off);
         for (int i = tid; i < a.ring_words; i += NT) a.ring[i] = lr[i];
+    }
+    if (a.lds_occn_off >= 0) {
+        const int32_t *lc = occn_base(a);
+        for (int i = tid; i < a.ncells; i += NT) a.occ_n[i] = lc[i];
+    }
+    if (a.lds_occ_off >= 0) {
+        const int32_t *lo = occ_base(a);
+        for (int i = tid; i < a.ncells * a.occ_cap; i += NT) a.occ[i] = lo[i];
     }
 }
 
@@ -1597,6 +1640,24 @@ static void plan_lds(ecmc_system *s) {
         a.lds_ring_off = (int32_t)used;
         used += ring_b;
     }
+    // the far-cell list of a cell-bounding tagger (n entries at most), the occupant counts and the
+    // occupant lists, each when it fits after the others
+    a.lds_cand_off = a.lds_occn_off = a.lds_occ_off = -1;
+    const size_t cand_b = s->d_cand ? ((size_t)s->n * sizeof(int32_t) + 15) & ~(size_t)15 : 0;
+    if (cand_b && used + cand_b <= budget) {
+        a.lds_cand_off = (int32_t)used;
+        used += cand_b;
+    }
+    const size_t occn_b = ((size_t)s->ncells * sizeof(int32_t) + 15) & ~(size_t)15;
+    if (used + occn_b <= budget) {
+        a.lds_occn_off = (int32_t)used;
+        used += occn_b;
+    }
+    const size_t occ_b = ((size_t)s->ncells * s->occ_cap * sizeof(int32_t) + 15) & ~(size_t)15;
+    if (used + occ_b <= budget) {
+        a.lds_occ_off = (int32_t)used;
+        used += occ_b;
+    }
     s->lds_bytes = used;
     if (used > 65536) {
         const hipError_t r = s->dim == 2
@@ -1611,6 +1672,7 @@ static void plan_lds(ecmc_system *s) {
             a.lds_pos = 0;
             a.lds_ring = ring_b <= 65536 && ring_b <= budget ? 1 : 0;
             a.lds_ring_off = 0;
+            a.lds_cand_off = a.lds_occn_off = a.lds_occ_off = -1;
             s->lds_bytes = a.lds_ring ? ring_b : 0;
         }
     }
@@ -2200,6 +2262,7 @@ static int system_step(ecmc_system *s, int op, int tagger, ChainState &st) {
     if (op != 0 && (tagger < 0 || tagger >= s->desc.n_taggers)) return ecmc_fail(ECMC_ERR_ARGUMENT, "no such tagger");
     ChainArgs sa = s->args;   // one step works on global memory
     sa.lds_pos = sa.lds_ring = 0;
+    sa.lds_cand_off = sa.lds_occn_off = sa.lds_occ_off = -1;
     if (hipMemcpyAsync(s->d_args, &sa, sizeof(ChainArgs), hipMemcpyHostToDevice, s->stream) != hipSuccess)
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
     if (s->dim == 2) hipLaunchKernelGGL(chain_step_kernel<2>, dim3(1), dim3(NT), 0, s->stream, s->d_args, op, tagger, s->d_step_ev);
