@@ -186,6 +186,10 @@ __device__ __forceinline__ void mt_generate_block(uint32_t *mt, uint32_t *ring, 
 
 // Collective: make `need` unread words available in the ring.
 __device__ __forceinline__ void ring_ensure(ChainState &st, uint32_t *mt, const ChainArgs &a, uint64_t need) {
+    if (need + 2 * 624 > (uint64_t)a.ring_words) {   // the ring would overwrite unread words
+        st.err = ECMC_ERR_STATE;
+        return;
+    }
     while (st.produced < st.cursor + need) {
         mt_generate_block(mt, a.lds_ring ? reinterpret_cast<uint32_t *>(chain_dyn_lds + a.lds_ring_off) : a.ring,
                           (uint64_t)(a.ring_words - 1), st.produced);
@@ -1730,7 +1734,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     if (P > ECMC_MAX_LEAVES) return ecmc_fail(ECMC_ERR_CONFIGURATION, "more leaf units per composite object than ECMC_MAX_LEAVES");
     if (n % P != 0) return ecmc_fail(ECMC_ERR_CONFIGURATION, "the number of units is not a multiple of leaves_per_root");
     int ewald_tagger = -1, cb_tagger = -1;
-    int64_t max_cand = n;   // the most pair candidates one tagger creates in a leg
+    int64_t max_cand = 0;   // the most pair candidates one tagger creates in a leg
     for (int t = 0; t < desc->n_taggers; t++) {
         const ecmc_tagger_desc &tg = desc->taggers[t];
         if (tg.kind < 0 || tg.kind > ECMC_TAG_FACTOR_MAP) return ecmc_fail(ECMC_ERR_CONFIGURATION, "unknown tagger kind");
@@ -1862,6 +1866,17 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
         for (int d = 0; d < dim; d++) pos_soa[(size_t)d * n + i] = positions[(size_t)i * dim + d];
     std::vector<double> ch(n, 1.0);
     if (charges) for (int i = 0; i < n; i++) ch[i] = charges[i];
+    // the candidates of one create: all other units, the surplus (at most n), the occupants of
+    // the nearby cells (a cell holds at most occ_cap: an overflow stops the run) or the occupied
+    // far cells
+    for (int t = 0; t < desc->n_taggers; t++) {
+        const int kind = desc->taggers[t].kind;
+        int64_t m = 0;
+        if (kind == ECMC_TAG_PAIR_ALL || kind == ECMC_TAG_SURPLUS) m = n;
+        else if (kind == ECMC_TAG_PAIR_CELLS) m = (int64_t)grid.nb_stride * cap;
+        else if (kind == ECMC_TAG_CELL_BOUNDING) m = std::max<int64_t>((int64_t)grid.nb_stride * cap, grid.count);
+        max_cand = m > max_cand ? m : max_cand;
+    }
     // the ring holds every word a create may read plus a block in production: a power of two, so
     // that word indices wrap with a mask
     {
