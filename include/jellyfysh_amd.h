@@ -306,7 +306,9 @@ int ecmc_pair_derivative(int32_t potential, const double *par, int32_t dim, int6
                          const double *d_speed, const double *d_sep, const double *d_c1, const double *d_c2,
                          double system_length, double *d_out, void *stream);
 /* FP32 brackets lb <= t <= ub of the time displacement t that ecmc_pair_displacement returns for
- * the inverse-power potentials (ECMC_POT_INVERSE_POWER, _TRUNC; integer power 1..32), with the
+ * the inverse-power potentials (ECMC_POT_INVERSE_POWER, _TRUNC; integer power 1..32) and for the
+ * inverse-power Coulomb bound (ECMC_POT_INVERSE_POWER_COULOMB_BOUND, dim 3, par = {prefactor,
+ * system_length}; inverse_power_coulomb_bounding_potential.c:85-139), with the
  * potential change dU = -log(1 - u) / beta of the random pair whose leading MT word is word[i]
  * (u = random() of the words (word[i], next word), potential/inverse_power_potential.py:96-179
  * with the expovariate draw of event_handler/two_leaf_unit_event_handler.py).  lb = +inf only

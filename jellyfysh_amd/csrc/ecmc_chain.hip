@@ -130,12 +130,17 @@ struct ChainArgs {
     // images by k, j, i, then Fourier terms by i, j, k): kind << 24 | (i + 128) << 16 |
     // (j + 128) << 8 | (k + 128), built on the host
     const int32_t *ewald_codes;
-    int32_t ewald_ncodes, pad5;
+    int32_t ewald_ncodes, lds_charge_off;   // lds_charge_off < 0: the charges in global memory
 };
 
 // The cell occupant lists (ncells x occ_cap) and their counts, in LDS for a launch when placed there.
 __device__ __forceinline__ int32_t *occ_base(const ChainArgs &a) {
     return a.lds_occ_off >= 0 ? reinterpret_cast<int32_t *>(chain_dyn_lds + a.lds_occ_off) : a.occ;
+}
+// The charge of unit id (constant over a run): from LDS when the launch placed the charges there.
+__device__ __forceinline__ double charge_of(const ChainArgs &a, int id) {
+    if (a.lds_charge_off >= 0) return reinterpret_cast<const double *>(chain_dyn_lds + a.lds_charge_off)[id];
+    return a.charge[id];
 }
 __device__ __forceinline__ int32_t *occn_base(const ChainArgs &a) {
     return a.lds_occn_off >= 0 ? reinterpret_cast<int32_t *>(chain_dyn_lds + a.lds_occn_off) : a.occ_n;
@@ -407,7 +412,9 @@ __device__ __forceinline__ float block_min_float(float v, uint32_t *sh) {
 }
 
 // The candidate sets whose exact inversions are filtered by FP32 brackets first: two-leaf
-// inverse-power factors (plain and truncated-and-shifted) of an integer power up to 32.
+// inverse-power factors (plain and truncated-and-shifted) of an integer power up to 32.  The
+// Coulomb bound's sets are inverted exactly: its brackets (ipcb_time_bounds) cost nearly as much
+// as its exact inversion, and filtering them measured slower.
 __device__ __forceinline__ bool fp32_filter_applies(const ecmc_tagger_desc &tg) {
     if (tg.event_handler != ECMC_EH_TWO_LEAF) return false;
     if (tg.potential != ECMC_POT_INVERSE_POWER && tg.potential != ECMC_POT_INVERSE_POWER_TRUNC) return false;
@@ -508,7 +515,7 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
     const Time at = st.at;
     double ap[D];
     load_pos<D>(a, act, ap);
-    const double ca = tg.use_charge ? a.charge[act] : 1.0;
+    const double ca = tg.use_charge ? charge_of(a, act) : 1.0;
     const int jbase = st.ncand;
     Best best = {INFINITY, INFINITY, 0x7fffffff};
     uint32_t hsum = 0;
@@ -559,11 +566,11 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
             near_edge = near_edge || !(fabs(v) < edge);
             s[d] = v;
         }
-        const double ct = tg.use_charge ? a.charge[target] : 1.0;
-        const double pp = tg.par[1] * (ca * ct);
-        const float K = (float)fabs(pp);
+        const double ct = tg.use_charge ? charge_of(a, target) : 1.0;
         lb = 0.f;
         ub = INFINITY;
+        const double pp = tg.par[1] * (ca * ct);
+        const float K = (float)fabs(pp);
         if (!near_edge && K >= 0x1p-100f && K <= 0x1p100f) {
             const uint32_t w0 = ring_word(a, base + 2ull * j);
             ip_time_bounds<D>(s, dir, K, pp > 0.0, fp, two_over_p, ftrunc, dn(rc2f), up(rc2f),
@@ -623,7 +630,7 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
             double u = words_to_random(ring_word(a, base + 2ull * j), ring_word(a, base + 2ull * j + 1));
             dU = -cr_log(1.0 - u) / a.beta;
         }
-        const double ct = tg.use_charge ? a.charge[target] : 1.0;
+        const double ct = tg.use_charge ? charge_of(a, target) : 1.0;
         double dt;
         if (tg.event_handler == ECMC_EH_CELL_BOUNDING) {
             // two_leaf_unit_cell_bounding_potential_event_handler.py:171-177
@@ -905,7 +912,7 @@ template <int D>
 __device__ __forceinline__ bool create_cell_veto(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &tg, int tix) {
     Slot &sl = st.slot[tix];
     const int act = st.active, dir = st.adir;
-    double cf = (tg.use_charge ? a.charge[act] : 1.0) * 1.0 / tg.bpar[2];
+    double cf = (tg.use_charge ? charge_of(a, act) : 1.0) * 1.0 / tg.bpar[2];
     int b = 0;
     if (!(cf > 0.0)) { cf *= -1.0; b = 1; }
     const int wt = dir * 2 + b;
@@ -968,18 +975,25 @@ __device__ __forceinline__ bool create_cell_veto(const ChainArgs &a, ChainState 
 // butterfly over the lanes adds the 64 partial sums (commutative pairings, so every lane ends
 // with the same sum).  The order of the additions differs from the sequential loop's, so the
 // result agrees with it to rounding.  More than 256 terms fall back to the sequential function.
+// The sequential form where the wave form does not apply: called, not inlined (a rare path).
+__device__ __noinline__ static double ewald_derivative_call(const Ewald &e, double sx, double sy, double sz) {
+    return ewald_derivative(e, sx, sy, sz);
+}
 __device__ __forceinline__ double ewald_derivative_wave(const Ewald &e, const int32_t *codes, int ncodes, double sx, double sy,
                                         double sz) {
-    if (ncodes > 256 || !codes) return ewald_derivative(e, sx, sy, sz);
+    if (ncodes > 256 || !codes) return ewald_derivative_call(e, sx, sy, sz);
     const int lane = threadIdx.x & 63;
     const int n = e.fc + 1;
     const double L = e.L;
     int code[4];
 #pragma unroll
     for (int s = 0; s < 4; s++) code[s] = lane + 64 * s < ncodes ? codes[lane + 64 * s] : -1;
-    const double dcx = cos(e.tpol * sx), dsx = sin(e.tpol * sx);
-    const double dcy = cos(e.tpol * sy), dsy = sin(e.tpol * sy);
-    const double dcz = cos(e.tpol * sz), dsz = sin(e.tpol * sz);
+    // cos and sin of tpol s_d on lane d (the same functions per component), broadcast
+    double cd, sd;
+    sincos(e.tpol * (lane == 0 ? sx : (lane == 1 ? sy : sz)), &sd, &cd);
+    const double dcx = __shfl(cd, 0, 64), dsx = __shfl(sd, 0, 64);
+    const double dcy = __shfl(cd, 1, 64), dsy = __shfl(sd, 1, 64);
+    const double dcz = __shfl(cd, 2, 64), dsz = __shfl(sd, 2, 64);
     double sum = 0.0;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
@@ -1148,10 +1162,10 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
             if (pos_to_cell<D>(a, sl.apos) != sl.cb_cell) { st.err = ECMC_ERR_STATE; break; }
             double q3[3] = {0, 0, 0};
             for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
-            const double c1 = T.use_charge ? a.charge[act] : 1.0;
-            const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
+            const double c1 = T.use_charge ? charge_of(a, act) : 1.0;
+            const double c2 = T.use_charge ? charge_of(a, sl.target) : 1.0;
             double bound = cell_rate<D>(a, T, sl.cb_cell, sl.target, sl.adir, c1, c2) * sl.aspeed;
-            const double der = isnan(ewald_pre) ? ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
+            const double der = isnan(ewald_pre) ? ewald_derivative_call(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
             double real = T.par[3] * c1 * c2 * der * sl.aspeed;
             lift = false;
             if (real > 0) {
@@ -1165,10 +1179,10 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
             // event_handler_with_bounding_potential.py:75-101
             double q3[3] = {0, 0, 0};
             for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
-            const double c1 = T.use_charge ? a.charge[act] : 1.0;
-            const double c2 = T.use_charge ? a.charge[sl.target] : 1.0;
+            const double c1 = T.use_charge ? charge_of(a, act) : 1.0;
+            const double c2 = T.use_charge ? charge_of(a, sl.target) : 1.0;
             double bound = ipcb_derivative(T.bpar[0] * c1 * c2, q3[0], q3[1], q3[2]) * sl.aspeed;
-            const double der = isnan(ewald_pre) ? ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
+            const double der = isnan(ewald_pre) ? ewald_derivative_call(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
             double real = T.par[3] * c1 * c2 * der * sl.aspeed;
             lift = false;
             if (real > 0) {
@@ -1209,9 +1223,9 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
             separation_vector<D>(a, sl.apos, tp, s);
             double q3[3] = {0, 0, 0};
             for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
-            const double c1 = T.use_charge ? a.charge[act] : 1.0;
-            const double c2 = T.use_charge ? a.charge[tid2] : 1.0;
-            const double der = isnan(ewald_pre) ? ewald_derivative(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
+            const double c1 = T.use_charge ? charge_of(a, act) : 1.0;
+            const double c2 = T.use_charge ? charge_of(a, tid2) : 1.0;
+            const double der = isnan(ewald_pre) ? ewald_derivative_call(a.ewald, q3[0], q3[1], q3[2]) : ewald_pre;
             const double real = T.par[3] * c1 * c2 * der * sl.aspeed;
             if (real > 0) {
                 double u = words_to_random(ring_word(a, st.cursor), ring_word(a, st.cursor + 1));
@@ -1306,6 +1320,10 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
     if (a.lds_occn_off >= 0) {
         int32_t *lc = occn_base(a);
         for (int i = tid; i < a.ncells; i += NT) lc[i] = a.occ_n[i];
+    }
+    if (a.lds_charge_off >= 0) {
+        double *lq = reinterpret_cast<double *>(chain_dyn_lds + a.lds_charge_off);
+        for (int i = tid; i < a.n; i += NT) lq[i] = a.charge[i];
     }
     if (a.lds_occ_off >= 0) {
         int32_t *lo = occ_base(a);
@@ -1644,9 +1662,17 @@ static void plan_lds(ecmc_system *s) {
         a.lds_ring_off = (int32_t)used;
         used += ring_b;
     }
-    // the far-cell list of a cell-bounding tagger (n entries at most), the occupant counts and the
-    // occupant lists, each when it fits after the others
-    a.lds_cand_off = a.lds_occn_off = a.lds_occ_off = -1;
+    // the charges (read per candidate by the Coulomb taggers), the far-cell list of a
+    // cell-bounding tagger (n entries at most), the occupant counts and the occupant lists, each
+    // when it fits after the others
+    a.lds_cand_off = a.lds_occn_off = a.lds_occ_off = a.lds_charge_off = -1;
+    bool charged = false;
+    for (int t = 0; t < s->desc.n_taggers; t++) charged = charged || s->desc.taggers[t].use_charge;
+    const size_t charge_b = charged ? ((size_t)s->n * sizeof(double) + 15) & ~(size_t)15 : 0;
+    if (charge_b && used + charge_b <= budget) {
+        a.lds_charge_off = (int32_t)used;
+        used += charge_b;
+    }
     const size_t cand_b = s->d_cand ? ((size_t)s->n * sizeof(int32_t) + 15) & ~(size_t)15 : 0;
     if (cand_b && used + cand_b <= budget) {
         a.lds_cand_off = (int32_t)used;
@@ -1676,7 +1702,7 @@ static void plan_lds(ecmc_system *s) {
             a.lds_pos = 0;
             a.lds_ring = ring_b <= 65536 && ring_b <= budget ? 1 : 0;
             a.lds_ring_off = 0;
-            a.lds_cand_off = a.lds_occn_off = a.lds_occ_off = -1;
+            a.lds_cand_off = a.lds_occn_off = a.lds_occ_off = a.lds_charge_off = -1;
             s->lds_bytes = a.lds_ring ? ring_b : 0;
         }
     }
@@ -2277,7 +2303,7 @@ static int system_step(ecmc_system *s, int op, int tagger, ChainState &st) {
     if (op != 0 && (tagger < 0 || tagger >= s->desc.n_taggers)) return ecmc_fail(ECMC_ERR_ARGUMENT, "no such tagger");
     ChainArgs sa = s->args;   // one step works on global memory
     sa.lds_pos = sa.lds_ring = 0;
-    sa.lds_cand_off = sa.lds_occn_off = sa.lds_occ_off = -1;
+    sa.lds_cand_off = sa.lds_occn_off = sa.lds_occ_off = sa.lds_charge_off = -1;
     if (hipMemcpyAsync(s->d_args, &sa, sizeof(ChainArgs), hipMemcpyHostToDevice, s->stream) != hipSuccess)
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
     if (s->dim == 2) hipLaunchKernelGGL(chain_step_kernel<2>, dim3(1), dim3(NT), 0, s->stream, s->d_args, op, tagger, s->d_step_ev);

@@ -191,4 +191,38 @@ __device__ void ip_time_bounds(const double *s, int dir, float K, bool rep, int 
     ub = u == INF ? INF : up(u * isp_hi);
 }
 
+// FP32 brackets [lb, ub] of the time displacement of the inverse-power Coulomb bound
+// (ipcb_displacement, inverse_power_coulomb_bounding_potential.c:85-139) with prefactor magnitude
+// in [K_lo, K_hi], the squared perpendicular separation in [perp2_lo, perp2_hi], box length L
+// (half = L / 2) and potential change in [dU_lo, dU_hi].  The displacement is floor(dU / per) L
+// for the whole windings of the image sum, per = |pp| (1 / rho - 1 / sqrt(half^2 + rho^2)) the
+// rise over one winding, plus a remainder in [0, 3L/2]: the way to the start of the rising
+// flank (at most L) and the rise within it (at most half).  So lb = floor(q_lo) L and
+// ub = (floor(q_hi) + 3/2) L for q_lo <= dU / per <= q_hi, each with relative margins far above
+// the FP64 rounding of the exact inversion (2^-48 for per, whose cancellation is bounded since
+// rho <= half sqrt(2)).  An event is always certain to come, so lb stays finite (capped at 2^100,
+// which culls like any large bound).
+__device__ __forceinline__ void ipcb_time_bounds(float K_lo, float K_hi, float perp2_lo, float perp2_hi, float L,
+                                                 float dU_lo, float dU_hi, float isp_lo, float isp_hi, float &lb,
+                                                 float &ub) {
+    const float INF = __int_as_float(0x7f800000);
+    const float h = 0.5f * L;
+    const float h2_lo = dn(h * h), h2_hi = up(h * h);
+    // a = 1 / rho and b = 1 / sqrt(half^2 + rho^2), bracketed
+    const float a_hi = perp2_lo > 0x1p-100f ? up(rcp1(sqrt_dn(perp2_lo))) : INF;
+    const float a_lo = dn(rcp1(sqrt_up(perp2_hi)));
+    const float b_hi = up(rcp1(sqrt_dn(dn(h2_lo + perp2_lo))));
+    const float b_lo = dn(rcp1(sqrt_up(up(h2_hi + perp2_hi))));
+    const float d_lo = dn(a_lo - b_hi), d_hi = a_hi == INF ? INF : up(a_hi - b_lo);
+    const float per_lo = d_lo > 0.f ? dn(dn(K_lo * d_lo) * (1.f - 0x1p-20f)) : 0.f;
+    const float per_hi = d_hi == INF ? INF : up(up(K_hi * d_hi) * (1.f + 0x1p-20f));
+    const float q_lo = per_hi == INF ? 0.f : dn(dU_lo * dn(rcp1(per_hi)));
+    const float q_hi = per_lo > 0x1p-100f ? up(dU_hi * up(rcp1(per_lo))) : INF;
+    const float k_lo = floorf(q_lo), k_hi = floorf(q_hi);
+    const float l = k_lo > 0.f ? fminf(dn(dn(k_lo * L) * (1.f - 0x1p-20f)), 0x1p100f) : 0.f;
+    const float u = k_hi < 0x1p100f ? up(up((k_hi + 1.5f) * L) * (1.f + 0x1p-20f)) : INF;
+    lb = fminf(fmaxf(dn(l * isp_lo), 0.f), 0x1p100f);
+    ub = u == INF ? INF : up(u * isp_hi);
+}
+
 }  // namespace ecmc

@@ -29,11 +29,13 @@
 // the rarely taken double-double fallbacks: called, not inlined, so that their registers do
 // not add to those of the fast paths around every call site
 #define ECMC_SLOW static __device__ __noinline__
+#define ECMC_HD_SLOW static __host__ __device__ __noinline__
 #else
 #define ECMC_FN static inline
 #define ECMC_HD static inline
 #define ECMC_TABLE
 #define ECMC_SLOW static
+#define ECMC_HD_SLOW static
 #endif
 
 #include "ecmc_tables.h"
@@ -41,6 +43,10 @@
 namespace ecmc {
 
 // ---- Python float semantics ------------------------------------------------------------
+// fmod beyond one period (a reduction loop): called, not inlined, where the one-period paths
+// below leave it
+ECMC_HD_SLOW double fmod_call(double x, double y) { return fmod(x, y); }
+
 ECMC_HD double py_mod(double x, double y) {
     if (y > 0.0 && fabs(x) < 2.0 * y) {
         // within one period either way fmod's exact result is x, x - y or x + y (Sterbenz), and a
@@ -49,7 +55,7 @@ ECMC_HD double py_mod(double x, double y) {
         if (m < 0.0) m += y;
         return m == 0.0 ? 0.0 : m;
     }
-    double m = fmod(x, y);
+    double m = fmod_call(x, y);
     if (m != 0.0) {
         if ((y < 0) != (m < 0)) m += y;
     } else {
@@ -62,7 +68,7 @@ ECMC_HD double py_mod(double x, double y) {
 // 0 <= x < y, the exact x - y for y <= x < 2y (Sterbenz).
 ECMC_HD double fmod_near(double x, double y) {
     if (x >= 0.0 && y > 0.0 && x < 2.0 * y) return x < y ? x : x - y;
-    return fmod(x, y);
+    return fmod_call(x, y);
 }
 
 ECMC_HD void py_divmod(double vx, double wx, double &q, double &r) {
@@ -74,7 +80,7 @@ ECMC_HD void py_divmod(double vx, double wx, double &q, double &r) {
         r = vx - f;
         return;
     }
-    double mod = fmod(vx, wx);
+    double mod = fmod_call(vx, wx);
     double div = (vx - mod) / wx;
     if (mod != 0.0) {
         if ((wx < 0) != (mod < 0)) { mod += wx; div -= 1.0; }
@@ -274,11 +280,13 @@ ECMC_FN double log_fast(double x, double &lo) {
 }
 
 ECMC_SLOW double cr_log_slow(double x) { return dd_log(x).h; }
+// log of zero, negative, infinite, NaN and subnormal arguments
+ECMC_SLOW double log_special(double x) { return log(x); }
 
 // Correctly rounded natural logarithm: the fast path when its rounding is settled (all but
 // about one call in 500), else the double-double path.
 ECMC_FN double cr_log(double x) {
-    if (!(x > 0.0) || isinf(x) || x < 0x1p-1022) return log(x);
+    if (!(x > 0.0) || isinf(x) || x < 0x1p-1022) return log_special(x);
     if (x == 1.0) return 0.0;
     double lo;
     const double hi = log_fast(x, lo);

@@ -89,11 +89,12 @@ __global__ __launch_bounds__(PT) void pair_kernel(PairArgs a, int derivative) {
     pair_row<D>(a, i, derivative != 0);
 }
 
-// FP32 brackets of the time displacement of inverse-power rows (the chain kernel's candidate
-// filter), the potential change drawn from the leading word of the random pair.
+// FP32 brackets of the time displacement of inverse-power rows and of inverse-power Coulomb
+// bound rows (the chain kernel's candidate filter), the potential change drawn from the leading
+// word of the random pair.
 template <int D>
-__global__ __launch_bounds__(PT) void pair_bounds_kernel(int64_t n, int trunc, double power, double prefactor,
-                                                          double cutoff, const int32_t *dir, const double *speed,
+__global__ __launch_bounds__(PT) void pair_bounds_kernel(int64_t n, int trunc, int ipcb, double power, double prefactor,
+                                                          double cutoff, double L, const int32_t *dir, const double *speed,
                                                           const double *sep, const double *c1, const double *c2,
                                                           const uint32_t *word, double beta, float *lb, float *ub) {
     const int64_t i = (int64_t)blockIdx.x * PT + threadIdx.x;
@@ -106,10 +107,25 @@ __global__ __launch_bounds__(PT) void pair_bounds_kernel(int64_t n, int trunc, d
     const float ib = (float)(1.0 / beta), is = (float)(1.0 / (speed ? speed[i] : 1.0));
     const float rc2 = trunc ? (float)(cutoff * cutoff) : 0.f;
     float l = 0.f, u = __int_as_float(0x7f800000);
-    if (K >= 0x1p-100f && K <= 0x1p100f)
-        ip_time_bounds<D>(s, dir[i], K, pp > 0.0, (int)power, 2.f / (float)(int)power, trunc != 0, dn(rc2), up(rc2),
-                          dU_lower(word[i], ib * (1.f - 0x1p-22f)), dU_upper(word[i], ib * (1.f + 0x1p-22f)),
-                          is * (1.f - 0x1p-22f), is * (1.f + 0x1p-22f), l, u);
+    if (K >= 0x1p-100f && K <= 0x1p100f) {
+        const float dU_lo = dU_lower(word[i], ib * (1.f - 0x1p-22f)), dU_hi = dU_upper(word[i], ib * (1.f + 0x1p-22f));
+        if (ipcb) {
+            float lo2 = 0.f, hi2 = 0.f;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                if (d == dir[i]) continue;
+                const float m = fabsf((float)s[d]);
+                const float lo = fmaxf(dn(m) - 0x1p-110f, 0.f), hi = up(m) + 0x1p-110f;
+                lo2 += lo * lo;
+                hi2 += hi * hi;
+            }
+            ipcb_time_bounds(dn(K), up(K), dn(lo2), up(hi2), (float)L, dU_lo, dU_hi, is * (1.f - 0x1p-22f),
+                             is * (1.f + 0x1p-22f), l, u);
+        } else {
+            ip_time_bounds<D>(s, dir[i], K, pp > 0.0, (int)power, 2.f / (float)(int)power, trunc != 0, dn(rc2), up(rc2),
+                              dU_lo, dU_hi, is * (1.f - 0x1p-22f), is * (1.f + 0x1p-22f), l, u);
+        }
+    }
     lb[i] = l;
     ub[i] = u;
 }
@@ -178,22 +194,27 @@ extern "C" int ecmc_pair_time_bounds(int32_t potential, const double *par, int32
                                      const double *d_c1, const double *d_c2, const uint32_t *d_word, double beta,
                                      float *d_lb, float *d_ub, void *stream) {
     if (n < 0 || (dim != 2 && dim != 3) || !par || !(beta > 0.0)) return ecmc_fail(ECMC_ERR_ARGUMENT, "bad argument");
-    if (potential != POT_INVERSE_POWER && potential != POT_INVERSE_POWER_TRUNC)
-        return ecmc_fail(ECMC_ERR_CONFIGURATION, "time brackets exist for the inverse-power potentials only");
-    if (!(par[0] >= 1.0 && par[0] <= 32.0 && par[0] == floor(par[0])))
+    const int ipcb = potential == POT_INVERSE_POWER_COULOMB_BOUND;
+    if (potential != POT_INVERSE_POWER && potential != POT_INVERSE_POWER_TRUNC && !ipcb)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "time brackets exist for the inverse-power potentials and the Coulomb bound only");
+    if (ipcb && (dim != 3 || !(par[1] > 0.0)))
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "the Coulomb-bound brackets need three dimensions and par[1] = system length > 0");
+    if (!ipcb && !(par[0] >= 1.0 && par[0] <= 32.0 && par[0] == floor(par[0])))
         return ecmc_fail(ECMC_ERR_CONFIGURATION, "time brackets need an integer power from 1 to 32");
     if (n == 0) return ECMC_OK;
     if (!d_dir || !d_sep || !d_word || !d_lb || !d_ub) return ecmc_fail(ECMC_ERR_ARGUMENT, "null device pointer");
     const int trunc = potential == POT_INVERSE_POWER_TRUNC;
     const double cutoff = trunc ? par[2] : 0.0;
+    // the inverse-power rows: par = (power, prefactor[, cutoff]); the Coulomb bound: (prefactor, L)
+    const double power = ipcb ? 1.0 : par[0], prefactor = ipcb ? par[0] : par[1], L = ipcb ? par[1] : 0.0;
     dim3 grid((unsigned)((n + PT - 1) / PT));
     hipStream_t st = (hipStream_t)stream;
     if (dim == 2)
-        hipLaunchKernelGGL(pair_bounds_kernel<2>, grid, dim3(PT), 0, st, n, trunc, par[0], par[1], cutoff, d_dir, d_speed,
-                           d_sep, d_c1, d_c2, d_word, beta, d_lb, d_ub);
+        hipLaunchKernelGGL(pair_bounds_kernel<2>, grid, dim3(PT), 0, st, n, trunc, ipcb, power, prefactor, cutoff, L, d_dir,
+                           d_speed, d_sep, d_c1, d_c2, d_word, beta, d_lb, d_ub);
     else
-        hipLaunchKernelGGL(pair_bounds_kernel<3>, grid, dim3(PT), 0, st, n, trunc, par[0], par[1], cutoff, d_dir, d_speed,
-                           d_sep, d_c1, d_c2, d_word, beta, d_lb, d_ub);
+        hipLaunchKernelGGL(pair_bounds_kernel<3>, grid, dim3(PT), 0, st, n, trunc, ipcb, power, prefactor, cutoff, L, d_dir,
+                           d_speed, d_sep, d_c1, d_c2, d_word, beta, d_lb, d_ub);
     if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "bounds kernel launch failed");
     return ECMC_OK;
 }

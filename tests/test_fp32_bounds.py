@@ -114,3 +114,78 @@ def test_gpu_pair_time_brackets_hold_the_exact_displacement(device, dim):
                 checked += int(ok.sum())
                 certain += int(fin.sum())
     assert checked > 20 * n and certain > n
+
+
+@pytest.mark.gpu
+def test_gpu_coulomb_bound_time_brackets_hold_the_exact_displacement(device):
+    """ecmc_pair_time_bounds for the inverse-power Coulomb bound (the chain kernel's filter of the
+    merged-image Coulomb candidates) against ecmc_pair_displacement's exact inversion on the same
+    rows and the same random words: lb <= t <= ub on every row, lb always finite (an event always
+    comes).  Rows over the whole box and near the special cases: the target on the line of motion
+    (rho -> 0), at the corner (rho -> half sqrt 2), level with the active unit along the motion and
+    half a box away; both signs, two box lengths, two speeds.  The brackets must resolve the
+    windings: where the exact count floor(dU / per) is at least 2, lb > 0."""
+    import ctypes
+    import torch
+    rng = np.random.default_rng(77)
+    n = 200000
+    dev = lambda a, dt: torch.tensor(a, dtype=dt, device="cuda").contiguous()
+    checked = resolved = 0
+    for L in (8.0, 21.5):
+        for prefactor in (1.5837, 0.4):
+            print("Coulomb-bound brackets: L %g, prefactor %g" % (L, prefactor), flush=True)
+            h = L / 2
+            sep = rng.uniform(-h, h, (n, 3))
+            d = rng.integers(0, 3, n).astype(np.int32)
+            rows = np.arange(n)
+            k = rng.integers(0, 6, n)
+            for q in range(3):
+                sel = (k == 0) & (d != q)
+                sep[rows[sel], q] *= 1e-7                                   # on the line of motion
+                sel = (k == 1) & (d != q)
+                sep[rows[sel], q] = np.sign(sep[rows[sel], q]) * h * (1 - rng.uniform(0, 1e-9, sel.sum()))  # corner
+            sel = k == 2
+            sep[rows[sel], d[sel]] = rng.normal(0, 1e-9, sel.sum())          # level along the motion
+            sel = k == 3
+            sep[rows[sel], d[sel]] = -h * (1 - rng.uniform(0, 1e-12, sel.sum()))   # half a box away
+            speed = np.where(rng.random(n) < 0.5, 1.0, 0.7)
+            c1 = rng.choice([1.0, -1.0, 0.5, -0.5], n)
+            c2 = rng.choice([1.0, -1.0, 0.5], n)
+            w = rng.integers(0, 2 ** 32, (n, 2), dtype=np.uint64).astype(np.uint32)
+            beta = float(rng.uniform(0.5, 2.0))
+            u = ((w[:, 0] >> 5).astype(np.float64) * 67108864.0 + (w[:, 1] >> 6)) * (1.0 / 9007199254740992.0)
+            dU = -np.log(1.0 - u) / beta
+            par = np.array([prefactor, L, 0.0, 0.0])
+            pptr = par.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+            t_dir, t_speed, t_sep = dev(d, torch.int32), dev(speed, torch.float64), dev(sep, torch.float64)
+            t_c1, t_c2 = dev(c1, torch.float64), dev(c2, torch.float64)
+            t_word = dev(w[:, 0].view(np.int32), torch.int32)
+            disp = torch.empty(n, dtype=torch.float64, device="cuda")
+            lb = torch.empty(n, dtype=torch.float32, device="cuda")
+            ub = torch.empty(n, dtype=torch.float32, device="cuda")
+            device.check(device.lib().ecmc_pair_displacement(device.POT_INVERSE_POWER_COULOMB_BOUND, pptr, 3, n,
+                                                             device.dptr(t_dir), device.dptr(t_speed), device.dptr(t_sep),
+                                                             device.dptr(t_c1), device.dptr(t_c2),
+                                                             device.dptr(dev(dU, torch.float64)), L, device.dptr(disp), None))
+            device.check(device.lib().ecmc_pair_time_bounds(device.POT_INVERSE_POWER_COULOMB_BOUND, pptr, 3, n,
+                                                            device.dptr(t_dir), device.dptr(t_speed), device.dptr(t_sep),
+                                                            device.dptr(t_c1), device.dptr(t_c2), device.dptr(t_word), beta,
+                                                            device.dptr(lb), device.dptr(ub), None))
+            torch.cuda.synchronize()
+            t, lo, hi = disp.cpu().numpy(), lb.cpu().numpy().astype(np.float64), ub.cpu().numpy().astype(np.float64)
+            assert np.all(np.isfinite(lo))
+            ok = ~np.isnan(t)
+            assert ok.sum() > 0.99 * n
+            t, lo, hi = t[ok], lo[ok], hi[ok]
+            assert np.all(lo <= t), (L, prefactor, int(np.sum(lo > t)))
+            assert np.all(t <= hi), (L, prefactor, int(np.sum(t > hi)))
+            # the windings: floor(dU / per) >= 2 must give lb > 0
+            rho = np.sqrt(np.sum(sep ** 2, axis=1) - sep[rows, d] ** 2)[ok]
+            pp = np.abs(prefactor * c1 * c2)[ok]
+            with np.errstate(divide="ignore"):
+                per = pp * (1 / rho - 1 / np.sqrt(h * h + rho * rho))
+            wind = np.floor(dU[ok] / per) >= 2
+            assert np.all(lo[wind] > 0), (L, prefactor, int(np.sum(lo[wind] == 0)))
+            checked += int(ok.sum())
+            resolved += int(wind.sum())
+    assert checked > 3 * n and resolved > n
