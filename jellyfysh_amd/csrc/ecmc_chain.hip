@@ -314,22 +314,44 @@ __device__ __forceinline__ bool is_nearby(const ChainArgs &a, int dim, int c, in
     return true;
 }
 
+// Kernel variants by the features of the systems they run (FM): bit p for the two-leaf pair
+// potential ECMC_POT_p inverted exactly, FEAT_COULOMB for the Coulomb machinery (the bounding
+// event handlers with their thinning by the Ewald derivative, cell bounding, the cell veto) and
+// FEAT_FMAP for factor-type maps.  A variant leaves the code of the other features out of the
+// kernel (a smaller leg loop, fewer instruction-cache misses); the host launches the first
+// variant that covers every feature of its systems.
+constexpr int FEAT_COULOMB = 1 << 9, FEAT_FMAP = 1 << 10, FEAT_ALL = (1 << 11) - 1;
+constexpr int FEAT_IP_FILTER = (1 << ECMC_POT_INVERSE_POWER) | (1 << ECMC_POT_INVERSE_POWER_TRUNC);
+constexpr int VARIANT_LJ = FEAT_IP_FILTER;
+constexpr int VARIANT_CHARGED = (1 << ECMC_POT_INVERSE_POWER) | FEAT_COULOMB;
+constexpr int VARIANT_DIPOLES = (1 << ECMC_POT_INVERSE_POWER) | (1 << ECMC_POT_DISPLACED_EVEN_POWER) | FEAT_COULOMB | FEAT_FMAP;
+#define POT_BUILT(p) ((FM >> (p)) & 1)
+
 // Candidate time displacement of one pair (send_event_time of the pair event handlers).
-template <int D>
+template <int D, int FM>
 __device__ __forceinline__ double pair_dt(const ChainArgs &a, const ecmc_tagger_desc &tg, int dir, double speed, const double *s,
                           double c1, double c2, double dU) {
     if (tg.event_handler == ECMC_EH_TWO_LEAF) {
         switch (tg.potential) {
-        case ECMC_POT_HARD_SPHERE: return hs_displacement<D>(dir, speed, s, tg.par[0]);
-        case ECMC_POT_HARD_DIPOLE: return hd_displacement<D>(dir, speed, s, tg.par[0], tg.par[1]);
-        case ECMC_POT_INVERSE_POWER: return ip_displacement<D>(dir, speed, s, c1, c2, tg.par[0], tg.par[1], dU);
+        case ECMC_POT_HARD_SPHERE:
+            if (!POT_BUILT(ECMC_POT_HARD_SPHERE)) return NAN;
+            return hs_displacement<D>(dir, speed, s, tg.par[0]);
+        case ECMC_POT_HARD_DIPOLE:
+            if (!POT_BUILT(ECMC_POT_HARD_DIPOLE)) return NAN;
+            return hd_displacement<D>(dir, speed, s, tg.par[0], tg.par[1]);
+        case ECMC_POT_INVERSE_POWER:
+            if (!POT_BUILT(ECMC_POT_INVERSE_POWER)) return NAN;
+            return ip_displacement<D>(dir, speed, s, c1, c2, tg.par[0], tg.par[1], dU);
         case ECMC_POT_INVERSE_POWER_TRUNC:
+            if (!POT_BUILT(ECMC_POT_INVERSE_POWER_TRUNC)) return NAN;
             return ipt_displacement<D>(dir, speed, s, c1, c2, tg.par[0], tg.par[1], tg.par[2], dU);
         case ECMC_POT_LENNARD_JONES: {
+            if (!POT_BUILT(ECMC_POT_LENNARD_JONES)) return NAN;
             MexicanHat h = mh_lennard_jones(tg.par[0], tg.par[1]);
             return mh_displacement<D>(h, dir, speed, s, dU);
         }
         case ECMC_POT_DISPLACED_EVEN_POWER: {
+            if (!POT_BUILT(ECMC_POT_DISPLACED_EVEN_POWER)) return NAN;
             MexicanHat h = mh_displaced_even_power(tg.par[0], tg.par[1], tg.par[2]);
             return mh_displacement<D>(h, dir, speed, s, dU);
         }
@@ -337,6 +359,7 @@ __device__ __forceinline__ double pair_dt(const ChainArgs &a, const ecmc_tagger_
         }
     }
     // two_leaf_unit_bounding_potential_event_handler.py:138-143 with the Coulomb bound
+    if (!(FM & FEAT_COULOMB)) return NAN;
     double p[3] = {0, 0, 0};
 #pragma unroll
     for (int d = 0; d < 3; d++) p[d] = d < D ? s[(dir + d) % D] : 0.0;
@@ -422,7 +445,7 @@ __device__ __forceinline__ bool fp32_filter_applies(const ecmc_tagger_desc &tg) 
     return p >= 1.0 && p <= 32.0 && p == floor(p);
 }
 
-template <int D>
+template <int D, int FM>
 __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &tg, int tix,
                             int *nb_cell, int *nb_pref, Best *sb, uint32_t *sh, int *scan) {
     const int tid = threadIdx.x;
@@ -461,8 +484,8 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
     const int nc = nb_cell[31];
     const bool all = tg.kind == ECMC_TAG_PAIR_ALL;
     const bool sur = tg.kind == ECMC_TAG_SURPLUS;
-    const bool far = tg.kind == ECMC_TAG_CELL_BOUNDING;
-    const bool fmap = tg.kind == ECMC_TAG_FACTOR_MAP;
+    const bool far = (FM & FEAT_COULOMB) && tg.kind == ECMC_TAG_CELL_BOUNDING;
+    const bool fmap = (FM & FEAT_FMAP) && tg.kind == ECMC_TAG_FACTOR_MAP;
     // the far cells of this create (built and read within it): in LDS when the launch has room
     int32_t *const cand = a.lds_cand_off >= 0 ? reinterpret_cast<int32_t *>(chain_dyn_lds + a.lds_cand_off) : a.cand;
     if (far) {
@@ -542,7 +565,7 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
     // inverted exactly as before, so the result is the unfiltered one, unless a candidate left
     // out would have raised the reference's ValueError (a negative square root from rounding),
     // which the exact inversion flags.
-    const bool filt = fp32_filter_applies(tg) && M > 1;
+    const bool filt = (FM & FEAT_IP_FILTER) && fp32_filter_applies(tg) && M > 1;
     float lbk0 = 0.f, lbk1 = 0.f;
     double cull_above = INFINITY;
     const int fp = (int)tg.par[0];
@@ -632,13 +655,13 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
         }
         const double ct = tg.use_charge ? charge_of(a, target) : 1.0;
         double dt;
-        if (tg.event_handler == ECMC_EH_CELL_BOUNDING) {
+        if ((FM & FEAT_COULOMB) && tg.event_handler == ECMC_EH_CELL_BOUNDING) {
             // two_leaf_unit_cell_bounding_potential_event_handler.py:171-177
             const double rate = far ? cell_rate_of_cells<D>(a, tg, st.occ_cell, (int)cand[j], dir, ca, ct)
                                     : cell_rate<D>(a, tg, st.occ_cell, target, dir, ca, ct);
             dt = (rate > 0 ? dU / rate : INFINITY) / speed;
         } else {
-            dt = pair_dt<D>(a, tg, dir, speed, s, ca, ct, dU);
+            dt = pair_dt<D, FM>(a, tg, dir, speed, s, ca, ct, dU);
         }
         Time t = time_add(at, dt);
         Best c = {t.q, t.r, j};
@@ -1080,14 +1103,14 @@ struct LegShared {
 // The event handlers of tagger t send their candidate event times: the pair taggers' candidate
 // sets over the workgroup, the cell veto and the end of chain with their bounded ring refills,
 // the others on thread 0 (collective; tag_activator.py:228-289 for one tag of a create list).
-template <int D>
+template <int D, int FM>
 __device__ __forceinline__ void create_tagger(const ChainArgs &a, ChainState &st, uint32_t *mt, const ecmc_tagger_desc &c, int t,
                               LegShared &ls) {
     const int tid = threadIdx.x;
     if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL || c.kind == ECMC_TAG_SURPLUS ||
         c.kind == ECMC_TAG_CELL_BOUNDING || c.kind == ECMC_TAG_FACTOR_MAP) {
-        create_pair<D>(a, st, mt, c, t, ls.nb_cell, ls.nb_pref, ls.sb, ls.sh, ls.scan);
-    } else if (c.kind == ECMC_TAG_CELL_VETO) {
+        create_pair<D, FM>(a, st, mt, c, t, ls.nb_cell, ls.nb_pref, ls.sb, ls.sh, ls.scan);
+    } else if ((FM & FEAT_COULOMB) && c.kind == ECMC_TAG_CELL_VETO) {
         if (tid == 0) { st.slot[t].valid = 0; st.slot[t].cv_phase = 0; }
         __syncthreads();
         // the same bounded refill loop as the end-of-chain randint below
@@ -1132,7 +1155,7 @@ __device__ __forceinline__ bool commit_thins(const ecmc_tagger_desc &T) {
 // The out-state of tagger w's stored event, committed to the state (thread 0; the caller made
 // two ring words available when the event thins): send_out_state of its event handler and
 // insert_into_global_state (mediator/single_process_mediator.py:130-150).
-template <int D>
+template <int D, int FM>
 __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st, const ecmc_tagger_desc &T, int w, ecmc_event &ev,
                              double ewald_pre) {
     Slot &sl = st.slot[w];
@@ -1156,7 +1179,7 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
         bool lift = true;
         double s[D];
         separation_vector<D>(a, sl.apos, sl.tpos, s);
-        if (T.event_handler == ECMC_EH_CELL_BOUNDING) {
+        if ((FM & FEAT_COULOMB) && T.event_handler == ECMC_EH_CELL_BOUNDING) {
             // two_leaf_unit_cell_bounding_potential_event_handler.py:181-211: the active
             // unit is still in its cell; bound = the candidate's rate; thinning as below
             if (pos_to_cell<D>(a, sl.apos) != sl.cb_cell) { st.err = ECMC_ERR_STATE; break; }
@@ -1175,7 +1198,7 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
                 if (u < real) lift = true;
             }
             if (!lift) st.est[6] += 1.0;
-        } else if (T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
+        } else if ((FM & FEAT_COULOMB) && T.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) {
             // event_handler_with_bounding_potential.py:75-101
             double q3[3] = {0, 0, 0};
             for (int d = 0; d < D; d++) q3[d] = s[(sl.adir + d) % D];
@@ -1205,6 +1228,7 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
         break;
     }
     case ECMC_TAG_CELL_VETO: {
+        if (!(FM & FEAT_COULOMB)) { st.err = ECMC_ERR_CONFIGURATION; break; }
         // mediator.py:265-292: the occupant of the sampled cell, or none;
         // leaf_unit_cell_veto_event_handler.py:127-149 with
         // event_handler_with_bounding_potential.py:75-101: thinning of the stored rate
@@ -1299,7 +1323,7 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
 }
 
 // One workgroup runs one system's legs (the whole leg loop of the reference's mediator).
-template <int D>
+template <int D, int FM>
 __device__ __forceinline__ void chain_body(const ChainArgs &a) {
     __shared__ ChainState st;
     __shared__ ecmc_tagger_desc tg[ECMC_MAX_TAGGERS];
@@ -1361,7 +1385,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
             const ecmc_tagger_desc &w = tg[st.last_tagger];
             for (int ci = 0; ci < w.n_create && !st.err; ci++) {
                 const int t = w.create[ci];
-                create_tagger<D>(a, st, mt, tg[t], t, ls);
+                create_tagger<D, FM>(a, st, mt, tg[t], t, ls);
             }
             if (tid == 0) st.pending_creates = 0;
             __syncthreads();
@@ -1385,13 +1409,13 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
         const int w = flag;
         if (commit_thins(tg[w])) ring_ensure(st, mt, a, 2);
         double ewald_pre = NAN;
-        if (tid < 64) {
+        if ((FM & FEAT_COULOMB) && tid < 64) {
             double q3[3];
             if (thinning_separation<D>(a, st, tg[w], w, q3)) ewald_pre = ewald_derivative_wave(a.ewald, a.ewald_codes, a.ewald_ncodes, q3[0], q3[1], q3[2]);
         }
         if (tid == 0) {
             ecmc_event ev;
-            commit_event<D>(a, st, tg[w], w, ev, ewald_pre);
+            commit_event<D, FM>(a, st, tg[w], w, ev, ewald_pre);
             const ecmc_tagger_desc &T = tg[w];
             // trash by tag
             for (int i = 0; i < T.n_trash; i++) st.slot[T.trash[i]].valid = 0;
@@ -1430,9 +1454,9 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
 }
 
 // One system; its arguments in device memory (s->d_args), read at a uniform address.
-template <int D>
+template <int D, int FM>
 __global__ __launch_bounds__(NT) void chain_kernel(const ChainArgs *__restrict__ args) {
-    chain_body<D>(*args);
+    chain_body<D, FM>(*args);
 }
 
 // One step of a leg driven from the host, for the set-level event handler (the mediator and
@@ -1442,7 +1466,7 @@ __global__ __launch_bounds__(NT) void chain_kernel(const ChainArgs *__restrict__
 //   op 1  send_event_time of tagger t's handlers: its candidates' times, the earliest kept in
 //         slot t (time, target, in-state copies);
 //   op 2  send_out_state of tagger t's stored event, committed to the state; *out receives it.
-template <int D>
+template <int D, int FM>
 __global__ __launch_bounds__(NT) void chain_step_kernel(const ChainArgs *__restrict__ args, int op, int t,
                                                         ecmc_event *out) {
     const ChainArgs &a = *args;
@@ -1466,17 +1490,17 @@ __global__ __launch_bounds__(NT) void chain_step_kernel(const ChainArgs *__restr
         // the occupancy follows the active unit here too (a no-op after op 0 in the same leg), so
         // that a cell tagger never reads the cell of a unit that stopped being active
         if (a.occupancy && st.active >= 0) occupancy_update<D>(a, st, ls.scan);
-        create_tagger<D>(a, st, mt, tg[t], t, ls);
+        create_tagger<D, FM>(a, st, mt, tg[t], t, ls);
     } else {
         if (commit_thins(tg[t])) ring_ensure(st, mt, a, 2);
         double ewald_pre = NAN;
-        if (tid < 64) {
+        if ((FM & FEAT_COULOMB) && tid < 64) {
             double q3[3];
             if (thinning_separation<D>(a, st, tg[t], t, q3)) ewald_pre = ewald_derivative_wave(a.ewald, a.ewald_codes, a.ewald_ncodes, q3[0], q3[1], q3[2]);
         }
         if (tid == 0) {
             ecmc_event ev;
-            commit_event<D>(a, st, tg[t], t, ev, ewald_pre);
+            commit_event<D, FM>(a, st, tg[t], t, ev, ewald_pre);
             st.last = st.slot[t].t;
             *out = ev;
         }
@@ -1488,9 +1512,9 @@ __global__ __launch_bounds__(NT) void chain_step_kernel(const ChainArgs *__restr
 
 // Independent systems in one launch: workgroup b runs system b (its own positions, cell
 // occupancy, random stream and state), exactly as chain_kernel runs it alone.
-template <int D>
+template <int D, int FM>
 __global__ __launch_bounds__(NT) void chain_many_kernel(const ChainArgs *__restrict__ args) {
-    chain_body<D>(args[blockIdx.x]);   // the arguments stay in global memory, read at a uniform address
+    chain_body<D, FM>(args[blockIdx.x]);   // the arguments stay in global memory, read at a uniform address
 }
 
 // InnerPointEstimator.derivative_bound (inner_point_estimator.py:108-171) for the cell
@@ -1631,9 +1655,55 @@ struct ecmc_system {
     int32_t stop_mask = 0;
     int device = 0;          // the device current at creation, which holds its memory
     size_t lds_bytes = 0;    // dynamic LDS of a launch (positions and ring resident there)
+    int features = FEAT_ALL; // the kernel features the system uses (FEAT_*, potential bits)
     ChainArgs args;
     ChainArgs *d_args = nullptr;   // the arguments of the system's own launches, in device memory
 };
+
+// The kernel features a system's taggers use.
+static int system_features(const ecmc_system_desc &d) {
+    int f = 0;
+    for (int t = 0; t < d.n_taggers; t++) {
+        const ecmc_tagger_desc &tg = d.taggers[t];
+        const bool pair = tg.kind == ECMC_TAG_PAIR_CELLS || tg.kind == ECMC_TAG_PAIR_ALL || tg.kind == ECMC_TAG_SURPLUS ||
+                          tg.kind == ECMC_TAG_CELL_BOUNDING || tg.kind == ECMC_TAG_FACTOR_MAP;
+        if (pair) {
+            if (tg.event_handler == ECMC_EH_TWO_LEAF && tg.potential >= 0 && tg.potential < 9) f |= 1 << tg.potential;
+            else f |= FEAT_COULOMB;
+            if (tg.kind == ECMC_TAG_FACTOR_MAP) f |= FEAT_FMAP;
+            if (tg.kind == ECMC_TAG_CELL_BOUNDING) f |= FEAT_COULOMB;
+        }
+        if (tg.kind == ECMC_TAG_CELL_VETO) f |= FEAT_COULOMB;
+    }
+    return f;
+}
+
+// The kernel variants of the leg loop, in the order the host tries them (three dimensions; two
+// dimensions run the full kernel): the first whose features cover a launch's runs it.
+struct ChainVariant {
+    int features;
+    const void *one, *many;   // chain_kernel, chain_many_kernel
+};
+static const ChainVariant CHAIN_VARIANTS_3D[] = {
+    {VARIANT_LJ, reinterpret_cast<const void *>(&chain_kernel<3, VARIANT_LJ>),
+     reinterpret_cast<const void *>(&chain_many_kernel<3, VARIANT_LJ>)},
+    {VARIANT_CHARGED, reinterpret_cast<const void *>(&chain_kernel<3, VARIANT_CHARGED>),
+     reinterpret_cast<const void *>(&chain_many_kernel<3, VARIANT_CHARGED>)},
+    {VARIANT_DIPOLES, reinterpret_cast<const void *>(&chain_kernel<3, VARIANT_DIPOLES>),
+     reinterpret_cast<const void *>(&chain_many_kernel<3, VARIANT_DIPOLES>)},
+    {FEAT_ALL, reinterpret_cast<const void *>(&chain_kernel<3, FEAT_ALL>),
+     reinterpret_cast<const void *>(&chain_many_kernel<3, FEAT_ALL>)},
+};
+static const ChainVariant CHAIN_VARIANTS_2D[] = {
+    {FEAT_ALL, reinterpret_cast<const void *>(&chain_kernel<2, FEAT_ALL>),
+     reinterpret_cast<const void *>(&chain_many_kernel<2, FEAT_ALL>)},
+};
+static const ChainVariant &chain_variant(int dim, int features) {
+    if (dim == 2) return CHAIN_VARIANTS_2D[0];
+    for (const ChainVariant &v : CHAIN_VARIANTS_3D)
+        if ((features & ~v.features) == 0) return v;
+    return CHAIN_VARIANTS_3D[sizeof(CHAIN_VARIANTS_3D) / sizeof(CHAIN_VARIANTS_3D[0]) - 1];
+}
 
 // Dynamic LDS of the leg loop for a system of n units in dim dimensions with a ring of
 // ring_words words: the positions first, then the ring, each only while the whole fits next to
@@ -1643,8 +1713,7 @@ static void plan_lds(ecmc_system *s) {
     hipDeviceGetAttribute(&dev_max, hipDeviceAttributeMaxSharedMemoryPerBlock, s->device);
     hipFuncAttributes fa;
     size_t stat = 16384;
-    const hipError_t e = s->dim == 2 ? hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&chain_kernel<2>))
-                                     : hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&chain_kernel<3>));
+    const hipError_t e = hipFuncGetAttributes(&fa, chain_variant(s->dim, FEAT_ALL).one);
     if (e == hipSuccess) stat = fa.sharedSizeBytes;
     const size_t budget = dev_max > (int)stat ? (size_t)dev_max - stat : 0;
     const size_t pos_b = ((size_t)s->dim * s->n * sizeof(double) + 15) & ~(size_t)15;
@@ -1690,13 +1759,16 @@ static void plan_lds(ecmc_system *s) {
     }
     s->lds_bytes = used;
     if (used > 65536) {
-        const hipError_t r = s->dim == 2
-            ? hipFuncSetAttribute(reinterpret_cast<const void *>(&chain_kernel<2>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)used)
-            : hipFuncSetAttribute(reinterpret_cast<const void *>(&chain_kernel<3>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)used);
-        const hipError_t r2 = s->dim == 2
-            ? hipFuncSetAttribute(reinterpret_cast<const void *>(&chain_many_kernel<2>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)used)
-            : hipFuncSetAttribute(reinterpret_cast<const void *>(&chain_many_kernel<3>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)used);
-        if (r != hipSuccess || r2 != hipSuccess) {
+        // every variant may run the system (a launch of several systems takes the variant that
+        // covers all of them)
+        bool ok = true;
+        const ChainVariant *vs = s->dim == 2 ? CHAIN_VARIANTS_2D : CHAIN_VARIANTS_3D;
+        const size_t nv = s->dim == 2 ? sizeof(CHAIN_VARIANTS_2D) / sizeof(ChainVariant) : sizeof(CHAIN_VARIANTS_3D) / sizeof(ChainVariant);
+        for (size_t i = 0; i < nv; i++) {
+            ok = ok && hipFuncSetAttribute(vs[i].one, hipFuncAttributeMaxDynamicSharedMemorySize, (int)used) == hipSuccess;
+            ok = ok && hipFuncSetAttribute(vs[i].many, hipFuncAttributeMaxDynamicSharedMemorySize, (int)used) == hipSuccess;
+        }
+        if (!ok) {
             // keep within the default limit: the ring alone, or nothing
             (void)hipGetLastError();
             a.lds_pos = 0;
@@ -1847,6 +1919,7 @@ extern "C" int ecmc_system_create(const ecmc_system_desc *desc, const double *po
     }
     ecmc_system *s = new ecmc_system();
     s->desc = *desc;
+    s->features = system_features(*desc);
     if (hipGetDevice(&s->device) != hipSuccess) {
         delete s;
         return ecmc_fail(ECMC_ERR_HIP, "no current HIP device");
@@ -2167,8 +2240,10 @@ extern "C" int ecmc_system_run(ecmc_system *s, int64_t max_events, ecmc_event *h
     hipEventRecord(s->ev0, s->stream);
     if (hipMemcpyAsync(s->d_args, &a, sizeof(ChainArgs), hipMemcpyHostToDevice, s->stream) != hipSuccess)
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
-    if (s->dim == 2) hipLaunchKernelGGL(chain_kernel<2>, dim3(1), dim3(NT), s->lds_bytes, s->stream, s->d_args);
-    else hipLaunchKernelGGL(chain_kernel<3>, dim3(1), dim3(NT), s->lds_bytes, s->stream, s->d_args);
+    {
+        void *kargs[] = {&s->d_args};
+        hipLaunchKernel(chain_variant(s->dim, s->features).one, dim3(1), dim3(NT), kargs, s->lds_bytes, s->stream);
+    }
     hipEventRecord(s->ev1, s->stream);
     if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
     if (hipStreamSynchronize(s->stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
@@ -2208,8 +2283,10 @@ static int systems_run_staged(ecmc_system *const *systems, int32_t count, int64_
     if (hipMemcpyAsync(d_args, host_args, sizeof(ChainArgs) * count, hipMemcpyHostToDevice, s0->stream) != hipSuccess)
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
     hipEventRecord(s0->ev0, s0->stream);
-    if (dim == 2) hipLaunchKernelGGL(chain_many_kernel<2>, dim3(count), dim3(NT), lds, s0->stream, d_args);
-    else hipLaunchKernelGGL(chain_many_kernel<3>, dim3(count), dim3(NT), lds, s0->stream, d_args);
+    int features = 0;
+    for (int i = 0; i < count; i++) features |= systems[i]->features;
+    void *kargs[] = {&d_args};
+    hipLaunchKernel(chain_variant(dim, features).many, dim3(count), dim3(NT), kargs, lds, s0->stream);
     hipEventRecord(s0->ev1, s0->stream);
     const bool launched = hipGetLastError() == hipSuccess;
     const bool ran = hipStreamSynchronize(s0->stream) == hipSuccess;
@@ -2306,8 +2383,8 @@ static int system_step(ecmc_system *s, int op, int tagger, ChainState &st) {
     sa.lds_cand_off = sa.lds_occn_off = sa.lds_occ_off = sa.lds_charge_off = -1;
     if (hipMemcpyAsync(s->d_args, &sa, sizeof(ChainArgs), hipMemcpyHostToDevice, s->stream) != hipSuccess)
         return ecmc_fail(ECMC_ERR_HIP, "hipMemcpy of the launch arguments failed");
-    if (s->dim == 2) hipLaunchKernelGGL(chain_step_kernel<2>, dim3(1), dim3(NT), 0, s->stream, s->d_args, op, tagger, s->d_step_ev);
-    else hipLaunchKernelGGL(chain_step_kernel<3>, dim3(1), dim3(NT), 0, s->stream, s->d_args, op, tagger, s->d_step_ev);
+    if (s->dim == 2) hipLaunchKernelGGL((chain_step_kernel<2, FEAT_ALL>), dim3(1), dim3(NT), 0, s->stream, s->d_args, op, tagger, s->d_step_ev);
+    else hipLaunchKernelGGL((chain_step_kernel<3, FEAT_ALL>), dim3(1), dim3(NT), 0, s->stream, s->d_args, op, tagger, s->d_step_ev);
     if (hipGetLastError() != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel launch failed");
     if (hipStreamSynchronize(s->stream) != hipSuccess) return ecmc_fail(ECMC_ERR_HIP, "kernel execution failed");
     if (hipMemcpy(&st, s->d_st, sizeof(ChainState), hipMemcpyDeviceToHost) != hipSuccess)
