@@ -148,9 +148,9 @@ def plain_chain_rate(legs, pressure_seconds, equilibration_legs):
                events=int(done), kernel_ms=ms, wall_s=wall, events_per_sec=done / (ms * 1e-3))
     lds_b, lds_p, lds_r = chain.launch_lds()
     out["launch_lds"] = dict(bytes=lds_b, positions=lds_p, ring=lds_r)
-    out["roofline"] = plain_roofline("ecmc::chain_kernel<3>", ncand, 3, 27, done, ms, charged=False)
-    out["roofline"]["traffic"] = profiled_traffic("ecmc::chain_kernel<3>", legs,
-                                                  "chain_kernel_plain_lj_4096_counters.txt", "plain")
+    kname = "ecmc::chain_kernel<3,%d>" % chain.kernel_variant()[1]   # the specialized leg loop that ran
+    out["roofline"] = plain_roofline(kname, ncand, 3, 27, done, ms, charged=False)
+    out["roofline"]["traffic"] = profiled_traffic(kname, legs, "chain_kernel_plain_lj_4096_counters.txt", "plain")
     out["cpu_single_core"] = plain_cpu_single_core(desc, chain.positions(), seconds=4.0)
     # pressure: chunks of `legs` legs for `pressure_seconds`; the standard error comes from
     # 8 blocks of consecutive chunks (chunks of one chain are correlated)
@@ -271,6 +271,7 @@ def charged_spheres_rate(replicas, legs, far="cell_bounding"):
     k1, ms1 = run_together(chains[:1], legs)
     est = np.array([c.estimators() for c in chains])
     lds_b, lds_p, lds_r = chains[0].launch_lds()
+    kname = "ecmc::chain_many_kernel<3,%d>" % chains[0].kernel_variant()[1]
     for c in chains:
         c.close()
     # candidates per leg from a logged run of one more replica of the same system
@@ -279,9 +280,8 @@ def charged_spheres_rate(replicas, legs, far="cell_bounding"):
     probe = make_chain(sysd)
     ncand = float(probe.run(500)["ncand"].mean())
     probe.close()
-    roof = plain_roofline("ecmc::chain_many_kernel<3>", ncand, 3, 27, int(k.sum()), ms, charged=True)
-    roof["traffic"] = profiled_traffic("ecmc::chain_many_kernel<3>", legs,
-                                       "chain_many_kernel_charged_soft_spheres_counters.txt", far)
+    roof = plain_roofline(kname, ncand, 3, 27, int(k.sum()), ms, charged=True)
+    roof["traffic"] = profiled_traffic(kname, legs, "chain_many_kernel_charged_soft_spheres_counters.txt", far)
     from jellyfysh_amd.chain import make_desc
     desc = make_desc(3, 512, sysd["system_length"], sysd["beta"], sysd["cells_per_side"], sysd["taggers"], 999,
                      scheduler="heap", max_occupants=1)
@@ -320,14 +320,15 @@ def flexible_dipoles_rate(replicas, legs):
     wall = time.perf_counter() - t0
     k1, ms1 = run_together(chains[:1], legs)
     lds_b, lds_p, lds_r = chains[0].launch_lds()
+    kname = "ecmc::chain_many_kernel<3,%d>" % chains[0].kernel_variant()[1]
     probe = chains[1].run(300)
     ncand = float(probe["ncand"][1:].mean()) if len(probe) > 1 else 0.0
     desc = chains[0].desc
     for c in chains:
         c.close()
-    roof = plain_roofline("ecmc::chain_many_kernel<3>", ncand, 3, 0, int(k.sum()), ms, charged=True)
-    roof["traffic"] = profiled_traffic("ecmc::chain_many_kernel<3>", legs,
-                                       "chain_many_kernel_charged_soft_spheres_counters.txt", "flexible_dipoles")
+    roof = plain_roofline(kname, ncand, 3, 0, int(k.sum()), ms, charged=True)
+    roof["traffic"] = profiled_traffic(kname, legs, "chain_many_kernel_charged_soft_spheres_counters.txt",
+                                       "flexible_dipoles")
     p, q, w = starts[0]
     cpu1 = plain_cpu_single_core(desc, p, seconds=4.0, charges=q)
     return dict(replicas=replicas, n=len(starts[0][0]), legs_per_replica=legs, roofline=roof, cpu_single_core=cpu1,

@@ -193,6 +193,14 @@ class DeviceChain:
         L.check(L.lib().ecmc_system_launch_lds(self._h, ctypes.byref(b), ctypes.byref(p), ctypes.byref(r)))
         return b.value, bool(p.value), bool(r.value)
 
+    def kernel_variant(self):
+        """(features the system's taggers use, features of the specialized leg-loop kernel that
+        runs it): the kernel is ecmc::chain_kernel<dim, variant> alone and
+        ecmc::chain_many_kernel<dim, variant> in a launch of several systems of the same variant."""
+        f, v = ctypes.c_int32(), ctypes.c_int32()
+        L.check(L.lib().ecmc_system_kernel_variant(self._h, ctypes.byref(f), ctypes.byref(v)))
+        return f.value, v.value
+
     def last_kernel_ms(self):
         ms = ctypes.c_double()
         L.check(L.lib().ecmc_system_last_kernel_ms(self._h, ctypes.byref(ms)))

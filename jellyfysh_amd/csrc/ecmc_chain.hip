@@ -2483,6 +2483,13 @@ extern "C" int ecmc_system_launch_lds(const ecmc_system *s, int64_t *bytes, int3
     return ECMC_OK;
 }
 
+extern "C" int ecmc_system_kernel_variant(const ecmc_system *s, int32_t *features, int32_t *variant) {
+    if (!s) return ecmc_fail(ECMC_ERR_ARGUMENT, "null system");
+    if (features) *features = s->features;
+    if (variant) *variant = chain_variant(s->dim, s->features).features;
+    return ECMC_OK;
+}
+
 extern "C" int ecmc_system_last_kernel_ms(const ecmc_system *s, double *ms) {
     if (!s || !ms) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
     *ms = s->last_ms;

@@ -1,6 +1,7 @@
 """Parity of the plain GPU chain and of the device math against the oracle and the
 reference's own event logs.  Integer and event data must agree bit for bit; see
 test_oracle_golden.py for how the oracle itself is pinned."""
+import os
 import random
 
 import numpy as np
@@ -171,3 +172,35 @@ def test_gpu_lj_truncated_bit_exact_against_cr_oracle(device, oracle):
     e = oracle.Engine(S.oracle_config(meta, math_mode=oracle.MATH_CR), pos)
     ref = e.run(legs)
     assert first_difference(log, ref) is None, first_difference(log, ref)
+
+
+@pytest.mark.gpu
+def test_gpu_leg_loop_kernel_variants_cover_the_systems(device):
+    """Each system runs the first specialized leg-loop kernel whose features cover its taggers'
+    (ecmc_system_kernel_variant): LJ as truncated inverse powers the inverse-power variant, the
+    charged soft spheres the Coulomb variant, the flexible dipoles the factor-map variant, 2D hard
+    disks the full kernel; the parity tests of each workload run through these kernels."""
+    from jellyfysh_amd.chain import DeviceChain
+    from jellyfysh_amd.config import RunConfig
+    from jellyfysh_amd.workloads import charged_soft_spheres, make_chain
+    IP, IPT, DEP, HS = device.POT_INVERSE_POWER, device.POT_INVERSE_POWER_TRUNC, device.POT_DISPLACED_EVEN_POWER, \
+        device.POT_HARD_SPHERE
+    COUL, FMAP, ALL = 1 << 9, 1 << 10, (1 << 11) - 1
+
+    def variant(chain):
+        try:
+            return chain.kernel_variant()
+        finally:
+            chain.close()
+
+    meta, pos = S.lj_system(n_cells=3, cutoff=2.5)
+    assert variant(DeviceChain(S.device_desc(meta), pos)) == (1 << IPT, (1 << IP) | (1 << IPT))
+    f, v = variant(make_chain(charged_soft_spheres(n=32, system_length=4.0, seed=1, cells_per_side=4)))
+    assert f == (1 << IP) | COUL and v == f
+    cfg = RunConfig.from_file(os.path.join(os.path.dirname(S.__file__), "..", "jellyfysh_amd", "configs",
+                                           "flexible_dipoles.ini"))
+    p, q, w = cfg.initial_state(0)
+    f, v = variant(DeviceChain(cfg.desc(len(p), 0, rng_skip=w), p, q))
+    assert f == (1 << IP) | (1 << DEP) | COUL | FMAP and v == f
+    meta, pos = S.hard_disk_system(n=64)
+    assert variant(DeviceChain(S.device_desc(meta), pos)) == (1 << HS, ALL)
