@@ -186,9 +186,9 @@ int ecmc_system_last_kernel_ms(const ecmc_system *sys, double *ms);
  * counterpart: the device's memory placement, for diagnostics. */
 int ecmc_system_launch_lds(const ecmc_system *sys, int64_t *bytes, int32_t *positions, int32_t *ring);
 /* The leg-loop kernel that runs the system alone: *features gets the features its taggers use
- * (bit p for the two-leaf pair potential ECMC_POT_p inverted exactly, bit 9 the Coulomb
- * machinery: bounding event handlers with Ewald thinning, cell bounding, cell veto; bit 10
- * factor-type maps), *variant the features of the specialized kernel launched for them (its
+ * (bit p for the two-leaf pair potential ECMC_POT_p inverted exactly, bit 9 the merged-image
+ * Coulomb factors through the inverse-power Coulomb bound, bit 10 factor-type maps, bit 11 cell
+ * bounding, bit 12 the cell veto), *variant the features of the specialized kernel launched for them (its
  * template argument in the kernel's name, ecmc::chain_kernel<dim, variant>).  A launch of
  * several systems runs the first variant covering all of them.  No reference counterpart: which
  * compiled code runs, for diagnostics and profiles. */

@@ -315,15 +315,18 @@ __device__ __forceinline__ bool is_nearby(const ChainArgs &a, int dim, int c, in
 }
 
 // Kernel variants by the features of the systems they run (FM): bit p for the two-leaf pair
-// potential ECMC_POT_p inverted exactly, FEAT_COULOMB for the Coulomb machinery (the bounding
-// event handlers with their thinning by the Ewald derivative, cell bounding, the cell veto) and
-// FEAT_FMAP for factor-type maps.  A variant leaves the code of the other features out of the
-// kernel (a smaller leg loop, fewer instruction-cache misses); the host launches the first
-// variant that covers every feature of its systems.
-constexpr int FEAT_COULOMB = 1 << 9, FEAT_FMAP = 1 << 10, FEAT_ALL = (1 << 11) - 1;
+// potential ECMC_POT_p inverted exactly, FEAT_COULOMB for the merged-image Coulomb factors through
+// the inverse-power Coulomb bound (thinning by the Ewald derivative), FEAT_FMAP for factor-type
+// maps, FEAT_CELL_BOUNDING and FEAT_CELL_VETO for the two far-cell schemes.  A variant leaves the
+// code of the other features out of the kernel (a smaller leg loop, fewer instruction-cache
+// misses); the host launches the first variant that covers every feature of its systems.
+constexpr int FEAT_COULOMB = 1 << 9, FEAT_FMAP = 1 << 10, FEAT_CELL_BOUNDING = 1 << 11, FEAT_CELL_VETO = 1 << 12;
+constexpr int FEAT_ALL = (1 << 13) - 1;
+constexpr int FEAT_EWALD = FEAT_COULOMB | FEAT_CELL_BOUNDING | FEAT_CELL_VETO;   // thinning by the Ewald derivative
 constexpr int FEAT_IP_FILTER = (1 << ECMC_POT_INVERSE_POWER) | (1 << ECMC_POT_INVERSE_POWER_TRUNC);
 constexpr int VARIANT_LJ = FEAT_IP_FILTER;
-constexpr int VARIANT_CHARGED = (1 << ECMC_POT_INVERSE_POWER) | FEAT_COULOMB;
+constexpr int VARIANT_CHARGED_CB = (1 << ECMC_POT_INVERSE_POWER) | FEAT_COULOMB | FEAT_CELL_BOUNDING;
+constexpr int VARIANT_CHARGED_CV = (1 << ECMC_POT_INVERSE_POWER) | FEAT_COULOMB | FEAT_CELL_VETO;
 constexpr int VARIANT_DIPOLES = (1 << ECMC_POT_INVERSE_POWER) | (1 << ECMC_POT_DISPLACED_EVEN_POWER) | FEAT_COULOMB | FEAT_FMAP;
 #define POT_BUILT(p) ((FM >> (p)) & 1)
 
@@ -484,7 +487,7 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
     const int nc = nb_cell[31];
     const bool all = tg.kind == ECMC_TAG_PAIR_ALL;
     const bool sur = tg.kind == ECMC_TAG_SURPLUS;
-    const bool far = (FM & FEAT_COULOMB) && tg.kind == ECMC_TAG_CELL_BOUNDING;
+    const bool far = (FM & FEAT_CELL_BOUNDING) && tg.kind == ECMC_TAG_CELL_BOUNDING;
     const bool fmap = (FM & FEAT_FMAP) && tg.kind == ECMC_TAG_FACTOR_MAP;
     // the far cells of this create (built and read within it): in LDS when the launch has room
     int32_t *const cand = a.lds_cand_off >= 0 ? reinterpret_cast<int32_t *>(chain_dyn_lds + a.lds_cand_off) : a.cand;
@@ -655,7 +658,7 @@ __device__ __forceinline__ void create_pair(const ChainArgs &a, ChainState &st, 
         }
         const double ct = tg.use_charge ? charge_of(a, target) : 1.0;
         double dt;
-        if ((FM & FEAT_COULOMB) && tg.event_handler == ECMC_EH_CELL_BOUNDING) {
+        if ((FM & FEAT_CELL_BOUNDING) && tg.event_handler == ECMC_EH_CELL_BOUNDING) {
             // two_leaf_unit_cell_bounding_potential_event_handler.py:171-177
             const double rate = far ? cell_rate_of_cells<D>(a, tg, st.occ_cell, (int)cand[j], dir, ca, ct)
                                     : cell_rate<D>(a, tg, st.occ_cell, target, dir, ca, ct);
@@ -1110,7 +1113,7 @@ __device__ __forceinline__ void create_tagger(const ChainArgs &a, ChainState &st
     if (c.kind == ECMC_TAG_PAIR_CELLS || c.kind == ECMC_TAG_PAIR_ALL || c.kind == ECMC_TAG_SURPLUS ||
         c.kind == ECMC_TAG_CELL_BOUNDING || c.kind == ECMC_TAG_FACTOR_MAP) {
         create_pair<D, FM>(a, st, mt, c, t, ls.nb_cell, ls.nb_pref, ls.sb, ls.sh, ls.scan);
-    } else if ((FM & FEAT_COULOMB) && c.kind == ECMC_TAG_CELL_VETO) {
+    } else if ((FM & FEAT_CELL_VETO) && c.kind == ECMC_TAG_CELL_VETO) {
         if (tid == 0) { st.slot[t].valid = 0; st.slot[t].cv_phase = 0; }
         __syncthreads();
         // the same bounded refill loop as the end-of-chain randint below
@@ -1179,7 +1182,7 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
         bool lift = true;
         double s[D];
         separation_vector<D>(a, sl.apos, sl.tpos, s);
-        if ((FM & FEAT_COULOMB) && T.event_handler == ECMC_EH_CELL_BOUNDING) {
+        if ((FM & FEAT_CELL_BOUNDING) && T.event_handler == ECMC_EH_CELL_BOUNDING) {
             // two_leaf_unit_cell_bounding_potential_event_handler.py:181-211: the active
             // unit is still in its cell; bound = the candidate's rate; thinning as below
             if (pos_to_cell<D>(a, sl.apos) != sl.cb_cell) { st.err = ECMC_ERR_STATE; break; }
@@ -1228,7 +1231,7 @@ __device__ __forceinline__ void commit_event(const ChainArgs &a, ChainState &st,
         break;
     }
     case ECMC_TAG_CELL_VETO: {
-        if (!(FM & FEAT_COULOMB)) { st.err = ECMC_ERR_CONFIGURATION; break; }
+        if (!(FM & FEAT_CELL_VETO)) { st.err = ECMC_ERR_CONFIGURATION; break; }
         // mediator.py:265-292: the occupant of the sampled cell, or none;
         // leaf_unit_cell_veto_event_handler.py:127-149 with
         // event_handler_with_bounding_potential.py:75-101: thinning of the stored rate
@@ -1409,7 +1412,7 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
         const int w = flag;
         if (commit_thins(tg[w])) ring_ensure(st, mt, a, 2);
         double ewald_pre = NAN;
-        if ((FM & FEAT_COULOMB) && tid < 64) {
+        if ((FM & FEAT_EWALD) && tid < 64) {
             double q3[3];
             if (thinning_separation<D>(a, st, tg[w], w, q3)) ewald_pre = ewald_derivative_wave(a.ewald, a.ewald_codes, a.ewald_ncodes, q3[0], q3[1], q3[2]);
         }
@@ -1494,7 +1497,7 @@ __global__ __launch_bounds__(NT) void chain_step_kernel(const ChainArgs *__restr
     } else {
         if (commit_thins(tg[t])) ring_ensure(st, mt, a, 2);
         double ewald_pre = NAN;
-        if ((FM & FEAT_COULOMB) && tid < 64) {
+        if ((FM & FEAT_EWALD) && tid < 64) {
             double q3[3];
             if (thinning_separation<D>(a, st, tg[t], t, q3)) ewald_pre = ewald_derivative_wave(a.ewald, a.ewald_codes, a.ewald_ncodes, q3[0], q3[1], q3[2]);
         }
@@ -1669,11 +1672,13 @@ static int system_features(const ecmc_system_desc &d) {
                           tg.kind == ECMC_TAG_CELL_BOUNDING || tg.kind == ECMC_TAG_FACTOR_MAP;
         if (pair) {
             if (tg.event_handler == ECMC_EH_TWO_LEAF && tg.potential >= 0 && tg.potential < 9) f |= 1 << tg.potential;
-            else f |= FEAT_COULOMB;
+            else if (tg.event_handler == ECMC_EH_TWO_LEAF_BOUNDING) f |= FEAT_COULOMB;
+            else if (tg.event_handler == ECMC_EH_CELL_BOUNDING) f |= FEAT_CELL_BOUNDING;
+            else f |= FEAT_ALL;
             if (tg.kind == ECMC_TAG_FACTOR_MAP) f |= FEAT_FMAP;
-            if (tg.kind == ECMC_TAG_CELL_BOUNDING) f |= FEAT_COULOMB;
+            if (tg.kind == ECMC_TAG_CELL_BOUNDING) f |= FEAT_CELL_BOUNDING;
         }
-        if (tg.kind == ECMC_TAG_CELL_VETO) f |= FEAT_COULOMB;
+        if (tg.kind == ECMC_TAG_CELL_VETO) f |= FEAT_CELL_VETO;
     }
     return f;
 }
@@ -1687,8 +1692,10 @@ struct ChainVariant {
 static const ChainVariant CHAIN_VARIANTS_3D[] = {
     {VARIANT_LJ, reinterpret_cast<const void *>(&chain_kernel<3, VARIANT_LJ>),
      reinterpret_cast<const void *>(&chain_many_kernel<3, VARIANT_LJ>)},
-    {VARIANT_CHARGED, reinterpret_cast<const void *>(&chain_kernel<3, VARIANT_CHARGED>),
-     reinterpret_cast<const void *>(&chain_many_kernel<3, VARIANT_CHARGED>)},
+    {VARIANT_CHARGED_CB, reinterpret_cast<const void *>(&chain_kernel<3, VARIANT_CHARGED_CB>),
+     reinterpret_cast<const void *>(&chain_many_kernel<3, VARIANT_CHARGED_CB>)},
+    {VARIANT_CHARGED_CV, reinterpret_cast<const void *>(&chain_kernel<3, VARIANT_CHARGED_CV>),
+     reinterpret_cast<const void *>(&chain_many_kernel<3, VARIANT_CHARGED_CV>)},
     {VARIANT_DIPOLES, reinterpret_cast<const void *>(&chain_kernel<3, VARIANT_DIPOLES>),
      reinterpret_cast<const void *>(&chain_many_kernel<3, VARIANT_DIPOLES>)},
     {FEAT_ALL, reinterpret_cast<const void *>(&chain_kernel<3, FEAT_ALL>),

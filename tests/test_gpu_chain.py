@@ -185,7 +185,7 @@ def test_gpu_leg_loop_kernel_variants_cover_the_systems(device):
     from jellyfysh_amd.workloads import charged_soft_spheres, make_chain
     IP, IPT, DEP, HS = device.POT_INVERSE_POWER, device.POT_INVERSE_POWER_TRUNC, device.POT_DISPLACED_EVEN_POWER, \
         device.POT_HARD_SPHERE
-    COUL, FMAP, ALL = 1 << 9, 1 << 10, (1 << 11) - 1
+    COUL, FMAP, CB, ALL = 1 << 9, 1 << 10, 1 << 11, (1 << 13) - 1
 
     def variant(chain):
         try:
@@ -196,7 +196,7 @@ def test_gpu_leg_loop_kernel_variants_cover_the_systems(device):
     meta, pos = S.lj_system(n_cells=3, cutoff=2.5)
     assert variant(DeviceChain(S.device_desc(meta), pos)) == (1 << IPT, (1 << IP) | (1 << IPT))
     f, v = variant(make_chain(charged_soft_spheres(n=32, system_length=4.0, seed=1, cells_per_side=4)))
-    assert f == (1 << IP) | COUL and v == f
+    assert f == (1 << IP) | COUL | CB and v == f
     cfg = RunConfig.from_file(os.path.join(os.path.dirname(S.__file__), "..", "jellyfysh_amd", "configs",
                                            "flexible_dipoles.ini"))
     p, q, w = cfg.initial_state(0)
