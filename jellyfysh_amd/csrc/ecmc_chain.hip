@@ -1394,18 +1394,34 @@ __device__ __forceinline__ void chain_body(const ChainArgs &a) {
             __syncthreads();
             if (st.err) break;
         }
-        // earliest stored event (ties: first pushed)
-        if (tid == 0) {
-            int w = -1;
-            for (int t = 0; t < a.n_taggers; t++) {
-                const Slot &s = st.slot[t];
-                if (!s.valid) continue;
-                if (w < 0 || time_lt(s.t, st.slot[w].t) || (time_eq(s.t, st.slot[w].t) && s.seq < st.slot[w].seq)) w = t;
+        // earliest stored event (ties: first pushed): lane t of the first wavefront holds slot t,
+        // a butterfly over the ECMC_MAX_TAGGERS lanes keeps the least (time, push counter) of the
+        // valid slots (push counters are distinct, so the result is the sequential scan's)
+        static_assert(ECMC_MAX_TAGGERS == 16, "the select butterfly spans 16 lanes");
+        if (tid < 64) {
+            const Slot &sl = st.slot[tid < ECMC_MAX_TAGGERS ? tid : 0];
+            const bool v = tid < a.n_taggers && sl.valid;
+            double q = v ? sl.t.q : 0.0, r = v ? sl.t.r : 0.0;
+            int64_t sq = v ? sl.seq : 0;
+            int w = v ? tid : -1;
+#pragma unroll
+            for (int off = 1; off < ECMC_MAX_TAGGERS; off <<= 1) {
+                const double oq = __shfl_xor(q, off, 64), orr = __shfl_xor(r, off, 64);
+                const int64_t os = __shfl_xor((long long)sq, off, 64);
+                const int ow = __shfl_xor(w, off, 64);
+                if (ow >= 0 && (w < 0 || oq < q || (oq == q && (orr < r || (orr == r && os < sq))))) {
+                    q = oq;
+                    r = orr;
+                    sq = os;
+                    w = ow;
+                }
             }
-            if (w < 0) st.err = ECMC_ERR_SCHEDULER;
-            else if (time_lt(st.slot[w].t, st.last)) st.err = ECMC_ERR_SCHEDULER;
-            else st.last = st.slot[w].t;
-            flag = w;
+            if (tid == 0) {
+                if (w < 0) st.err = ECMC_ERR_SCHEDULER;
+                else if (time_lt(st.slot[w].t, st.last)) st.err = ECMC_ERR_SCHEDULER;
+                else st.last = st.slot[w].t;
+                flag = w;
+            }
         }
         __syncthreads();
         if (st.err) break;
