@@ -10,18 +10,24 @@
 // Mapping to the hardware:
 //   * the candidate set of a pair tagger (active unit x occupants of the nearby cells,
 //     excluded_cells_tagger.py:129-132) is spread over the 256 threads of the workgroup in
-//     canonical order (nearby cells in itertools.product order, occupants in list order);
-//     positions are read from the SoA arrays in HBM (x[], y[], z[]), one candidate per lane;
+//     canonical order (nearby cells in itertools.product order, occupants in list order),
+//     one candidate per lane; positions (SoA x[], y[], z[]), the random ring, charges and cell
+//     lists are read from LDS where the launch placed them, else from HBM;
 //   * each candidate's exponential draw is the 2 MT19937 words at its rank in the stream;
-//     words come from a ring in HBM that the workgroup refills 624 words at a time with the
+//     words come from a ring that the workgroup refills 624 words at a time with the
 //     three-phase parallel twist, so the draws are those of CPython's random in the
 //     reference's call order;
+//   * the inverse-power candidates are first bracketed in FP32 and only those that can be the
+//     earliest are inverted exactly in FP64;
 //   * the earliest candidate is found by a wavefront shuffle min-reduce over (time, push
 //     rank) followed by a 4-wave LDS reduction; the scheduler keeps, per tagger, the
 //     earliest of its pushed events (all events of one tagger are trashed together), and
-//     ties go to the first pushed event as in list_scheduler.py:129-156.
+//     ties go to the first pushed event as in list_scheduler.py:129-156; the earliest stored
+//     event is chosen by a butterfly over the first wavefront's lanes, one slot per lane;
 //   * bookkeeping that is sequential in the reference (occupancy lists, end of chain,
-//     sampling, out-states) runs on thread 0 between barriers.
+//     sampling, out-states) runs on thread 0 between barriers;
+//   * the leg loop is compiled once per feature set (FEAT_*) and each launch runs the smallest
+//     variant that covers its systems.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
