@@ -405,11 +405,37 @@ __device__ __forceinline__ bool best_lt(const Best &x, const Best &y) {
     return false;
 }
 
-// Collective workgroup min-reduce of (time, index) and sum of the hash terms.
+// One butterfly step within the 16-lane rows by a DPP lane permutation (CTRL: 0xB1 swaps
+// neighbours, 0x4E pairs of neighbours, 0x141 mirrors each half row, 0x140 the row): after the
+// four steps every lane of a row holds the row's least (time, index) and its hash sum.
+template <int CTRL>
+__device__ __forceinline__ int dpp_move(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false); }
+template <int CTRL>
+__device__ __forceinline__ double dpp_move_d(double v) {
+    return __hiloint2double(dpp_move<CTRL>(__double2hiint(v)), dpp_move<CTRL>(__double2loint(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ void best_dpp_step(Best &b, uint32_t &h) {
+    Best o;
+    o.q = dpp_move_d<CTRL>(b.q);
+    o.r = dpp_move_d<CTRL>(b.r);
+    o.j = dpp_move<CTRL>(b.j);
+    const uint32_t oh = (uint32_t)dpp_move<CTRL>((int)h);
+    if (best_lt(o, b)) b = o;
+    h += oh;
+}
+
+// Collective workgroup min-reduce of (time, index) and sum of the hash terms: DPP steps within
+// each row, shuffles across the rows, LDS across the wavefronts.  The order is total (indices
+// are distinct) and the sum wraps, so the result does not depend on the reduction order.
 __device__ __forceinline__ void block_reduce(Best &b, uint32_t &h, Best *sb, uint32_t *sh) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    best_dpp_step<0xB1>(b, h);
+    best_dpp_step<0x4E>(b, h);
+    best_dpp_step<0x141>(b, h);
+    best_dpp_step<0x140>(b, h);
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
+    for (int off = 32; off >= 16; off >>= 1) {
         Best o;
         o.q = __shfl_xor(b.q, off, 64);
         o.r = __shfl_xor(b.r, off, 64);
@@ -433,8 +459,12 @@ __device__ __forceinline__ void block_reduce(Best &b, uint32_t &h, Best *sb, uin
 
 // Collective workgroup minimum of a non-negative float (+inf allowed); sh holds NT / 64 words.
 __device__ __forceinline__ float block_min_float(float v, uint32_t *sh) {
+    v = fminf(v, __int_as_float(dpp_move<0xB1>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(dpp_move<0x4E>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(dpp_move<0x141>(__float_as_int(v))));
+    v = fminf(v, __int_as_float(dpp_move<0x140>(__float_as_int(v))));
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off >= 16; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = __float_as_uint(v);
     __syncthreads();
     float r = __uint_as_float(sh[0]);
