@@ -193,6 +193,9 @@ int ecmc_system_launch_lds(const ecmc_system *sys, int64_t *bytes, int32_t *posi
  * several systems runs the first variant covering all of them.  No reference counterpart: which
  * compiled code runs, for diagnostics and profiles. */
 int ecmc_system_kernel_variant(const ecmc_system *sys, int32_t *features, int32_t *variant);
+/* The same for a descriptor, before any system exists (no device call).  No reference
+ * counterpart. */
+int ecmc_desc_kernel_variant(const ecmc_system_desc *desc, int32_t *features, int32_t *variant);
 /* Independent systems advanced together: one launch, one workgroup per system, each system
  * committing up to max_events legs exactly as ecmc_system_run would (no event logs).  All
  * systems have one dimension and appear once; n_committed (count entries, may be NULL) gets

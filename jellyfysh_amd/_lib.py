@@ -86,6 +86,7 @@ def _signatures():
         "ecmc_system_launch_lds": (i32, [vp, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_int32)]),
         "ecmc_system_kernel_variant": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+        "ecmc_desc_kernel_variant": (i32, [vp, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
         "ecmc_systems_run": (i32, [vp, i32, i64, pi64]),
         "ecmc_systems_run_noalloc": (i32, [vp, i32, i64, pi64]),
         "ecmc_system_begin_leg": (i32, [vp]),

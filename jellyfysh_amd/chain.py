@@ -49,6 +49,14 @@ def fill_factor_map(g, factor_map, leaves_per_root):
             g.fmap_target[k][e] = int(tl)
 
 
+def kernel_variant_of(desc):
+    """(features the descriptor's taggers use, features of the specialized leg-loop kernel that
+    would run its system): ecmc_desc_kernel_variant, no device needed."""
+    f, v = ctypes.c_int32(), ctypes.c_int32()
+    L.check(L.lib().ecmc_desc_kernel_variant(ctypes.byref(desc), ctypes.byref(f), ctypes.byref(v)))
+    return f.value, v.value
+
+
 def make_desc(dim, n, system_length, beta, cells_per_side, taggers, seed, scheduler="heap", neighbor_layers=1,
               max_occupants=-1, rng_skip=0, leaves_per_root=1):
     """Fill an ecmc_system_desc.

@@ -2549,6 +2549,17 @@ extern "C" int ecmc_system_kernel_variant(const ecmc_system *s, int32_t *feature
     return ECMC_OK;
 }
 
+extern "C" int ecmc_desc_kernel_variant(const ecmc_system_desc *desc, int32_t *features, int32_t *variant) {
+    if (!desc) return ecmc_fail(ECMC_ERR_ARGUMENT, "null descriptor");
+    if (desc->dim != 2 && desc->dim != 3) return ecmc_fail(ECMC_ERR_CONFIGURATION, "dimension must be 2 or 3");
+    if (desc->n_taggers < 0 || desc->n_taggers > ECMC_MAX_TAGGERS)
+        return ecmc_fail(ECMC_ERR_CONFIGURATION, "number of taggers out of range");
+    const int f = system_features(*desc);
+    if (features) *features = f;
+    if (variant) *variant = chain_variant(desc->dim, f).features;
+    return ECMC_OK;
+}
+
 extern "C" int ecmc_system_last_kernel_ms(const ecmc_system *s, double *ms) {
     if (!s || !ms) return ecmc_fail(ECMC_ERR_ARGUMENT, "null argument");
     *ms = s->last_ms;
